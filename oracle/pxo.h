@@ -1,0 +1,175 @@
+/*
+ * pxo.h -- CPU oracle for the PeriX PDDO hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is a plain-C FP64 restatement of the reference algorithms, used by the
+ * test-suite as the parity checker and by bench.py as the timed CPU baseline.
+ * It is never linked into, loaded by, or called from the product library
+ * (perix_amd/csrc); the product path fails loudly when its HIP library is
+ * missing instead of falling back here.
+ *
+ * Every function cites the reference file:line it restates.  Built with
+ * -ffp-contract=off so that the x86-64 arithmetic matches the reference's
+ * own (SSE2, no FMA) build bit for bit.
+ *
+ * Conventions (same as the reference):
+ *   node ids inside neighbour lists are 1-based; coords are interleaved x,y,z
+ *   (z = 0 in 2D); displacement / unknown vectors are node-major (i*ndof+a).
+ */
+#ifndef PXO_H
+#define PXO_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- a1: structured point cloud with one face-ghost layer ---------------- */
+typedef struct {
+    int dim;            /* 2 (quad4) or 3 (hex8) */
+    int nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} pxo_grid;
+
+/* Number of PD nodes (bulk + ghosts) and bulk nodes. */
+int pxo_mesh_sizes(const pxo_grid *g, int *n_nodes, int *n_bulk);
+/* coords[3N], vols[N], mirror[N] (1-based mirror bulk id, 0 for bulk),
+ * wall[N] (-1 bulk, else 0..5 = left,right,bottom,top,back,front),
+ * dxyz[3] = DX,DY,DZ.  Returns 0 on success. */
+int pxo_mesh_build(const pxo_grid *g, double *coords, double *vols, int *mirror,
+                   int *wall, double *dxyz);
+/* Horizon radius factor*dx*(1+1e-6) with dx = |xmax-xmin|/nx. */
+double pxo_horizon(const pxo_grid *g, double factor);
+
+/* ---- a2: families in the reference sweep order --------------------------- */
+/* Two calls: with nbr==NULL fills off[N+1] and returns nb; then fills nbr. */
+long long pxo_families(int n, const double *coords, int is3d, double r,
+                       int *off, int *nbr);
+
+/* ---- a3: crack cut ------------------------------------------------------- */
+typedef struct {
+    int nseg;  const double *seg;    /* nseg x {x1,y1,x2,y2} */
+    int nquad; const double *quad;   /* nquad x 12 corners   */
+} pxo_cracks;
+int pxo_crack_cuts_bond(const pxo_cracks *c, int n, const double *coords,
+                        const int *mirror, int i1, int j1);
+/* health seed (1.0 or 0.0) for every directed bond (force-only cracks) */
+void pxo_seed_health(const pxo_cracks *c, int n, const double *coords,
+                     const int *mirror, const int *off, const int *nbr,
+                     double *health);
+
+/* ---- a4: CSR pattern ----------------------------------------------------- */
+long long pxo_csr_pattern(int n, int ndof, const int *off, const int *nbr,
+                          const int *mirror, int *rowptr, int *cols);
+
+/* ---- a5-a8: PDDO operators ---------------------------------------------- */
+int pxo_nop(int dim, int order);
+void pxo_multi_indices(int dim, int order, int *qp, int *qq, int *qr);
+/* Full operator vectors (nop per bond, family order) and per-node flags
+ * (bit0 = degenerate, bit1 = usable<nop).  Returns number of degenerate nodes. */
+int pxo_operators(int n, int dim, int order, const double *coords,
+                  const double *vols, const int *off, const int *nbr,
+                  double delta, double dx_char, int vol_corr,
+                  double *ops, unsigned char *flags, int nthreads);
+
+/* ---- a11: Poisson CSR assembly ------------------------------------------ */
+void pxo_assemble_poisson(int n, int dim, int nop, const int *off, const int *nbr,
+                          const double *ops, const int *rowptr, const int *cols,
+                          double sigma, const double *f_node, double f_const,
+                          const double *u, double *kvals, double *rhs);
+
+/* ---- a12/a13: Cahn-Hilliard -------------------------------------------- */
+void pxo_ch_geometry(int n, int dim, int nop, const double *coords,
+                     const int *off, const int *nbr, const double *ops,
+                     const int *mirror, double *lapbond, int *rev,
+                     unsigned char *isghost);
+void pxo_ch_mobility(int n, const double *u, double M0, double *mnode);
+void pxo_assemble_ch(int n, const int *off, const int *nbr, const int *rev,
+                     const double *lapbond, const unsigned char *isghost,
+                     const double *vol, const double *mnode,
+                     double kappa, double chi, double dt,
+                     const int *rowptr, const int *cols,
+                     const double *u, const double *uold,
+                     double *kvals, double *rhs);
+
+/* ---- a15-a17: explicit PDDO fracture ------------------------------------ */
+typedef struct {
+    int dim;
+    double C11, C12, C66, rho, bx, by, bz, sc;
+    int tension_only, damage_on;
+    double nu; int plane_stress;   /* used only with a per-node E array */
+} pxo_frac;
+void pxo_elastic_constants(int dim, int plane_stress, double E, double nu,
+                           double *C11, double *C12, double *C66);
+double pxo_critical_stretch(double G0, double E, double delta, int dim,
+                            int plane_stress);
+double pxo_stable_dt(int dim, int plane_stress, double E, double nu,
+                     double rho, const double *dxyz);
+/* Element op cache (5 per bond in 2D, 9 in 3D) from the full operators. */
+void pxo_frac_opcache(int n, int dim, const int *off, const double *ops,
+                      double *opc);
+/* One preprocessIteration: damage then Navier force then accel.
+ * node_E (optional, may be NULL): per-node Young's modulus; node i's family
+ * sum then uses C(E_i) (same formulas as the uniform constants). */
+void pxo_frac_accel(int n, const pxo_frac *p, const double *coords,
+                    const int *off, const int *nbr, const double *opc,
+                    double *health, const double *u, const double *node_E,
+                    double *accel, int nthreads);
+
+/* ---- a20: BC presets ---------------------------------------------------- */
+typedef struct {
+    int type;          /* 0 = neumann (mirror copy), 1 = dirichlet */
+    int direct;        /* dirichlet: 1 = write g(t), 0 = reflection 2g-u_b */
+    int ncomp;         /* constrained components (0-based) */
+    int comp[3];
+    double value[3], velocity[3];
+    int has_box; double box[6];   /* xlo,xhi,ylo,yhi,zlo,zhi (lo>hi = open) */
+    int nnodes; const int *nodes; /* 1-based node ids */
+} pxo_bc;
+void pxo_bc_preset(int n, int ndof, const double *coords, const int *mirror,
+                   int nbc, const pxo_bc *bcs, double *u, double t);
+
+/* ---- a19: central-difference driver ------------------------------------- */
+/* Step schedule of TimeStepping::solveExplicitDynamics: returns the number of
+ * steps; fills dts[k], ts[k] (tNew) when non-NULL (capacity cap). */
+int pxo_schedule(double dt, double total, int cap, double *dts, double *ts);
+/* Runs the whole explicit loop.  u/uold/v0 are N*dim.  On entry u holds the
+ * preset initial state; health the seeded bond health.  break_step (optional,
+ * nb ints) receives the step at which each bond broke (-1 = seeded broken,
+ * 0 = broken by the t=0 rate evaluation, k = during step k, INT32_MAX = intact).  Returns steps done (negative = blow-up at -step). */
+int pxo_explicit_run(int n, const pxo_frac *p, const double *coords,
+                     const int *off, const int *nbr, const double *opc,
+                     const int *mirror, int nbc, const pxo_bc *bcs,
+                     double dt, double total, double *u, double *uold,
+                     const double *v0, double *health, int *break_step,
+                     const double *node_E, int nthreads);
+
+/* ---- a21: RandomIC (std::mt19937_64, top 53 bits) ------------------------ */
+void pxo_random_ic(uint64_t seed, double lo, double hi, int nnodes,
+                   const int *nodes, int ndof, int ndofs_set,
+                   const int *dofs, double *values);
+
+/* ---- a22: adjoint of the explicit step (new; no reference) -------------- */
+/* Uniform steps t_k = t_{k-1}+dt, k=1..nsteps (the schedule of the explicit
+ * driver when no step is clamped).
+ * Loss J = sum_s u^N[sens_s*dim+comp]^2.  Health frozen from break_step.
+ * node_E = per-node Young's modulus (required; C constants of node i follow E_i).
+ * grad_e receives dJ/dE_i. Returns J. */
+double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
+                        const int *off, const int *nbr, const double *opc,
+                        const int *mirror, int nbc, const pxo_bc *bcs,
+                        double dt, int nsteps, const double *u0,
+                        const double *v0, const int *break_step,
+                        const double *node_E, int nsens, const int *sens,
+                        int comp, double *grad_e, int nthreads);
+/* Forward of the same frozen-health map (for finite differences). */
+double pxo_frac_loss_frozen(int n, const pxo_frac *p, const double *coords,
+                            const int *off, const int *nbr, const double *opc,
+                            const int *mirror, int nbc, const pxo_bc *bcs,
+                            double dt, int nsteps, const double *u0,
+                            const double *v0, const int *break_step,
+                            const double *node_E, int nsens, const int *sens,
+                            int comp, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+/*
+ * pxo_elements.c -- oracle (test infrastructure): Poisson and Cahn-Hilliard
+ * CSR assembly, restated from the reference's serial assembler.
+ */
+#include "pxo.h"
+#include <math.h>
+#include <string.h>
+
+/* SparseMatrix::findEntry (SparseMatrix.cpp:187-197): binary search */
+static int find_slot(const int *rowptr, const int *cols, int row, int col)
+{
+    int lo = rowptr[row], hi = rowptr[row + 1];
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (cols[mid] < col) lo = mid + 1; else hi = mid;
+    }
+    return (lo < rowptr[row + 1] && cols[lo] == col) ? lo : -1;
+}
+
+/* ElementBase::pdLaplacian (ElementBase.h:317-321) on the full op vector */
+static double lap_of(const double *op, int dim)
+{
+    if (dim >= 3) return op[3] + op[6] + op[8];   /* d2/dx2 + d2/dy2 + d2/dz2 */
+    return op[2] + op[4];                        /* d2/dx2 + d2/dy2 */
+}
+
+/* PDSystem::formResidualAndJacobian (FormResidualAndJacobian.cpp:25-132)
+ * with PoissonElement callbacks (PoissonElement.cpp:26-58). ndof = 1.
+ * f_node (optional) generalises the reference's scalar source m_F. */
+void pxo_assemble_poisson(int n, int dim, int nop, const int *off, const int *nbr,
+                          const double *ops, const int *rowptr, const int *cols,
+                          double sigma, const double *f_node, double f_const,
+                          const double *u, double *kvals, double *rhs)
+{
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[n]);
+    memset(rhs, 0, sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int d = find_slot(rowptr, cols, i, i);
+        rhs[i] += f_node ? f_node[i] : f_const;
+        kvals[d] += -0.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const double coeff = sigma * lap_of(ops + (size_t)t * nop, dim);
+            rhs[i] += coeff * (u[j] - u[i]);
+            kvals[d] += -(-coeff);
+            kvals[find_slot(rowptr, cols, i, j)] += -coeff;
+        }
+    }
+}
+
+/* CahnHilliardElement::preprocessIteration one-off geometry
+ * (CahnHilliardElement.cpp:93-153). */
+void pxo_ch_geometry(int n, int dim, int nop, const double *coords,
+                     const int *off, const int *nbr, const double *ops,
+                     const int *mirror, double *lapbond, int *rev,
+                     unsigned char *isghost)
+{
+    for (int i = 0; i < n; i++) isghost[i] = (mirror && mirror[i] > 0) ? 1 : 0;
+    for (int i = 0; i < n; i++) {
+        const double xi0 = coords[3 * i], yi0 = coords[3 * i + 1];
+        const double zi0 = (dim >= 3) ? coords[3 * i + 2] : 0.0;
+        const double qi = xi0 * xi0 + yi0 * yi0 + zi0 * zi0;
+        double lapq = 0.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const double w = lap_of(ops + (size_t)t * nop, dim);
+            lapbond[t] = w;
+            if (!isghost[j]) {
+                const double xj = coords[3 * j], yj = coords[3 * j + 1];
+                const double zj = (dim >= 3) ? coords[3 * j + 2] : 0.0;
+                lapq += w * (xj * xj + yj * yj + zj * zj - qi);
+            }
+        }
+        const double lam = (lapq > 1.0e-12) ? (2.0 * (double)dim / lapq) : 1.0;
+        for (int t = off[i]; t < off[i + 1]; t++)
+            if (!isghost[nbr[t] - 1]) lapbond[t] *= lam;
+    }
+    for (int i = 0; i < n; i++)
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            int r = -1;
+            for (int s = off[j]; s < off[j + 1]; s++)
+                if (nbr[s] == i + 1) { r = s; break; }
+            rev[t] = r;
+        }
+}
+
+static double clampc(double c)
+{
+    const double e = 1.0e-8;
+    return fmax(e, fmin(1.0 - e, c));
+}
+
+/* mobility M(c)=M0 c(1-c) (CahnHilliardElement.cpp:46-49), per node */
+void pxo_ch_mobility(int n, const double *u, double M0, double *mnode)
+{
+    for (int i = 0; i < n; i++) {
+        const double cc = clampc(u[2 * i]);
+        mnode[i] = M0 * cc * (1.0 - cc);
+    }
+}
+
+/* Serial assembler with the CH callbacks (CahnHilliardElement.cpp:178-292).
+ * ndof = 2 (c, mu). */
+void pxo_assemble_ch(int n, const int *off, const int *nbr, const int *rev,
+                     const double *lapbond, const unsigned char *isghost,
+                     const double *vol, const double *mnode,
+                     double kappa, double chi, double dt,
+                     const int *rowptr, const int *cols,
+                     const double *u, const double *uold,
+                     double *kvals, double *rhs)
+{
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[2 * n]);
+    memset(rhs, 0, sizeof(double) * (size_t)n * 2);
+    for (int i = 0; i < n; i++) {
+        const int r0 = 2 * i, r1 = 2 * i + 1;
+        const int d00 = find_slot(rowptr, cols, r0, r0), d01 = find_slot(rowptr, cols, r0, r1);
+        const int d10 = find_slot(rowptr, cols, r1, r0), d11 = find_slot(rowptr, cols, r1, r1);
+        const double c = u[r0], mu = u[r1], cold = uold[r0];
+        const double cc = clampc(c);
+        const double fp = log(cc / (1.0 - cc)) + chi * (1.0 - 2.0 * cc);
+        const double fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * chi;
+        /* nodal block: R=[(c-cold)/dt, mu-f'], K_node=[[1/dt,0],[-f'',1]] */
+        rhs[r0] += (c - cold) / dt;
+        rhs[r1] += mu - fp;
+        kvals[d00] += -(1.0 / dt);
+        kvals[d01] += -0.0;
+        kvals[d10] += -(-fpp);
+        kvals[d11] += -1.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const int ghostJ = isghost[j];
+            const double g2ij = lapbond[t];
+            double coeff_c = 0.0;
+            if (!ghostJ) {
+                const double Mi = mnode[i], Mj = mnode[j], Vi = vol[i], Vj = vol[j];
+                const int rv = rev[t];
+                double wij;
+                if (rv >= 0 && Vi > 0.0) wij = 0.5 * (Mi * g2ij + (Vj / Vi) * Mj * lapbond[rv]);
+                else wij = Mi * g2ij;
+                coeff_c = -wij;
+            }
+            const double coeff_mu = ghostJ ? 0.0 : (kappa * g2ij);
+            const double dC = u[2 * j] - u[r0];
+            const double dMu = u[2 * j + 1] - u[r1];
+            /* LocalR = [coeff_c dMu, coeff_mu dC];
+             * K_II = [[0,-coeff_c],[-coeff_mu,0]], K_IJ = -K_II */
+            rhs[r0] += coeff_c * dMu;
+            rhs[r1] += coeff_mu * dC;
+            const int j0 = 2 * j, j1 = 2 * j + 1;
+            kvals[d00] += -0.0;
+            kvals[d01] += -(-coeff_c);
+            kvals[d10] += -(-coeff_mu);
+            kvals[d11] += -0.0;
+            kvals[find_slot(rowptr, cols, r0, j0)] += -0.0;
+            kvals[find_slot(rowptr, cols, r0, j1)] += -coeff_c;
+            kvals[find_slot(rowptr, cols, r1, j0)] += -coeff_mu;
+            kvals[find_slot(rowptr, cols, r1, j1)] += -0.0;
+        }
+    }
+}

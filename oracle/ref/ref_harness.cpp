@@ -1,0 +1,246 @@
+// ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A thin extern "C" driver over the reference's own C++ classes, compiled
+// together with the reference sources where they lie under the read-only
+// reference tree (see oracle/ref/Makefile).  It is used only to produce
+// golden vectors and to validate the C oracle (oracle/pxo_*.c).  Nothing in
+// the product links it.
+//
+// The harness reads object state the reference keeps private (PDProblem's
+// members, the explicit element's health/op caches, the CH element's bond
+// Laplacian) by compiling the reference headers with `private` mapped to
+// `public`; standard headers are included first so that only the reference
+// classes are affected.  Object layout is unchanged by this.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+#include <fstream>
+#include <sstream>
+#include <iostream>
+#include <chrono>
+#include <filesystem>
+#include <nlohmann/json.hpp>
+#include <Eigen/Dense>
+#define private public
+#define protected public
+#include "PDProblem/PDProblem.h"
+#include "ElmtSystem/ExplicitPDDOFracElement.h"
+#include "ElmtSystem/CahnHilliardElement.h"
+#include "ElmtSystem/PoissonElement.h"
+#undef private
+#undef protected
+
+namespace {
+struct RefCase {
+    PDProblem prob;
+    std::string deck;
+};
+}
+
+extern "C" {
+
+// Runs PDProblem::init on a JSON deck (mesh, families, operators setup).
+void *ref_open(const char *deck)
+{
+    auto *c = new RefCase();
+    c->deck = deck;
+    std::string a0 = "perix", a1 = "-i", a2 = deck;
+    char *argv[3] = {a0.data(), a1.data(), a2.data()};
+    c->prob.init(3, argv);
+    return c;
+}
+
+void ref_close(void *h) { delete static_cast<RefCase *>(h); }
+
+// out: N, nb, dim, ndof, nnz (0 for explicit), nbulk
+void ref_info(void *h, long long *out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const auto &md = p.m_PDMesh.getDataConstRef();
+    long long nb = 0;
+    for (const auto &f : md.NodesNeighNodesID) nb += (long long)f.size();
+    out[0] = md.NodesNum;
+    out[1] = nb;
+    out[2] = p.m_Mesh.getMeshDataRef().MeshDim;
+    out[3] = p.m_ElmtSystem.getMaxDofsPerNode();
+    out[4] = p.m_ElmtSystem.isExplicit() ? 0 : p.m_Matrix.getNNZNum();
+    out[5] = md.BulkElmtsNum;
+}
+
+double ref_horizon(void *h)
+{
+    return static_cast<RefCase *>(h)->prob.m_PDMesh.getDataConstRef().HorizonRadius;
+}
+
+void ref_dxyz(void *h, double *d)
+{
+    const auto &md = static_cast<RefCase *>(h)->prob.m_PDMesh.getDataConstRef();
+    d[0] = md.DX; d[1] = md.DY; d[2] = md.DZ;
+}
+
+void ref_mesh(void *h, double *coords, double *vols, int *mirror)
+{
+    const auto &md = static_cast<RefCase *>(h)->prob.m_PDMesh.getDataConstRef();
+    std::copy(md.NodeCoords.begin(), md.NodeCoords.end(), coords);
+    std::copy(md.NodeVolumes.begin(), md.NodeVolumes.end(), vols);
+    for (int i = 0; i < md.NodesNum; i++)
+        mirror[i] = (i < (int)md.GhostMirrorBulkID.size()) ? md.GhostMirrorBulkID[i] : 0;
+}
+
+void ref_families(void *h, int *off, int *nbr)
+{
+    const auto &md = static_cast<RefCase *>(h)->prob.m_PDMesh.getDataConstRef();
+    long long p = 0;
+    off[0] = 0;
+    for (int i = 0; i < md.NodesNum; i++) {
+        for (int j : md.NodesNeighNodesID[i]) nbr[p++] = j;
+        off[i + 1] = (int)p;
+    }
+}
+
+// returns the group size; fills ids (1-based) when non-null
+int ref_group(void *h, const char *name, int *ids)
+{
+    const auto &v = static_cast<RefCase *>(h)->prob.m_PDMesh.getNodeIDsViaPhyName(name);
+    if (ids) std::copy(v.begin(), v.end(), ids);
+    return (int)v.size();
+}
+
+// full operator vectors (nop per bond, family order); flags[i]=1 if degenerate
+int ref_ops(void *h, double *ops, unsigned char *flags)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const auto &md = p.m_PDMesh.getDataConstRef();
+    PDOperators op = p.m_Operators;
+    op.setAbortOnDegenerate(false);
+    const int nop = op.getOperatorsVecSize();
+    long long t = 0;
+    for (int i = 1; i <= md.NodesNum; i++) {
+        op.calcAMatrix(i, md);
+        if (flags) flags[i - 1] = op.m_LastFamilyDegenerate ? 1 : 0;
+        for (int j : md.NodesNeighNodesID[i - 1]) {
+            op.calcOperators(i, j, md);
+            for (int k = 1; k <= nop; k++) ops[t * nop + k - 1] = op.m_OperatorsVec(k);
+            t++;
+        }
+    }
+    return nop;
+}
+
+// CSR pattern of the implicit matrix
+void ref_csr(void *h, int *rowptr, int *cols)
+{
+    auto &m = static_cast<RefCase *>(h)->prob.m_Matrix;
+    const auto r = m.getCSRRowsIndexCopy();
+    const auto c = m.getCSRColsIndexCopy();
+    std::copy(r.begin(), r.end(), rowptr);
+    std::copy(c.begin(), c.end(), cols);
+}
+
+// Initial state exactly as PDProblem::run: zero, ICs, BC presets (t=0).
+void ref_initial_state(void *h, double *u, double *v0)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    p.m_U.setToZeros();
+    p.m_InitialVelocity.setToZeros();
+    p.m_ICSystem.applyInitialConditions(p.m_PDMesh, ndof, p.m_U, p.m_InitialVelocity);
+    p.m_BCSystem.presetSolution(p.m_PDMesh, ndof, p.m_U);
+    p.m_Uold = p.m_U;
+    const int n = p.m_U.getSize();
+    std::copy(p.m_U.getDataPtr(), p.m_U.getDataPtr() + n, u);
+    if (v0) std::copy(p.m_InitialVelocity.getDataPtr(), p.m_InitialVelocity.getDataPtr() + n, v0);
+}
+
+// BC preset on a caller vector at time t (BCSystem::presetSolution)
+void ref_preset(void *h, double *u, double t)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    VectorXd U(p.m_U.getSize());
+    std::copy(u, u + U.getSize(), U.getDataPtr());
+    p.m_BCSystem.presetSolution(p.m_PDMesh, ndof, U, t);
+    std::copy(U.getDataPtr(), U.getDataPtr() + U.getSize(), u);
+}
+
+// One formResidualAndJacobian at (u, uold) with Info.Dt = dt (serial path).
+void ref_assemble(void *h, const double *u, const double *uold, double dt,
+                  double *kvals, double *rhs)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int n = p.m_U.getSize();
+    VectorXd U(n), Uo(n);
+    std::copy(u, u + n, U.getDataPtr());
+    std::copy(uold, uold + n, Uo.getDataPtr());
+    LocalElmtInfo info;
+    info.Dt = dt; info.T = dt; info.Timestep = 1;
+    p.m_PDSystem.formResidualAndJacobian(p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, info,
+                                         U, Uo, p.m_Matrix, p.m_RHS);
+    const int nnz = p.m_Matrix.getNNZNum();
+    std::copy(p.m_Matrix.getCSRValuesPtr(), p.m_Matrix.getCSRValuesPtr() + nnz, kvals);
+    std::copy(p.m_RHS.getDataPtr(), p.m_RHS.getDataPtr() + n, rhs);
+}
+
+// CH element geometry (valid after one ref_assemble): lapbond, rev
+void ref_ch_geometry(void *h, double *lapbond, int *rev, double *mnode)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<CahnHilliardElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    std::copy(e->m_LapBond.begin(), e->m_LapBond.end(), lapbond);
+    std::copy(e->m_Rev.begin(), e->m_Rev.end(), rev);
+    if (mnode) std::copy(e->m_Mnode.begin(), e->m_Mnode.end(), mnode);
+}
+
+// Explicit run from the deck's TimeStepping block.  Returns steps, fills the
+// final u, uold, health and the element's op cache (NOps per bond).
+int ref_explicit(void *h, double *u, double *uold, double *health, double *opc,
+                 double *sc_out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    p.m_U.setToZeros();
+    p.m_InitialVelocity.setToZeros();
+    p.m_ICSystem.applyInitialConditions(p.m_PDMesh, ndof, p.m_U, p.m_InitialVelocity);
+    p.m_BCSystem.presetSolution(p.m_PDMesh, ndof, p.m_U);
+    p.m_Uold = p.m_U;
+    std::function<void(const int &, const double &)> none;
+    const bool ok = p.m_TimeStepping.solveExplicitDynamics(
+        p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, p.m_BCSystem, p.m_PDSystem, ndof,
+        p.m_U, p.m_Uold, p.m_InitialVelocity, p.m_RHS, none);
+    const int n = p.m_U.getSize();
+    std::copy(p.m_U.getDataPtr(), p.m_U.getDataPtr() + n, u);
+    std::copy(p.m_Uold.getDataPtr(), p.m_Uold.getDataPtr() + n, uold);
+    auto *e = dynamic_cast<ExplicitPDDOFracElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (e) {
+        std::copy(e->m_HealthFlat.begin(), e->m_HealthFlat.end(), health);
+        std::copy(e->m_OpCache.begin(), e->m_OpCache.end(), opc);
+        if (sc_out) *sc_out = e->m_Sc;
+    }
+    return ok ? p.m_TimeStepping.m_LastSteps : -p.m_TimeStepping.m_LastSteps - 1;
+}
+
+// Elastic constants of the explicit element (C11, C12, C66, rho, sc).
+void ref_frac_consts(void *h, double *out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<ExplicitPDDOFracElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    out[0] = e->m_C11; out[1] = e->m_C12; out[2] = e->m_C66; out[3] = e->m_Rho;
+    out[4] = e->criticalStretch(p.m_PDMesh.getDataConstRef().HorizonRadius);
+}
+
+} // extern "C"

@@ -1,0 +1,142 @@
+"""ctypes driver of oracle/_ref/libperix_ref.so -- TEST INFRASTRUCTURE ONLY.
+
+The library is the reference's own C++ built from its sources by
+oracle/ref/Makefile plus a small harness.  It is used in this container to
+write golden vectors (tests/golden/) and to check the C oracle; it is optional
+on machines where it was not built.
+"""
+import ctypes as C
+import json
+import os
+import tempfile
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "_ref", "libperix_ref.so")
+_LIB = None
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+
+def available():
+    return os.path.exists(_PATH)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(_PATH)
+        _LIB.ref_open.restype = C.c_void_p
+        _LIB.ref_open.argtypes = [C.c_char_p]
+        for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
+                  "ref_initial_state", "ref_preset", "ref_assemble", "ref_ch_geometry",
+                  "ref_explicit", "ref_frac_consts", "ref_dxyz"):
+            getattr(_LIB, f).argtypes = None
+        _LIB.ref_horizon.restype = C.c_double
+        _LIB.ref_group.restype = C.c_int
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(D) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(I) if a is not None else None
+
+
+def mesh_block(dim, nx, ny, nz=1, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    m = {"type": "perix", "dim": dim, "nx": nx, "ny": ny, "xmin": lo[0], "xmax": hi[0],
+         "ymin": lo[1], "ymax": hi[1], "meshtype": "quad4" if dim == 2 else "hex8",
+         "savemesh": False}
+    if dim == 3:
+        m.update({"nz": nz, "zmin": lo[2], "zmax": hi[2]})
+    return m
+
+
+class Case:
+    """One reference problem built from a JSON deck."""
+
+    def __init__(self, deck):
+        self._dir = tempfile.mkdtemp(prefix="pxref_")
+        self.path = os.path.join(self._dir, "deck.json")
+        with open(self.path, "w") as f:
+            json.dump(deck, f, indent=1)
+        L = lib()
+        self.h = C.c_void_p(L.ref_open(self.path.encode()))
+        info = np.zeros(6, np.int64)
+        L.ref_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong)))
+        self.n, self.nb, self.dim, self.ndof, self.nnz, self.nbulk = (int(x) for x in info)
+        self.delta = L.ref_horizon(self.h)
+
+    def close(self):
+        if self.h:
+            lib().ref_close(self.h)
+            self.h = None
+
+    def mesh(self):
+        c = np.zeros(3 * self.n); v = np.zeros(self.n); m = np.zeros(self.n, np.int32)
+        lib().ref_mesh(self.h, _dp(c), _dp(v), _ip(m))
+        return c, v, m
+
+    def dxyz(self):
+        d = np.zeros(3)
+        lib().ref_dxyz(self.h, _dp(d))
+        return d
+
+    def families(self):
+        off = np.zeros(self.n + 1, np.int32); nbr = np.zeros(self.nb, np.int32)
+        lib().ref_families(self.h, _ip(off), _ip(nbr))
+        return off, nbr
+
+    def group(self, name):
+        L = lib()
+        k = L.ref_group(self.h, name.encode(), None)
+        ids = np.zeros(k, np.int32)
+        if k:
+            L.ref_group(self.h, name.encode(), _ip(ids))
+        return ids
+
+    def ops(self, nop):
+        o = np.zeros(self.nb * nop); fl = np.zeros(self.n, np.uint8)
+        lib().ref_ops(self.h, _dp(o), fl.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        return o.reshape(self.nb, nop), fl
+
+    def csr(self):
+        rp = np.zeros(self.n * self.ndof + 1, np.int32); cols = np.zeros(self.nnz, np.int32)
+        lib().ref_csr(self.h, _ip(rp), _ip(cols))
+        return rp, cols
+
+    def initial_state(self):
+        u = np.zeros(self.n * self.ndof); v = np.zeros(self.n * self.ndof)
+        lib().ref_initial_state(self.h, _dp(u), _dp(v))
+        return u, v
+
+    def preset(self, u, t):
+        u = np.ascontiguousarray(u, float).copy()
+        lib().ref_preset(self.h, _dp(u), C.c_double(t))
+        return u
+
+    def assemble(self, u, uold, dt):
+        k = np.zeros(self.nnz); r = np.zeros(self.n * self.ndof)
+        lib().ref_assemble(self.h, _dp(np.ascontiguousarray(u, float)),
+                           _dp(np.ascontiguousarray(uold, float)), C.c_double(dt), _dp(k), _dp(r))
+        return k, r
+
+    def ch_geometry(self):
+        lap = np.zeros(self.nb); rev = np.zeros(self.nb, np.int32); mn = np.zeros(self.n)
+        lib().ref_ch_geometry(self.h, _dp(lap), _ip(rev), _dp(mn))
+        return lap, rev, mn
+
+    def frac_consts(self):
+        o = np.zeros(5)
+        lib().ref_frac_consts(self.h, _dp(o))
+        return o
+
+    def explicit(self):
+        nd = self.n * self.ndof
+        nops = 9 if self.dim == 3 else 5
+        u = np.zeros(nd); uo = np.zeros(nd); h = np.zeros(self.nb); opc = np.zeros(self.nb * nops)
+        sc = C.c_double()
+        steps = lib().ref_explicit(self.h, _dp(u), _dp(uo), _dp(h), _dp(opc), C.byref(sc))
+        return steps, u, uo, h, opc.reshape(self.nb, nops), sc.value
