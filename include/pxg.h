@@ -1,0 +1,239 @@
+/*
+ * pxg.h -- C ABI of the MI355X PDDO hot-path library (libpxg.so).
+ *
+ * Plain C: pointers, sizes and int status codes only.  It is the device seam
+ * that the reference's host code reaches through perix_cuda::* in
+ * include/PDSystem/CudaAssemble.h:22-259 (POD structs so the device compiler
+ * never parses framework headers), widened to the pieces the reference keeps
+ * on the CPU: family construction, the operator build and the whole
+ * central-difference loop.
+ *
+ * Conventions (same as the reference):
+ *   - node ids inside neighbour lists are 1-based; CSR indices are 0-based;
+ *   - coordinates are interleaved x,y,z per node (z = 0 in 2D);
+ *   - solution vectors are node-major, index i*ndof + a;
+ *   - every host pointer is borrowed for the duration of the call.
+ * Errors: every entry point returns 0 on success or a negative PXG_ERR_*
+ * code; pxg_last_error() gives the message.  As in the reference
+ * (CudaAssembleKernels.cu:534-540), a failing call leaves the caller free to
+ * take its CPU path.  A context is used by one host thread at a time.
+ */
+#ifndef PXG_H
+#define PXG_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXG_OK 0
+#define PXG_ERR_ARG (-1)      /* invalid argument / size */
+#define PXG_ERR_HIP (-2)      /* HIP runtime failure */
+#define PXG_ERR_STATE (-3)    /* call order (e.g. operators before families) */
+#define PXG_ERR_FAULT (-4)    /* injected fault (pxg_set_fault) */
+#define PXG_ERR_DEGENERATE (-5) /* degenerate moment matrix */
+#define PXG_ERR_BLOWUP (-6)   /* explicit blow-up guard tripped */
+
+#define PXG_FP64 0            /* plain FP64 (bit-exact with the reference) */
+#define PXG_FP32C 1           /* FP32 operators + compensated FP32 family sums */
+
+typedef struct pxg_ctx pxg_ctx;
+
+/* ---- lifetime ------------------------------------------------------------
+ * Replaces the static device contexts g_ctx/g_epfctx/g_chctx and
+ * perix_cuda::shutdown (CudaAssemble.h:257; CudaAssembleKernels.cu:505-560,
+ * 1068-1107) with an explicit handle. */
+int pxg_create(pxg_ctx **out, int device);
+void pxg_destroy(pxg_ctx *ctx);
+const char *pxg_last_error(void);
+/* Host-test fault injection: the next entry point that uses the device
+ * returns PXG_ERR_FAULT without launching (code 0 disables).  Mirrors the
+ * `return false` seam failure of CudaAssembleKernels.cu:1112,1193. */
+int pxg_set_fault(pxg_ctx *ctx, int code);
+/* Blocks until all queued work of the context has finished. */
+int pxg_sync(pxg_ctx *ctx);
+
+/* ---- a1: structured point cloud (host) ----------------------------------
+ * Restates the quad4/hex8 background generator (Lagrange2DQuad4MeshGenerator
+ * .cpp:40-153, Lagrange3DHex8MeshGenerator.cpp:74-130) and the structured
+ * branch of PDMesh::createPDMesh (CreatePDMesh.cpp:451-799): cell-centre
+ * points in element order, then one ghost per boundary face in group order
+ * left,right,bottom,top[,back,front].  wall[i] = -1 for bulk, else the wall
+ * index in that order (the "<wall>nodes_ghost" groups). */
+typedef struct {
+    int dim, nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} pxg_grid;
+int pxg_structured_sizes(const pxg_grid *g, int *n, int *nbulk);
+int pxg_structured_cloud(const pxg_grid *g, double *coords, double *vols, int *mirror,
+                         int *wall, double *dxyz);
+/* HorizonRadius = factor * |xmax-xmin|/nx * (1+1e-6) (ReadPDMeshBlock.cpp:17-23,
+ * PDMesh.h:37-41) */
+double pxg_horizon_radius(const pxg_grid *g, double factor);
+/* a21: RandomIC::apply (RandomIC.cpp:13-42), std::mt19937_64 top 53 bits;
+ * dofs are 1-based. */
+int pxg_random_ic(uint64_t seed, double lo, double hi, int nnodes, const int *nodes,
+                  int ndof, int ndofs_set, const int *dofs, double *values);
+/* Step schedule of TimeStepping::solveExplicitDynamics (TimeStepping.cpp:
+ * 497-510); returns the step count, fills dts/ts (tNew) up to cap entries. */
+int pxg_schedule(double dt, double total, int cap, double *dts, double *ts);
+
+/* ---- a1: point cloud upload ----------------------------------------------
+ * Uploads PDMeshData::NodeCoords / NodeVolumes / GhostMirrorBulkID
+ * (PDMeshData.h; built by PDMesh::createPDMesh, CreatePDMesh.cpp:451-799)
+ * and DX/DY/DZ.  coords[3n], vols[n], mirror[n] (1-based, 0 = bulk). */
+int pxg_set_points(pxg_ctx *ctx, int n, int dim, const double *coords,
+                   const double *vols, const int *mirror, const double *dxyz);
+
+/* ---- a2: horizon families on the device ---------------------------------
+ * Replaces PDMesh::createNeighborNodes (CreatePDMesh.cpp:910-1130).  The
+ * lists come out in the reference's insertion order.  r is HorizonRadius
+ * (PDMesh.h:37-41, margin included).  nb_out receives the directed bond count. */
+int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out);
+/* Copies the flat families (ExplicitPDDOFracElement.cpp:115-153 layout):
+ * off[n+1], nbr[nb] (1-based). */
+int pxg_get_families(pxg_ctx *ctx, int *off, int *nbr);
+
+/* ---- a3: force-only crack seeds ------------------------------------------
+ * Replaces the force-only health seeding of ExplicitPDDOFracElement.cpp:
+ * 134-155 through initialCrackCutsBond (PDMesh.cpp:104-159): segments
+ * {x1,y1,x2,y2} (through-thickness in 3D) and planar quads (12 doubles). */
+int pxg_set_cracks(pxg_ctx *ctx, int nseg, const double *seg, int nquad,
+                   const double *quad);
+/* health seed per directed bond (1 intact, 0 cut), family order */
+int pxg_get_seed(pxg_ctx *ctx, unsigned char *seed);
+
+/* ---- a4: CSR pattern -----------------------------------------------------
+ * Replaces SparseMatrix::initFromPDMesh + setup (SparseMatrix.cpp:73-185).
+ * rowptr[n*ndof+1], cols[nnz] (0-based, sorted). */
+int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out);
+int pxg_get_csr(pxg_ctx *ctx, int *rowptr, int *cols);
+
+/* ---- a5-a8: PDDO moment matrix and g-weights (FP64) ---------------------
+ * Replaces PDOperators::calcAMatrixCompute/calcOperatorsCompute
+ * (CalcPDOperators.cpp:108-354) and the device twin opsBuildKernel
+ * (CudaAssembleKernels.cu:208-243).  dx_char = max(DX,DY,DZ) or 0 to disable
+ * the volume correction.  ndeg_out receives the degenerate-node count
+ * (usable bonds < nop, or inconsistent moment system). */
+int pxg_build_operators(pxg_ctx *ctx, int order, double delta, double dx_char,
+                        int *ndeg_out);
+/* ops[nb*nop] in the reference slot order (PDOperators.cpp:51-75), flags[n]
+ * (bit0 degenerate, bit1 usable<nop); either pointer may be NULL. */
+int pxg_get_operators(pxg_ctx *ctx, double *ops, unsigned char *flags);
+
+/* ---- a10-a11: Poisson CSR assembly --------------------------------------
+ * Replaces perix_cuda::assemble for ELMT_POISSON (CudaAssemble.h:93-95) and
+ * the serial/OpenMP assemblers (FormResidualAndJacobian*.cpp) with
+ * PoissonElement (PoissonElement.cpp:26-58).  K = -dF/dU, rhs = +F.
+ * f_node (optional, n) generalises the reference's scalar source f_const.
+ * Host pointers: u[n], kvals[nnz], rhs[n]. */
+int pxg_assemble_poisson(pxg_ctx *ctx, double sigma, const double *f_node,
+                         double f_const, const double *u, double *kvals,
+                         double *rhs);
+/* Same on device pointers (inputs already resident in HBM). */
+int pxg_assemble_poisson_dev(pxg_ctx *ctx, double sigma, const double *f_node_dev,
+                             double f_const, const double *u_dev,
+                             double *kvals_dev, double *rhs_dev);
+
+/* ---- a12-a13: Cahn-Hilliard ---------------------------------------------
+ * Replaces perix_cuda::assembleCahnHilliard (CudaAssemble.h:139-142) and the
+ * host-built geometry of CahnHilliardElement::preprocessIteration
+ * (CahnHilliardElement.cpp:93-153), which here is built on the device.
+ * Requires pxg_build_csr(ctx, 2, ...) and the operators. */
+int pxg_ch_setup(pxg_ctx *ctx);
+/* lambda-corrected g2 per bond, reverse-bond index, ghost flags (optional) */
+int pxg_ch_get_geometry(pxg_ctx *ctx, double *lapbond, int *rev,
+                        unsigned char *isghost);
+/* Picard-frozen mobility M_i = M0 c_i(1-c_i) (CahnHilliardElement.cpp:
+ * 155-175) then assembly (:178-292).  u, uold: 2n (c, mu); kvals[nnz];
+ * rhs[2n]. */
+int pxg_assemble_ch(pxg_ctx *ctx, double kappa, double chi, double M0,
+                    double dt, const double *u, const double *uold,
+                    double *kvals, double *rhs);
+int pxg_assemble_ch_dev(pxg_ctx *ctx, double kappa, double chi, double M0,
+                        double dt, const double *u_dev, const double *uold_dev,
+                        double *kvals_dev, double *rhs_dev);
+
+/* ---- a15-a20: explicit PDDO fracture --------------------------------------
+ * Material / damage parameters (ExplicitPDDOFracParams, CudaAssemble.h:
+ * 225-235).  sc is the critical stretch itself (FractureCriterion.h:30-43
+ * evaluated by the caller); sc <= 0 or damage_on = 0 disables bond failure. */
+typedef struct {
+    int dim;
+    double C11, C12, C66, rho;
+    double bx, by, bz;
+    double sc;
+    int tension_only, damage_on;
+} pxg_frac_params;
+
+/* Builds the per-bond Navier operators from the g-weights (the one-off
+ * op cache of ExplicitPDDOFracElement.cpp:106-172) and seeds health from
+ * the cracks.  precision = PXG_FP64 or PXG_FP32C.  node_E (optional, n, host)
+ * gives a per-node Young's modulus with Poisson ratio nu (then node i's
+ * family uses C(E_i); C11/C12/C66 of params are ignored). */
+int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precision,
+                   const double *node_E, double nu, int plane_stress);
+
+/* BC presets run after every central-difference update
+ * (BCSystem::presetSolution, BCSystem.cpp:114-129 -> DirichletBC.cpp:107-147,
+ * NeumannBC.cpp:88-105), in list order, last writer wins. */
+typedef struct {
+    int type;          /* 0 neumann (copy mirror bulk), 1 dirichlet */
+    int direct;        /* dirichlet: 1 writes g(t), 0 reflects 2g - u_bulk */
+    int ncomp;         /* number of constrained components */
+    int comp[3];       /* 0-based components */
+    double value[3];   /* g(0) per component */
+    double velocity[3];/* dg/dt per component */
+    int has_box;       /* inclusive coordinate box filter (DirichletBC.cpp:89-98) */
+    double box[6];     /* xlo,xhi,ylo,yhi,zlo,zhi; lo > hi leaves the axis open */
+    int nnodes;
+    const int *nodes;  /* 1-based node ids */
+} pxg_bc;
+int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs);
+
+/* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
+ * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):
+ * damage with u, then health-gated Navier force, accel = (L+b)/rho.
+ * u[n*dim] host; health_inout[nb] as double 0/1 (uploaded on the first call,
+ * kept on the device afterwards, written back); accel_out[n*dim]. */
+int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout,
+                   double *accel_out, int *any_broke);
+
+/* Device-resident central-difference loop (TimeStepping::solveExplicitDynamics,
+ * TimeStepping.cpp:417-599): the host passes the state once; every step runs
+ * damage + force + update + BC preset + blow-up guard on the device.
+ * pxg_frac_init: u0 = preset initial state (n*dim), v0 initial velocity (may be
+ * NULL); performs the t=0 rate evaluation and u^{-1}.  Steps follow the
+ * reference schedule for (dt, total); nsteps_max (>0) stops early after that
+ * many steps (for timing).  Returns PXG_ERR_BLOWUP when the guard trips. */
+int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, double dt);
+int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_max,
+                 int *steps_done);
+/* Downloads u^n, u^{n-1} (n*dim), health (nb, 0/1 bytes) and break steps
+ * (nb ints: -1 seeded, 0 broken at t=0, k during step k, INT32_MAX intact);
+ * any pointer may be NULL. */
+int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold,
+                       unsigned char *health, int *break_step);
+/* Device pointer of the displacement u^n and u^{n-1} (valid until the next
+ * pxg_frac_* call). */
+int pxg_frac_state_ptrs(pxg_ctx *ctx, double **u_dev, double **uold_dev);
+
+/* ---- a22: adjoint of the explicit step (new; no reference counterpart) ----
+ * Gradient of J = sum_s u^K[sens_s*dim+comp]^2 with respect to the per-node
+ * Young's modulus given to pxg_frac_setup, with bond health frozen from the
+ * recorded forward break steps, uniform steps t_k = t_{k-1} + dt, and
+ * displacement checkpoints every `ckpt` steps (recomputed in segments).
+ * u0 (n*dim host, preset initial state), v0 (optional).  grad[n] host. */
+int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0,
+                     double dt, int nsteps, int ckpt, int nsens,
+                     const int *sens, int comp, double *J_out, double *grad);
+/* Frozen-health forward loss of the same map (finite-difference checks);
+ * node_E (optional, host, n) replaces the per-node modulus for this call. */
+int pxg_frac_loss(pxg_ctx *ctx, const double *u0, const double *v0, double dt,
+                  int nsteps, const double *node_E, int nsens, const int *sens,
+                  int comp, double *J_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

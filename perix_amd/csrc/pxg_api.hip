@@ -1,0 +1,67 @@
+// pxg_api.hip -- context lifetime, error reporting and fault injection.
+#include "pxg_internal.h"
+
+namespace pxg {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+} // namespace pxg
+
+using namespace pxg;
+
+extern "C" int pxg_create(pxg_ctx **out, int device)
+{
+    PXG_CHECK(out, PXG_ERR_ARG, "pxg_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    PXG_HIP(hipGetDeviceCount(&ndev));
+    PXG_CHECK(device >= 0 && device < ndev, PXG_ERR_ARG, "pxg_create: no such device");
+    PXG_HIP(hipSetDevice(device));
+    pxg_ctx *c = new pxg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error(std::string("pxg_create: ") + hipGetErrorString(e));
+        return PXG_ERR_HIP;
+    }
+    *out = c;
+    return PXG_OK;
+}
+
+template <typename T> static void rel(dbuf<T> &b) { b.release(); }
+
+extern "C" void pxg_destroy(pxg_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    rel(c->coords); rel(c->xp); rel(c->vols); rel(c->mirror); rel(c->off); rel(c->nbr);
+    rel(c->crack_seg); rel(c->crack_quad); rel(c->seed); rel(c->ops); rel(c->flags);
+    rel(c->rowptr); rel(c->cols); rel(c->slot); rel(c->diag); rel(c->mslot); rel(c->lap);
+    rel(c->rev); rel(c->isghost); rel(c->mnode); rel(c->su); rel(c->suo); rel(c->sk);
+    rel(c->sr); rel(c->sf);
+    FracState &F = c->frac;
+    rel(F.P64); rel(F.P32); rel(F.health); rel(F.break_step); rel(F.u[0]); rel(F.u[1]);
+    rel(F.preset_mask); rel(F.bc.node); rel(F.bc.src); rel(F.bc.mode); rel(F.bc.val0);
+    rel(F.bc.vel); rel(F.gmax); rel(F.ghist); rel(F.gstate); rel(F.nodeE); rel(F.Phat);
+    rel(F.rev); rel(F.accel); rel(F.anybroke); rel(F.bc.inv_off); rel(F.bc.inv_q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *pxg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pxg_set_fault(pxg_ctx *ctx, int code)
+{
+    PXG_CHECK(ctx, PXG_ERR_ARG, "pxg_set_fault: ctx is NULL");
+    ctx->fault = code;
+    return PXG_OK;
+}
+
+extern "C" int pxg_sync(pxg_ctx *ctx)
+{
+    PXG_CHECK(ctx, PXG_ERR_ARG, "pxg_sync: ctx is NULL");
+    PXG_HIP(hipSetDevice(ctx->device));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    return PXG_OK;
+}

@@ -1,0 +1,745 @@
+// pxg_explicit.hip -- matrix-free explicit PDDO fracture on the device.
+//
+// Reference: ExplicitPDDOFracElement::preprocessIteration (per-step loop,
+// ExplicitPDDOFracElement.cpp:237-299), the central-difference driver
+// TimeStepping::solveExplicitDynamics (TimeStepping.cpp:417-599) and the BC
+// presets (BCSystem.cpp:114-129, DirichletBC.cpp:107-147, NeumannBC.cpp:88-105).
+//
+// One step = two launches:
+//   k_frac_step   bond-parallel tiles.  A 256-thread block owns T nodes and
+//                 streams their bonds (contiguous in CSR order, so every load
+//                 of neighbour ids, operators and health is coalesced).  Each
+//                 lane evaluates damage and the Navier bond force for its
+//                 bonds and parks the force in LDS; then one lane per node
+//                 sums its bonds in family order (FP64: the reference's exact
+//                 sum; FP32C: TwoSum-compensated FP32 pair), computes the
+//                 acceleration and the central-difference update, and folds
+//                 |u| into the blow-up guard maximum.
+//   k_bc_preset   the merged BC program (one thread per preset target node)
+//                 followed, in the last block, by the blow-up guard decision.
+// Damage is evaluated in FP64 in both precisions; only the FP32C force sum
+// runs in FP32.
+#include "pxg_internal.h"
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace pxg {
+
+enum { MODE_CD = 0, MODE_VAR = 1, MODE_ACCEL = 2, MODE_INIT = 3 };
+
+struct StepArgs {
+    int n;
+    long long nb;
+    const int *off;
+    const int *nbr;
+    const double *xp;
+    const double *P64;
+    const float *P32;
+    unsigned char *health;
+    int *break_step;
+    int *anybroke;
+    int step;
+    const double *u;       // u^n
+    const double *uold;    // u^{n-1} (MODE_CD/VAR) or v0 (MODE_INIT)
+    double *out;           // u^{n+1} / accel / u^{-1}
+    double sc;
+    int tension_only, damage;
+    double b0, b1, b2, rho;
+    double c1, c2;         // MODE_CD: dt2; MODE_VAR: rho_, w; MODE_INIT: dt, half
+    const unsigned char *mask;
+    unsigned long long *gmax;
+    const int *gstate;
+    const double *nodeE;   // optional: per-node scale applied to the family sum
+    int frozen;            // >0: health frozen from break_step (no damage), 0: live
+    int frozen_k;          // bond t is intact iff break_step[t] > frozen_k
+};
+
+template <int DIM> struct TileCfg;
+template <> struct TileCfg<2> { static constexpr int T = 32; static constexpr int K = 4; };
+template <> struct TileCfg<3> { static constexpr int T = 8; static constexpr int K = 4; };
+
+template <int DIM, bool F32, int MODE>
+__global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
+{
+    constexpr int T = TileCfg<DIM>::T;
+    constexpr int K = TileCfg<DIM>::K;
+    constexpr int CH = 256 * K;
+    constexpr int NC = (DIM == 3) ? 6 : 3;
+    using TermT = typename std::conditional<F32, float, double>::type;
+    __shared__ int s_off[T + 1];
+    __shared__ double s_u[T][DIM];
+    __shared__ double s_x[T][DIM];
+    __shared__ TermT s_f[CH][DIM];
+    __shared__ unsigned char s_ok[CH];
+    __shared__ unsigned long long s_max[4];
+
+    if (a.gstate && a.gstate[0]) return;            // blow-up guard tripped earlier
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * T;
+    const int nt = min(T, a.n - i0);
+    if (tid <= nt) s_off[tid] = a.off[i0 + tid];
+    if (tid < nt) {
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            s_u[tid][c] = a.u[(size_t)(i0 + tid) * DIM + c];
+            s_x[tid][c] = a.xp[(size_t)(i0 + tid) * DIM + c];
+        }
+    }
+    __syncthreads();
+    const int bt0 = s_off[0], bt1 = s_off[nt];
+    // per-node accumulators live in the lane that owns the node
+    double L[DIM];
+    TermT Ls[DIM], Lc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) { L[c] = 0.0; Ls[c] = 0; Lc[c] = 0; }
+    int broke = 0;
+
+    for (int cs = bt0; cs < bt1; cs += CH) {
+        const int ce = min(bt1, cs + CH);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int lt = tid + k * 256;
+            const int t = cs + lt;
+            if (t >= ce) continue;
+            // local source node: last ln with s_off[ln] <= t
+            int lo = 0, hi = nt - 1;
+            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= t) lo = mid; else hi = mid - 1; }
+            const int ln = lo;
+            const unsigned char ok = a.frozen ? (a.break_step[t] > a.frozen_k) : a.health[t];
+            if (!ok) { s_ok[lt] = 0; continue; }
+            const int j = a.nbr[t] - 1;
+            double uj[DIM], du[DIM];
+#pragma unroll
+            for (int c = 0; c < DIM; c++) { uj[c] = a.u[(size_t)j * DIM + c]; du[c] = uj[c] - s_u[ln][c]; }
+            if (a.damage && !a.frozen) {
+                // stretch of ExplicitPDDOFracElement.cpp:249-266
+                double xi[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; c++) xi[c] = a.xp[(size_t)j * DIM + c] - s_x[ln][c];
+                double rr = xi[0] * xi[0] + xi[1] * xi[1];
+                if (DIM == 3) rr = rr + xi[DIM - 1] * xi[DIM - 1];
+                const double r = sqrt(rr);
+                if (r > 0.0) {
+                    const double yx = xi[0] + du[0], yy = xi[1] + du[1];
+                    double yy2 = yx * yx + yy * yy;
+                    if (DIM == 3) { const double yz = xi[DIM - 1] + du[DIM - 1]; yy2 = yy2 + yz * yz; }
+                    const double s = (sqrt(yy2) - r) / r;
+                    if (a.tension_only ? (s > a.sc) : (fabs(s) > a.sc)) {
+                        a.health[t] = 0;
+                        if (a.break_step) a.break_step[t] = a.step;
+                        broke = 1;
+                        s_ok[lt] = 0;
+                        continue;
+                    }
+                }
+            }
+            // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295)
+            if (F32) {
+                float p[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) p[c] = a.P32[(size_t)c * a.nb + t];
+                float d[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; c++) d[c] = (float)du[c];
+                if (DIM == 2) {
+                    s_f[lt][0] = (TermT)fmaf(p[0], d[0], p[2] * d[1]);
+                    s_f[lt][1] = (TermT)fmaf(p[2], d[0], p[1] * d[1]);
+                } else {
+                    s_f[lt][0] = (TermT)fmaf(p[0], d[0], fmaf(p[3], d[1], p[4] * d[DIM - 1]));
+                    s_f[lt][1] = (TermT)fmaf(p[3], d[0], fmaf(p[1], d[1], p[5] * d[DIM - 1]));
+                    s_f[lt][DIM - 1] = (TermT)fmaf(p[4], d[0], fmaf(p[5], d[1], p[2] * d[DIM - 1]));
+                }
+            } else {
+                double p[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) p[c] = a.P64[(size_t)c * a.nb + t];
+                if (DIM == 2) {
+                    s_f[lt][0] = (TermT)(p[0] * du[0] + p[2] * du[1]);
+                    s_f[lt][1] = (TermT)(p[2] * du[0] + p[1] * du[1]);
+                } else {
+                    s_f[lt][0] = (TermT)(p[0] * du[0] + p[3] * du[1] + p[4] * du[DIM - 1]);
+                    s_f[lt][1] = (TermT)(p[3] * du[0] + p[1] * du[1] + p[5] * du[DIM - 1]);
+                    s_f[lt][DIM - 1] = (TermT)(p[4] * du[0] + p[5] * du[1] + p[2] * du[DIM - 1]);
+                }
+            }
+            s_ok[lt] = 1;
+        }
+        __syncthreads();
+        if (tid < nt) {
+            const int tb = max(s_off[tid], cs), te = min(s_off[tid + 1], ce);
+            for (int t = tb; t < te; t++) {
+                const int lt = t - cs;
+                if (!s_ok[lt]) continue;
+                if (F32) {
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) {
+                        // TwoSum (Knuth): s + f = s' + e exactly
+                        const float f = (float)s_f[lt][c];
+                        const float sn = (float)Ls[c] + f;
+                        const float bp = sn - (float)Ls[c];
+                        const float e = ((float)Ls[c] - (sn - bp)) + (f - bp);
+                        Lc[c] = (TermT)((float)Lc[c] + e);
+                        Ls[c] = (TermT)sn;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) L[c] += (double)s_f[lt][c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (broke && a.anybroke) atomicAdd(a.anybroke, 1);
+
+    unsigned long long mymax = 0;
+    if (tid < nt) {
+        const int i = i0 + tid;
+        if (F32) {
+#pragma unroll
+            for (int c = 0; c < DIM; c++) L[c] = (double)(float)Ls[c] + (double)(float)Lc[c];
+        }
+        const double bb[3] = {a.b0, a.b1, a.b2};
+        const double escale = a.nodeE ? a.nodeE[i] : 1.0;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            const double Lv = a.nodeE ? escale * L[c] : L[c];
+            const double acc = (Lv + bb[c]) / a.rho;
+            const double ui = s_u[tid][c];
+            double v;
+            if (MODE == MODE_CD) v = 2.0 * ui - a.uold[(size_t)i * DIM + c] + a.c1 * acc;
+            else if (MODE == MODE_VAR) v = ui + a.c1 * (ui - a.uold[(size_t)i * DIM + c]) + a.c2 * acc;
+            else if (MODE == MODE_ACCEL) v = acc;
+            else v = ui - a.c1 * a.uold[(size_t)i * DIM + c] + a.c2 * acc;
+            a.out[(size_t)i * DIM + c] = v;
+            if ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i]) {
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(v));
+                mymax = bits > mymax ? bits : mymax;
+            }
+        }
+    }
+    if (MODE == MODE_CD || MODE == MODE_VAR) {
+        for (int m = 32; m >= 1; m >>= 1) {
+            const unsigned long long o = __shfl_xor(mymax, m);
+            mymax = o > mymax ? o : mymax;
+        }
+        if ((tid & 63) == 0) s_max[tid >> 6] = mymax;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long m = s_max[0];
+            for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
+            if (m) atomicMax(&a.gmax[0], m);
+        }
+    }
+}
+
+// BC presets on the freshly updated u^{n+1}, then the blow-up guard
+// (TimeStepping.cpp:550-573) decided by the last block of the launch.
+__global__ void __launch_bounds__(256) k_bc_preset(int count, int dim, const int *node, const int *src,
+                                                   const int *mode, const double *val0, const double *vel,
+                                                   double t, double *u, unsigned long long *gmax,
+                                                   double *ghist, int *gstate, int step)
+{
+    __shared__ unsigned long long s_max[4];
+    __shared__ int s_last;
+    if (gstate[0]) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long mymax = 0;
+    if (q < count) {
+        const int i = node[q], b = src[q];
+        for (int c = 0; c < dim; c++) {
+            const int m = mode[q * 3 + c];
+            if (m == 1) u[(size_t)i * dim + c] = u[(size_t)b * dim + c];
+            else if (m == 2) {
+                const double g = val0[q * 3 + c] + vel[q * 3 + c] * t;
+                u[(size_t)i * dim + c] = 2.0 * g - u[(size_t)b * dim + c];
+            } else if (m == 3) {
+                u[(size_t)i * dim + c] = val0[q * 3 + c] + vel[q * 3 + c] * t;
+            }
+        }
+        for (int c = 0; c < dim; c++) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(u[(size_t)i * dim + c]));
+            mymax = bits > mymax ? bits : mymax;
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        const unsigned long long o = __shfl_xor(mymax, m);
+        mymax = o > mymax ? o : mymax;
+    }
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mymax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = s_max[0];
+        for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
+        if (m) atomicMax(&gmax[1], m);
+        __threadfence();
+        const unsigned long long prev = atomicAdd(&gmax[2], 1ULL);
+        s_last = (prev == (unsigned long long)(gridDim.x - 1));
+    }
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();
+        const unsigned long long g0 = atomicAdd(&gmax[0], 0ULL), g1 = atomicAdd(&gmax[1], 0ULL);
+        const unsigned long long m = g0 > g1 ? g0 : g1;
+        const bool finite = m <= (unsigned long long)__double_as_longlong(DBL_MAX);
+        const double umax = __longlong_as_double((long long)m);
+        const int slot = (step - 1) % 50;
+        const double past = (step > 50) ? ghist[slot] : -1.0;
+        const double cap = (past >= 0.0) ? 1.0e3 * fmax(1.0, past) : INFINITY;
+        if (!finite || umax > cap) { gstate[0] = 1; gstate[1] = step; }
+        else ghist[slot] = umax;
+        gstate[2] = step;
+        gmax[0] = 0; gmax[1] = 0; gmax[2] = 0;
+    }
+}
+
+// per-bond Navier operators from the g-weights (full slot order of
+// PDOperators.cpp:51-75); component-major layout P[c*nb + t].
+__global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double *ops,
+                           const double *nodeE, double C11, double C12, double C66, double nu,
+                           int plane_stress, long long nb, double *P64, float *P32, double *Phat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c11 = C11, c12 = C12, c66 = C66;
+    if (nodeE) {
+        // ExplicitPDDOFracElement::recalcElasticConstants (:37-52) at E_i
+        const double E = nodeE[i];
+        if (E <= 0.0) { c11 = c12 = c66 = 0.0; }
+        else if (dim >= 3 || !plane_stress) {
+            const double d = (1.0 + nu) * (1.0 - 2.0 * nu);
+            c11 = E * (1.0 - nu) / d; c12 = E * nu / d; c66 = E / (2.0 * (1.0 + nu));
+        } else {
+            const double d = 1.0 - nu * nu;
+            c11 = E / d; c12 = E * nu / d; c66 = E / (2.0 * (1.0 + nu));
+        }
+    }
+    double h11 = 0, h12 = 0, h66 = 0;
+    if (Phat) {
+        if (dim >= 3 || !plane_stress) {
+            const double d = (1.0 + nu) * (1.0 - 2.0 * nu);
+            h11 = 1.0 * (1.0 - nu) / d; h12 = 1.0 * nu / d; h66 = 1.0 / (2.0 * (1.0 + nu));
+        } else {
+            const double d = 1.0 - nu * nu;
+            h11 = 1.0 / d; h12 = 1.0 * nu / d; h66 = 1.0 / (2.0 * (1.0 + nu));
+        }
+    }
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        const double *op = ops + (size_t)t * nop;
+        double p[6], q[6];
+        int nc;
+        if (dim >= 3) {
+            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+            const double aa = c12 + c66;
+            p[0] = c11 * Gxx + c66 * (Gyy + Gzz);
+            p[1] = c11 * Gyy + c66 * (Gxx + Gzz);
+            p[2] = c11 * Gzz + c66 * (Gxx + Gyy);
+            p[3] = aa * Gxy; p[4] = aa * Gxz; p[5] = aa * Gyz;
+            const double ha = h12 + h66;
+            q[0] = h11 * Gxx + h66 * (Gyy + Gzz);
+            q[1] = h11 * Gyy + h66 * (Gxx + Gzz);
+            q[2] = h11 * Gzz + h66 * (Gxx + Gyy);
+            q[3] = ha * Gxy; q[4] = ha * Gxz; q[5] = ha * Gyz;
+            nc = 6;
+        } else {
+            const double Gxx = op[2], Gxy = op[3], Gyy = op[4];
+            p[0] = c11 * Gxx + c66 * Gyy;
+            p[1] = c66 * Gxx + c11 * Gyy;
+            p[2] = (c12 + c66) * Gxy;
+            q[0] = h11 * Gxx + h66 * Gyy;
+            q[1] = h66 * Gxx + h11 * Gyy;
+            q[2] = (h12 + h66) * Gxy;
+            nc = 3;
+        }
+        for (int c = 0; c < nc; c++) {
+            if (P64) P64[(size_t)c * nb + t] = p[c];
+            if (P32) P32[(size_t)c * nb + t] = (float)p[c];
+            if (Phat) Phat[(size_t)c * nb + t] = q[c];
+        }
+    }
+}
+
+__global__ void k_health_init(long long nb, const unsigned char *seed, unsigned char *health, int *bs)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    health[t] = seed[t];
+    bs[t] = seed[t] ? INT_MAX : -1;
+}
+
+__global__ void k_health_from_double(long long nb, const double *h, unsigned char *health)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    health[t] = (h[t] > 0.0) ? 1 : 0;
+}
+
+__global__ void k_health_to_double(long long nb, const unsigned char *health, double *h)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    h[t] = health[t] ? 1.0 : 0.0;
+}
+
+__global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        const int j = nbr[t] - 1;
+        int r = -1;
+        for (int s = off[j]; s < off[j + 1]; s++) if (nbr[s] == i + 1) { r = s; break; }
+        rev[t] = r;
+    }
+}
+
+} // namespace pxg
+
+using namespace pxg;
+
+static unsigned gridn(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+namespace pxg {
+
+// launches one k_frac_step for the context's dimension / precision / mode
+int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
+                double c1, double c2, int step, bool guard, int frozen_k)
+{
+    FracState &F = ctx->frac;
+    StepArgs a;
+    a.n = ctx->n; a.nb = ctx->nb; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
+    a.anybroke = F.anybroke.p; a.step = step; a.u = u; a.uold = uold; a.out = out;
+    a.sc = F.sc; a.tension_only = F.tension_only; a.damage = (F.damage_on && F.sc > 0.0) ? 1 : 0;
+    a.b0 = F.b[0]; a.b1 = F.b[1]; a.b2 = F.b[2]; a.rho = F.rho; a.c1 = c1; a.c2 = c2;
+    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = guard ? F.gstate.p : nullptr;
+    a.nodeE = nullptr;
+    a.frozen = (frozen_k > INT_MIN) ? 1 : 0; a.frozen_k = frozen_k;
+    const bool f32 = (F.precision == PXG_FP32C) && !a.frozen;
+    hipStream_t st = ctx->stream;
+#define PXG_LAUNCH(D, FF, M)                                                                        \
+    hipLaunchKernelGGL((k_frac_step<D, FF, M>), dim3(gridn(ctx->n, TileCfg<D>::T)), dim3(256), 0, st, a)
+#define PXG_MODES(D, FF)                                                                            \
+    switch (mode) {                                                                                 \
+    case MODE_CD: PXG_LAUNCH(D, FF, MODE_CD); break;                                                \
+    case MODE_VAR: PXG_LAUNCH(D, FF, MODE_VAR); break;                                              \
+    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL); break;                                          \
+    default: PXG_LAUNCH(D, FF, MODE_INIT); break;                                                   \
+    }
+    if (ctx->dim == 2) { if (f32) { PXG_MODES(2, true) } else { PXG_MODES(2, false) } }
+    else { if (f32) { PXG_MODES(3, true) } else { PXG_MODES(3, false) } }
+#undef PXG_MODES
+#undef PXG_LAUNCH
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev)
+{
+    FracState &F = ctx->frac;
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
+                       ctx->nop, ctx->off.p, ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress,
+                       ctx->nb, F.P64.p, (float *)nullptr, (double *)nullptr);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int launch_preset(pxg_ctx *ctx, double *u, double t, int step)
+{
+    FracState &F = ctx->frac;
+    const int cnt = F.bc.count;
+    hipLaunchKernelGGL(k_bc_preset, dim3(cnt > 0 ? gridn(cnt, 256) : 1), dim3(256), 0, ctx->stream, cnt,
+                       ctx->dim, F.bc.node.p, F.bc.src.p, F.bc.mode.p, F.bc.val0.p, F.bc.vel.p, t, u,
+                       F.gmax.p, F.ghist.p, F.gstate.p, step);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+} // namespace pxg
+
+extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precision,
+                              const double *node_E, double nu, int plane_stress)
+{
+    PXG_CHECK(ctx && p, PXG_ERR_ARG, "pxg_frac_setup: bad arguments");
+    PXG_CHECK(ctx->nop > 0, PXG_ERR_STATE, "pxg_frac_setup: build the operators first");
+    PXG_CHECK(p->dim == ctx->dim, PXG_ERR_ARG, "pxg_frac_setup: dim mismatch");
+    PXG_CHECK(precision == PXG_FP64 || precision == PXG_FP32C, PXG_ERR_ARG, "pxg_frac_setup: bad precision");
+    PXG_CHECK(p->rho > 0.0, PXG_ERR_ARG, "pxg_frac_setup: rho must be > 0");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_setup: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracState &F = ctx->frac;
+    hipStream_t st = ctx->stream;
+    F.dim = p->dim; F.C11 = p->C11; F.C12 = p->C12; F.C66 = p->C66; F.rho = p->rho;
+    F.b[0] = p->bx; F.b[1] = p->by; F.b[2] = p->bz; F.sc = p->sc;
+    F.tension_only = p->tension_only; F.damage_on = p->damage_on;
+    F.precision = precision; F.nu = nu; F.plane_stress = plane_stress;
+    F.has_nodeE = node_E ? 1 : 0;
+    const int nc = (ctx->dim == 3) ? 6 : 3;
+    const long long nb = ctx->nb;
+    if (node_E) {
+        PXG_HIP(F.nodeE.alloc(ctx->n));
+        PXG_HIP(hipMemcpyAsync(F.nodeE.p, node_E, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
+        PXG_HIP(F.Phat.alloc((size_t)nb * nc));
+        PXG_HIP(F.rev.alloc(nb));
+        hipLaunchKernelGGL(k_rev, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, ctx->nbr.p, F.rev.p);
+    } else {
+        F.nodeE.release(); F.Phat.release(); F.rev.release();
+    }
+    if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)nb * nc)); F.P32.release(); }
+    else { PXG_HIP(F.P32.alloc((size_t)nb * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)nb * nc)); }
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->dim, ctx->nop,
+                       ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
+                       plane_stress, nb, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
+    PXG_HIP(hipGetLastError());
+    PXG_HIP(F.health.alloc(nb));
+    PXG_HIP(F.break_step.alloc(nb));
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(nb, 256)), dim3(256), 0, st, nb, ctx->seed.p, F.health.p,
+                       F.break_step.p);
+    for (int k = 0; k < 2; k++) PXG_HIP(F.u[k].alloc((size_t)ctx->n * ctx->dim));
+    PXG_HIP(F.accel.alloc((size_t)ctx->n * ctx->dim));
+    PXG_HIP(F.preset_mask.alloc(ctx->n));
+    PXG_HIP(hipMemsetAsync(F.preset_mask.p, 0, ctx->n, st));
+    PXG_HIP(F.gmax.alloc(4));
+    PXG_HIP(F.ghist.alloc(50));
+    PXG_HIP(F.gstate.alloc(4));
+    PXG_HIP(F.anybroke.alloc(1));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.ghist.p, 0, sizeof(double) * 50, st));
+    PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.anybroke.p, 0, sizeof(int), st));
+    F.bc.count = 0;
+    F.health_uploaded = 0;
+    F.cur = 0; F.t = 0.0; F.step = 0; F.dt_prev = 0.0;
+    PXG_HIP(hipStreamSynchronize(st));
+    F.ready = 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_set_bcs: call pxg_frac_setup first");
+    PXG_CHECK(nbc >= 0 && (nbc == 0 || bcs), PXG_ERR_ARG, "pxg_frac_set_bcs: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const int n = ctx->n, dim = ctx->dim;
+    // resolve the BC list (later entries override earlier ones) into one
+    // program entry per target node
+    std::vector<int> slot(n, -1), tnode, tsrc, tmode;
+    std::vector<double> tval, tvel;
+    for (int e = 0; e < nbc; e++) {
+        const pxg_bc &b = bcs[e];
+        PXG_CHECK(b.ncomp >= 0 && b.ncomp <= dim && (b.nnodes == 0 || b.nodes), PXG_ERR_ARG,
+                  "pxg_frac_set_bcs: bad BC entry");
+        for (int q = 0; q < b.nnodes; q++) {
+            const int id = b.nodes[q];
+            PXG_CHECK(id >= 1 && id <= n, PXG_ERR_ARG, "pxg_frac_set_bcs: node outside the PD mesh");
+            const int mb = ctx->hmirror[id - 1];
+            if (b.type == 0) {
+                PXG_CHECK(mb >= 1, PXG_ERR_ARG, "pxg_frac_set_bcs: neumann node is not a boundary ghost "
+                                                "with a mirror bulk (NeumannBC.cpp:60-78)");
+            } else if (b.has_box) {
+                bool in = true;
+                for (int ax = 0; ax < 3; ax++) {
+                    const double lo = b.box[2 * ax], hi = b.box[2 * ax + 1];
+                    if (lo > hi) continue;
+                    const double x = ctx->hcoords[3 * (size_t)(id - 1) + ax];
+                    if (x < lo || x > hi) { in = false; break; }
+                }
+                if (!in) continue;
+            }
+            int s = slot[id - 1];
+            if (s < 0) {
+                s = (int)tnode.size();
+                slot[id - 1] = s;
+                tnode.push_back(id - 1); tsrc.push_back(mb >= 1 ? mb - 1 : -1);
+                for (int c = 0; c < 3; c++) { tmode.push_back(0); tval.push_back(0.0); tvel.push_back(0.0); }
+            }
+            for (int k = 0; k < b.ncomp; k++) {
+                const int c = b.comp[k];
+                PXG_CHECK(c >= 0 && c < dim, PXG_ERR_ARG, "pxg_frac_set_bcs: component out of range");
+                int m;
+                if (b.type == 0) m = 1;
+                else m = (mb >= 1 && !b.direct) ? 2 : 3;
+                tmode[3 * s + c] = m;
+                tval[3 * s + c] = b.value[k];
+                tvel[3 * s + c] = b.velocity[k];
+            }
+        }
+    }
+    // the merged program is exact when no target is read as a mirror source
+    for (size_t s = 0; s < tnode.size(); s++) {
+        bool reads = false;
+        for (int c = 0; c < 3; c++) reads |= (tmode[3 * s + c] == 1 || tmode[3 * s + c] == 2);
+        if (reads && tsrc[s] >= 0) {
+            PXG_CHECK(slot[tsrc[s]] < 0, PXG_ERR_ARG,
+                      "pxg_frac_set_bcs: a BC reads a node that another BC writes; chained presets "
+                      "are not supported on the device");
+        }
+    }
+    FracState &F = ctx->frac;
+    const int cnt = (int)tnode.size();
+    F.bc.count = cnt;
+    std::vector<unsigned char> mask(n, 0);
+    for (int s = 0; s < cnt; s++) mask[tnode[s]] = 1;
+    const int m1 = cnt > 0 ? cnt : 1;
+    PXG_HIP(F.bc.node.alloc(m1)); PXG_HIP(F.bc.src.alloc(m1)); PXG_HIP(F.bc.mode.alloc(3 * m1));
+    PXG_HIP(F.bc.val0.alloc(3 * m1)); PXG_HIP(F.bc.vel.alloc(3 * m1));
+    hipStream_t st = ctx->stream;
+    if (cnt > 0) {
+        PXG_HIP(hipMemcpyAsync(F.bc.node.p, tnode.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, st));
+        PXG_HIP(hipMemcpyAsync(F.bc.src.p, tsrc.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, st));
+        PXG_HIP(hipMemcpyAsync(F.bc.mode.p, tmode.data(), sizeof(int) * 3 * cnt, hipMemcpyHostToDevice, st));
+        PXG_HIP(hipMemcpyAsync(F.bc.val0.p, tval.data(), sizeof(double) * 3 * cnt, hipMemcpyHostToDevice, st));
+        PXG_HIP(hipMemcpyAsync(F.bc.vel.p, tvel.data(), sizeof(double) * 3 * cnt, hipMemcpyHostToDevice, st));
+    }
+    PXG_HIP(hipMemcpyAsync(F.preset_mask.p, mask.data(), n, hipMemcpyHostToDevice, st));
+    // inverse of the source map (adjoint of the preset), entries in program order
+    std::vector<int> ioff(n + 1, 0), iq;
+    for (int s2 = 0; s2 < cnt; s2++) if (tsrc[s2] >= 0) ioff[tsrc[s2] + 1]++;
+    for (int m = 0; m < n; m++) ioff[m + 1] += ioff[m];
+    iq.assign(ioff[n] > 0 ? ioff[n] : 1, 0);
+    {
+        std::vector<int> cur(ioff.begin(), ioff.end() - 1);
+        for (int s2 = 0; s2 < cnt; s2++) if (tsrc[s2] >= 0) iq[cur[tsrc[s2]]++] = s2;
+    }
+    PXG_HIP(F.bc.inv_off.alloc(n + 1));
+    PXG_HIP(F.bc.inv_q.alloc(iq.size()));
+    PXG_HIP(hipMemcpyAsync(F.bc.inv_off.p, ioff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.bc.inv_q.p, iq.data(), sizeof(int) * iq.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout, double *accel_out,
+                              int *any_broke)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_accel: call pxg_frac_setup first");
+    PXG_CHECK(u && accel_out, PXG_ERR_ARG, "pxg_frac_accel: bad arguments");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_accel: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracState &F = ctx->frac;
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)ctx->n * ctx->dim;
+    if (health_inout && !F.health_uploaded) {
+        PXG_HIP(ctx->sk.alloc(ctx->nb));
+        PXG_HIP(hipMemcpyAsync(ctx->sk.p, health_inout, sizeof(double) * ctx->nb, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_health_from_double, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, ctx->sk.p,
+                           F.health.p);
+        F.health_uploaded = 1;
+    }
+    PXG_HIP(hipMemcpyAsync(F.u[F.cur].p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemsetAsync(F.anybroke.p, 0, sizeof(int), st));
+    int rc = launch_step(ctx, MODE_ACCEL, F.u[F.cur].p, F.u[F.cur].p, F.accel.p, 0.0, 0.0, F.step, false, INT_MIN);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(accel_out, F.accel.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    int broke = 0;
+    PXG_HIP(hipMemcpyAsync(&broke, F.anybroke.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    if (broke && health_inout) {
+        PXG_HIP(ctx->sk.alloc(ctx->nb));
+        hipLaunchKernelGGL(k_health_to_double, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, F.health.p,
+                           ctx->sk.p);
+        PXG_HIP(hipMemcpyAsync(health_inout, ctx->sk.p, sizeof(double) * ctx->nb, hipMemcpyDeviceToHost, st));
+        PXG_HIP(hipStreamSynchronize(st));
+    }
+    if (any_broke) *any_broke = broke ? 1 : 0;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, double dt)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_init: call pxg_frac_setup first");
+    PXG_CHECK(u0 && dt > 0.0, PXG_ERR_ARG, "pxg_frac_init: bad arguments");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_init: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracState &F = ctx->frac;
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)ctx->n * ctx->dim;
+    F.cur = 0;
+    PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    else PXG_HIP(hipMemsetAsync(F.accel.p, 0, sizeof(double) * nd, st));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
+    // t = 0 rate evaluation (damage included), then u^{-1} = u0 - dt v0 + dt^2/2 a0
+    const double half = 0.5 * dt * dt;
+    int rc = launch_step(ctx, MODE_INIT, F.u[0].p, F.accel.p, F.u[1].p, dt, half, 0, false, INT_MIN);
+    if (rc) return rc;
+    PXG_HIP(hipStreamSynchronize(st));
+    F.t = 0.0; F.dt_prev = dt; F.step = 0; F.cur0 = 0;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_max, int *steps_done)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_run: call pxg_frac_setup first");
+    PXG_CHECK(dt > 0.0 && total > 0.0, PXG_ERR_ARG, "pxg_frac_run: dt and total_time must be positive");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_run: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracState &F = ctx->frac;
+    const double tEps = 1.0e-12 * total;
+    int done = 0;
+    int rc = PXG_OK;
+    while (F.t < total - tEps && (nsteps_max <= 0 || done < nsteps_max)) {
+        const int step = F.step + 1;
+        const double dtA = fmin(dt, total - F.t);
+        const double tNew = F.t + dtA;
+        double *ucur = F.u[F.cur].p, *uold = F.u[F.cur ^ 1].p;
+        if (dtA == F.dt_prev) {
+            rc = launch_step(ctx, MODE_CD, ucur, uold, uold, dtA * dtA, 0.0, step, true, INT_MIN);
+        } else {
+            const double rr = dtA / F.dt_prev;
+            const double w = 0.5 * dtA * (dtA + F.dt_prev);
+            rc = launch_step(ctx, MODE_VAR, ucur, uold, uold, rr, w, step, true, INT_MIN);
+        }
+        if (rc) return rc;
+        rc = launch_preset(ctx, uold, tNew, step);
+        if (rc) return rc;
+        F.cur ^= 1;
+        F.t = tNew; F.dt_prev = dtA; F.step = step;
+        done++;
+        if ((done & 255) == 0) {
+            int gs[3];
+            PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
+            PXG_HIP(hipStreamSynchronize(ctx->stream));
+            if (gs[0]) break;
+        }
+    }
+    int gs[3];
+    PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    if (gs[0]) {
+        // state at the aborted step: the buffer written by that step
+        const int abort_step = gs[1];
+        F.cur = F.cur0 ^ (abort_step & 1);
+        F.step = abort_step;
+        if (steps_done) *steps_done = abort_step - 1;
+        ::pxg::set_error("pxg_frac_run: solution blew up at step " + std::to_string(abort_step) +
+                         " (non-finite or 1e3x growth over 50 steps; reduce dt below the CFL limit)");
+        return PXG_ERR_BLOWUP;
+    }
+    if (steps_done) *steps_done = done;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold, unsigned char *health, int *break_step)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_get_state: no explicit state");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracState &F = ctx->frac;
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)ctx->n * ctx->dim;
+    if (u) PXG_HIP(hipMemcpyAsync(u, F.u[F.cur].p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    if (uold) PXG_HIP(hipMemcpyAsync(uold, F.u[F.cur ^ 1].p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    if (health) PXG_HIP(hipMemcpyAsync(health, F.health.p, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    if (break_step) PXG_HIP(hipMemcpyAsync(break_step, F.break_step.p, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_state_ptrs(pxg_ctx *ctx, double **u_dev, double **uold_dev)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_state_ptrs: no explicit state");
+    if (u_dev) *u_dev = ctx->frac.u[ctx->frac.cur].p;
+    if (uold_dev) *uold_dev = ctx->frac.u[ctx->frac.cur ^ 1].p;
+    return PXG_OK;
+}

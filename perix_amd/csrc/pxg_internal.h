@@ -1,0 +1,156 @@
+// pxg_internal.h -- shared state and helpers of the PDDO hot-path library.
+//
+// One pxg_ctx owns the device copy of one PD problem on one GPU: the point
+// cloud, the horizon families, the PDDO operators, the implicit CSR pattern
+// and the explicit-fracture state.  All device memory belongs to the context
+// and is released by pxg_destroy.  One host thread per context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "pxg.h"
+
+namespace pxg {
+
+constexpr int kMaxNop = 9;        // Order-2 slots: 5 (2D), 9 (3D)
+constexpr int kWave = 64;
+
+// thread-local last error message (pxg_last_error)
+void set_error(const std::string &msg);
+
+#define PXG_HIP(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            ::pxg::set_error(std::string("HIP error ") + hipGetErrorString(e_) +      \
+                             " at " + __FILE__ + ":" + std::to_string(__LINE__) +     \
+                             " (" #expr ")");                                         \
+            return PXG_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+#define PXG_CHECK(cond, code, msg)                                                    \
+    do {                                                                              \
+        if (!(cond)) { ::pxg::set_error(msg); return (code); }                        \
+    } while (0)
+
+// device buffer with size bookkeeping
+template <typename T>
+struct dbuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        if (count == n && p) return hipSuccess;
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(&p, sizeof(T) * count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct BcNodeProgram {
+    // one entry per preset-target node (all its components handled together)
+    dbuf<int> node;          // 0-based target node
+    dbuf<int> src;           // 0-based mirror bulk node (-1 none)
+    dbuf<int> mode;          // per node*3+comp: 0 untouched, 1 copy, 2 reflect, 3 direct
+    dbuf<double> val0;       // per node*3+comp: g(0)
+    dbuf<double> vel;        // per node*3+comp: dg/dt
+    int count = 0;
+    bool merged = true;      // single-stage (parallel) program
+    dbuf<int> inv_off;       // N+1: program entries whose mirror source is node m
+    dbuf<int> inv_q;
+};
+
+struct FracState {
+    int ready = 0;
+    int precision = PXG_FP64;
+    int dim = 2;
+    double C11 = 0, C12 = 0, C66 = 0, rho = 1, b[3] = {0, 0, 0}, sc = 0;
+    int tension_only = 1, damage_on = 1;
+    // per-bond force operators P (CSR bond order): 3 (2D) or 6 (3D) components
+    dbuf<double> P64;        // nb*ncomp, component-major (SoA)
+    dbuf<float> P32;
+    dbuf<unsigned char> health;   // nb
+    dbuf<int> break_step;         // nb (recorded break step, INT_MAX intact)
+    dbuf<double> u[2];            // ping-pong displacement buffers (N*dim)
+    int cur = 0;                  // u[cur] = u^n, u[cur^1] = u^{n-1}
+    dbuf<unsigned char> preset_mask;   // N: node is a preset target
+    BcNodeProgram bc;
+    // blow-up guard state
+    dbuf<unsigned long long> gmax;     // [0] CD part, [1] preset part, [2] block counter
+    dbuf<double> ghist;                // 50-entry ring of max|u|
+    dbuf<int> gstate;                  // [0] aborted flag, [1] abort step, [2] steps done
+    // node-E (optional): per-node Young's modulus for the adjoint configuration
+    dbuf<double> nodeE;
+    double nu = 0.25;
+    int plane_stress = 0;
+    int has_nodeE = 0;
+    dbuf<double> Phat;            // nb*ncomp: operators with C(E=1) (adjoint)
+    dbuf<int> rev;                // nb: reverse bond index
+    dbuf<double> accel;           // N*dim scratch (drop-in accel / init)
+    dbuf<int> anybroke;           // [0] breaks in the last launch
+    int health_uploaded = 0;
+    // schedule state of the device-resident loop
+    double t = 0.0, dt_prev = 0.0;
+    int step = 0;
+    int cur0 = 0;                 // buffer index at step 0
+};
+
+int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
+                double c1, double c2, int step, bool guard, int frozen_k);
+int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
+int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
+
+} // namespace pxg
+
+struct pxg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int fault = 0;                  // host-test fault injection (pxg_set_fault)
+    // ---- point cloud ----
+    int n = 0, dim = 2, nbulk = 0;
+    double dxyz[3] = {0, 0, 0};
+    pxg::dbuf<double> coords;       // 3N interleaved (reference layout)
+    pxg::dbuf<double> xp;           // dim-packed coordinates (gathers)
+    pxg::dbuf<double> vols;
+    pxg::dbuf<int> mirror;          // 1-based, 0 = bulk
+    std::vector<double> hcoords;    // host copies for BC box filters
+    std::vector<int> hmirror;
+    // ---- families ----
+    double delta = 0;
+    long long nb = 0;
+    pxg::dbuf<int> off;             // N+1
+    pxg::dbuf<int> nbr;             // nb, 1-based
+    // ---- cracks ----
+    pxg::dbuf<double> crack_seg;    // 4 per segment
+    pxg::dbuf<double> crack_quad;   // 12 per quad
+    int nseg = 0, nquad = 0;
+    pxg::dbuf<unsigned char> seed;  // nb health seed (1 intact, 0 cut)
+    // ---- operators ----
+    int order = 0, nop = 0;
+    pxg::dbuf<double> ops;          // nb*nop (bond-major, reference slot order)
+    pxg::dbuf<unsigned char> flags; // N
+    int ndegenerate = 0;
+    // ---- implicit CSR ----
+    int csr_ndof = 0;
+    long long nnz = 0;
+    pxg::dbuf<int> rowptr, cols;
+    pxg::dbuf<int> slot;            // nb: CSR index of (row i*ndof, col j*ndof)
+    pxg::dbuf<int> diag;            // N: CSR index of (i*ndof, i*ndof)
+    pxg::dbuf<int> mslot;           // N: CSR index of mirror block (-1 if in family/none)
+    pxg::dbuf<double> lap;          // nb: pdLaplacian per bond (Poisson) / lambda-corrected (CH)
+    pxg::dbuf<int> rev;             // nb
+    pxg::dbuf<unsigned char> isghost;
+    pxg::dbuf<double> mnode;
+    int ch_ready = 0;
+    // scratch for host-pointer entry points
+    pxg::dbuf<double> su, suo, sk, sr, sf;
+    // ---- explicit fracture ----
+    pxg::FracState frac;
+};

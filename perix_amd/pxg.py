@@ -1,0 +1,395 @@
+"""ctypes binding of libpxg.so (the C ABI in include/pxg.h).
+
+Every call goes to the HIP library; a missing library or a failing device
+call raises PxgError -- there is no host fallback in this package.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+FP64 = 0
+FP32C = 1
+_LIB = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+U8 = C.POINTER(C.c_ubyte)
+
+
+class PxgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pxg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+                ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double)]
+
+
+class FracParams(C.Structure):
+    _fields_ = [("dim", C.c_int), ("C11", C.c_double), ("C12", C.c_double),
+                ("C66", C.c_double), ("rho", C.c_double), ("bx", C.c_double),
+                ("by", C.c_double), ("bz", C.c_double), ("sc", C.c_double),
+                ("tension_only", C.c_int), ("damage_on", C.c_int)]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", C.c_int), ("direct", C.c_int), ("ncomp", C.c_int),
+                ("comp", C.c_int * 3), ("value", C.c_double * 3),
+                ("velocity", C.c_double * 3), ("has_box", C.c_int),
+                ("box", C.c_double * 6), ("nnodes", C.c_int), ("nodes", I)]
+
+
+# every symbol of include/pxg.h (checked by the ABI test)
+SYMBOLS = [
+    "pxg_create", "pxg_destroy", "pxg_last_error", "pxg_set_fault", "pxg_sync",
+    "pxg_structured_sizes", "pxg_structured_cloud", "pxg_horizon_radius", "pxg_random_ic",
+    "pxg_schedule", "pxg_set_points", "pxg_build_families", "pxg_get_families",
+    "pxg_set_cracks", "pxg_get_seed", "pxg_build_csr", "pxg_get_csr", "pxg_build_operators",
+    "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev", "pxg_ch_setup",
+    "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
+    "pxg_frac_set_bcs", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
+    "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_adjoint", "pxg_frac_loss",
+]
+
+
+def lib_path():
+    return os.path.join(_HERE, "libpxg.so")
+
+
+def load():
+    """Loads libpxg.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            raise PxgError(-99, "libpxg.so not built (run __graft_entry__.build() or "
+                                "python perix_amd/build.py)")
+        L = C.CDLL(p)
+        L.pxg_last_error.restype = C.c_char_p
+        L.pxg_horizon_radius.restype = C.c_double
+        L.pxg_destroy.restype = None
+        L.pxg_random_ic.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_int, I, C.c_int,
+                                    C.c_int, I, D]
+        for name in ("pxg_build_operators",):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, I]
+        L.pxg_build_families.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)]
+        L.pxg_build_csr.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
+        L.pxg_assemble_poisson.argtypes = [C.c_void_p, C.c_double, D, C.c_double, D, D, D]
+        L.pxg_assemble_poisson_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
+                                               C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pxg_assemble_ch.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, D, D, D, D]
+        L.pxg_assemble_ch_dev.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
+        L.pxg_frac_setup.argtypes = [C.c_void_p, C.POINTER(FracParams), C.c_int, D, C.c_double,
+                                     C.c_int]
+        L.pxg_frac_init.argtypes = [C.c_void_p, D, D, C.c_double]
+        L.pxg_frac_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, I]
+        L.pxg_frac_adjoint.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       I, C.c_int, D, D]
+        L.pxg_frac_loss.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, D, C.c_int, I,
+                                    C.c_int, D]
+        L.pxg_frac_state_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_void_p)]
+        _LIB = L
+    return _LIB
+
+
+def available_devices():
+    """Number of HIP devices (0 when none), without creating a context."""
+    try:
+        import torch
+        return torch.cuda.device_count()
+    except Exception:
+        return 0
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(I)
+
+
+def _up(a):
+    return None if a is None else a.ctypes.data_as(U8)
+
+
+def _check(rc):
+    if rc != 0:
+        raise PxgError(rc, load().pxg_last_error().decode(errors="replace"))
+
+
+def grid_struct(g):
+    return Grid(g.dim, g.nx, g.ny, g.nz if g.dim == 3 else 0, g.lo[0], g.hi[0], g.lo[1],
+                g.hi[1], g.lo[2] if g.dim == 3 else 0.0, g.hi[2] if g.dim == 3 else 0.0)
+
+
+def make_bcs(specs):
+    """specs: dicts {type: 'neumann'|'dirichlet', nodes (1-based), comps (0-based),
+    value, velocity, direct, box=(xlo,xhi,ylo,yhi,zlo,zhi)}."""
+    arr = (BC * max(1, len(specs)))()
+    keep = []
+    for e, s in enumerate(specs):
+        b = arr[e]
+        b.type = 0 if s["type"] == "neumann" else 1
+        b.direct = int(bool(s.get("direct", False)))
+        comps = list(s["comps"])
+        b.ncomp = len(comps)
+        val = list(s.get("value", [0.0] * len(comps)))
+        vel = list(s.get("velocity", [0.0] * len(comps)))
+        if len(val) == 1:
+            val = val * len(comps)
+        if len(vel) == 1:
+            vel = vel * len(comps)
+        for k, c in enumerate(comps):
+            b.comp[k] = c
+            b.value[k] = val[k]
+            b.velocity[k] = vel[k]
+        box = s.get("box")
+        b.has_box = int(box is not None)
+        for k in range(6):
+            b.box[k] = box[k] if box is not None else (1.0 if k % 2 == 0 else -1.0)
+        nodes = np.ascontiguousarray(s["nodes"], np.int32)
+        keep.append(nodes)
+        b.nnodes = len(nodes)
+        b.nodes = _ip(nodes)
+    return arr, len(specs), keep
+
+
+class Context:
+    """One PD problem on one GPU (pxg_ctx)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        _check(L.pxg_create(C.byref(h), device))
+        self.h = h
+        self.n = 0
+        self.dim = 0
+        self.nb = 0
+        self.nop = 0
+        self.nnz = 0
+        self.ndof = 0
+
+    def close(self):
+        if self.h:
+            load().pxg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fault(self, code):
+        _check(load().pxg_set_fault(self.h, code))
+
+    def sync(self):
+        _check(load().pxg_sync(self.h))
+
+    # -- a1-a4 ---------------------------------------------------------------
+    def set_points(self, coords, vols, mirror, dxyz, dim):
+        coords = np.ascontiguousarray(coords, float)
+        vols = np.ascontiguousarray(vols, float)
+        mirror = np.ascontiguousarray(mirror, np.int32)
+        dxyz = np.ascontiguousarray(dxyz, float)
+        self.n = len(vols)
+        self.dim = dim
+        _check(load().pxg_set_points(self.h, self.n, dim, _dp(coords), _dp(vols), _ip(mirror),
+                                     _dp(dxyz)))
+
+    def build_families(self, r):
+        nb = C.c_longlong()
+        _check(load().pxg_build_families(self.h, C.c_double(r), C.byref(nb)))
+        self.nb = nb.value
+        return self.nb
+
+    def families(self):
+        off = np.zeros(self.n + 1, np.int32)
+        nbr = np.zeros(self.nb, np.int32)
+        _check(load().pxg_get_families(self.h, _ip(off), _ip(nbr)))
+        return off, nbr
+
+    def set_cracks(self, segs=(), quads=()):
+        seg = np.ascontiguousarray(np.array(segs, float).reshape(-1))
+        quad = np.ascontiguousarray(np.array(quads, float).reshape(-1))
+        _check(load().pxg_set_cracks(self.h, len(segs), _dp(seg) if len(segs) else None,
+                                     len(quads), _dp(quad) if len(quads) else None))
+
+    def seed(self):
+        s = np.zeros(self.nb, np.uint8)
+        _check(load().pxg_get_seed(self.h, _up(s)))
+        return s
+
+    def build_csr(self, ndof):
+        nnz = C.c_longlong()
+        _check(load().pxg_build_csr(self.h, ndof, C.byref(nnz)))
+        self.nnz = nnz.value
+        self.ndof = ndof
+        return self.nnz
+
+    def csr(self):
+        rp = np.zeros(self.n * self.ndof + 1, np.int32)
+        cols = np.zeros(self.nnz, np.int32)
+        _check(load().pxg_get_csr(self.h, _ip(rp), _ip(cols)))
+        return rp, cols
+
+    # -- a5-a8 ---------------------------------------------------------------
+    def build_operators(self, delta, dx_char, order=2):
+        nd = C.c_int()
+        _check(load().pxg_build_operators(self.h, order, C.c_double(delta), C.c_double(dx_char),
+                                          C.byref(nd)))
+        self.nop = 5 if self.dim == 2 else 9
+        return nd.value
+
+    def operators(self):
+        ops = np.zeros(self.nb * self.nop)
+        fl = np.zeros(self.n, np.uint8)
+        _check(load().pxg_get_operators(self.h, _dp(ops), _up(fl)))
+        return ops.reshape(self.nb, self.nop), fl
+
+    # -- a10-a13 ---------------------------------------------------------------
+    def assemble_poisson(self, sigma, u, f_node=None, f_const=0.0):
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n)
+        fn = None if f_node is None else np.ascontiguousarray(f_node, float)
+        _check(load().pxg_assemble_poisson(self.h, sigma, _dp(fn), f_const,
+                                           _dp(np.ascontiguousarray(u, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_poisson_dev(self, sigma, u_ptr, k_ptr, r_ptr, f_ptr=None, f_const=0.0):
+        _check(load().pxg_assemble_poisson_dev(self.h, sigma, f_ptr, f_const, u_ptr, k_ptr, r_ptr))
+
+    def ch_setup(self):
+        _check(load().pxg_ch_setup(self.h))
+
+    def ch_geometry(self):
+        lap = np.zeros(self.nb)
+        rev = np.zeros(self.nb, np.int32)
+        isg = np.zeros(self.n, np.uint8)
+        _check(load().pxg_ch_get_geometry(self.h, _dp(lap), _ip(rev), _up(isg)))
+        return lap, rev, isg
+
+    def assemble_ch(self, kappa, chi, M0, dt, u, uold):
+        k = np.zeros(self.nnz)
+        r = np.zeros(2 * self.n)
+        _check(load().pxg_assemble_ch(self.h, kappa, chi, M0, dt,
+                                      _dp(np.ascontiguousarray(u, float)),
+                                      _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_ch_dev(self, kappa, chi, M0, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
+        _check(load().pxg_assemble_ch_dev(self.h, kappa, chi, M0, dt, u_ptr, uold_ptr, k_ptr,
+                                          r_ptr))
+
+    # -- a15-a20 ---------------------------------------------------------------
+    def frac_setup(self, params, precision=FP64, node_E=None, nu=0.25, plane_stress=False):
+        ne = None if node_E is None else np.ascontiguousarray(node_E, float)
+        _check(load().pxg_frac_setup(self.h, C.byref(params), precision, _dp(ne), nu,
+                                     int(plane_stress)))
+
+    def frac_set_bcs(self, specs):
+        arr, n, keep = make_bcs(specs)
+        _check(load().pxg_frac_set_bcs(self.h, n, arr))
+
+    def frac_accel(self, u, health):
+        a = np.zeros(self.n * self.dim)
+        ab = C.c_int()
+        _check(load().pxg_frac_accel(self.h, _dp(np.ascontiguousarray(u, float)), _dp(health),
+                                     _dp(a), C.byref(ab)))
+        return a, ab.value
+
+    def frac_init(self, u0, dt, v0=None):
+        v = None if v0 is None else np.ascontiguousarray(v0, float)
+        _check(load().pxg_frac_init(self.h, _dp(np.ascontiguousarray(u0, float)), _dp(v), dt))
+
+    def frac_run(self, dt, total, nsteps_max=0):
+        done = C.c_int()
+        rc = load().pxg_frac_run(self.h, dt, total, nsteps_max, C.byref(done))
+        if rc != 0:
+            err = PxgError(rc, load().pxg_last_error().decode(errors="replace"))
+            err.steps = done.value
+            raise err
+        return done.value
+
+    def frac_state(self):
+        nd = self.n * self.dim
+        u = np.zeros(nd)
+        uo = np.zeros(nd)
+        h = np.zeros(self.nb, np.uint8)
+        bs = np.zeros(self.nb, np.int32)
+        _check(load().pxg_frac_get_state(self.h, _dp(u), _dp(uo), _up(h), _ip(bs)))
+        return u, uo, h, bs
+
+    def frac_state_ptrs(self):
+        a = C.c_void_p()
+        b = C.c_void_p()
+        _check(load().pxg_frac_state_ptrs(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # -- a22 ---------------------------------------------------------------------
+    def frac_adjoint(self, u0, dt, nsteps, sens, comp, ckpt=50, v0=None):
+        g = np.zeros(self.n)
+        J = C.c_double()
+        sens = np.ascontiguousarray(sens, np.int32)
+        v = None if v0 is None else np.ascontiguousarray(v0, float)
+        _check(load().pxg_frac_adjoint(self.h, _dp(np.ascontiguousarray(u0, float)), _dp(v), dt,
+                                       nsteps, ckpt, len(sens), _ip(sens), comp, C.byref(J),
+                                       _dp(g)))
+        return J.value, g
+
+    def frac_loss(self, u0, dt, nsteps, sens, comp, node_E=None, v0=None):
+        J = C.c_double()
+        sens = np.ascontiguousarray(sens, np.int32)
+        v = None if v0 is None else np.ascontiguousarray(v0, float)
+        ne = None if node_E is None else np.ascontiguousarray(node_E, float)
+        _check(load().pxg_frac_loss(self.h, _dp(np.ascontiguousarray(u0, float)), _dp(v), dt,
+                                    nsteps, _dp(ne), len(sens), _ip(sens), comp, C.byref(J)))
+        return J.value
+
+
+def structured_cloud(g):
+    L = load()
+    gs = grid_struct(g)
+    n = C.c_int()
+    nb = C.c_int()
+    _check(L.pxg_structured_sizes(C.byref(gs), C.byref(n), C.byref(nb)))
+    coords = np.zeros(3 * n.value)
+    vols = np.zeros(n.value)
+    mirror = np.zeros(n.value, np.int32)
+    wall = np.zeros(n.value, np.int32)
+    dxyz = np.zeros(3)
+    _check(L.pxg_structured_cloud(C.byref(gs), _dp(coords), _dp(vols), _ip(mirror), _ip(wall),
+                                  _dp(dxyz)))
+    return coords, vols, mirror, wall, dxyz, nb.value
+
+
+def horizon(g, factor):
+    gs = grid_struct(g)
+    return load().pxg_horizon_radius(C.byref(gs), C.c_double(factor))
+
+
+def random_ic(seed, lo, hi, nodes, ndof, dofs, values):
+    nodes = np.ascontiguousarray(nodes, np.int32)
+    dofs = np.ascontiguousarray(dofs, np.int32)
+    _check(load().pxg_random_ic(C.c_uint64(seed), lo, hi, len(nodes), _ip(nodes), ndof,
+                                len(dofs), _ip(dofs), _dp(values)))
+    return values
+
+
+def schedule(dt, total):
+    L = load()
+    L.pxg_schedule.argtypes = [C.c_double, C.c_double, C.c_int, D, D]
+    n = L.pxg_schedule(dt, total, 0, None, None)
+    dts = np.zeros(n)
+    ts = np.zeros(n)
+    L.pxg_schedule(dt, total, n, _dp(dts), _dp(ts))
+    return dts, ts

@@ -1,0 +1,119 @@
+"""GPU parity of the explicit fracture loop against the oracle.  FP64 must
+reproduce the oracle trajectory bit for bit (displacements, u_old, broken
+bond set and break steps) when both start from the same operators; the
+FP32C form reports its error against FP64."""
+import numpy as np
+import pytest
+
+from perix_amd import pxg, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(oracle, setup, nsteps_total, ops=None):
+    """Oracle trajectory; ops=None uses the oracle's own g-weights, otherwise
+    the given ones (e.g. the device's, to compare the step arithmetic alone)."""
+    s = setup
+    cl = s.cloud
+    g = cl.grid
+    m = oracle.Mesh(oracle.make_grid(g.dim, g.nx, g.ny, g.nz, hi=g.hi))
+    if ops is None:
+        ops, _ = oracle.operators(m)
+    opc = oracle.frac_opcache(m, ops)
+    p = oracle.frac_params(g.dim, problems.E_STEEL, problems.NU, problems.RHO, s.params.sc)
+    h = oracle.seed_health(m, segs=s.cracks)
+    bcs = oracle.make_bcs(s.bcs)
+    u = s.u0.copy()
+    uo = np.zeros_like(u)
+    steps, bs = oracle.explicit_run(m, p, opc, bcs, s.dt, s.total, u, uo, h)
+    return steps, u, uo, h, bs
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_explicit_fp64_bit_exact(oracle, dim):
+    if dim == 2:
+        s = problems.explicit_plate(2, 80, 40, hi=(0.02, 0.01, 1.0), sc=0.003,
+                                    cracks=[(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+                                    box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=300)
+    else:
+        s = problems.explicit_plate(3, 24, 12, 6, hi=(0.024, 0.012, 0.006), sc=0.003,
+                                    cracks=[(0.012, 0.0, 0.012, 0.0012)],
+                                    box=(0.006, 0.009, 1.0, -1.0, 1.0, -1.0), nsteps=200)
+    print("dim", dim, "N", s.cloud.n, flush=True)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    done = ctx.frac_run(s.dt, s.total)
+    u, uo, h, bs = ctx.frac_state()
+    dev_ops, _ = ctx.operators()
+    # same operators: the step arithmetic must agree bit for bit
+    steps, ru, ruo, rh, rbs = _oracle_run(oracle, s, s.nsteps, ops=dev_ops)
+    assert done == steps
+    assert (rh == 0).sum() > 0, "case must break bonds"
+    assert np.array_equal(h.astype(float), rh)
+    assert np.array_equal(bs, rbs)
+    assert np.array_equal(u.view(np.uint64), ru.view(np.uint64))
+    assert np.array_equal(uo.view(np.uint64), ruo.view(np.uint64))
+    # end to end (each side with its own g-weights): same broken-bond set,
+    # displacements within FP64 round-off of the operators
+    steps2, ru2, _, rh2, rbs2 = _oracle_run(oracle, s, s.nsteps)
+    rel = np.linalg.norm(u - ru2) / np.linalg.norm(ru2)
+    print("end-to-end rel-L2 %.3e, broken-bond mismatches %d" % (rel, int((h.astype(float) != rh2).sum())))
+    assert rel <= 1e-12
+    assert np.array_equal(h.astype(float), rh2)
+    ctx.close()
+
+
+def test_explicit_fp32c_error(oracle):
+    s64 = problems.explicit_plate(2, 80, 40, hi=(0.02, 0.01, 1.0), sc=0.003,
+                                  cracks=[(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+                                  box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=300)
+    s32 = problems.explicit_plate(2, 80, 40, hi=(0.02, 0.01, 1.0), sc=0.003,
+                                  cracks=[(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+                                  box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=300,
+                                  precision=pxg.FP32C)
+    for s in (s64, s32):
+        s.ctx.frac_init(s.u0, s.dt)
+        s.ctx.frac_run(s.dt, s.total)
+    u64, _, h64, _ = s64.ctx.frac_state()
+    u32, _, h32, _ = s32.ctx.frac_state()
+    rel = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
+    mism = int((h32 != h64).sum())
+    print("FP32C rel-L2 %.3e, max abs %.3e, broken-bond mismatches %d of %d broken"
+          % (rel, np.abs(u32 - u64).max(), mism, int((h64 == 0).sum())))
+    assert rel <= 1e-5
+    s64.ctx.close(); s32.ctx.close()
+
+
+def test_drop_in_accel(oracle):
+    """pxg_frac_accel = one preprocessIteration (damage + force + accel)."""
+    s = problems.explicit_plate(2, 40, 20, hi=(0.02, 0.01, 1.0), sc=0.003,
+                                cracks=[(0.005, 0.0, 0.005, 0.005)],
+                                box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=10)
+    cl = s.cloud
+    m = oracle.Mesh(oracle.make_grid(2, 40, 20, hi=(0.02, 0.01, 1.0)))
+    ops, _ = s.ctx.operators()
+    opc = oracle.frac_opcache(m, ops)
+    p = oracle.frac_params(2, problems.E_STEEL, problems.NU, problems.RHO, s.params.sc)
+    h_ref = oracle.seed_health(m, segs=s.cracks)
+    u = np.random.RandomState(1).standard_normal(cl.n * 2) * 2e-5
+    a_ref = oracle.frac_accel(m, p, opc, h_ref, u)
+    h = oracle.seed_health(m, segs=s.cracks)
+    a, broke = s.ctx.frac_accel(u, h)
+    assert np.array_equal(a.view(np.uint64), a_ref.view(np.uint64))
+    assert np.array_equal(h, h_ref)
+    assert broke == int((h_ref == 0).sum() > (oracle.seed_health(m, segs=s.cracks) == 0).sum())
+    s.ctx.close()
+
+
+def test_blowup_guard(oracle):
+    """dt far above the CFL limit trips the guard at the same step as the oracle."""
+    s = problems.explicit_plate(2, 40, 20, hi=(0.02, 0.01, 1.0), sc=0.0, cracks=[],
+                                box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=400, cfl=3.0)
+    s.ctx.frac_init(s.u0, s.dt)
+    with pytest.raises(pxg.PxgError) as ei:
+        s.ctx.frac_run(s.dt, s.total)
+    steps, ru, ruo, rh, rbs = _oracle_run(oracle, s, s.nsteps)
+    assert steps < 0
+    assert ei.value.code == -6
+    assert ei.value.steps == -steps - 1
+    s.ctx.close()

@@ -1,0 +1,87 @@
+"""GPU parity: families, crack seeds, CSR pattern and g-weights against the
+oracle (which is itself pinned to the reference build and its golden files).
+Families, seeds and CSR must be bit-exact; g-weights within 1e-12 relative
+(max over bonds, normalised by the largest operator of the same slot)."""
+import numpy as np
+import pytest
+
+import perix_amd
+from perix_amd import pxg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (2, 20, 12, 1, (1.0, 0.6, 1.0)),
+    (2, 100, 100, 1, (1.0, 1.0, 1.0)),       # config 1 grid
+    (2, 400, 200, 1, (0.2, 0.1, 1.0)),       # config 2 grid
+    (3, 7, 6, 5, (0.7, 0.6, 0.5)),
+    (3, 24, 12, 6, (0.024, 0.012, 0.006)),
+]
+
+
+def _setup(oracle, dim, nx, ny, nz, hi, cracks=()):
+    g = perix_amd.StructuredGrid(dim, nx, ny, nz, hi=hi)
+    cloud = perix_amd.build_structured_cloud(g)
+    delta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+    if cracks:
+        ctx.set_cracks(segs=cracks)
+    ctx.build_families(delta)
+    m = oracle.Mesh(oracle.make_grid(dim, nx, ny, nz, hi=hi))
+    return ctx, cloud, delta, m
+
+
+@pytest.mark.parametrize("dim,nx,ny,nz,hi", CASES)
+def test_families_bit_exact(oracle, dim, nx, ny, nz, hi):
+    print("case", dim, nx, ny, nz, flush=True)
+    ctx, cloud, delta, m = _setup(oracle, dim, nx, ny, nz, hi)
+    off, nbr = ctx.families()
+    assert ctx.nb == m.nb
+    assert np.array_equal(off, m.off)
+    assert np.array_equal(nbr, m.nbr)        # reference insertion order
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,nx,ny,nz,hi", CASES)
+def test_operators(oracle, dim, nx, ny, nz, hi):
+    print("case", dim, nx, ny, nz, flush=True)
+    ctx, cloud, delta, m = _setup(oracle, dim, nx, ny, nz, hi)
+    nd = ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ops, fl = ctx.operators()
+    ref, rfl = oracle.operators(m)
+    assert nd == int((rfl & 1).sum())
+    assert np.array_equal(fl, rfl)
+    scale = np.abs(ref).max(axis=0)
+    err = (np.abs(ops - ref) / scale).max()
+    exact = np.mean(ops == ref)
+    print("ops max rel err %.3e, bit-exact fraction %.6f" % (err, exact))
+    assert err <= 1e-12
+    ctx.close()
+
+
+def test_crack_seed_and_csr(oracle):
+    cr = [(0.05, 0.0, 0.05, 0.05), (0.15, 0.0, 0.15, 0.05)]
+    ctx, cloud, delta, m = _setup(oracle, 2, 80, 40, 1, (0.2, 0.1, 1.0), cracks=cr)
+    seed = ctx.seed()
+    ref = oracle.seed_health(m, segs=cr)
+    assert np.array_equal(seed.astype(float), ref)
+    assert (seed == 0).sum() > 0
+    for ndof in (1, 2):
+        ctx.build_csr(ndof)
+        rp, cols = ctx.csr()
+        rrp, rcols = oracle.csr_pattern(m, ndof)
+        assert np.array_equal(rp, rrp) and np.array_equal(cols, rcols)
+    ctx.close()
+
+
+def test_crack_plane_seed(oracle):
+    # 3D through-thickness notch given as a quad (bondPiercesQuad)
+    quad = [0.012, -0.001, -0.001, 0.012, 0.003, -0.001, 0.012, 0.003, 0.007, 0.012, -0.001, 0.007]
+    ctx, cloud, delta, m = _setup(oracle, 3, 24, 12, 6, (0.024, 0.012, 0.006))
+    ctx.set_cracks(quads=[quad])
+    seed = ctx.seed()
+    ref = oracle.seed_health(m, quads=[quad])
+    assert np.array_equal(seed.astype(float), ref)
+    assert (seed == 0).sum() > 0
+    ctx.close()
