@@ -1,0 +1,364 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit PDDO fracture hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the 2D plate
+0.2 m x 0.1 m at dx = 0.5 mm (400 x 200 cells, 81,200 PD nodes, 2,251,924
+directed bonds), two 50 mm force-only edge notches, a 16.5 m/s impactor
+patch, E = 191 GPa, nu = 0.25, rho = 8000 kg/m^3, plane strain, critical
+stretch 0.01, CFL-limited dt (0.9 x dx/c), 2000 central-difference steps.
+
+A "step" is one explicit time step (damage, Navier force, central-difference
+update, BC presets, blow-up guard) over all nodes.  Inputs are resident in
+HBM; the one-off family / g-weight build is timed separately.  `value` is the
+FP32-compensated form's node-steps/s; the FP64 form, the FP32C error against
+FP64, the adjoint/forward cost (200 x 100 plate, 500 steps) and the 3D block
+(160 x 80 x 40, 556,800 nodes) are reported beside it.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "explicit PDDO node-steps/s FP32-comp vs FP64 at stated err; adjoint/fwd cost"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class quiet_stdout:
+    """Redirects fd 1 to fd 2 (the reference prints progress on stdout)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def frac_bytes(nb, n, dim, precision):
+    """Algorithmic HBM bytes of one step launch: per directed bond the
+    neighbour id (4 B), the Navier operators (3 or 6 values) and the health
+    byte; per node u^n, u^{n-1} read, u^{n+1} written and x read."""
+    ncomp = 3 if dim == 2 else 6
+    opb = 4 if precision == 1 else 8
+    per_bond = 4 + ncomp * opb + 1
+    per_node = 4 * dim * 8
+    return nb * per_bond + n * per_node
+
+
+def timed_run(ctx, s, steps, warmup, profile=False):
+    """Warm-up pass, then a fresh run of `steps` steps timed on the host
+    clock (device synchronised on both sides).  Returns seconds."""
+    total = steps * s.dt
+    if warmup > 0:
+        ctx.frac_init(s.u0, s.dt)
+        ctx.frac_run(s.dt, total, nsteps_max=warmup)
+    ctx.frac_init(s.u0, s.dt)
+    ctx.sync()
+    if profile:
+        ctx.frac_profile(True)
+    t0 = time.perf_counter()
+    done = ctx.frac_run(s.dt, total, nsteps_max=steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    if profile:
+        ms, launches = ctx.frac_kernel_time()
+        ctx.frac_profile(False)
+        return t1 - t0, done, ms, launches
+    return t1 - t0, done
+
+
+def plate_case(problems, pxg, precision, steps):
+    t0 = time.perf_counter()
+    s = problems.plate_config2(precision=precision, nsteps=steps)
+    s.ctx.sync()
+    return s, time.perf_counter() - t0
+
+
+def cpu_reference_baseline(problems, n_nodes, dt, budget_s=12.0):
+    """The reference's own OpenMP explicit loop (oracle/_ref, compiled from the
+    reference sources) on this host, on a bounded sample of the same plate;
+    falls back to the C oracle when that build is absent."""
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    from oracle import refpy
+    if refpy.available():
+        import perix_amd
+        g = perix_amd.StructuredGrid(2, 400, 200, hi=(0.2, 0.1, 1.0))
+        delta = perix_amd.horizon_radius(g, 3.015)
+        G0 = problems.g0_for_stretch(0.01, problems.E_STEEL, delta, 2)
+
+        def deck(nsteps):
+            d = {"Mesh": refpy.mesh_block(2, 400, 200, hi=(0.2, 0.1, 1.0)),
+                 "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+                 "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                                "Cracks": [{"x1": 0.05, "y1": 0.0, "x2": 0.05, "y2": 0.05},
+                                           {"x1": 0.15, "y1": 0.0, "x2": 0.15, "y2": 0.05}]},
+                 "DOFs": ["ux", "uy"],
+                 "ElmtSystem": {"f": {"type": "explicit_pddo_frac", "params": {
+                     "E": problems.E_STEEL, "nu": problems.NU, "rho": problems.RHO, "G0": G0,
+                     "state": "plane_strain", "tension_only": True, "damage": True}}},
+                 "BCSystem": {}, "JobSystem": {"type": "transient", "assemble": "openmp"},
+                 "TimeStepping": {"dt": dt, "total_time": nsteps * dt, "verbose": False}}
+            for w in ("left", "right", "bottom", "top"):
+                d["BCSystem"]["n_" + w] = {"type": "neumann", "phygroup": w + "nodes_ghost",
+                                          "dofs": ["ux", "uy"], "value": [0.0, 0.0]}
+            d["BCSystem"]["impactor"] = {"type": "dirichlet", "phygroup": "bottomnodes_ghost",
+                                         "dofs": ["ux", "uy"], "value": [0.0, 0.0],
+                                         "velocity": [0.0, 16.5], "direct": True,
+                                         "box": {"xmin": 0.075, "xmax": 0.125}}
+            return d
+
+        def run(nsteps):
+            with quiet_stdout():
+                c = refpy.Case(deck(nsteps))
+                steps, setup_s, loop_s, _ = c.explicit_timed()
+                c.close()
+            return steps, setup_s, loop_s
+
+        steps, _, loop_s = run(10)
+        per_step = loop_s / max(steps, 1)
+        sample = int(min(2000, max(20, budget_s / max(per_step, 1e-6))))
+        steps, setup_s, loop_s = run(sample)
+        return {"value": n_nodes * steps / loop_s, "unit": "node-steps/s", "cores": cores,
+                "kind": "reference",
+                "sample": "reference OpenMP explicit loop (oracle/_ref built from the reference "
+                          "sources), first %d steps of the same 400x200 plate; one-off "
+                          "operator build (%.2f s) excluded" % (steps, setup_s)}
+    from oracle import pxo
+    m = pxo.Mesh(pxo.make_grid(2, 400, 200, hi=(0.2, 0.1, 1.0)))
+    ops, _ = pxo.operators(m)
+    opc = pxo.frac_opcache(m, ops)
+    G0 = problems.g0_for_stretch(0.01, problems.E_STEEL, m.delta, 2)
+    sc = problems.critical_stretch(G0, problems.E_STEEL, m.delta, 2)
+    p = pxo.frac_params(2, problems.E_STEEL, problems.NU, problems.RHO, sc)
+    h = pxo.seed_health(m, segs=[(0.05, 0.0, 0.05, 0.05), (0.15, 0.0, 0.15, 0.05)])
+    specs = [{"type": "neumann", "nodes": m.group(w), "comps": [0, 1]} for w in range(4)]
+    specs.append({"type": "dirichlet", "nodes": m.group(2), "comps": [0, 1], "value": [0, 0],
+                  "velocity": [0, 16.5], "direct": True,
+                  "box": (0.075, 0.125, 1.0, -1.0, 1.0, -1.0)})
+    bcs = pxo.make_bcs(specs)
+    u = np.zeros(2 * m.n)
+    pxo.bc_preset(m, 2, bcs, u, 0.0)
+    uo = np.zeros_like(u)
+    nsteps = 100
+    t0 = time.perf_counter()
+    pxo.explicit_run(m, p, opc, bcs, dt, nsteps * dt, u, uo, h)
+    loop_s = time.perf_counter() - t0
+    return {"value": m.n * nsteps / loop_s, "unit": "node-steps/s", "cores": cores,
+            "kind": "port", "sample": "C oracle OpenMP loop, first %d steps" % nsteps}
+
+
+def adjoint_case(problems, pxg, nsteps=500, nsens=64, ndirs=20, seed=2024):
+    """Config 4: 200 x 100 plate (dx = 1 mm), E_i = 191 GPa (1 + 0.05 N(0,1)),
+    J = sum over 64 sensors of u_y^2 at step 500, checkpoints every 50 steps."""
+    rs = np.random.RandomState(seed)
+    probe = problems.explicit_plate(2, 200, 100, nsteps=nsteps)
+    n, nbulk = probe.cloud.n, probe.cloud.nbulk
+    probe.ctx.close()
+    E = problems.E_STEEL * (1.0 + 0.05 * rs.standard_normal(n))
+    sens = rs.choice(nbulk, nsens, replace=False).astype(np.int32)
+    s = problems.explicit_plate(2, 200, 100, nsteps=nsteps, node_E=E)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    ctx.frac_run(s.dt, s.total)          # records break steps
+    ctx.sync()
+    # warm, then time: one forward (frozen health) and one adjoint
+    ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1)
+    ctx.frac_adjoint(s.u0, s.dt, nsteps, sens, 1, ckpt=50)
+    t0 = time.perf_counter()
+    ctx.frac_init(s.u0, s.dt)
+    ctx.frac_run(s.dt, s.total)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J0 = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1)
+    t_fwd_frozen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, g = ctx.frac_adjoint(s.u0, s.dt, nsteps, sens, 1, ckpt=50)
+    t_adj = time.perf_counter() - t0
+    worst = 0.0
+    hh = 1e-6
+    for _ in range(ndirs):
+        v = rs.standard_normal(n) * E
+        Jp = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1, node_E=E + hh * v)
+        Jm = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1, node_E=E - hh * v)
+        fd = (Jp - Jm) / (2 * hh)
+        ad = float(g @ v)
+        worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
+    ctx.close()
+    return {"workload": "200x100 plate, 500 steps, 64 sensors, checkpoint every 50",
+            "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "reverse_over_forward": (t_adj - t_fwd_frozen) / t_fwd,
+            "fd_max_rel_err_20dirs": worst, "J": J, "J_forward": J0}
+
+
+def block_case(problems, pxg, steps=100):
+    """Config 5: 3D block 160 x 80 x 40 (556,800 PD nodes), FP32C and FP64."""
+    out = {}
+    states = {}
+    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
+        t0 = time.perf_counter()
+        s = problems.block_config5(precision=prec, nsteps=steps)
+        s.ctx.sync()
+        setup = time.perf_counter() - t0
+        sec, done, ms, launches = timed_run(s.ctx, s, steps, 5, profile=True)
+        sec2, done2 = timed_run(s.ctx, s, steps, 0)
+        u, _, h, _ = s.ctx.frac_state()
+        states[name] = (u, h)
+        nb, n = s.ctx.nb, s.cloud.n
+        avg_ms = ms / max(launches, 1)
+        gbs = frac_bytes(nb, n, 3, prec) / (avg_ms * 1e-3) / 1e9
+        out[name] = {"node_steps_per_s": n * done2 / sec2, "ms_per_step": sec2 * 1e3 / done2,
+                     "step_kernel_ms": avg_ms, "step_kernel_GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS, "setup_s": setup}
+        out["nodes"] = n
+        out["bonds"] = nb
+        s.ctx.close()
+    u32, h32 = states["fp32c"]
+    u64, h64 = states["fp64"]
+    out["fp32c_rel_l2_err"] = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
+    out["fp32c_broken_bond_mismatch"] = int((h32 != h64).sum())
+    out["fp32c_over_fp64"] = out["fp32c"]["node_steps_per_s"] / out["fp64"]["node_steps_per_s"]
+    out["steps"] = steps
+    return out
+
+
+def copy_calibration():
+    """A plain device copy of known size for calibrating rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters (64 MiB read, 64 MiB written, 10 launches)."""
+    import torch
+    a = torch.empty(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8,
+                      "launches": 10}))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and 3D block figures")
+    ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU slab "
+              "decomposition is a design (see the sharding notes), not a measured path. "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import perix_amd
+    from perix_amd import pxg, problems
+
+    K, W = args.steps, args.warmup
+    res = {}
+    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
+        log("plate %s: setup" % name)
+        s, setup_s = plate_case(problems, pxg, prec, K)
+        ctx = s.ctx
+        sec, done = timed_run(ctx, s, K, W)
+        log("plate %s: %d steps in %.3f s" % (name, done, sec))
+        sec_p, done_p, ms, launches = timed_run(ctx, s, K, 0, profile=True)
+        u, _, h, _ = ctx.frac_state()
+        res[name] = {"sec": sec, "done": done, "setup_s": setup_s, "kernel_ms": ms,
+                     "launches": launches, "u": u, "h": h, "n": s.cloud.n, "nb": ctx.nb}
+        ctx.close()
+    n, nb = res["fp32c"]["n"], res["fp32c"]["nb"]
+    f32, f64 = res["fp32c"], res["fp64"]
+    value = n * f32["done"] / f32["sec"]
+    v64 = n * f64["done"] / f64["sec"]
+    u32, u64 = f32["u"], f64["u"]
+    rel = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
+    mism = int((f32["h"] != f64["h"]).sum())
+    avg_ms = f32["kernel_ms"] / max(f32["launches"], 1)
+    bytes32 = frac_bytes(nb, n, 2, 1)
+    achieved = bytes32 / (avg_ms * 1e-3) / 1e9
+    avg64 = f64["kernel_ms"] / max(f64["launches"], 1)
+    achieved64 = frac_bytes(nb, n, 2, 0) / (avg64 * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("frac_step_plate400x200_fp32c_bytes_per_launch")
+
+    dt = f32.get("dt")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": f32["sec"] * 1e3 / f32["done"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 compensated family sums + f64 damage/update",
+        "data": "synthetic",
+        "config": {"workload": "explicit_pddo_frac 2D plate 0.2x0.1 m, 400x200 cells "
+                               "(81,200 PD nodes, %d directed bonds), two force-only edge "
+                               "notches, 16.5 m/s impactor, %d CFL-limited steps" % (nb, K),
+                   "nodes": n, "bonds": nb, "horizon": "3.015 dx", "order": 2},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_frac_step<2,fp32c>", "algorithmic_bytes_per_launch": bytes32,
+                     "avg_launch_ms": avg_ms,
+                     "note": "the 43.5 MB working set of the 2D plate stays in the 256 MB "
+                             "Infinity Cache, so this kernel is latency/launch-bound; the HBM "
+                             "claim is made on the 3D block (extra.block3d)"},
+        "fp64": {"node_steps_per_s": v64, "ms_per_step": f64["sec"] * 1e3 / f64["done"],
+                 "step_kernel_ms": avg64, "step_kernel_GBs": achieved64},
+        "fp32c_over_fp64": value / v64,
+        "fp32c_error_vs_fp64": {"rel_l2": rel, "max_abs": float(np.abs(u32 - u64).max()),
+                                "broken_bond_mismatch": mism,
+                                "broken_bonds_fp64": int((f64["h"] == 0).sum())},
+        "setup_s": {"fp32c": f32["setup_s"], "fp64": f64["setup_s"]},
+    }
+    if not args.no_cpu and not args.only_plate:
+        log("cpu baseline")
+        s0 = problems.plate_config2(precision=pxg.FP64, nsteps=K)
+        dt = s0.dt
+        s0.ctx.close()
+        try:
+            out["cpu_baseline"] = cpu_reference_baseline(problems, n, dt)
+        except Exception as e:   # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "unit": "node-steps/s", "cores": os.cpu_count(),
+                                   "kind": "reference", "sample": "failed: %s" % e}
+    if not args.no_extra and not args.only_plate:
+        log("adjoint (200x100, 500 steps)")
+        out["adjoint"] = adjoint_case(problems, pxg)
+        log("3D block (160x80x40)")
+        out["block3d"] = block_case(problems, pxg)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

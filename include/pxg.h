@@ -203,7 +203,8 @@ int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout,
  * TimeStepping.cpp:417-599): the host passes the state once; every step runs
  * damage + force + update + BC preset + blow-up guard on the device.
  * pxg_frac_init: u0 = preset initial state (n*dim), v0 initial velocity (may be
- * NULL); performs the t=0 rate evaluation and u^{-1}.  Steps follow the
+ * NULL); resets bond health to the crack seed, performs the t=0 rate
+ * evaluation and u^{-1}.  Steps follow the
  * reference schedule for (dt, total); nsteps_max (>0) stops early after that
  * many steps (for timing).  Returns PXG_ERR_BLOWUP when the guard trips. */
 int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, double dt);
@@ -217,6 +218,11 @@ int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold,
 /* Device pointer of the displacement u^n and u^{n-1} (valid until the next
  * pxg_frac_* call). */
 int pxg_frac_state_ptrs(pxg_ctx *ctx, double **u_dev, double **uold_dev);
+
+/* Per-launch timing of the step kernel inside pxg_frac_run (HIP events on
+ * the context's stream): enable, run, then read the summed kernel time. */
+int pxg_frac_profile(pxg_ctx *ctx, int enable);
+int pxg_frac_kernel_time(pxg_ctx *ctx, double *ms_total, int *launches);
 
 /* ---- a22: adjoint of the explicit step (new; no reference counterpart) ----
  * Gradient of J = sum_s u^K[sens_s*dim+comp]^2 with respect to the per-node

@@ -33,6 +33,8 @@ def lib():
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
+        _LIB.ref_explicit_timed.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), D]
         _LIB.ref_group.restype = C.c_int
     return _LIB
 
@@ -132,6 +134,13 @@ class Case:
         o = np.zeros(5)
         lib().ref_frac_consts(self.h, _dp(o))
         return o
+
+    def explicit_timed(self):
+        """(steps, one-off setup seconds, loop seconds, final u)."""
+        u = np.zeros(self.n * self.ndof)
+        a = C.c_double(); b = C.c_double()
+        steps = lib().ref_explicit_timed(self.h, C.byref(a), C.byref(b), _dp(u))
+        return steps, a.value, b.value, u
 
     def explicit(self):
         nd = self.n * self.ndof

@@ -655,6 +655,10 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     hipStream_t st = ctx->stream;
     const size_t nd = (size_t)ctx->n * ctx->dim;
     F.cur = 0;
+    // a new run starts from the seeded bond health
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, ctx->seed.p,
+                       F.health.p, F.break_step.p);
+    PXG_HIP(hipGetLastError());
     PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     else PXG_HIP(hipMemsetAsync(F.accel.p, 0, sizeof(double) * nd, st));
@@ -684,6 +688,15 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
         const double dtA = fmin(dt, total - F.t);
         const double tNew = F.t + dtA;
         double *ucur = F.u[F.cur].p, *uold = F.u[F.cur ^ 1].p;
+        if (F.prof) {
+            if ((size_t)(2 * F.nev + 2) > F.ev.size()) {
+                hipEvent_t a, b;
+                PXG_HIP(hipEventCreate(&a));
+                PXG_HIP(hipEventCreate(&b));
+                F.ev.push_back(a); F.ev.push_back(b);
+            }
+            PXG_HIP(hipEventRecord(F.ev[2 * F.nev], ctx->stream));
+        }
         if (dtA == F.dt_prev) {
             rc = launch_step(ctx, MODE_CD, ucur, uold, uold, dtA * dtA, 0.0, step, true, INT_MIN);
         } else {
@@ -692,6 +705,7 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
             rc = launch_step(ctx, MODE_VAR, ucur, uold, uold, rr, w, step, true, INT_MIN);
         }
         if (rc) return rc;
+        if (F.prof) { PXG_HIP(hipEventRecord(F.ev[2 * F.nev + 1], ctx->stream)); F.nev++; }
         rc = launch_preset(ctx, uold, tNew, step);
         if (rc) return rc;
         F.cur ^= 1;
@@ -741,5 +755,30 @@ extern "C" int pxg_frac_state_ptrs(pxg_ctx *ctx, double **u_dev, double **uold_d
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_state_ptrs: no explicit state");
     if (u_dev) *u_dev = ctx->frac.u[ctx->frac.cur].p;
     if (uold_dev) *uold_dev = ctx->frac.u[ctx->frac.cur ^ 1].p;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_profile(pxg_ctx *ctx, int enable)
+{
+    PXG_CHECK(ctx, PXG_ERR_ARG, "pxg_frac_profile: ctx is NULL");
+    ctx->frac.prof = enable ? 1 : 0;
+    ctx->frac.nev = 0;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_kernel_time(pxg_ctx *ctx, double *ms_total, int *launches)
+{
+    PXG_CHECK(ctx && ms_total && launches, PXG_ERR_ARG, "pxg_frac_kernel_time: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    FracState &F = ctx->frac;
+    double tot = 0.0;
+    for (int k = 0; k < F.nev; k++) {
+        float ms = 0.0f;
+        PXG_HIP(hipEventElapsedTime(&ms, F.ev[2 * k], F.ev[2 * k + 1]));
+        tot += ms;
+    }
+    *ms_total = tot;
+    *launches = F.nev;
     return PXG_OK;
 }

@@ -100,6 +100,10 @@ struct FracState {
     double t = 0.0, dt_prev = 0.0;
     int step = 0;
     int cur0 = 0;                 // buffer index at step 0
+    // optional per-launch timing of the step kernel (pxg_frac_profile)
+    int prof = 0;
+    std::vector<hipEvent_t> ev;   // start/stop pairs
+    int nev = 0;
 };
 
 int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,

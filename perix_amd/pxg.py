@@ -52,7 +52,8 @@ SYMBOLS = [
     "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
-    "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_adjoint", "pxg_frac_loss",
+    "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
+    "pxg_frac_adjoint", "pxg_frac_loss",
 ]
 
 
@@ -319,6 +320,16 @@ class Context:
             err.steps = done.value
             raise err
         return done.value
+
+    def frac_profile(self, enable):
+        _check(load().pxg_frac_profile(self.h, int(enable)))
+
+    def frac_kernel_time(self):
+        """(summed step-kernel milliseconds, launches) since profiling was enabled."""
+        ms = C.c_double()
+        n = C.c_int()
+        _check(load().pxg_frac_kernel_time(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def frac_state(self):
         nd = self.n * self.dim
