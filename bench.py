@@ -262,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and 3D block figures")
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
+    ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -278,6 +279,9 @@ def main():
     from perix_amd import pxg, problems
 
     K, W = args.steps, args.warmup
+    if args.only_block:
+        print(json.dumps({"block3d": block_case(problems, pxg, steps=K)}))
+        return 0
     res = {}
     for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
         log("plate %s: setup" % name)

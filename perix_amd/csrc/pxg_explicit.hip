@@ -56,29 +56,44 @@ struct StepArgs {
 };
 
 template <int DIM> struct TileCfg;
-template <> struct TileCfg<2> { static constexpr int T = 32; static constexpr int K = 4; };
-template <> struct TileCfg<3> { static constexpr int T = 8; static constexpr int K = 4; };
+// T nodes per 256-thread block, S = 256/T lanes per node (S divides 64, so a
+// node's lanes share one wavefront), R bond slots per lane and round.
+template <> struct TileCfg<2> { static constexpr int T = 32; static constexpr int R = 4; };
+template <> struct TileCfg<3> { static constexpr int T = 8; static constexpr int R = 4; };
+
+// TwoSum (Knuth): a + b = s + e exactly, FP32
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    const float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
 
 template <int DIM, bool F32, int MODE>
 __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 {
     constexpr int T = TileCfg<DIM>::T;
-    constexpr int K = TileCfg<DIM>::K;
-    constexpr int CH = 256 * K;
+    constexpr int S = 256 / T;
+    constexpr int R = TileCfg<DIM>::R;
+    constexpr int MAXS = S * R;            // slots per round
     constexpr int NC = (DIM == 3) ? 6 : 3;
-    using TermT = typename std::conditional<F32, float, double>::type;
-    __shared__ int s_off[T + 1];
+    // FP64: per-bond forces go through LDS so that one lane per node can add
+    // them in family order; FP32C reduces in registers and needs no buffer.
+    constexpr int FS = F32 ? 1 : MAXS;
+    __shared__ double s_f[T][FS][DIM];
+    __shared__ unsigned char s_ok[T][FS];
     __shared__ double s_u[T][DIM];
     __shared__ double s_x[T][DIM];
-    __shared__ TermT s_f[CH][DIM];
-    __shared__ unsigned char s_ok[CH];
+    __shared__ int s_b0[T + 1];
+    __shared__ int s_maxfam;
     __shared__ unsigned long long s_max[4];
 
     if (a.gstate && a.gstate[0]) return;            // blow-up guard tripped earlier
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * T;
     const int nt = min(T, a.n - i0);
-    if (tid <= nt) s_off[tid] = a.off[i0 + tid];
+    if (tid == 0) s_maxfam = 0;
+    if (tid <= nt) s_b0[tid] = a.off[i0 + tid];
     if (tid < nt) {
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
@@ -87,130 +102,183 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         }
     }
     __syncthreads();
-    const int bt0 = s_off[0], bt1 = s_off[nt];
-    // per-node accumulators live in the lane that owns the node
-    double L[DIM];
-    TermT Ls[DIM], Lc[DIM];
+    if (tid < nt) atomicMax(&s_maxfam, s_b0[tid + 1] - s_b0[tid]);
+    __syncthreads();
+    const int ln = tid / S, sl = tid % S;
+    const bool nvalid = ln < nt;
+    const int b0 = nvalid ? s_b0[ln] : 0;
+    const int fam = nvalid ? s_b0[ln + 1] - b0 : 0;
+    const int maxfam = s_maxfam;
+    double ui[DIM], xi[DIM];
 #pragma unroll
-    for (int c = 0; c < DIM; c++) { L[c] = 0.0; Ls[c] = 0; Lc[c] = 0; }
+    for (int c = 0; c < DIM; c++) { ui[c] = nvalid ? s_u[ln][c] : 0.0; xi[c] = nvalid ? s_x[ln][c] : 0.0; }
+
+    double L[DIM];                     // FP64 sum (lane tid < nt owns node tid)
+    float Ls[DIM], Lc[DIM];            // FP32C compensated pair (every lane)
+#pragma unroll
+    for (int c = 0; c < DIM; c++) { L[c] = 0.0; Ls[c] = 0.0f; Lc[c] = 0.0f; }
     int broke = 0;
 
-    for (int cs = bt0; cs < bt1; cs += CH) {
-        const int ce = min(bt1, cs + CH);
+    for (int rb = 0; rb < maxfam; rb += MAXS) {
+        // ---- issue every load of the lane's R bonds first ----------------
+        int tt[R], jj[R];
+        unsigned char ok[R];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int lt = tid + k * 256;
-            const int t = cs + lt;
-            if (t >= ce) continue;
-            // local source node: last ln with s_off[ln] <= t
-            int lo = 0, hi = nt - 1;
-            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= t) lo = mid; else hi = mid - 1; }
-            const int ln = lo;
-            const unsigned char ok = a.frozen ? (a.break_step[t] > a.frozen_k) : a.health[t];
-            if (!ok) { s_ok[lt] = 0; continue; }
-            const int j = a.nbr[t] - 1;
-            double uj[DIM], du[DIM];
-#pragma unroll
-            for (int c = 0; c < DIM; c++) { uj[c] = a.u[(size_t)j * DIM + c]; du[c] = uj[c] - s_u[ln][c]; }
-            if (a.damage && !a.frozen) {
-                // stretch of ExplicitPDDOFracElement.cpp:249-266
-                double xi[DIM];
-#pragma unroll
-                for (int c = 0; c < DIM; c++) xi[c] = a.xp[(size_t)j * DIM + c] - s_x[ln][c];
-                double rr = xi[0] * xi[0] + xi[1] * xi[1];
-                if (DIM == 3) rr = rr + xi[DIM - 1] * xi[DIM - 1];
-                const double r = sqrt(rr);
-                if (r > 0.0) {
-                    const double yx = xi[0] + du[0], yy = xi[1] + du[1];
-                    double yy2 = yx * yx + yy * yy;
-                    if (DIM == 3) { const double yz = xi[DIM - 1] + du[DIM - 1]; yy2 = yy2 + yz * yz; }
-                    const double s = (sqrt(yy2) - r) / r;
-                    if (a.tension_only ? (s > a.sc) : (fabs(s) > a.sc)) {
-                        a.health[t] = 0;
-                        if (a.break_step) a.break_step[t] = a.step;
-                        broke = 1;
-                        s_ok[lt] = 0;
-                        continue;
-                    }
-                }
+        for (int r = 0; r < R; r++) {
+            const int slot = rb + sl + r * S;
+            const bool v = slot < fam;
+            tt[r] = v ? b0 + slot : -1;
+            ok[r] = 0; jj[r] = 0;
+            if (v) {
+                ok[r] = a.frozen ? (unsigned char)(a.break_step[tt[r]] > a.frozen_k) : a.health[tt[r]];
+                jj[r] = a.nbr[tt[r]] - 1;
             }
-            // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295)
-            if (F32) {
-                float p[NC];
-#pragma unroll
-                for (int c = 0; c < NC; c++) p[c] = a.P32[(size_t)c * a.nb + t];
-                float d[DIM];
-#pragma unroll
-                for (int c = 0; c < DIM; c++) d[c] = (float)du[c];
-                if (DIM == 2) {
-                    s_f[lt][0] = (TermT)fmaf(p[0], d[0], p[2] * d[1]);
-                    s_f[lt][1] = (TermT)fmaf(p[2], d[0], p[1] * d[1]);
-                } else {
-                    s_f[lt][0] = (TermT)fmaf(p[0], d[0], fmaf(p[3], d[1], p[4] * d[DIM - 1]));
-                    s_f[lt][1] = (TermT)fmaf(p[3], d[0], fmaf(p[1], d[1], p[5] * d[DIM - 1]));
-                    s_f[lt][DIM - 1] = (TermT)fmaf(p[4], d[0], fmaf(p[5], d[1], p[2] * d[DIM - 1]));
-                }
-            } else {
-                double p[NC];
-#pragma unroll
-                for (int c = 0; c < NC; c++) p[c] = a.P64[(size_t)c * a.nb + t];
-                if (DIM == 2) {
-                    s_f[lt][0] = (TermT)(p[0] * du[0] + p[2] * du[1]);
-                    s_f[lt][1] = (TermT)(p[2] * du[0] + p[1] * du[1]);
-                } else {
-                    s_f[lt][0] = (TermT)(p[0] * du[0] + p[3] * du[1] + p[4] * du[DIM - 1]);
-                    s_f[lt][1] = (TermT)(p[3] * du[0] + p[1] * du[1] + p[5] * du[DIM - 1]);
-                    s_f[lt][DIM - 1] = (TermT)(p[4] * du[0] + p[5] * du[1] + p[2] * du[DIM - 1]);
-                }
-            }
-            s_ok[lt] = 1;
         }
-        __syncthreads();
-        if (tid < nt) {
-            const int tb = max(s_off[tid], cs), te = min(s_off[tid + 1], ce);
-            for (int t = tb; t < te; t++) {
-                const int lt = t - cs;
-                if (!s_ok[lt]) continue;
+        double uj[R][DIM], xj[R][DIM];
+        double p64[R][F32 ? 1 : NC];
+        float p32[R][F32 ? NC : 1];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (tt[r] >= 0) {
+#pragma unroll
+                for (int c = 0; c < DIM; c++) uj[r][c] = a.u[(size_t)jj[r] * DIM + c];
+                if (a.damage && !a.frozen) {
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) xj[r][c] = a.xp[(size_t)jj[r] * DIM + c];
+                }
                 if (F32) {
 #pragma unroll
-                    for (int c = 0; c < DIM; c++) {
-                        // TwoSum (Knuth): s + f = s' + e exactly
-                        const float f = (float)s_f[lt][c];
-                        const float sn = (float)Ls[c] + f;
-                        const float bp = sn - (float)Ls[c];
-                        const float e = ((float)Ls[c] - (sn - bp)) + (f - bp);
-                        Lc[c] = (TermT)((float)Lc[c] + e);
-                        Ls[c] = (TermT)sn;
-                    }
+                    for (int c = 0; c < NC; c++) p32[r][F32 ? c : 0] = a.P32[(size_t)c * a.nb + tt[r]];
                 } else {
 #pragma unroll
-                    for (int c = 0; c < DIM; c++) L[c] += (double)s_f[lt][c];
+                    for (int c = 0; c < NC; c++) p64[r][F32 ? 0 : c] = a.P64[(size_t)c * a.nb + tt[r]];
                 }
             }
         }
-        __syncthreads();
+        // ---- damage (FP64) and bond force ---------------------------------
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int slot = sl + r * S;          // slot within this round
+            bool live = tt[r] >= 0 && ok[r];
+            double du[DIM];
+#pragma unroll
+            for (int c = 0; c < DIM; c++) du[c] = live ? uj[r][c] - ui[c] : 0.0;
+            if (live && a.damage && !a.frozen) {
+                // stretch of ExplicitPDDOFracElement.cpp:249-266
+                double xs[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; c++) xs[c] = xj[r][c] - xi[c];
+                double rr = xs[0] * xs[0] + xs[1] * xs[1];
+                if (DIM == 3) rr = rr + xs[DIM - 1] * xs[DIM - 1];
+                const double rl = sqrt(rr);
+                if (rl > 0.0) {
+                    const double yx = xs[0] + du[0], yy = xs[1] + du[1];
+                    double yy2 = yx * yx + yy * yy;
+                    if (DIM == 3) { const double yz = xs[DIM - 1] + du[DIM - 1]; yy2 = yy2 + yz * yz; }
+                    const double s = (sqrt(yy2) - rl) / rl;
+                    if (a.tension_only ? (s > a.sc) : (fabs(s) > a.sc)) {
+                        a.health[tt[r]] = 0;
+                        if (a.break_step) a.break_step[tt[r]] = a.step;
+                        broke = 1;
+                        live = false;
+                    }
+                }
+            }
+            if (F32) {
+                // health-gated Navier force in FP32, compensated accumulation
+                float f[DIM];
+                if (live) {
+                    float d[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) d[c] = (float)du[c];
+                    const float *p = p32[r];
+                    if (DIM == 2) {
+                        f[0] = fmaf(p[0], d[0], p[2] * d[1]);
+                        f[1] = fmaf(p[2], d[0], p[1] * d[1]);
+                    } else {
+                        f[0] = fmaf(p[0], d[0], fmaf(p[3], d[1], p[4] * d[DIM - 1]));
+                        f[1] = fmaf(p[3], d[0], fmaf(p[1], d[1], p[5] * d[DIM - 1]));
+                        f[DIM - 1] = fmaf(p[4], d[0], fmaf(p[5], d[1], p[2] * d[DIM - 1]));
+                    }
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) {
+                        float sn, e;
+                        two_sum(Ls[c], f[c], sn, e);
+                        Ls[c] = sn;
+                        Lc[c] += e;
+                    }
+                }
+            } else {
+                // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295)
+                if (live) {
+                    const double *p = p64[r];
+                    if (DIM == 2) {
+                        s_f[ln][slot][0] = p[0] * du[0] + p[2] * du[1];
+                        s_f[ln][slot][1] = p[2] * du[0] + p[1] * du[1];
+                    } else {
+                        s_f[ln][slot][0] = p[0] * du[0] + p[3] * du[1] + p[4] * du[DIM - 1];
+                        s_f[ln][slot][1] = p[3] * du[0] + p[1] * du[1] + p[5] * du[DIM - 1];
+                        s_f[ln][slot][DIM - 1] = p[4] * du[0] + p[5] * du[1] + p[2] * du[DIM - 1];
+                    }
+                }
+                if (nvalid) s_ok[ln][slot] = live ? 1 : 0;
+            }
+        }
+        if (!F32) {
+            __syncthreads();
+            // the reference's family-order sum, one lane per node
+            if (tid < nt) {
+                const int ne = min(MAXS, s_b0[tid + 1] - s_b0[tid] - rb);
+                for (int q = 0; q < ne; q++) {
+                    if (!s_ok[tid][q]) continue;
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) L[c] += s_f[tid][q][c];
+                }
+            }
+            __syncthreads();
+        }
     }
     if (broke && a.anybroke) atomicAdd(a.anybroke, 1);
 
-    unsigned long long mymax = 0;
-    if (tid < nt) {
-        const int i = i0 + tid;
-        if (F32) {
+    int own;                              // lane that finalises node `node`
+    int node;
+    if (F32) {
+        // combine the S lanes of each node: pairwise TwoSum of the partial
+        // sums, compensations added (lanes of one node share a wavefront)
 #pragma unroll
-            for (int c = 0; c < DIM; c++) L[c] = (double)(float)Ls[c] + (double)(float)Lc[c];
+        for (int m = S / 2; m >= 1; m >>= 1) {
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                const float os = __shfl_xor(Ls[c], m);
+                const float oc = __shfl_xor(Lc[c], m);
+                float sn, e;
+                two_sum(Ls[c], os, sn, e);
+                Ls[c] = sn;
+                Lc[c] = Lc[c] + oc + e;
+            }
         }
+        own = (sl == 0) && nvalid;
+        node = ln;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) L[c] = (double)Ls[c] + (double)Lc[c];
+    } else {
+        own = tid < nt;
+        node = tid;
+    }
+
+    unsigned long long mymax = 0;
+    if (own) {
+        const int i = i0 + node;
         const double bb[3] = {a.b0, a.b1, a.b2};
-        const double escale = a.nodeE ? a.nodeE[i] : 1.0;
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
-            const double Lv = a.nodeE ? escale * L[c] : L[c];
-            const double acc = (Lv + bb[c]) / a.rho;
-            const double ui = s_u[tid][c];
+            const double acc = (L[c] + bb[c]) / a.rho;
+            const double uu = s_u[node][c];
             double v;
-            if (MODE == MODE_CD) v = 2.0 * ui - a.uold[(size_t)i * DIM + c] + a.c1 * acc;
-            else if (MODE == MODE_VAR) v = ui + a.c1 * (ui - a.uold[(size_t)i * DIM + c]) + a.c2 * acc;
+            if (MODE == MODE_CD) v = 2.0 * uu - a.uold[(size_t)i * DIM + c] + a.c1 * acc;
+            else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - a.uold[(size_t)i * DIM + c]) + a.c2 * acc;
             else if (MODE == MODE_ACCEL) v = acc;
-            else v = ui - a.c1 * a.uold[(size_t)i * DIM + c] + a.c2 * acc;
+            else v = uu - a.c1 * a.uold[(size_t)i * DIM + c] + a.c2 * acc;
             a.out[(size_t)i * DIM + c] = v;
             if ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i]) {
                 const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(v));
