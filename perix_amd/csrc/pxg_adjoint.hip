@@ -54,15 +54,15 @@ __global__ void k_preset_T(int n, int dim, const double *lam, const unsigned cha
 
 // P(u) product with a symmetric per-bond operator (3 or 6 components)
 template <int DIM>
-__device__ __forceinline__ void pmul(const double *P, long long nb, long long t, const double *v, double *o)
+__device__ __forceinline__ void pmul(const double *P, long long ps, long long t, const double *v, double *o)
 {
     if (DIM == 2) {
-        const double p11 = P[t], p22 = P[nb + t], p12 = P[2 * nb + t];
+        const double p11 = P[t], p22 = P[ps + t], p12 = P[2 * ps + t];
         o[0] = p11 * v[0] + p12 * v[1];
         o[1] = p12 * v[0] + p22 * v[1];
     } else {
-        const double p11 = P[t], p22 = P[nb + t], p33 = P[2 * nb + t];
-        const double p12 = P[3 * nb + t], p13 = P[4 * nb + t], p23 = P[5 * nb + t];
+        const double p11 = P[t], p22 = P[ps + t], p33 = P[2 * ps + t];
+        const double p12 = P[3 * ps + t], p13 = P[4 * ps + t], p23 = P[5 * ps + t];
         o[0] = p11 * v[0] + p12 * v[1] + p13 * v[2];
         o[1] = p12 * v[0] + p22 * v[1] + p23 * v[2];
         o[2] = p13 * v[0] + p23 * v[1] + p33 * v[2];
@@ -71,7 +71,7 @@ __device__ __forceinline__ void pmul(const double *P, long long nb, long long t,
 
 // One reverse step (LAST = the u^{-1} term only).
 template <int DIM, bool LAST>
-__global__ void __launch_bounds__(256) k_adj_step(int n, long long nb, const int *off, const int *nbr,
+__global__ void __launch_bounds__(256) k_adj_step(int n, long long ps, const int *off, const int *nbr,
                                                   const int *rev, const int *bs, int k, const double *Phat,
                                                   const double *E, const double *u, const double *wbar,
                                                   double *lam, double *lamp, double *Ebar, double cf)
@@ -107,11 +107,11 @@ __global__ void __launch_bounds__(256) k_adj_step(int n, long long nb, const int
             for (int c = 0; c < DIM; c++) { a[c] = 0.0; zz[c] = 0.0; }
             if (bs[t] > k) {
                 for (int c = 0; c < DIM; c++) du[c] = u[(size_t)j * DIM + c] - s_u[lo][c];
-                pmul<DIM>(Phat, nb, t, du, a);
+                pmul<DIM>(Phat, ps, t, du, a);
                 if (!LAST) {
                     double ew[DIM], o[DIM];
                     for (int c = 0; c < DIM; c++) ew[c] = E[i] * s_w[lo][c];
-                    pmul<DIM>(Phat, nb, t, ew, o);
+                    pmul<DIM>(Phat, ps, t, ew, o);
                     for (int c = 0; c < DIM; c++) zz[c] -= o[c];
                 }
             }
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_adj_step(int n, long long nb, const int
                 if (r >= 0 && bs[r] > k) {
                     double ew[DIM], o[DIM];
                     for (int c = 0; c < DIM; c++) ew[c] = E[j] * wbar[(size_t)j * DIM + c];
-                    pmul<DIM>(Phat, nb, r, ew, o);
+                    pmul<DIM>(Phat, ps, r, ew, o);
                     for (int c = 0; c < DIM; c++) zz[c] += o[c];
                 }
             }
@@ -324,11 +324,11 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
                                F.preset_mask.p, F.bc.node.p, F.bc.mode.p, F.bc.inv_off.p, F.bc.inv_q.p,
                                tslot.p, wbar.p);
             if (dim == 2)
-                hipLaunchKernelGGL((k_adj_step<2, false>), dim3(gridn2(n, T)), dim3(256), 0, st, n, ctx->nb,
+                hipLaunchKernelGGL((k_adj_step<2, false>), dim3(gridn2(n, T)), dim3(256), 0, st, n, F.ps,
                                    ctx->off.p, ctx->nbr.p, F.rev.p, F.break_step.p, k, F.Phat.p, F.nodeE.p,
                                    ukm1, wbar.p, lam.p, lamp.p, Ebar.p, c);
             else
-                hipLaunchKernelGGL((k_adj_step<3, false>), dim3(gridn2(n, T)), dim3(256), 0, st, n, ctx->nb,
+                hipLaunchKernelGGL((k_adj_step<3, false>), dim3(gridn2(n, T)), dim3(256), 0, st, n, F.ps,
                                    ctx->off.p, ctx->nbr.p, F.rev.p, F.break_step.p, k, F.Phat.p, F.nodeE.p,
                                    ukm1, wbar.p, lam.p, lamp.p, Ebar.p, c);
             PXG_HIP(hipGetLastError());
@@ -336,11 +336,11 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
     }
     // u^{-1} term with h^0 at u^0 (checkpoint 0 holds u^{-1}, u^0)
     if (dim == 2)
-        hipLaunchKernelGGL((k_adj_step<2, true>), dim3(gridn2(n, T)), dim3(256), 0, st, n, ctx->nb, ctx->off.p,
+        hipLaunchKernelGGL((k_adj_step<2, true>), dim3(gridn2(n, T)), dim3(256), 0, st, n, F.ps, ctx->off.p,
                            ctx->nbr.p, F.rev.p, F.break_step.p, 0, F.Phat.p, F.nodeE.p, cks.p + nd, lamp.p,
                            lam.p, lamp.p, Ebar.p, 0.5 * c);
     else
-        hipLaunchKernelGGL((k_adj_step<3, true>), dim3(gridn2(n, T)), dim3(256), 0, st, n, ctx->nb, ctx->off.p,
+        hipLaunchKernelGGL((k_adj_step<3, true>), dim3(gridn2(n, T)), dim3(256), 0, st, n, F.ps, ctx->off.p,
                            ctx->nbr.p, F.rev.p, F.break_step.p, 0, F.Phat.p, F.nodeE.p, cks.p + nd, lamp.p,
                            lam.p, lamp.p, Ebar.p, 0.5 * c);
     PXG_HIP(hipGetLastError());

@@ -6,300 +6,18 @@
 // presets (BCSystem.cpp:114-129, DirichletBC.cpp:107-147, NeumannBC.cpp:88-105).
 //
 // One step = two launches:
-//   k_frac_step   bond-parallel tiles.  A 256-thread block owns T nodes and
-//                 streams their bonds (contiguous in CSR order, so every load
-//                 of neighbour ids, operators and health is coalesced).  Each
-//                 lane evaluates damage and the Navier bond force for its
-//                 bonds and parks the force in LDS; then one lane per node
-//                 sums its bonds in family order (FP64: the reference's exact
-//                 sum; FP32C: TwoSum-compensated FP32 pair), computes the
-//                 acceleration and the central-difference update, and folds
-//                 |u| into the blow-up guard maximum.
+//   k_frac_step   damage, Navier family sum and central-difference update
+//                 (pxg_step.hip), folding |u| into the blow-up guard maximum.
 //   k_bc_preset   the merged BC program (one thread per preset target node)
 //                 followed, in the last block, by the blow-up guard decision.
-// Damage is evaluated in FP64 in both precisions; only the FP32C force sum
-// runs in FP32.
+// This file holds the BC and guard kernel, the per-bond operator build, the
+// health bookkeeping and the host side of the explicit loop.
 #include "pxg_internal.h"
 #include <cfloat>
 #include <climits>
 #include <cmath>
 
 namespace pxg {
-
-enum { MODE_CD = 0, MODE_VAR = 1, MODE_ACCEL = 2, MODE_INIT = 3 };
-
-struct StepArgs {
-    int n;
-    long long nb;
-    const int *off;
-    const int *nbr;
-    const double *xp;
-    const double *P64;
-    const float *P32;
-    unsigned char *health;
-    int *break_step;
-    int *anybroke;
-    int step;
-    const double *u;       // u^n
-    const double *uold;    // u^{n-1} (MODE_CD/VAR) or v0 (MODE_INIT)
-    double *out;           // u^{n+1} / accel / u^{-1}
-    double sc;
-    int tension_only, damage;
-    double b0, b1, b2, rho;
-    double c1, c2;         // MODE_CD: dt2; MODE_VAR: rho_, w; MODE_INIT: dt, half
-    const unsigned char *mask;
-    unsigned long long *gmax;
-    const int *gstate;
-    const double *nodeE;   // optional: per-node scale applied to the family sum
-    int frozen;            // >0: health frozen from break_step (no damage), 0: live
-    int frozen_k;          // bond t is intact iff break_step[t] > frozen_k
-};
-
-template <int DIM> struct TileCfg;
-// T nodes per 256-thread block, S = 256/T lanes per node (S divides 64, so a
-// node's lanes share one wavefront), R bond slots per lane and round.
-template <> struct TileCfg<2> { static constexpr int T = 32; static constexpr int R = 4; };
-template <> struct TileCfg<3> { static constexpr int T = 8; static constexpr int R = 4; };
-
-// TwoSum (Knuth): a + b = s + e exactly, FP32
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
-{
-    s = a + b;
-    const float bp = s - a;
-    e = (a - (s - bp)) + (b - bp);
-}
-
-template <int DIM, bool F32, int MODE>
-__global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
-{
-    constexpr int T = TileCfg<DIM>::T;
-    constexpr int S = 256 / T;
-    constexpr int R = TileCfg<DIM>::R;
-    constexpr int MAXS = S * R;            // slots per round
-    constexpr int NC = (DIM == 3) ? 6 : 3;
-    // FP64: per-bond forces go through LDS so that one lane per node can add
-    // them in family order; FP32C reduces in registers and needs no buffer.
-    constexpr int FS = F32 ? 1 : MAXS;
-    __shared__ double s_f[T][FS][DIM];
-    __shared__ unsigned char s_ok[T][FS];
-    __shared__ double s_u[T][DIM];
-    __shared__ double s_x[T][DIM];
-    __shared__ int s_b0[T + 1];
-    __shared__ int s_maxfam;
-    __shared__ unsigned long long s_max[4];
-
-    if (a.gstate && a.gstate[0]) return;            // blow-up guard tripped earlier
-    const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * T;
-    const int nt = min(T, a.n - i0);
-    if (tid == 0) s_maxfam = 0;
-    if (tid <= nt) s_b0[tid] = a.off[i0 + tid];
-    if (tid < nt) {
-#pragma unroll
-        for (int c = 0; c < DIM; c++) {
-            s_u[tid][c] = a.u[(size_t)(i0 + tid) * DIM + c];
-            s_x[tid][c] = a.xp[(size_t)(i0 + tid) * DIM + c];
-        }
-    }
-    __syncthreads();
-    if (tid < nt) atomicMax(&s_maxfam, s_b0[tid + 1] - s_b0[tid]);
-    __syncthreads();
-    const int ln = tid / S, sl = tid % S;
-    const bool nvalid = ln < nt;
-    const int b0 = nvalid ? s_b0[ln] : 0;
-    const int fam = nvalid ? s_b0[ln + 1] - b0 : 0;
-    const int maxfam = s_maxfam;
-    double ui[DIM], xi[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; c++) { ui[c] = nvalid ? s_u[ln][c] : 0.0; xi[c] = nvalid ? s_x[ln][c] : 0.0; }
-
-    double L[DIM];                     // FP64 sum (lane tid < nt owns node tid)
-    float Ls[DIM], Lc[DIM];            // FP32C compensated pair (every lane)
-#pragma unroll
-    for (int c = 0; c < DIM; c++) { L[c] = 0.0; Ls[c] = 0.0f; Lc[c] = 0.0f; }
-    int broke = 0;
-
-    for (int rb = 0; rb < maxfam; rb += MAXS) {
-        // ---- issue every load of the lane's R bonds first ----------------
-        int tt[R], jj[R];
-        unsigned char ok[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int slot = rb + sl + r * S;
-            const bool v = slot < fam;
-            tt[r] = v ? b0 + slot : -1;
-            ok[r] = 0; jj[r] = 0;
-            if (v) {
-                ok[r] = a.frozen ? (unsigned char)(a.break_step[tt[r]] > a.frozen_k) : a.health[tt[r]];
-                jj[r] = a.nbr[tt[r]] - 1;
-            }
-        }
-        double uj[R][DIM], xj[R][DIM];
-        double p64[R][F32 ? 1 : NC];
-        float p32[R][F32 ? NC : 1];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (tt[r] >= 0) {
-#pragma unroll
-                for (int c = 0; c < DIM; c++) uj[r][c] = a.u[(size_t)jj[r] * DIM + c];
-                if (a.damage && !a.frozen) {
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) xj[r][c] = a.xp[(size_t)jj[r] * DIM + c];
-                }
-                if (F32) {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) p32[r][F32 ? c : 0] = a.P32[(size_t)c * a.nb + tt[r]];
-                } else {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) p64[r][F32 ? 0 : c] = a.P64[(size_t)c * a.nb + tt[r]];
-                }
-            }
-        }
-        // ---- damage (FP64) and bond force ---------------------------------
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int slot = sl + r * S;          // slot within this round
-            bool live = tt[r] >= 0 && ok[r];
-            double du[DIM];
-#pragma unroll
-            for (int c = 0; c < DIM; c++) du[c] = live ? uj[r][c] - ui[c] : 0.0;
-            if (live && a.damage && !a.frozen) {
-                // stretch of ExplicitPDDOFracElement.cpp:249-266
-                double xs[DIM];
-#pragma unroll
-                for (int c = 0; c < DIM; c++) xs[c] = xj[r][c] - xi[c];
-                double rr = xs[0] * xs[0] + xs[1] * xs[1];
-                if (DIM == 3) rr = rr + xs[DIM - 1] * xs[DIM - 1];
-                const double rl = sqrt(rr);
-                if (rl > 0.0) {
-                    const double yx = xs[0] + du[0], yy = xs[1] + du[1];
-                    double yy2 = yx * yx + yy * yy;
-                    if (DIM == 3) { const double yz = xs[DIM - 1] + du[DIM - 1]; yy2 = yy2 + yz * yz; }
-                    const double s = (sqrt(yy2) - rl) / rl;
-                    if (a.tension_only ? (s > a.sc) : (fabs(s) > a.sc)) {
-                        a.health[tt[r]] = 0;
-                        if (a.break_step) a.break_step[tt[r]] = a.step;
-                        broke = 1;
-                        live = false;
-                    }
-                }
-            }
-            if (F32) {
-                // health-gated Navier force in FP32, compensated accumulation
-                float f[DIM];
-                if (live) {
-                    float d[DIM];
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) d[c] = (float)du[c];
-                    const float *p = p32[r];
-                    if (DIM == 2) {
-                        f[0] = fmaf(p[0], d[0], p[2] * d[1]);
-                        f[1] = fmaf(p[2], d[0], p[1] * d[1]);
-                    } else {
-                        f[0] = fmaf(p[0], d[0], fmaf(p[3], d[1], p[4] * d[DIM - 1]));
-                        f[1] = fmaf(p[3], d[0], fmaf(p[1], d[1], p[5] * d[DIM - 1]));
-                        f[DIM - 1] = fmaf(p[4], d[0], fmaf(p[5], d[1], p[2] * d[DIM - 1]));
-                    }
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) {
-                        float sn, e;
-                        two_sum(Ls[c], f[c], sn, e);
-                        Ls[c] = sn;
-                        Lc[c] += e;
-                    }
-                }
-            } else {
-                // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295)
-                if (live) {
-                    const double *p = p64[r];
-                    if (DIM == 2) {
-                        s_f[ln][slot][0] = p[0] * du[0] + p[2] * du[1];
-                        s_f[ln][slot][1] = p[2] * du[0] + p[1] * du[1];
-                    } else {
-                        s_f[ln][slot][0] = p[0] * du[0] + p[3] * du[1] + p[4] * du[DIM - 1];
-                        s_f[ln][slot][1] = p[3] * du[0] + p[1] * du[1] + p[5] * du[DIM - 1];
-                        s_f[ln][slot][DIM - 1] = p[4] * du[0] + p[5] * du[1] + p[2] * du[DIM - 1];
-                    }
-                }
-                if (nvalid) s_ok[ln][slot] = live ? 1 : 0;
-            }
-        }
-        if (!F32) {
-            __syncthreads();
-            // the reference's family-order sum, one lane per node
-            if (tid < nt) {
-                const int ne = min(MAXS, s_b0[tid + 1] - s_b0[tid] - rb);
-                for (int q = 0; q < ne; q++) {
-                    if (!s_ok[tid][q]) continue;
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) L[c] += s_f[tid][q][c];
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (broke && a.anybroke) atomicAdd(a.anybroke, 1);
-
-    int own;                              // lane that finalises node `node`
-    int node;
-    if (F32) {
-        // combine the S lanes of each node: pairwise TwoSum of the partial
-        // sums, compensations added (lanes of one node share a wavefront)
-#pragma unroll
-        for (int m = S / 2; m >= 1; m >>= 1) {
-#pragma unroll
-            for (int c = 0; c < DIM; c++) {
-                const float os = __shfl_xor(Ls[c], m);
-                const float oc = __shfl_xor(Lc[c], m);
-                float sn, e;
-                two_sum(Ls[c], os, sn, e);
-                Ls[c] = sn;
-                Lc[c] = Lc[c] + oc + e;
-            }
-        }
-        own = (sl == 0) && nvalid;
-        node = ln;
-#pragma unroll
-        for (int c = 0; c < DIM; c++) L[c] = (double)Ls[c] + (double)Lc[c];
-    } else {
-        own = tid < nt;
-        node = tid;
-    }
-
-    unsigned long long mymax = 0;
-    if (own) {
-        const int i = i0 + node;
-        const double bb[3] = {a.b0, a.b1, a.b2};
-#pragma unroll
-        for (int c = 0; c < DIM; c++) {
-            const double acc = (L[c] + bb[c]) / a.rho;
-            const double uu = s_u[node][c];
-            double v;
-            if (MODE == MODE_CD) v = 2.0 * uu - a.uold[(size_t)i * DIM + c] + a.c1 * acc;
-            else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - a.uold[(size_t)i * DIM + c]) + a.c2 * acc;
-            else if (MODE == MODE_ACCEL) v = acc;
-            else v = uu - a.c1 * a.uold[(size_t)i * DIM + c] + a.c2 * acc;
-            a.out[(size_t)i * DIM + c] = v;
-            if ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i]) {
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(v));
-                mymax = bits > mymax ? bits : mymax;
-            }
-        }
-    }
-    if (MODE == MODE_CD || MODE == MODE_VAR) {
-        for (int m = 32; m >= 1; m >>= 1) {
-            const unsigned long long o = __shfl_xor(mymax, m);
-            mymax = o > mymax ? o : mymax;
-        }
-        if ((tid & 63) == 0) s_max[tid >> 6] = mymax;
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long m = s_max[0];
-            for (int w = 1; w < 4; w++) m = s_max[w] > m ? s_max[w] : m;
-            if (m) atomicMax(&a.gmax[0], m);
-        }
-    }
-}
 
 // BC presets on the freshly updated u^{n+1}, then the blow-up guard
 // (TimeStepping.cpp:550-573) decided by the last block of the launch.
@@ -362,10 +80,11 @@ __global__ void __launch_bounds__(256) k_bc_preset(int count, int dim, const int
 }
 
 // per-bond Navier operators from the g-weights (full slot order of
-// PDOperators.cpp:51-75); component-major layout P[c*nb + t].
+// PDOperators.cpp:51-75); component-major layout P[c*ps + t], ps = nb
+// rounded up to a multiple of 64 so that every component starts aligned.
 __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double *ops,
                            const double *nodeE, double C11, double C12, double C66, double nu,
-                           int plane_stress, long long nb, double *P64, float *P32, double *Phat)
+                           int plane_stress, long long ps, double *P64, float *P32, double *Phat)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -420,9 +139,9 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
             nc = 3;
         }
         for (int c = 0; c < nc; c++) {
-            if (P64) P64[(size_t)c * nb + t] = p[c];
-            if (P32) P32[(size_t)c * nb + t] = (float)p[c];
-            if (Phat) Phat[(size_t)c * nb + t] = q[c];
+            if (P64) P64[(size_t)c * ps + t] = p[c];
+            if (P32) P32[(size_t)c * ps + t] = (float)p[c];
+            if (Phat) Phat[(size_t)c * ps + t] = q[c];
         }
     }
 }
@@ -469,45 +188,12 @@ static unsigned gridn(long long n, int b) { return (unsigned)((n + b - 1) / b); 
 
 namespace pxg {
 
-// launches one k_frac_step for the context's dimension / precision / mode
-int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
-                double c1, double c2, int step, bool guard, int frozen_k)
-{
-    FracState &F = ctx->frac;
-    StepArgs a;
-    a.n = ctx->n; a.nb = ctx->nb; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
-    a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
-    a.anybroke = F.anybroke.p; a.step = step; a.u = u; a.uold = uold; a.out = out;
-    a.sc = F.sc; a.tension_only = F.tension_only; a.damage = (F.damage_on && F.sc > 0.0) ? 1 : 0;
-    a.b0 = F.b[0]; a.b1 = F.b[1]; a.b2 = F.b[2]; a.rho = F.rho; a.c1 = c1; a.c2 = c2;
-    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = guard ? F.gstate.p : nullptr;
-    a.nodeE = nullptr;
-    a.frozen = (frozen_k > INT_MIN) ? 1 : 0; a.frozen_k = frozen_k;
-    const bool f32 = (F.precision == PXG_FP32C) && !a.frozen;
-    hipStream_t st = ctx->stream;
-#define PXG_LAUNCH(D, FF, M)                                                                        \
-    hipLaunchKernelGGL((k_frac_step<D, FF, M>), dim3(gridn(ctx->n, TileCfg<D>::T)), dim3(256), 0, st, a)
-#define PXG_MODES(D, FF)                                                                            \
-    switch (mode) {                                                                                 \
-    case MODE_CD: PXG_LAUNCH(D, FF, MODE_CD); break;                                                \
-    case MODE_VAR: PXG_LAUNCH(D, FF, MODE_VAR); break;                                              \
-    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL); break;                                          \
-    default: PXG_LAUNCH(D, FF, MODE_INIT); break;                                                   \
-    }
-    if (ctx->dim == 2) { if (f32) { PXG_MODES(2, true) } else { PXG_MODES(2, false) } }
-    else { if (f32) { PXG_MODES(3, true) } else { PXG_MODES(3, false) } }
-#undef PXG_MODES
-#undef PXG_LAUNCH
-    PXG_HIP(hipGetLastError());
-    return PXG_OK;
-}
-
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev)
 {
     FracState &F = ctx->frac;
     hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
                        ctx->nop, ctx->off.p, ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress,
-                       ctx->nb, F.P64.p, (float *)nullptr, (double *)nullptr);
+                       F.ps, F.P64.p, (float *)nullptr, (double *)nullptr);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
@@ -544,20 +230,21 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     F.has_nodeE = node_E ? 1 : 0;
     const int nc = (ctx->dim == 3) ? 6 : 3;
     const long long nb = ctx->nb;
+    F.ps = (nb + 63) / 64 * 64;
     if (node_E) {
         PXG_HIP(F.nodeE.alloc(ctx->n));
         PXG_HIP(hipMemcpyAsync(F.nodeE.p, node_E, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
-        PXG_HIP(F.Phat.alloc((size_t)nb * nc));
+        PXG_HIP(F.Phat.alloc((size_t)F.ps * nc));
         PXG_HIP(F.rev.alloc(nb));
         hipLaunchKernelGGL(k_rev, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, ctx->nbr.p, F.rev.p);
     } else {
         F.nodeE.release(); F.Phat.release(); F.rev.release();
     }
-    if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)nb * nc)); F.P32.release(); }
-    else { PXG_HIP(F.P32.alloc((size_t)nb * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)nb * nc)); }
+    if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)F.ps * nc)); F.P32.release(); }
+    else { PXG_HIP(F.P32.alloc((size_t)F.ps * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)F.ps * nc)); }
     hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->dim, ctx->nop,
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
-                       plane_stress, nb, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
+                       plane_stress, F.ps, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
     PXG_HIP(F.health.alloc(nb));
     PXG_HIP(F.break_step.alloc(nb));

@@ -35,17 +35,26 @@ void set_error(const std::string &msg);
         if (!(cond)) { ::pxg::set_error(msg); return (code); }                        \
     } while (0)
 
-// device buffer with size bookkeeping
+// device buffer with size bookkeeping.  Every allocation carries at least 64
+// zeroed elements of slack past `count` (rounded to a multiple of 64), so
+// kernels may read aligned vectors that run past the last element.
 template <typename T>
 struct dbuf {
     T *p = nullptr;
     size_t n = 0;
+    static constexpr size_t SLACK = 64;
+    static size_t padded(size_t count) { return (count + 2 * SLACK - 1) / SLACK * SLACK; }
     hipError_t alloc(size_t count) {
         if (count == n && p) return hipSuccess;
         release();
         n = count;
         if (count == 0) return hipSuccess;
-        return hipMalloc(&p, sizeof(T) * count);
+        hipError_t e = hipMalloc(&p, sizeof(T) * padded(count));
+        if (e != hipSuccess) { p = nullptr; n = 0; return e; }
+        e = hipMemset(p, 0, sizeof(T) * padded(count));
+        // the context streams are non-blocking: finish the fill before any
+        // of them can touch the buffer
+        return e != hipSuccess ? e : hipDeviceSynchronize();
     }
     void release() {
         if (p) (void)hipFree(p);
@@ -74,7 +83,9 @@ struct FracState {
     double C11 = 0, C12 = 0, C66 = 0, rho = 1, b[3] = {0, 0, 0}, sc = 0;
     int tension_only = 1, damage_on = 1;
     // per-bond force operators P (CSR bond order): 3 (2D) or 6 (3D) components
-    dbuf<double> P64;        // nb*ncomp, component-major (SoA)
+    // component-major: component c of bond t at [c*ps + t]
+    long long ps = 0;        // component stride: nb rounded up to a multiple of 64
+    dbuf<double> P64;
     dbuf<float> P32;
     dbuf<unsigned char> health;   // nb
     dbuf<int> break_step;         // nb (recorded break step, INT_MAX intact)
@@ -106,6 +117,9 @@ struct FracState {
     int nev = 0;
 };
 
+// launch_step modes: u^{n+1} by the fixed-dt central difference, by the
+// variable-dt form of the last step, the acceleration alone, or u^{-1}
+enum { MODE_CD = 0, MODE_VAR = 1, MODE_ACCEL = 2, MODE_INIT = 3 };
 int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
                 double c1, double c2, int step, bool guard, int frozen_k);
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
