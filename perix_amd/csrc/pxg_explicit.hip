@@ -5,13 +5,13 @@
 // TimeStepping::solveExplicitDynamics (TimeStepping.cpp:417-599) and the BC
 // presets (BCSystem.cpp:114-129, DirichletBC.cpp:107-147, NeumannBC.cpp:88-105).
 //
-// One step = two launches:
-//   k_frac_step   damage, Navier family sum and central-difference update
-//                 (pxg_step.hip), folding |u| into the blow-up guard maximum.
-//   k_bc_preset   the merged BC program (one thread per preset target node)
-//                 followed, in the last block, by the blow-up guard decision.
-// This file holds the BC and guard kernel, the per-bond operator build, the
-// health bookkeeping and the host side of the explicit loop.
+// pxg_frac_run takes one launch per step: k_frac_step (pxg_step.hip) with
+// damage, the Navier family sum, the central-difference update, the merged
+// BC program and the blow-up guard folded in.  The adjoint's frozen-health
+// recompute runs the step without presets followed by k_bc_preset (one
+// thread per preset target node, guard decided by the launch's last block).
+// This file holds k_bc_preset, the per-bond operator build, the health
+// bookkeeping and the host side of the explicit loop.
 #include "pxg_internal.h"
 #include <cfloat>
 #include <climits>
@@ -248,18 +248,30 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_HIP(hipGetLastError());
     PXG_HIP(F.health.alloc(nb));
     PXG_HIP(F.break_step.alloc(nb));
+    {
+        const int rc = build_step_gather(ctx);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_health_init, dim3(gridn(nb, 256)), dim3(256), 0, st, nb, ctx->seed.p, F.health.p,
                        F.break_step.p);
     for (int k = 0; k < 2; k++) PXG_HIP(F.u[k].alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.accel.alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.preset_mask.alloc(ctx->n));
     PXG_HIP(hipMemsetAsync(F.preset_mask.p, 0, ctx->n, st));
-    PXG_HIP(F.gmax.alloc(4));
+    PXG_HIP(F.gmax.alloc(8));
     PXG_HIP(F.ghist.alloc(50));
+    PXG_HIP(F.ghist64.alloc(64));
+    // empty BC program until pxg_frac_set_bcs
+    PXG_HIP(F.bc.prog_of.alloc(ctx->n));
+    PXG_HIP(hipMemsetAsync(F.bc.prog_of.p, 0xFF, sizeof(int) * ctx->n, st));
+    PXG_HIP(F.bc.inv_off.alloc((size_t)ctx->n + 1));
+    PXG_HIP(hipMemsetAsync(F.bc.inv_off.p, 0, sizeof(int) * ((size_t)ctx->n + 1), st));
+    PXG_HIP(F.bc.inv_q.alloc(1));
     PXG_HIP(F.gstate.alloc(4));
     PXG_HIP(F.anybroke.alloc(1));
-    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 8, st));
     PXG_HIP(hipMemsetAsync(F.ghist.p, 0, sizeof(double) * 50, st));
+    PXG_HIP(hipMemsetAsync(F.ghist64.p, 0, sizeof(double) * 64, st));
     PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
     PXG_HIP(hipMemsetAsync(F.anybroke.p, 0, sizeof(int), st));
     F.bc.count = 0;
@@ -347,6 +359,8 @@ extern "C" int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
         PXG_HIP(hipMemcpyAsync(F.bc.vel.p, tvel.data(), sizeof(double) * 3 * cnt, hipMemcpyHostToDevice, st));
     }
     PXG_HIP(hipMemcpyAsync(F.preset_mask.p, mask.data(), n, hipMemcpyHostToDevice, st));
+    PXG_HIP(F.bc.prog_of.alloc(n));
+    PXG_HIP(hipMemcpyAsync(F.bc.prog_of.p, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     // inverse of the source map (adjoint of the preset), entries in program order
     std::vector<int> ioff(n + 1, 0), iq;
     for (int s2 = 0; s2 < cnt; s2++) if (tsrc[s2] >= 0) ioff[tsrc[s2] + 1]++;
@@ -417,8 +431,9 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     else PXG_HIP(hipMemsetAsync(F.accel.p, 0, sizeof(double) * nd, st));
-    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 8, st));
     PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
+    PXG_HIP(hipMemsetAsync(F.ghist64.p, 0, sizeof(double) * 64, st));
     // t = 0 rate evaluation (damage included), then u^{-1} = u0 - dt v0 + dt^2/2 a0
     const double half = 0.5 * dt * dt;
     int rc = launch_step(ctx, MODE_INIT, F.u[0].p, F.accel.p, F.u[1].p, dt, half, 0, false, INT_MIN);
@@ -452,17 +467,17 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
             }
             PXG_HIP(hipEventRecord(F.ev[2 * F.nev], ctx->stream));
         }
+        // one launch per step: damage, forces, update, BC presets at tNew and
+        // the guard decision on the previous step
         if (dtA == F.dt_prev) {
-            rc = launch_step(ctx, MODE_CD, ucur, uold, uold, dtA * dtA, 0.0, step, true, INT_MIN);
+            rc = launch_step_fused(ctx, MODE_CD, ucur, uold, uold, dtA * dtA, 0.0, step, tNew);
         } else {
             const double rr = dtA / F.dt_prev;
             const double w = 0.5 * dtA * (dtA + F.dt_prev);
-            rc = launch_step(ctx, MODE_VAR, ucur, uold, uold, rr, w, step, true, INT_MIN);
+            rc = launch_step_fused(ctx, MODE_VAR, ucur, uold, uold, rr, w, step, tNew);
         }
         if (rc) return rc;
         if (F.prof) { PXG_HIP(hipEventRecord(F.ev[2 * F.nev + 1], ctx->stream)); F.nev++; }
-        rc = launch_preset(ctx, uold, tNew, step);
-        if (rc) return rc;
         F.cur ^= 1;
         F.t = tNew; F.dt_prev = dtA; F.step = step;
         done++;
@@ -472,6 +487,10 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
             PXG_HIP(hipStreamSynchronize(ctx->stream));
             if (gs[0]) break;
         }
+    }
+    if (done > 0) {
+        rc = launch_guard_final(ctx, F.step);
+        if (rc) return rc;
     }
     int gs[3];
     PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));

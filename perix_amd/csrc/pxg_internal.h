@@ -72,6 +72,7 @@ struct BcNodeProgram {
     dbuf<double> vel;        // per node*3+comp: dg/dt
     int count = 0;
     bool merged = true;      // single-stage (parallel) program
+    dbuf<int> prog_of;       // N: program entry of a target node, -1 otherwise
     dbuf<int> inv_off;       // N+1: program entries whose mirror source is node m
     dbuf<int> inv_q;
 };
@@ -87,6 +88,15 @@ struct FracState {
     long long ps = 0;        // component stride: nb rounded up to a multiple of 64
     dbuf<double> P64;
     dbuf<float> P32;
+    // LDS neighbour staging of the step kernel: the step block b (npb
+    // consecutive nodes) copies the neighbour rows listed in
+    // runs[roff[b] .. roff[b+1]) -- {first node, count, first LDS slot, 0} --
+    // into LDS, and bond t reads its neighbour from LDS slot gslot[t]
+    // (GSLOT_GLOBAL: read it from global memory via nbr[t]).
+    int npb = 0, gcap = 0;        // nodes per step block, LDS slots per block
+    dbuf<int> roff;
+    dbuf<int4> runs;
+    dbuf<unsigned short> gslot;
     dbuf<unsigned char> health;   // nb
     dbuf<int> break_step;         // nb (recorded break step, INT_MAX intact)
     dbuf<double> u[2];            // ping-pong displacement buffers (N*dim)
@@ -94,8 +104,11 @@ struct FracState {
     dbuf<unsigned char> preset_mask;   // N: node is a preset target
     BcNodeProgram bc;
     // blow-up guard state
-    dbuf<unsigned long long> gmax;     // [0] CD part, [1] preset part, [2] block counter
-    dbuf<double> ghist;                // 50-entry ring of max|u|
+    // separate-preset path (k_bc_preset): [0] CD part, [1] preset part,
+    // [2] block counter; fused path: max|u^{s}| of step s in [4 + s % 3]
+    dbuf<unsigned long long> gmax;
+    dbuf<double> ghist;                // 50-entry ring of max|u| (separate-preset path)
+    dbuf<double> ghist64;              // max|u^{s}| at [s % 64] (fused path)
     dbuf<int> gstate;                  // [0] aborted flag, [1] abort step, [2] steps done
     // node-E (optional): per-node Young's modulus for the adjoint configuration
     dbuf<double> nodeE;
@@ -122,6 +135,14 @@ struct FracState {
 enum { MODE_CD = 0, MODE_VAR = 1, MODE_ACCEL = 2, MODE_INIT = 3 };
 int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
                 double c1, double c2, int step, bool guard, int frozen_k);
+// MODE_CD / MODE_VAR step with the BC presets at time tpre and the blow-up
+// guard folded in (the guard decision of step `step`-1 is taken at the start
+// of the launch; launch_guard_final decides the last step of a run)
+int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
+                      double c1, double c2, int step, double tpre);
+int launch_guard_final(pxg_ctx *ctx, int step);
+int build_step_gather(pxg_ctx *ctx);
+constexpr unsigned short GSLOT_GLOBAL = 0xFFFF;
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
 

@@ -5,15 +5,22 @@
 // central-difference update of TimeStepping::solveExplicitDynamics
 // (TimeStepping.cpp:417-599).
 //
-// Work layout.  A wavefront owns 64/S consecutive nodes and gives each node S
-// adjacent lanes.  A node's bonds [off[i], off[i+1]) are contiguous; the lanes
-// walk them in groups of G consecutive bonds aligned to G, so one G-wide
-// vector load fetches a group's neighbour ids, its health bytes and each
-// operator component (all bond arrays are allocated with zeroed slack and the
-// operator components are strided by a multiple of 64, so every such load is
-// aligned and in bounds).  A group that straddles two families is loaded by
-// both nodes and masked by each.  Wavefronts never wait for each other: the
-// kernel has no block-level barrier.
+// Work layout.  A 256-thread block owns 4*64/S consecutive nodes; each
+// wavefront owns 64/S of them and gives each node S adjacent lanes.
+//  * Neighbour rows are staged in LDS first: the block's distinct neighbours
+//    form a few runs of consecutive node ids (rows of the grid), each copied
+//    from u and x with coalesced loads; a bond then reads its neighbour from
+//    the LDS slot recorded for it at setup.  Gathering straight from global
+//    memory costs one cache-line lookup per lane and load, which saturates
+//    the texture units long before HBM; the staged copy reads each row once
+//    per block (about 3.5x fewer rows than bonds).
+//  * A node's bonds [off[i], off[i+1]) are contiguous; the lanes walk them in
+//    groups of G consecutive bonds aligned to G, so one G-wide vector load
+//    fetches a group's LDS slots, its health bytes and each operator
+//    component (all bond arrays are allocated with zeroed slack and the
+//    operator components are strided by a multiple of 64, so every such load
+//    is aligned and in bounds).  A group that straddles two families is
+//    loaded by both nodes and masked by each.
 //
 // Damage.  The reference breaks bond (i,j) when s = (|y| - |xi|)/|xi| > sc
 // (tension-only) or |s| > sc.  The kernel first decides from
@@ -31,9 +38,11 @@
 // pair); the S lanes of a node combine their pairs with a lane butterfly
 // (DPP within 16 lanes), and the FP64 update consumes sum + compensation.
 #include "pxg_internal.h"
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <thread>
 
 namespace pxg {
 
@@ -43,6 +52,10 @@ struct StepArgs {
     long long ps;              // component stride of the operator arrays
     const int *off;
     const int *nbr;
+    const int *roff;           // neighbour-row runs per block (LDS staging)
+    const int4 *runs;
+    const unsigned short *gslot;
+    int gcap;                  // LDS slots per block
     const double *xp;
     const double *P64;
     const float *P32;
@@ -59,17 +72,52 @@ struct StepArgs {
     double c1, c2;             // MODE_CD: dt2; MODE_VAR: rho_, w; MODE_INIT: dt, half
     const unsigned char *mask;
     unsigned long long *gmax;
-    const int *gstate;
+    int *gstate;
     int frozen;                // >0: health frozen from break_step (no damage)
     int frozen_k;              // bond t is intact iff break_step[t] > frozen_k
+    // fused BC presets (merged program, see BcNodeProgram) and guard history
+    const int *prog_of, *bnode, *bmode, *inv_off, *inv_q;
+    const double *bval0, *bvel;
+    double tpre;               // time of the presets (t^{n+1})
+    double *ghist64;
 };
 
+// blow-up guard of TimeStepping.cpp:550-573 for step s (fused path): trips
+// on a non-finite max|u^s| or on growth beyond 1e3 * max(1, max|u| of step
+// s-50); max|u^s| was accumulated in gmax[4 + s % 3] by the launch of step s.
+__device__ __forceinline__ unsigned long long guard_max(int s, unsigned long long *gmax)
+{
+    return __hip_atomic_load(&gmax[4 + s % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double guard_past(int s, const double *h64)
+{
+    return (s > 50) ? h64[(s - 50) & 63] : -1.0;
+}
+__device__ __forceinline__ bool guard_trips(unsigned long long m, double past, double &umax)
+{
+    const bool finite = m <= (unsigned long long)__double_as_longlong(DBL_MAX);
+    umax = __longlong_as_double((long long)m);
+    const double cap = (past >= 0.0) ? 1.0e3 * fmax(1.0, past) : INFINITY;
+    return !finite || umax > cap;
+}
+
+// records the decision on step s (one thread)
+__device__ __forceinline__ void guard_record(int s, bool trip, double umax, double *h64, int *gstate)
+{
+    if (trip) { gstate[0] = 1; gstate[1] = s; }
+    else h64[s & 63] = umax;
+    gstate[2] = s;
+}
+
 template <int DIM, bool F32> struct StepCfg;
-// S lanes per node (S divides 64), G bonds per lane group.
-template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; };
-template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; };
-template <> struct StepCfg<3, true> { static constexpr int S = 16, G = 4; };
-template <> struct StepCfg<3, false> { static constexpr int S = 16, G = 4; };
+// S lanes per node (S divides 64), G bonds per lane group, STAGE: neighbour
+// rows staged in LDS.  The 2D plate is latency-bound (its working set lives
+// in the Infinity Cache) and runs faster gathering straight from global
+// memory; the 3D block saturates the texture units without staging.
+template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false; };
+template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false; };
+template <> struct StepCfg<3, true> { static constexpr int S = 16, G = 4; static constexpr bool STAGE = true; };
+template <> struct StepCfg<3, false> { static constexpr int S = 16, G = 2; static constexpr bool STAGE = true; };
 
 template <int B> struct VecT;
 template <> struct VecT<2> { using T = unsigned short; };
@@ -161,27 +209,80 @@ __device__ __forceinline__ double pick(const double (&v)[DIM], int c)
     return (DIM == 3 && c == 2) ? v[DIM - 1] : (c == 1 ? v[1] : v[0]);
 }
 
-template <int DIM, bool F32, int MODE>
+template <int DIM, bool F32, int MODE, bool FUSED>
 __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 {
     constexpr int S = StepCfg<DIM, F32>::S;
     constexpr int G = StepCfg<DIM, F32>::G;
+    constexpr bool STAGE = StepCfg<DIM, F32>::STAGE;
     constexpr int NPW = 64 / S;               // nodes per wavefront
     constexpr int W = G * S;                  // bonds per node and round
     constexpr int NC = (DIM == 3) ? 6 : 3;
     constexpr int ROW = W * DIM + 1;          // LDS row per node (odd: no bank aliasing)
-    __shared__ double s_f[F32 ? 1 : 4 * NPW * ROW];
+    constexpr int FROWS = F32 ? 0 : 4 * NPW * ROW;
+    // dynamic LDS: [FP64 force rows][u of gcap slots][x of gcap slots]
+    extern __shared__ double s_dyn[];
+    double *s_f = s_dyn;
+    double *s_u = s_dyn + FROWS;
+    double *s_x = s_u + (size_t)a.gcap * DIM;
 
-    if (a.gstate && a.gstate[0]) return;      // blow-up guard tripped earlier
+    if constexpr (!FUSED) {
+        if (a.gstate && a.gstate[0]) return;  // blow-up guard tripped earlier
+    }
     const int blk = xcd_block(blockIdx.x, a.nblk);
     if (blk < 0) return;
+    // fused guard: the loads are issued here and consumed only after the
+    // LDS staging, before the first global write
+    int g_tripped = 0;
+    unsigned long long g_max = 0;
+    double g_past = -1.0;
+    if constexpr (FUSED) {
+        g_tripped = a.gstate[0];
+        if (a.step >= 2) { g_max = guard_max(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool live_dmg = a.damage && !a.frozen;
+    if constexpr (STAGE) {
+        // stage the block's neighbour rows (runs of consecutive node ids) in
+        // LDS: each run is a contiguous span of u and x, copied coalesced
+        const int r1 = a.roff[blk + 1];
+        for (int r = a.roff[blk] + wv; r < r1; r += 4) {
+            const int4 R = a.runs[r];                 // first node, count, first slot
+            const int len = R.y * DIM;
+            const double *gu = a.u + (size_t)R.x * DIM;
+            double *lu = s_u + R.z * DIM;
+            for (int e = lane; e < len; e += 64) lu[e] = gu[e];
+            if (live_dmg) {
+                const double *gx = a.xp + (size_t)R.x * DIM;
+                double *lx = s_x + R.z * DIM;
+                for (int e = lane; e < len; e += 64) lx[e] = gx[e];
+            }
+        }
+        __syncthreads();
+    }
     const int ln = lane / S, sl = lane % S;
     const int i = (blk * 4 + wv) * NPW + ln;
     const bool nvalid = i < a.n;
     int b0 = 0, b1 = 0;
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
-    const bool live_dmg = a.damage && !a.frozen;
+    // BC program of this node (fused path): own entry, entries it mirrors
+    int bq = -1, bp0 = 0, bp1 = 0;
+    if constexpr (FUSED) {
+        if (nvalid && sl < DIM) { bq = a.prog_of[i]; bp0 = a.inv_off[i]; bp1 = a.inv_off[i + 1]; }
+        if (g_tripped) return;                // the guard tripped earlier
+        if (a.step >= 2) {
+            // every block takes the guard decision on the previous step from
+            // the same data; block 0 records it and clears the maximum slot
+            // of step+1 (last used by step-2, already decided)
+            double umax;
+            const bool trip = guard_trips(g_max, g_past, umax);
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                guard_record(a.step - 1, trip, umax, a.ghist64, a.gstate);
+                a.gmax[4 + (a.step + 1) % 3] = 0;
+            }
+            if (trip) return;
+        }
+    }
     double ui[DIM], xi[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
@@ -203,8 +304,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     using F64G = double[F32 ? 1 : G][DIM];
     using F32G = float[F32 ? G : 1][DIM];
     auto group = [&](unsigned t0, F64G &f64, F32G &f32) {
-        int jv[G];
-        vload(a.nbr + t0, jv);
+        unsigned short sv[STAGE ? G : 1];
+        int jv[STAGE ? 1 : G];
+        if constexpr (STAGE) vload(a.gslot + t0, sv);
+        else vload(a.nbr + t0, jv);
         bool ok[G];
         if (a.frozen) {
             int bs[G];
@@ -232,13 +335,40 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 #pragma unroll
             for (int c = 0; c < DIM; c++) du[c] = 0.0;
             if (live) {
-                const unsigned jo = (unsigned)(jv[k] - 1) * (unsigned)(DIM * sizeof(double));
-                const double *uj = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.u) + jo);
+                // neighbour row: the block's LDS copy, or global memory for
+                // a block whose neighbourhood did not fit the LDS slots
+                double uj[DIM], xj[DIM];
+                if constexpr (STAGE) {
+                    if (sv[k] != GSLOT_GLOBAL) {
+                        const int q = sv[k] * DIM;
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) {
+                            uj[c] = s_u[q + c];
+                            xj[c] = live_dmg ? s_x[q + c] : 0.0;
+                        }
+                    } else {
+                        const size_t q = (size_t)(a.nbr[t] - 1) * DIM;
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) {
+                            uj[c] = a.u[q + c];
+                            xj[c] = live_dmg ? a.xp[q + c] : 0.0;
+                        }
+                    }
+                } else {
+                    // 32-bit byte offset from the array base
+                    const unsigned jo = (unsigned)(jv[k] - 1) * (unsigned)(DIM * sizeof(double));
+                    const double *gu = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.u) + jo);
+                    const double *gx = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.xp) + jo);
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) {
+                        uj[c] = gu[c];
+                        xj[c] = live_dmg ? gx[c] : 0.0;
+                    }
+                }
 #pragma unroll
                 for (int c = 0; c < DIM; c++) du[c] = uj[c] - ui[c];
                 if (live_dmg) {
                     // stretch of ExplicitPDDOFracElement.cpp:249-266
-                    const double *xj = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.xp) + jo);
                     double xs[DIM];
 #pragma unroll
                     for (int c = 0; c < DIM; c++) xs[c] = xj[c] - xi[c];
@@ -361,72 +491,231 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 
     // central-difference update, one lane per component
     unsigned long long mymax = 0;
+    auto fold = [&](double w) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(w));
+        mymax = bits > mymax ? bits : mymax;
+    };
     if (nvalid && sl < DIM) {
         const int c = sl;
-        const double bb = (c == 0) ? a.b0 : (c == 1 ? a.b1 : a.b2);
-        const double acc = (Lown + bb) / a.rho;
-        const double uu = pick(ui, c);
         const size_t idx = (size_t)i * DIM + c;
-        double v;
-        if (MODE == MODE_CD) v = 2.0 * uu - a.uold[idx] + a.c1 * acc;
-        else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - a.uold[idx]) + a.c2 * acc;
-        else if (MODE == MODE_ACCEL) v = acc;
-        else v = uu - a.c1 * a.uold[idx] + a.c2 * acc;
-        a.out[idx] = v;
-        if ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i])
-            mymax = (unsigned long long)__double_as_longlong(fabs(v));
+        // preset mode of this component (fused path): 0 none, 1 copy and
+        // 2 reflection (written by the mirror source's lane), 3 direct
+        const int q = bq;
+        int m = 0;
+        if constexpr (FUSED) {
+            if (q >= 0) m = a.bmode[q * 3 + c];
+        }
+        double v = 0.0;
+        if (m == 0) {
+            const double bb = (c == 0) ? a.b0 : (c == 1 ? a.b1 : a.b2);
+            const double acc = (Lown + bb) / a.rho;
+            const double uu = pick(ui, c);
+            if (MODE == MODE_CD) v = 2.0 * uu - a.uold[idx] + a.c1 * acc;
+            else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - a.uold[idx]) + a.c2 * acc;
+            else if (MODE == MODE_ACCEL) v = acc;
+            else v = uu - a.c1 * a.uold[idx] + a.c2 * acc;
+            a.out[idx] = v;
+            if (FUSED || ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i])) fold(v);
+        } else if (m == 3) {
+            // direct Dirichlet value (DirichletBC.cpp:107-147)
+            v = a.bval0[q * 3 + c] + a.bvel[q * 3 + c] * a.tpre;
+            a.out[idx] = v;
+            fold(v);
+        }
+        if constexpr (FUSED) {
+            // this node as the mirror source of preset targets (never itself
+            // a target: the program has no chains), BCSystem.cpp:114-129
+            for (int p = bp0; p < bp1; p++) {
+                const int qq = a.inv_q[p];
+                const int mm = a.bmode[qq * 3 + c];
+                if (mm != 1 && mm != 2) continue;
+                double w = v;
+                if (mm == 2) {
+                    const double g = a.bval0[qq * 3 + c] + a.bvel[qq * 3 + c] * a.tpre;
+                    w = 2.0 * g - v;
+                }
+                a.out[(size_t)a.bnode[qq] * DIM + c] = w;
+                fold(w);
+            }
+        }
     }
     if (MODE == MODE_CD || MODE == MODE_VAR) {
-        // blow-up guard maximum: |v| bit patterns order like the values
+        // blow-up guard maximum: |u| bit patterns order like the values
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) {
             const unsigned long long o = __shfl_xor(mymax, m);
             mymax = o > mymax ? o : mymax;
         }
+        unsigned long long *slot = FUSED ? &a.gmax[4 + a.step % 3] : &a.gmax[0];
         if (lane == 0 && mymax) {
-            const unsigned long long cur = __hip_atomic_load(&a.gmax[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (mymax > cur) atomicMax(&a.gmax[0], mymax);
+            const unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (mymax > cur) atomicMax(slot, mymax);
         }
     }
 }
 
+// guard decision on the last step of a fused run
+__global__ void k_guard_final(int s, unsigned long long *gmax, double *h64, int *gstate)
+{
+    if (threadIdx.x != 0 || gstate[0] || s < 1) return;
+    double umax;
+    const bool trip = guard_trips(guard_max(s, gmax), guard_past(s, h64), umax);
+    guard_record(s, trip, umax, h64, gstate);
+}
+
 static unsigned gridn(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// LDS doubles of the FP64 family-sum rows (zero for FP32C)
+template <int D, bool FF>
+constexpr size_t step_force_rows()
+{
+    return FF ? 0 : (size_t)4 * (64 / StepCfg<D, FF>::S) * (StepCfg<D, FF>::G * StepCfg<D, FF>::S * D + 1);
+}
+
+// nodes per step block; both precisions tile nodes alike, so one LDS
+// neighbour table serves either
+template <int D>
+constexpr int step_npb()
+{
+    static_assert(StepCfg<D, true>::S == StepCfg<D, false>::S, "precisions must share the node tiling");
+    return 4 * (64 / StepCfg<D, true>::S);
+}
+
+
 // launches one k_frac_step for the context's dimension / precision / mode
-int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
-                double c1, double c2, int step, bool guard, int frozen_k)
+static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out, double c1,
+                      double c2, int step, bool guard, int frozen_k, bool fused, double tpre)
 {
     FracState &F = ctx->frac;
     StepArgs a;
     a.n = ctx->n; a.ps = F.ps; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.roff = F.roff.p; a.runs = F.runs.p; a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
     a.anybroke = F.anybroke.p; a.step = step; a.u = u; a.uold = uold; a.out = out;
     a.sc = F.sc; a.tension_only = F.tension_only; a.damage = (F.damage_on && F.sc > 0.0) ? 1 : 0;
     a.chi = (1.0 + F.sc) * (1.0 + F.sc);
     a.clo = (F.sc < 1.0) ? (1.0 - F.sc) * (1.0 - F.sc) : -1.0;
     a.b0 = F.b[0]; a.b1 = F.b[1]; a.b2 = F.b[2]; a.rho = F.rho; a.c1 = c1; a.c2 = c2;
-    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = guard ? F.gstate.p : nullptr;
+    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = (guard || fused) ? F.gstate.p : nullptr;
     a.frozen = (frozen_k > INT_MIN) ? 1 : 0; a.frozen_k = frozen_k;
+    a.prog_of = F.bc.prog_of.p; a.bnode = F.bc.node.p; a.bmode = F.bc.mode.p;
+    a.inv_off = F.bc.inv_off.p; a.inv_q = F.bc.inv_q.p; a.bval0 = F.bc.val0.p; a.bvel = F.bc.vel.p;
+    a.tpre = tpre; a.ghist64 = F.ghist64.p;
     const bool f32 = (F.precision == PXG_FP32C) && !a.frozen;
+    const bool staged = (ctx->dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
+    PXG_CHECK(F.npb == (ctx->dim == 3 ? step_npb<3>() : step_npb<2>()) && (!staged || F.roff.p), PXG_ERR_STATE,
+              "launch_step: neighbour staging tables missing (pxg_frac_setup)");
+    PXG_CHECK(!fused || ((mode == MODE_CD || mode == MODE_VAR) && a.prog_of && a.inv_off && !a.frozen),
+              PXG_ERR_STATE, "launch_step_fused: CD/VAR steps with a BC program only (pxg_frac_set_bcs)");
     hipStream_t st = ctx->stream;
-#define PXG_LAUNCH(D, FF, M)                                                                        \
-    do {                                                                                            \
-        constexpr int npb = 4 * (64 / StepCfg<D, FF>::S);                                           \
-        a.nblk = (int)gridn(ctx->n, npb);                                                           \
-        hipLaunchKernelGGL((k_frac_step<D, FF, M>), dim3(8 * gridn(a.nblk, 8)), dim3(256), 0, st, a); \
-    } while (0)
+    a.nblk = (int)gridn(ctx->n, F.npb);
+    const dim3 grid(8 * gridn(a.nblk, 8));
+#define PXG_LAUNCH(D, FF, M, FU)                                                                    \
+    hipLaunchKernelGGL((k_frac_step<D, FF, M, FU>), grid, dim3(256),                                  \
+                       sizeof(double) * (step_force_rows<D, FF>() + 2 * (size_t)F.gcap * D), st, a)
 #define PXG_MODES(D, FF)                                                                            \
     switch (mode) {                                                                                 \
-    case MODE_CD: PXG_LAUNCH(D, FF, MODE_CD); break;                                                \
-    case MODE_VAR: PXG_LAUNCH(D, FF, MODE_VAR); break;                                              \
-    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL); break;                                          \
-    default: PXG_LAUNCH(D, FF, MODE_INIT); break;                                                   \
+    case MODE_CD: if (fused) PXG_LAUNCH(D, FF, MODE_CD, true); else PXG_LAUNCH(D, FF, MODE_CD, false); break; \
+    case MODE_VAR: if (fused) PXG_LAUNCH(D, FF, MODE_VAR, true); else PXG_LAUNCH(D, FF, MODE_VAR, false); break; \
+    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL, false); break;                                   \
+    default: PXG_LAUNCH(D, FF, MODE_INIT, false); break;                                            \
     }
     if (ctx->dim == 2) { if (f32) { PXG_MODES(2, true) } else { PXG_MODES(2, false) } }
     else { if (f32) { PXG_MODES(3, true) } else { PXG_MODES(3, false) } }
 #undef PXG_MODES
 #undef PXG_LAUNCH
     PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+// launches one k_frac_step for the context's dimension / precision / mode
+int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
+                double c1, double c2, int step, bool guard, int frozen_k)
+{
+    return launch_any(ctx, mode, u, uold, out, c1, c2, step, guard, frozen_k, false, 0.0);
+}
+
+int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
+                      double c1, double c2, int step, double tpre)
+{
+    return launch_any(ctx, mode, u, uold, out, c1, c2, step, true, INT_MIN, true, tpre);
+}
+
+int launch_guard_final(pxg_ctx *ctx, int step)
+{
+    FracState &F = ctx->frac;
+    hipLaunchKernelGGL(k_guard_final, dim3(1), dim3(64), 0, ctx->stream, step, F.gmax.p, F.ghist64.p, F.gstate.p);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+// Builds the LDS staging tables of the step kernel from the families: per
+// step block the sorted distinct neighbour ids, compressed into runs of
+// consecutive ids, and per bond its LDS slot.  Blocks whose neighbourhood
+// exceeds the slot budget read their neighbours from global memory.
+int build_step_gather(pxg_ctx *ctx)
+{
+    FracState &F = ctx->frac;
+    const int n = ctx->n, dim = ctx->dim;
+    const long long nb = ctx->nb;
+    const int npb = (dim == 3) ? step_npb<3>() : step_npb<2>();
+    const bool staged = (dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
+    F.npb = npb;
+    if (!staged) {
+        F.gcap = 0;
+        F.roff.release(); F.runs.release(); F.gslot.release();
+        return PXG_OK;
+    }
+    // slot budget: 640 rows (30 KB of u and x) in 3D, 512 (16 KB) in 2D
+    const int capmax = (dim == 3) ? 640 : 512;
+    const int nblk = (int)gridn(n, npb);
+    std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
+    PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (nb > 0)
+        PXG_HIP(hipMemcpyAsync(nbr.data(), ctx->nbr.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, ctx->stream));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<unsigned short> slot((size_t)(nb > 0 ? nb : 1), GSLOT_GLOBAL);
+    std::vector<std::vector<int4>> bruns((size_t)nblk);
+    std::vector<int> bcount((size_t)nblk, 0);
+    auto work = [&](int first, int step) {
+        std::vector<int> ids;
+        for (int b = first; b < nblk; b += step) {
+            const long long t0 = off[(size_t)b * npb], t1 = off[std::min<size_t>((size_t)n, (size_t)(b + 1) * npb)];
+            ids.assign(nbr.begin() + t0, nbr.begin() + t1);
+            std::sort(ids.begin(), ids.end());
+            ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+            if ((int)ids.size() > capmax) continue;           // global gathers for this block
+            bcount[b] = (int)ids.size();
+            std::vector<int4> &rr = bruns[b];
+            for (size_t q = 0; q < ids.size(); q++) {
+                if (q == 0 || ids[q] != ids[q - 1] + 1) rr.push_back(make_int4(ids[q] - 1, 0, (int)q, 0));
+                rr.back().y++;
+            }
+            for (long long t = t0; t < t1; t++)
+                slot[(size_t)t] = (unsigned short)(std::lower_bound(ids.begin(), ids.end(), nbr[(size_t)t]) - ids.begin());
+        }
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nth; k++) pool.emplace_back(work, k, nth);
+    for (auto &th : pool) th.join();
+    std::vector<int> roff((size_t)nblk + 1, 0);
+    int gcap = 0;
+    for (int b = 0; b < nblk; b++) {
+        roff[(size_t)b + 1] = roff[(size_t)b] + (int)bruns[b].size();
+        gcap = std::max(gcap, bcount[b]);
+    }
+    std::vector<int4> runs((size_t)std::max(1, roff[(size_t)nblk]));
+    for (int b = 0; b < nblk; b++) std::copy(bruns[b].begin(), bruns[b].end(), runs.begin() + roff[(size_t)b]);
+    PXG_HIP(F.roff.alloc(roff.size()));
+    PXG_HIP(F.runs.alloc(runs.size()));
+    PXG_HIP(F.gslot.alloc(slot.size()));
+    PXG_HIP(hipMemcpyAsync(F.roff.p, roff.data(), sizeof(int) * roff.size(), hipMemcpyHostToDevice, ctx->stream));
+    PXG_HIP(hipMemcpyAsync(F.runs.p, runs.data(), sizeof(int4) * runs.size(), hipMemcpyHostToDevice, ctx->stream));
+    PXG_HIP(hipMemcpyAsync(F.gslot.p, slot.data(), sizeof(unsigned short) * slot.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    F.gcap = std::max(gcap, 1);
     return PXG_OK;
 }
 
