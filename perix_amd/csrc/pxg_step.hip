@@ -34,9 +34,10 @@
 // -0.0, the additive identity), then one lane per component adds the node's
 // bonds in family order, which is the reference's exact summation order.
 // FP32C: forces are formed in FP32 from FP32 operators and FP32 displacement
-// differences and accumulated per lane with TwoSum (an FP32 sum/compensation
-// pair); the S lanes of a node combine their pairs with a lane butterfly
-// (DPP within 16 lanes), and the FP64 update consumes sum + compensation.
+// differences; each lane adds a group's G forces pairwise and accumulates the
+// group sums with TwoSum (an FP32 sum/compensation pair); the S lanes of a
+// node combine their pairs with a lane butterfly (DPP within 16 lanes), and
+// the FP64 update consumes sum + compensation.
 #include "pxg_internal.h"
 #include <algorithm>
 #include <cfloat>
@@ -114,10 +115,10 @@ template <int DIM, bool F32> struct StepCfg;
 // rows staged in LDS.  The 2D plate is latency-bound (its working set lives
 // in the Infinity Cache) and runs faster gathering straight from global
 // memory; the 3D block saturates the texture units without staging.
-template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false; };
-template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false; };
-template <> struct StepCfg<3, true> { static constexpr int S = 16, G = 4; static constexpr bool STAGE = true; };
-template <> struct StepCfg<3, false> { static constexpr int S = 16, G = 2; static constexpr bool STAGE = true; };
+template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = true; };
+template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = false; };
+template <> struct StepCfg<3, true> { static constexpr int S = 16, G = 4; static constexpr bool STAGE = true, PF2 = false; };
+template <> struct StepCfg<3, false> { static constexpr int S = 16, G = 2; static constexpr bool STAGE = true, PF2 = false; };
 
 template <int B> struct VecT;
 template <> struct VecT<2> { using T = unsigned short; };
@@ -167,6 +168,28 @@ __device__ __forceinline__ float partner(float v)
     else return __shfl_xor(v, M);
 }
 
+// a group's G bond forces are added pairwise in FP32 and the group sum
+// enters the lane's TwoSum-compensated (sum, compensation) pair
+template <int G, int DIM>
+__device__ __forceinline__ void accumulate_group(const float (&f)[G][DIM], float (&s)[DIM], float (&c)[DIM])
+{
+#pragma unroll
+    for (int k = 0; k < DIM; k++) {
+        float g;
+        if constexpr (G == 4) g = (f[0][k] + f[1][k]) + (f[2][k] + f[3][k]);
+        else if constexpr (G == 2) g = f[0][k] + f[1][k];
+        else {
+            g = f[0][k];
+#pragma unroll
+            for (int q = 1; q < G; q++) g += f[q][k];
+        }
+        float sn, e;
+        two_sum(s[k], g, sn, e);
+        s[k] = sn;
+        c[k] += e;
+    }
+}
+
 template <int M, int S, int DIM>
 __device__ __forceinline__ void butterfly(float (&s)[DIM], float (&c)[DIM])
 {
@@ -209,17 +232,30 @@ __device__ __forceinline__ double pick(const double (&v)[DIM], int c)
     return (DIM == 3 && c == 2) ? v[DIM - 1] : (c == 1 ? v[1] : v[0]);
 }
 
+// stream data of one group of G consecutive bonds (one vector load per array)
+template <int DIM, bool F32, int G, bool STAGE>
+struct GroupData {
+    static constexpr int NC = (DIM == 3) ? 6 : 3;
+    unsigned short sv[STAGE ? G : 1];      // LDS slots of the neighbours
+    int jv[STAGE ? 1 : G];                 // neighbour ids (1-based)
+    bool ok[G];                            // bond intact
+    float p32[F32 ? NC : 1][G];
+    double p64[F32 ? 1 : NC][G];
+};
+
 template <int DIM, bool F32, int MODE, bool FUSED>
 __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 {
     constexpr int S = StepCfg<DIM, F32>::S;
     constexpr int G = StepCfg<DIM, F32>::G;
     constexpr bool STAGE = StepCfg<DIM, F32>::STAGE;
+    constexpr bool PF2 = StepCfg<DIM, F32>::PF2;
     constexpr int NPW = 64 / S;               // nodes per wavefront
     constexpr int W = G * S;                  // bonds per node and round
     constexpr int NC = (DIM == 3) ? 6 : 3;
     constexpr int ROW = W * DIM + 1;          // LDS row per node (odd: no bank aliasing)
     constexpr int FROWS = F32 ? 0 : 4 * NPW * ROW;
+    using GD = GroupData<DIM, F32, G, STAGE>;
     // dynamic LDS: [FP64 force rows][u of gcap slots][x of gcap slots]
     extern __shared__ double s_dyn[];
     double *s_f = s_dyn;
@@ -231,8 +267,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     }
     const int blk = xcd_block(blockIdx.x, a.nblk);
     if (blk < 0) return;
-    // fused guard: the loads are issued here and consumed only after the
-    // LDS staging, before the first global write
+    // Everything that does not read LDS is issued before the staging
+    // barrier: the fused guard's inputs, the node's own data and the first
+    // bond group's stream loads; the guard is decided before the first write.
     int g_tripped = 0;
     unsigned long long g_max = 0;
     double g_past = -1.0;
@@ -242,6 +279,57 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool live_dmg = a.damage && !a.frozen;
+    const int ln = lane / S, sl = lane % S;
+    const int i = (blk * 4 + wv) * NPW + ln;
+    const bool nvalid = i < a.n;
+    int b0 = 0, b1 = 0;
+    if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
+    // BC program of this node (fused path): own entry, entries it mirrors
+    int bq = -1, bp0 = 0, bp1 = 0;
+    if constexpr (FUSED) {
+        if (nvalid && sl < DIM) { bq = a.prog_of[i]; bp0 = a.inv_off[i]; bp1 = a.inv_off[i + 1]; }
+    }
+    double ui[DIM], xi[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) {
+        ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
+        xi[c] = (nvalid && live_dmg) ? a.xp[(size_t)i * DIM + c] : 0.0;
+    }
+    const int A = b0 & ~(G - 1);
+    int broke = 0;
+
+    // bond-array bases; offsets below are unsigned 32-bit element indices
+    const float *P32c[F32 ? NC : 1];
+    const double *P64c[F32 ? 1 : NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if constexpr (F32) P32c[c] = a.P32 + c * a.ps;
+        else P64c[c] = a.P64 + c * a.ps;
+    }
+    auto load_group = [&](unsigned t0, GD &d) {
+        if constexpr (STAGE) vload(a.gslot + t0, d.sv);
+        else vload(a.nbr + t0, d.jv);
+        if (a.frozen) {
+            int bs[G];
+            vload(a.break_step + t0, bs);
+#pragma unroll
+            for (int k = 0; k < G; k++) d.ok[k] = bs[k] > a.frozen_k;
+        } else {
+            unsigned char hv[G];
+            vload(a.health + t0, hv);
+#pragma unroll
+            for (int k = 0; k < G; k++) d.ok[k] = hv[k] != 0;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if constexpr (F32) vload(P32c[c] + t0, d.p32[c]);
+            else vload(P64c[c] + t0, d.p64[c]);
+        }
+    };
+    GD d;
+    const int tfirst = A + G * sl;
+    if (tfirst < b1) load_group((unsigned)tfirst, d);
+
     if constexpr (STAGE) {
         // stage the block's neighbour rows (runs of consecutive node ids) in
         // LDS: each run is a contiguous span of u and x, copied coalesced
@@ -260,15 +348,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         }
         __syncthreads();
     }
-    const int ln = lane / S, sl = lane % S;
-    const int i = (blk * 4 + wv) * NPW + ln;
-    const bool nvalid = i < a.n;
-    int b0 = 0, b1 = 0;
-    if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
-    // BC program of this node (fused path): own entry, entries it mirrors
-    int bq = -1, bp0 = 0, bp1 = 0;
     if constexpr (FUSED) {
-        if (nvalid && sl < DIM) { bq = a.prog_of[i]; bp0 = a.inv_off[i]; bp1 = a.inv_off[i + 1]; }
         if (g_tripped) return;                // the guard tripped earlier
         if (a.step >= 2) {
             // every block takes the guard decision on the previous step from
@@ -283,64 +363,25 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             if (trip) return;
         }
     }
-    double ui[DIM], xi[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; c++) {
-        ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
-        xi[c] = (nvalid && live_dmg) ? a.xp[(size_t)i * DIM + c] : 0.0;
-    }
-    const int A = b0 & ~(G - 1);
-    int broke = 0;
 
-    // one group of G bonds: loads, damage, forces (dead bonds give f = 0)
-    // bond-array bases; offsets below are unsigned 32-bit element indices
-    const float *P32c[F32 ? NC : 1];
-    const double *P64c[F32 ? 1 : NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        if constexpr (F32) P32c[c] = a.P32 + c * a.ps;
-        else P64c[c] = a.P64 + c * a.ps;
-    }
+    // damage and forces of one loaded group (dead bonds give f = 0 / -0.0)
     using F64G = double[F32 ? 1 : G][DIM];
     using F32G = float[F32 ? G : 1][DIM];
-    auto group = [&](unsigned t0, F64G &f64, F32G &f32) {
-        unsigned short sv[STAGE ? G : 1];
-        int jv[STAGE ? 1 : G];
-        if constexpr (STAGE) vload(a.gslot + t0, sv);
-        else vload(a.nbr + t0, jv);
-        bool ok[G];
-        if (a.frozen) {
-            int bs[G];
-            vload(a.break_step + t0, bs);
-#pragma unroll
-            for (int k = 0; k < G; k++) ok[k] = bs[k] > a.frozen_k;
-        } else {
-            unsigned char hv[G];
-            vload(a.health + t0, hv);
-#pragma unroll
-            for (int k = 0; k < G; k++) ok[k] = hv[k] != 0;
-        }
-        float p32[F32 ? NC : 1][G];
-        double p64[F32 ? 1 : NC][G];
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-            if constexpr (F32) vload(P32c[c] + t0, p32[c]);
-            else vload(P64c[c] + t0, p64[c]);
-        }
+    auto compute_group = [&](unsigned t0, const GD &dd, F64G &f64, F32G &f32) {
 #pragma unroll
         for (int k = 0; k < G; k++) {
             const int t = (int)t0 + k;
-            bool live = ok[k] && t >= b0 && t < b1;
+            bool live = dd.ok[k] && t >= b0 && t < b1;
             double du[DIM];
 #pragma unroll
             for (int c = 0; c < DIM; c++) du[c] = 0.0;
             if (live) {
-                // neighbour row: the block's LDS copy, or global memory for
-                // a block whose neighbourhood did not fit the LDS slots
+                // neighbour row: the block's LDS copy, or global memory in a
+                // block whose neighbourhood did not fit the LDS slots
                 double uj[DIM], xj[DIM];
                 if constexpr (STAGE) {
-                    if (sv[k] != GSLOT_GLOBAL) {
-                        const int q = sv[k] * DIM;
+                    if (dd.sv[k] != GSLOT_GLOBAL) {
+                        const int q = dd.sv[k] * DIM;
 #pragma unroll
                         for (int c = 0; c < DIM; c++) {
                             uj[c] = s_u[q + c];
@@ -356,7 +397,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                     }
                 } else {
                     // 32-bit byte offset from the array base
-                    const unsigned jo = (unsigned)(jv[k] - 1) * (unsigned)(DIM * sizeof(double));
+                    const unsigned jo = (unsigned)(dd.jv[k] - 1) * (unsigned)(DIM * sizeof(double));
                     const double *gu = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.u) + jo);
                     const double *gx = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.xp) + jo);
 #pragma unroll
@@ -388,6 +429,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                             unsure = unsure || fabs(ql) <= band;
                         }
                         if (unsure) {
+                            // near the threshold: the reference's own expression
                             const double rl = sqrt(rr);
                             const double s = (sqrt(yy2) - rl) / rl;
                             brk = a.tension_only ? (s > a.sc) : (fabs(s) > a.sc);
@@ -402,27 +444,30 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 }
             }
             if constexpr (F32) {
-                float d[DIM];
+                float dv[DIM];
 #pragma unroll
-                for (int c = 0; c < DIM; c++) d[c] = live ? (float)du[c] : 0.0f;
+                for (int c = 0; c < DIM; c++) dv[c] = live ? (float)du[c] : 0.0f;
+                const auto &p = dd.p32;
                 if (DIM == 2) {
-                    f32[k][0] = fmaf(p32[0][k], d[0], p32[2][k] * d[1]);
-                    f32[k][1] = fmaf(p32[2][k], d[0], p32[1][k] * d[1]);
+                    f32[k][0] = fmaf(p[0][k], dv[0], p[2][k] * dv[1]);
+                    f32[k][1] = fmaf(p[2][k], dv[0], p[1][k] * dv[1]);
                 } else {
-                    f32[k][0] = fmaf(p32[0][k], d[0], fmaf(p32[3][k], d[1], p32[4][k] * d[DIM - 1]));
-                    f32[k][1] = fmaf(p32[3][k], d[0], fmaf(p32[1][k], d[1], p32[5][k] * d[DIM - 1]));
-                    f32[k][DIM - 1] = fmaf(p32[4][k], d[0], fmaf(p32[5][k], d[1], p32[2][k] * d[DIM - 1]));
+                    f32[k][0] = fmaf(p[0][k], dv[0], fmaf(p[3][k], dv[1], p[4][k] * dv[DIM - 1]));
+                    f32[k][1] = fmaf(p[3][k], dv[0], fmaf(p[1][k], dv[1], p[5][k] * dv[DIM - 1]));
+                    f32[k][DIM - 1] = fmaf(p[4][k], dv[0], fmaf(p[5][k], dv[1], p[2][k] * dv[DIM - 1]));
                 }
             } else {
-                // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295)
+                // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295);
+                // a dead bond contributes -0.0, the additive identity
+                const auto &p = dd.p64;
                 if (live) {
                     if (DIM == 2) {
-                        f64[k][0] = p64[0][k] * du[0] + p64[2][k] * du[1];
-                        f64[k][1] = p64[2][k] * du[0] + p64[1][k] * du[1];
+                        f64[k][0] = p[0][k] * du[0] + p[2][k] * du[1];
+                        f64[k][1] = p[2][k] * du[0] + p[1][k] * du[1];
                     } else {
-                        f64[k][0] = p64[0][k] * du[0] + p64[3][k] * du[1] + p64[4][k] * du[DIM - 1];
-                        f64[k][1] = p64[3][k] * du[0] + p64[1][k] * du[1] + p64[5][k] * du[DIM - 1];
-                        f64[k][DIM - 1] = p64[4][k] * du[0] + p64[5][k] * du[1] + p64[2][k] * du[DIM - 1];
+                        f64[k][0] = p[0][k] * du[0] + p[3][k] * du[1] + p[4][k] * du[DIM - 1];
+                        f64[k][1] = p[3][k] * du[0] + p[1][k] * du[1] + p[5][k] * du[DIM - 1];
+                        f64[k][DIM - 1] = p[4][k] * du[0] + p[5][k] * du[1] + p[2][k] * du[DIM - 1];
                     }
                 } else {
 #pragma unroll
@@ -437,19 +482,23 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         float Ls[DIM], Lc[DIM];
 #pragma unroll
         for (int c = 0; c < DIM; c++) { Ls[c] = 0.0f; Lc[c] = 0.0f; }
-        for (int t0 = A + G * sl; t0 < b1; t0 += W) {
-            F64G f64;
-            F32G f32;
-            group((unsigned)t0, f64, f32);
-#pragma unroll
-            for (int k = 0; k < G; k++)
-#pragma unroll
-                for (int c = 0; c < DIM; c++) {
-                    float sn, e;
-                    two_sum(Ls[c], f32[k][c], sn, e);
-                    Ls[c] = sn;
-                    Lc[c] += e;
-                }
+        for (int t0 = tfirst; t0 < b1; t0 += W) {
+            if constexpr (PF2) {
+                // the next group's loads fly while this one is computed
+                GD dn;
+                if (t0 + W < b1) load_group((unsigned)(t0 + W), dn);
+                F64G f64;
+                F32G f32;
+                compute_group((unsigned)t0, d, f64, f32);
+                accumulate_group<G, DIM>(f32, Ls, Lc);
+                d = dn;
+            } else {
+                if (t0 != tfirst) load_group((unsigned)t0, d);
+                F64G f64;
+                F32G f32;
+                compute_group((unsigned)t0, d, f64, f32);
+                accumulate_group<G, DIM>(f32, Ls, Lc);
+            }
         }
         butterfly<1, S, DIM>(Ls, Lc);
         double L[DIM];
@@ -466,8 +515,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             const int t0 = base + G * sl;
             F64G f64;
             if (t0 < b1) {
+                if (r > 0) load_group((unsigned)t0, d);
                 F32G f32;
-                group((unsigned)t0, f64, f32);
+                compute_group((unsigned)t0, d, f64, f32);
             } else {
 #pragma unroll
                 for (int k = 0; k < G; k++)
