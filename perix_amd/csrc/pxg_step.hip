@@ -250,6 +250,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     constexpr int G = StepCfg<DIM, F32>::G;
     constexpr bool STAGE = StepCfg<DIM, F32>::STAGE;
     constexpr bool PF2 = StepCfg<DIM, F32>::PF2;
+    static_assert(!(STAGE && PF2), "double-buffered groups rely on the first group loaded up front");
     constexpr int NPW = 64 / S;               // nodes per wavefront
     constexpr int W = G * S;                  // bonds per node and round
     constexpr int NC = (DIM == 3) ? 6 : 3;
@@ -328,7 +329,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     };
     GD d;
     const int tfirst = A + G * sl;
-    if (tfirst < b1) load_group((unsigned)tfirst, d);
+    // the first group's stream loads overlap the staging of the neighbour
+    // rows -- except where rows are staged: there the extra bytes in flight
+    // push the shared rows out of L2 before the next block reuses them
+    if (!STAGE && tfirst < b1) load_group((unsigned)tfirst, d);
 
     if constexpr (STAGE) {
         // stage the block's neighbour rows (runs of consecutive node ids) in
@@ -493,7 +497,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 accumulate_group<G, DIM>(f32, Ls, Lc);
                 d = dn;
             } else {
-                if (t0 != tfirst) load_group((unsigned)t0, d);
+                if (STAGE || t0 != tfirst) load_group((unsigned)t0, d);
                 F64G f64;
                 F32G f32;
                 compute_group((unsigned)t0, d, f64, f32);
@@ -515,7 +519,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             const int t0 = base + G * sl;
             F64G f64;
             if (t0 < b1) {
-                if (r > 0) load_group((unsigned)t0, d);
+                if (STAGE || r > 0) load_group((unsigned)t0, d);
                 F32G f32;
                 compute_group((unsigned)t0, d, f64, f32);
             } else {

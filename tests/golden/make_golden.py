@@ -1,0 +1,149 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own build.
+
+Run in the development container, where oracle/ref/Makefile has compiled the
+reference's C++ sources into oracle/_ref/libperix_ref.so:
+
+    make -C oracle/ref && python tests/golden/make_golden.py
+
+Every fixture holds inputs (the JSON deck parameters) and the reference's
+outputs, nothing else.  The tests compare the C oracle and the HIP library
+against these arrays, so neither needs the reference at test time.
+
+Fixtures (all small, .npz):
+  mesh_fam_2d.npz   20x12 grid on [0,1]x[0,0.6]: cloud, horizon, families,
+                    g-weights, CSR, Poisson K/residual for a seeded field
+  mesh_fam_3d.npz   6x5x4 grid on [0,0.6]x[0,0.5]x[0,0.4]: same, 10-term basis
+  explicit_2d.npz   40x20 plate, two force-only notches, 16.5 m/s impactor,
+                    400 FP64 steps: final u, u_prev, bond health
+  explicit_3d.npz   16x8x4 block, one notch, impactor, 200 FP64 steps
+  ch_2d.npz         16x16 Cahn-Hilliard: RandomIC + Neumann presets, CSR K and
+                    residual for a perturbed state (Picard-frozen mobility)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import refpy  # noqa: E402
+
+E_STEEL, NU, RHO = 191e9, 0.25, 8000.0
+
+
+def deck_poisson(dim, nx, ny, nz, hi):
+    return {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+            "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+            "DOFs": ["u"],
+            "ElmtSystem": {"p": {"type": "poisson", "params": {"sigma": 1.3, "f": 0.7}}},
+            "BCSystem": {"l": {"type": "dirichlet", "phygroup": "leftnodes_ghost", "dofs": ["u"],
+                               "value": 0.0}},
+            "JobSystem": {"type": "static", "assemble": "serial"}}
+
+
+def mesh_fixture(name, dim, nx, ny, nz, hi):
+    c = refpy.Case(deck_poisson(dim, nx, ny, nz, hi))
+    coords, vols, mirror = c.mesh()
+    off, nbr = c.families()
+    nop = 9 if dim == 3 else 5
+    ops, flags = c.ops(nop)
+    rowptr, cols = c.csr()
+    u = np.random.RandomState(7).rand(c.n)
+    k, r = c.assemble(u, u, 0.0)
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), factor=3.015, sigma=1.3, f=0.7,
+                        n=c.n, nbulk=c.nbulk, delta=c.delta, dxyz=c.dxyz(), coords=coords,
+                        vols=vols, mirror=mirror, off=off, nbr=nbr, ops=ops, flags=flags,
+                        rowptr=rowptr, cols=cols, u=u, K=k, R=r)
+    c.close()
+
+
+def deck_frac(dim, nx, ny, nz, hi, G0, dt, total, cracks, box):
+    d = {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                        "Cracks": [{"x1": q[0], "y1": q[1], "x2": q[2], "y2": q[3]} for q in cracks]},
+         "DOFs": ["ux", "uy"] + (["uz"] if dim == 3 else []),
+         "ElmtSystem": {"f": {"type": "explicit_pddo_frac",
+                              "params": {"E": E_STEEL, "nu": NU, "rho": RHO, "G0": G0,
+                                         "tension_only": True, "damage": True}}},
+         "BCSystem": {}, "JobSystem": {"type": "transient", "assemble": "openmp"},
+         "TimeStepping": {"dt": dt, "total_time": total}}
+    if dim == 2:
+        d["ElmtSystem"]["f"]["params"]["state"] = "plane_strain"
+    walls = ["left", "right", "bottom", "top"] + (["back", "front"] if dim == 3 else [])
+    dofs = d["DOFs"]
+    for w in walls:
+        d["BCSystem"]["n_" + w] = {"type": "neumann", "phygroup": w + "nodes_ghost", "dofs": dofs,
+                                   "value": [0.0] * len(dofs)}
+    d["BCSystem"]["imp"] = {"type": "dirichlet", "phygroup": "bottomnodes_ghost", "dofs": dofs,
+                            "value": [0.0] * len(dofs),
+                            "velocity": [0.0, 16.5] + ([0.0] if dim == 3 else []), "direct": True,
+                            "box": {"xmin": box[0], "xmax": box[1]}}
+    return d
+
+
+def explicit_fixture(name, dim, nx, ny, nz, hi, cracks, box, nsteps, sc_target=0.003):
+    # G0 from the target stretch (FractureCriterion.h:30-43 inverted); the
+    # reference recomputes sc from G0, and the fixture records both
+    c0 = refpy.Case(deck_poisson(dim, nx, ny, nz, hi))
+    delta, dxyz = c0.delta, c0.dxyz()
+    c0.close()
+    G0 = sc_target ** 2 * (12 * E_STEEL * delta / (5 * np.pi) if dim == 2 else 6 * E_STEEL * delta / 5)
+    c11 = E_STEEL * (1 - NU) / ((1 + NU) * (1 - 2 * NU))
+    cfl = float(max(dxyz)) / np.sqrt(c11 / RHO)
+    dt = 0.5 * cfl
+    total = nsteps * dt
+    c = refpy.Case(deck_frac(dim, nx, ny, nz, hi, G0, dt, total, cracks, box))
+    steps, u, uold, health, opc, sc = c.explicit()
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), cracks=np.array(cracks, float),
+                        box=np.array(box, float), E=E_STEEL, nu=NU, rho=RHO, G0=G0, sc=sc, dt=dt,
+                        total=total, nsteps=nsteps, steps=steps, u=u, uold=uold,
+                        health=health.astype(np.uint8))
+    c.close()
+
+
+def ch_fixture(name, nx):
+    kappa = 1e-2 * (1.0 / nx) ** 2
+    d = {"Mesh": refpy.mesh_block(2, nx, nx),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2}, "DOFs": ["c", "mu"],
+         "ElmtSystem": {"ch": {"type": "cahnhilliard", "params": {"chi": 3.0, "kappa": kappa, "M": 1.0}}},
+         "BCSystem": {w: {"type": "neumann", "phygroup": w + "nodes_ghost", "dofs": ["c", "mu"],
+                          "value": [0.0, 0.0]} for w in ["left", "right", "bottom", "top"]},
+         "ICSystem": {"noise": {"type": "random", "dof": "c", "min": 0.49, "max": 0.51, "seed": 12345},
+                      "mu0": {"type": "constant", "dof": "mu", "value": 0.0}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 1e-3, "total_time": 2e-2}}
+    c = refpy.Case(d)
+    u0, _ = c.initial_state()
+    u1 = u0.copy()
+    u1[1::2] = np.random.RandomState(3).rand(c.n) * 0.1
+    u1[0::2] += 1e-3
+    k, r = c.assemble(u1, u0, 1e-3)
+    lap, rev, mnode = c.ch_geometry()
+    np.savez_compressed(os.path.join(HERE, name), nx=nx, kappa=kappa, chi=3.0, M0=1.0, dt=1e-3,
+                        seed=12345, u0=u0, u1=u1, K=k, R=r, lap=lap, rev=rev, mnode=mnode)
+    c.close()
+
+
+def main():
+    if not refpy.available():
+        raise SystemExit("oracle/_ref/libperix_ref.so is missing: run `make -C oracle/ref` first")
+    mesh_fixture("mesh_fam_2d.npz", 2, 20, 12, 1, (1.0, 0.6, 1.0))
+    mesh_fixture("mesh_fam_3d.npz", 3, 6, 5, 4, (0.6, 0.5, 0.4))
+    explicit_fixture("explicit_2d.npz", 2, 40, 20, 1, (0.02, 0.01, 1.0),
+                     [(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+                     [0.0075, 0.0125, -1.0, 1.0, -1.0, 1.0], 400)
+    explicit_fixture("explicit_3d.npz", 3, 16, 8, 4, (0.016, 0.008, 0.004),
+                     [(0.008, 0.0, 0.008, 0.0008)], [0.004, 0.006, -1.0, 1.0, -1.0, 1.0], 200)
+    ch_fixture("ch_2d.npz", 16)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print("%-20s %8d bytes" % (f, os.path.getsize(os.path.join(HERE, f))))
+
+
+if __name__ == "__main__":
+    main()

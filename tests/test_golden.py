@@ -1,0 +1,99 @@
+"""The C oracle against the golden fixtures written from the reference's own
+build (tests/golden/make_golden.py).  CPU only.
+
+Bit-exact: point cloud, horizon, families (members and order), CSR pattern,
+g-weights, Poisson and Cahn-Hilliard assembly, the FP64 explicit trajectory
+and its broken-bond set, the RandomIC field.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pxo
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def mesh_of(z):
+    nx, ny, nz = (int(v) for v in z["grid"])
+    dim = int(z["dim"])
+    return pxo.Mesh(pxo.make_grid(dim, nx, ny, nz, hi=tuple(z["hi"])), factor=float(z["factor"]))
+
+
+def same(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    if a.dtype.kind == "f":
+        return a.shape == b.shape and np.array_equal(a.view(np.uint64), np.asarray(b, np.float64).view(np.uint64))
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["mesh_fam_2d.npz", "mesh_fam_3d.npz"])
+def test_mesh_families_ops_csr_poisson(name):
+    z = load(name)
+    m = mesh_of(z)
+    assert m.n == int(z["n"]) and m.nbulk == int(z["nbulk"])
+    assert same(m.coords, z["coords"]), "node coordinates"
+    assert same(m.vols, z["vols"]), "node volumes"
+    assert np.array_equal(m.mirror, z["mirror"]), "ghost mirrors"
+    assert m.delta == float(z["delta"]), "horizon radius"
+    assert same(m.dxyz, z["dxyz"])
+    assert np.array_equal(m.off, z["off"]) and np.array_equal(m.nbr, z["nbr"]), "families (members and order)"
+    ops, flags = pxo.operators(m)
+    assert same(ops, z["ops"]), "g-weights (Eigen FullPivLU restated bit for bit)"
+    assert np.array_equal(flags & 1, z["flags"])
+    rp, cols = pxo.csr_pattern(m, 1)
+    assert np.array_equal(rp, z["rowptr"]) and np.array_equal(cols, z["cols"]), "CSR pattern"
+    k, r = pxo.assemble_poisson(m, ops, rp, cols, float(z["sigma"]), z["u"], f_const=float(z["f"]))
+    assert same(k, z["K"]) and same(r, z["R"]), "Poisson Jacobian / residual"
+
+
+@pytest.mark.parametrize("name", ["explicit_2d.npz", "explicit_3d.npz"])
+def test_explicit_trajectory(name):
+    z = load(name)
+    dim = int(z["dim"])
+    m = mesh_of(dict(z, factor=3.015))
+    E, nu, rho = float(z["E"]), float(z["nu"]), float(z["rho"])
+    sc = pxo.critical_stretch(float(z["G0"]), E, m.delta, dim, False)
+    assert sc == float(z["sc"]), "critical stretch from G0 (FractureCriterion.h:30-43)"
+    ops, _ = pxo.operators(m)
+    opc = pxo.frac_opcache(m, ops)
+    p = pxo.frac_params(dim, E, nu, rho, sc)
+    h = pxo.seed_health(m, segs=[tuple(c) for c in z["cracks"]])
+    walls = ["left", "right", "bottom", "top"] + (["back", "front"] if dim == 3 else [])
+    comps = list(range(dim))
+    specs = [{"type": "neumann", "nodes": m.group(pxo.WALLS[w]), "comps": comps} for w in walls]
+    specs.append({"type": "dirichlet", "nodes": m.group(2), "comps": comps, "value": [0.0] * dim,
+                  "velocity": [0.0, 16.5] + ([0.0] if dim == 3 else []), "direct": True,
+                  "box": list(z["box"])})
+    bcs = pxo.make_bcs(specs)
+    u = np.zeros(m.n * dim); uo = np.zeros(m.n * dim)
+    pxo.bc_preset(m, dim, bcs, u, 0.0)
+    steps, _ = pxo.explicit_run(m, p, opc, bcs, float(z["dt"]), float(z["total"]), u, uo, h)
+    assert steps == int(z["steps"])
+    assert same(u, z["u"]) and same(uo, z["uold"]), "FP64 displacement history"
+    assert np.array_equal((h > 0).astype(np.uint8), z["health"]), "broken-bond set"
+
+
+def test_cahn_hilliard_assembly_and_random_ic():
+    z = load("ch_2d.npz")
+    nx = int(z["nx"])
+    m = pxo.Mesh(pxo.make_grid(2, nx, nx))
+    u = np.zeros(2 * m.n)
+    pxo.random_ic(int(z["seed"]), 0.49, 0.51, np.arange(1, m.n + 1), 2, [1], u)
+    specs = [{"type": "neumann", "nodes": m.group(w), "comps": [0, 1]} for w in range(4)]
+    pxo.bc_preset(m, 2, pxo.make_bcs(specs), u, 0.0)
+    assert same(u, z["u0"]), "RandomIC (std::mt19937_64) + Neumann presets"
+    ops, _ = pxo.operators(m)
+    lap, rev, isg = pxo.ch_geometry(m, ops)
+    assert same(lap, z["lap"]) and np.array_equal(rev, z["rev"])
+    mn = pxo.ch_mobility(m, z["u1"], float(z["M0"]))
+    assert same(mn, z["mnode"])
+    rp, cols = pxo.csr_pattern(m, 2)
+    k, r = pxo.assemble_ch(m, lap, rev, isg, mn, float(z["kappa"]), float(z["chi"]), float(z["dt"]),
+                           rp, cols, z["u1"], z["u0"])
+    assert same(k, z["K"]) and same(r, z["R"]), "Cahn-Hilliard Jacobian / residual"
