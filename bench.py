@@ -225,9 +225,14 @@ def block_case(problems, pxg, steps=100):
         nb, n = s.ctx.nb, s.cloud.n
         avg_ms = ms / max(launches, 1)
         gbs = frac_bytes(nb, n, 3, prec) / (avg_ms * 1e-3) / 1e9
+        alg = frac_bytes(nb, n, 3, prec)
         out[name] = {"node_steps_per_s": n * done2 / sec2, "ms_per_step": sec2 * 1e3 / done2,
-                     "step_kernel_ms": avg_ms, "step_kernel_GBs": gbs,
-                     "hbm_frac": gbs / HBM_PEAK_GBS, "setup_s": setup}
+                     "step_kernel_ms": avg_ms, "setup_s": setup,
+                     "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": gbs / HBM_PEAK_GBS,
+                                  "traffic": profiled_traffic("frac_step_block160x80x40", name),
+                                  "kernel": "k_frac_step<3,%s>" % name,
+                                  "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms}}
         out["nodes"] = n
         out["bonds"] = nb
         s.ctx.close()
@@ -238,6 +243,35 @@ def block_case(problems, pxg, steps=100):
     out["fp32c_over_fp64"] = out["fp32c"]["node_steps_per_s"] / out["fp64"]["node_steps_per_s"]
     out["steps"] = steps
     return out
+
+
+def profiled_traffic(kernel_case, prec):
+    """HBM bytes per launch of a step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), or None."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        t = json.load(fh)
+    return t.get(kernel_case, {}).get(prec, {}).get("traffic_bytes")
+
+
+def stream_ceiling(nbytes=1_750_000_000, reps=10):
+    """Device-wide read bandwidth of a plain reduction over nbytes (the volume
+    of the 3D block's bond streams): the practical ceiling for the step."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    x.sum()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        x.sum()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    print(json.dumps({"stream_read_bytes": x.numel() * 4, "ms": ms,
+                      "GBs": x.numel() * 4 / (ms * 1e-3) / 1e9}))
 
 
 def copy_calibration():
@@ -264,9 +298,13 @@ def main():
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
     ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--stream-ceiling", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return 0
+    if args.stream_ceiling:
+        stream_ceiling()
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -306,11 +344,7 @@ def main():
     achieved = bytes32 / (avg_ms * 1e-3) / 1e9
     avg64 = f64["kernel_ms"] / max(f64["launches"], 1)
     achieved64 = frac_bytes(nb, n, 2, 0) / (avg64 * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("frac_step_plate400x200_fp32c_bytes_per_launch")
+    traffic = profiled_traffic("frac_step_plate400x200", "fp32c")
 
     dt = f32.get("dt")
     out = {
@@ -336,7 +370,7 @@ def main():
                      "avg_launch_ms": avg_ms,
                      "note": "the 43.5 MB working set of the 2D plate stays in the 256 MB "
                              "Infinity Cache, so this kernel is latency/launch-bound; the HBM "
-                             "claim is made on the 3D block (extra.block3d)"},
+                             "roofline claim is made on the 3D block (block3d.fp32c.roofline)"},
         "fp64": {"node_steps_per_s": v64, "ms_per_step": f64["sec"] * 1e3 / f64["done"],
                  "step_kernel_ms": avg64, "step_kernel_GBs": achieved64},
         "fp32c_over_fp64": value / v64,
