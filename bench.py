@@ -297,6 +297,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and 3D block figures")
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
     ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
+    ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--stream-ceiling", action="store_true")
     args = ap.parse_args()
@@ -319,6 +320,9 @@ def main():
     K, W = args.steps, args.warmup
     if args.only_block:
         print(json.dumps({"block3d": block_case(problems, pxg, steps=K)}))
+        return 0
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_case(problems, pxg)}))
         return 0
     res = {}
     for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):

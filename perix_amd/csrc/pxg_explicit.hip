@@ -246,6 +246,12 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
                        plane_stress, F.ps, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
+    if (node_E) {
+        const int rc = launch_transpose_ops(ctx);
+        if (rc) return rc;
+    } else {
+        F.PhatT.release();
+    }
     PXG_HIP(F.health.alloc(nb));
     PXG_HIP(F.break_step.alloc(nb));
     {

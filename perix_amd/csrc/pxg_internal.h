@@ -6,6 +6,7 @@
 // and is released by pxg_destroy.  One host thread per context.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -57,10 +58,14 @@ struct dbuf {
         return e != hipSuccess ? e : hipDeviceSynchronize();
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFree(p);       // hipFree waits for the device to finish with it
         p = nullptr;
         n = 0;
     }
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
 };
 
 struct BcNodeProgram {
@@ -75,6 +80,14 @@ struct BcNodeProgram {
     dbuf<int> prog_of;       // N: program entry of a target node, -1 otherwise
     dbuf<int> inv_off;       // N+1: program entries whose mirror source is node m
     dbuf<int> inv_q;
+};
+
+// adjoint workspace, kept between calls and reallocated on size changes
+struct AdjWork {
+    dbuf<double> cks;        // 2 * ncheckpoints * N*dim: (u^{k-1}, u^k) pairs
+    dbuf<double> seg;        // (ckpt + 2) * N*dim: one recomputed segment
+    dbuf<double> lam, lamp, wbar, ew, Ebar, dJ, eo;
+    dbuf<int> ds;            // sensor node ids
 };
 
 struct FracState {
@@ -116,7 +129,9 @@ struct FracState {
     int plane_stress = 0;
     int has_nodeE = 0;
     dbuf<double> Phat;            // nb*ncomp: operators with C(E=1) (adjoint)
+    dbuf<double> PhatT;           // PhatT[c*ps + t] = Phat[c*ps + rev[t]] (adjoint)
     dbuf<int> rev;                // nb: reverse bond index
+    AdjWork adj;
     dbuf<double> accel;           // N*dim scratch (drop-in accel / init)
     dbuf<int> anybroke;           // [0] breaks in the last launch
     int health_uploaded = 0;
@@ -135,16 +150,19 @@ struct FracState {
 enum { MODE_CD = 0, MODE_VAR = 1, MODE_ACCEL = 2, MODE_INIT = 3 };
 int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
                 double c1, double c2, int step, bool guard, int frozen_k);
-// MODE_CD / MODE_VAR step with the BC presets at time tpre and the blow-up
-// guard folded in (the guard decision of step `step`-1 is taken at the start
-// of the launch; launch_guard_final decides the last step of a run)
+// MODE_CD / MODE_VAR step with the BC presets at time tpre folded in and,
+// with `guard`, the blow-up guard (the decision on step `step`-1 is taken at
+// the start of the launch; launch_guard_final decides the last step of a
+// run).  frozen_k > INT_MIN freezes health from the recorded break steps.
 int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
-                      double c1, double c2, int step, double tpre);
+                      double c1, double c2, int step, double tpre, bool guard = true,
+                      int frozen_k = INT_MIN);
 int launch_guard_final(pxg_ctx *ctx, int step);
 int build_step_gather(pxg_ctx *ctx);
 constexpr unsigned short GSLOT_GLOBAL = 0xFFFF;
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
+int launch_transpose_ops(pxg_ctx *ctx);
 
 } // namespace pxg
 

@@ -81,6 +81,7 @@ struct StepArgs {
     const double *bval0, *bvel;
     double tpre;               // time of the presets (t^{n+1})
     double *ghist64;
+    int guard_on;              // fused path: blow-up guard active
 };
 
 // blow-up guard of TimeStepping.cpp:550-573 for step s (fused path): trips
@@ -275,8 +276,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     unsigned long long g_max = 0;
     double g_past = -1.0;
     if constexpr (FUSED) {
-        g_tripped = a.gstate[0];
-        if (a.step >= 2) { g_max = guard_max(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
+        if (a.guard_on) {
+            g_tripped = a.gstate[0];
+            if (a.step >= 2) { g_max = guard_max(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
+        }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool live_dmg = a.damage && !a.frozen;
@@ -354,7 +357,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     }
     if constexpr (FUSED) {
         if (g_tripped) return;                // the guard tripped earlier
-        if (a.step >= 2) {
+        if (a.guard_on && a.step >= 2) {
             // every block takes the guard decision on the previous step from
             // the same data; block 0 records it and clears the maximum slot
             // of step+1 (last used by step-2, already decided)
@@ -601,7 +604,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             mymax = o > mymax ? o : mymax;
         }
         unsigned long long *slot = FUSED ? &a.gmax[4 + a.step % 3] : &a.gmax[0];
-        if (lane == 0 && mymax) {
+        if (lane == 0 && mymax && (!FUSED || a.guard_on)) {
             const unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (mymax > cur) atomicMax(slot, mymax);
         }
@@ -650,16 +653,16 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     a.chi = (1.0 + F.sc) * (1.0 + F.sc);
     a.clo = (F.sc < 1.0) ? (1.0 - F.sc) * (1.0 - F.sc) : -1.0;
     a.b0 = F.b[0]; a.b1 = F.b[1]; a.b2 = F.b[2]; a.rho = F.rho; a.c1 = c1; a.c2 = c2;
-    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = (guard || fused) ? F.gstate.p : nullptr;
+    a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = guard ? F.gstate.p : nullptr;
     a.frozen = (frozen_k > INT_MIN) ? 1 : 0; a.frozen_k = frozen_k;
     a.prog_of = F.bc.prog_of.p; a.bnode = F.bc.node.p; a.bmode = F.bc.mode.p;
     a.inv_off = F.bc.inv_off.p; a.inv_q = F.bc.inv_q.p; a.bval0 = F.bc.val0.p; a.bvel = F.bc.vel.p;
-    a.tpre = tpre; a.ghist64 = F.ghist64.p;
+    a.tpre = tpre; a.ghist64 = F.ghist64.p; a.guard_on = guard ? 1 : 0;
     const bool f32 = (F.precision == PXG_FP32C) && !a.frozen;
     const bool staged = (ctx->dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
     PXG_CHECK(F.npb == (ctx->dim == 3 ? step_npb<3>() : step_npb<2>()) && (!staged || F.roff.p), PXG_ERR_STATE,
               "launch_step: neighbour staging tables missing (pxg_frac_setup)");
-    PXG_CHECK(!fused || ((mode == MODE_CD || mode == MODE_VAR) && a.prog_of && a.inv_off && !a.frozen),
+    PXG_CHECK(!fused || ((mode == MODE_CD || mode == MODE_VAR) && a.prog_of && a.inv_off),
               PXG_ERR_STATE, "launch_step_fused: CD/VAR steps with a BC program only (pxg_frac_set_bcs)");
     hipStream_t st = ctx->stream;
     a.nblk = (int)gridn(ctx->n, F.npb);
@@ -690,9 +693,9 @@ int launch_step(pxg_ctx *ctx, int mode, const double *u, const double *uold, dou
 }
 
 int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uold, double *out,
-                      double c1, double c2, int step, double tpre)
+                      double c1, double c2, int step, double tpre, bool guard, int frozen_k)
 {
-    return launch_any(ctx, mode, u, uold, out, c1, c2, step, true, INT_MIN, true, tpre);
+    return launch_any(ctx, mode, u, uold, out, c1, c2, step, guard, frozen_k, true, tpre);
 }
 
 int launch_guard_final(pxg_ctx *ctx, int step)
