@@ -101,15 +101,18 @@ struct FracState {
     long long ps = 0;        // component stride: nb rounded up to a multiple of 64
     dbuf<double> P64;
     dbuf<float> P32;
-    // LDS neighbour staging of the step kernel: the step block b (npb
-    // consecutive nodes) copies the neighbour rows listed in
-    // runs[roff[b] .. roff[b+1]) -- {first node, count, first LDS slot, 0} --
-    // into LDS, and bond t reads its neighbour from LDS slot gslot[t]
-    // (GSLOT_GLOBAL: read it from global memory via nbr[t]).
+    // LDS neighbour staging of the step kernel: step block b owns the npb
+    // nodes perm[b*npb ..] (a compact tile) and copies the neighbour rows
+    // ulist[roff[b] .. roff[b+1]) (node ids, slot order) into LDS; bond t
+    // reads its neighbour from LDS slot gslot[t].  A block whose
+    // neighbourhood exceeds the slot budget has bovf set and gathers its
+    // neighbours from global memory via nbr.
     int npb = 0, gcap = 0;        // nodes per step block, LDS slots per block
+    dbuf<int> perm;
     dbuf<int> roff;
-    dbuf<int4> runs;
+    dbuf<int> ulist;
     dbuf<unsigned short> gslot;
+    dbuf<unsigned char> bovf;
     dbuf<unsigned char> health;   // nb
     dbuf<int> break_step;         // nb (recorded break step, INT_MAX intact)
     dbuf<double> u[2];            // ping-pong displacement buffers (N*dim)
@@ -159,7 +162,6 @@ int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uol
                       int frozen_k = INT_MIN);
 int launch_guard_final(pxg_ctx *ctx, int step);
 int build_step_gather(pxg_ctx *ctx);
-constexpr unsigned short GSLOT_GLOBAL = 0xFFFF;
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
 int launch_transpose_ops(pxg_ctx *ctx);

@@ -5,18 +5,20 @@
 // central-difference update of TimeStepping::solveExplicitDynamics
 // (TimeStepping.cpp:417-599).
 //
-// Work layout.  A 256-thread block owns 4*64/S consecutive nodes; each
-// wavefront owns 64/S of them and gives each node S adjacent lanes.
-//  * Neighbour rows are staged in LDS first: the block's distinct neighbours
-//    form a few runs of consecutive node ids (rows of the grid), each copied
-//    from u and x with coalesced loads; a bond then reads its neighbour from
-//    the LDS slot recorded for it at setup.  Gathering straight from global
-//    memory costs one cache-line lookup per lane and load, which saturates
-//    the texture units long before HBM; the staged copy reads each row once
-//    per block (about 3.5x fewer rows than bonds).
+// Work layout.  A 256-thread block owns 4*64/S nodes; each wavefront owns
+// 64/S of them and gives each node S adjacent lanes.
+//  * 3D: neighbour rows are staged in LDS first.  Blocks own compact tiles
+//    of nodes (2 x 4 x 4 cells on a structured grid, via the tile order
+//    perm), whose horizon union is about 14 rows per node; the block copies
+//    those rows {x_j, u_j} into LDS with coalesced loads, and a bond reads its
+//    neighbour from the LDS slot recorded for it at setup.  Gathering straight
+//    from global memory costs one cache-line lookup per lane and load, which
+//    saturates the texture units long before HBM.  2D (latency-bound, in the
+//    Infinity Cache) gathers straight from global memory over consecutive
+//    node blocks.
 //  * A node's bonds [off[i], off[i+1]) are contiguous; the lanes walk them in
 //    groups of G consecutive bonds aligned to G, so one G-wide vector load
-//    fetches a group's LDS slots, its health bytes and each operator
+//    fetches a group's LDS slots (or ids), its health bytes and each operator
 //    component (all bond arrays are allocated with zeroed slack and the
 //    operator components are strided by a multiple of 64, so every such load
 //    is aligned and in bounds).  A group that straddles two families is
@@ -53,8 +55,10 @@ struct StepArgs {
     long long ps;              // component stride of the operator arrays
     const int *off;
     const int *nbr;
-    const int *roff;           // neighbour-row runs per block (LDS staging)
-    const int4 *runs;
+    const int *perm;           // LDS staging: nodes of step block b at perm[b*npb ..]
+    const int *roff;           // rows of block b: ulist[roff[b] .. roff[b+1])
+    const int *ulist;
+    const unsigned char *bovf; // block gathers from global memory (no LDS rows)
     const unsigned short *gslot;
     int gcap;                  // LDS slots per block
     const double *xp;
@@ -118,8 +122,8 @@ template <int DIM, bool F32> struct StepCfg;
 // memory; the 3D block saturates the texture units without staging.
 template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = true; };
 template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = false; };
-template <> struct StepCfg<3, true> { static constexpr int S = 16, G = 4; static constexpr bool STAGE = true, PF2 = false; };
-template <> struct StepCfg<3, false> { static constexpr int S = 16, G = 2; static constexpr bool STAGE = true, PF2 = false; };
+template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = true, PF2 = false; };
+template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; static constexpr bool STAGE = true, PF2 = false; };
 
 template <int B> struct VecT;
 template <> struct VecT<2> { using T = unsigned short; };
@@ -258,11 +262,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     constexpr int ROW = W * DIM + 1;          // LDS row per node (odd: no bank aliasing)
     constexpr int FROWS = F32 ? 0 : 4 * NPW * ROW;
     using GD = GroupData<DIM, F32, G, STAGE>;
-    // dynamic LDS: [FP64 force rows][u of gcap slots][x of gcap slots]
+    // dynamic LDS: [FP64 force rows][gcap neighbour records {x, u}]
     extern __shared__ double s_dyn[];
     double *s_f = s_dyn;
-    double *s_u = s_dyn + FROWS;
-    double *s_x = s_u + (size_t)a.gcap * DIM;
+    double *s_rec = s_dyn + FROWS;
 
     if constexpr (!FUSED) {
         if (a.gstate && a.gstate[0]) return;  // blow-up guard tripped earlier
@@ -284,8 +287,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool live_dmg = a.damage && !a.frozen;
     const int ln = lane / S, sl = lane % S;
-    const int i = (blk * 4 + wv) * NPW + ln;
-    const bool nvalid = i < a.n;
+    const int gi = (blk * 4 + wv) * NPW + ln;
+    const bool nvalid = gi < a.n;
+    const int i = (STAGE && nvalid) ? a.perm[gi] : gi;
+    const bool govf = STAGE && a.bovf[blk];
     int b0 = 0, b1 = 0;
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
     // BC program of this node (fused path): own entry, entries it mirrors
@@ -338,20 +343,16 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     if (!STAGE && tfirst < b1) load_group((unsigned)tfirst, d);
 
     if constexpr (STAGE) {
-        // stage the block's neighbour rows (runs of consecutive node ids) in
-        // LDS: each run is a contiguous span of u and x, copied coalesced
-        const int r1 = a.roff[blk + 1];
-        for (int r = a.roff[blk] + wv; r < r1; r += 4) {
-            const int4 R = a.runs[r];                 // first node, count, first slot
-            const int len = R.y * DIM;
-            const double *gu = a.u + (size_t)R.x * DIM;
-            double *lu = s_u + R.z * DIM;
-            for (int e = lane; e < len; e += 64) lu[e] = gu[e];
-            if (live_dmg) {
-                const double *gx = a.xp + (size_t)R.x * DIM;
-                double *lx = s_x + R.z * DIM;
-                for (int e = lane; e < len; e += 64) lx[e] = gx[e];
-            }
+        // stage the block's neighbour rows as LDS records {x_j, u_j}; the
+        // rows are sorted node ids, so consecutive lanes read runs of
+        // consecutive nodes (coalesced)
+        const int r0 = a.roff[blk];
+        const int len = (a.roff[blk + 1] - r0) * (2 * DIM);
+        const int *rows = a.ulist + r0;
+        for (int e = threadIdx.x; e < len; e += 256) {
+            const int q = e / (2 * DIM), part = e - q * (2 * DIM);
+            const size_t j = (size_t)rows[q] * DIM;
+            s_rec[e] = (part < DIM) ? (live_dmg ? a.xp[j + part] : 0.0) : a.u[j + part - DIM];
         }
         __syncthreads();
     }
@@ -387,12 +388,12 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 // block whose neighbourhood did not fit the LDS slots
                 double uj[DIM], xj[DIM];
                 if constexpr (STAGE) {
-                    if (dd.sv[k] != GSLOT_GLOBAL) {
-                        const int q = dd.sv[k] * DIM;
+                    if (!govf) {
+                        const double *rec = s_rec + dd.sv[k] * (2 * DIM);
 #pragma unroll
                         for (int c = 0; c < DIM; c++) {
-                            uj[c] = s_u[q + c];
-                            xj[c] = live_dmg ? s_x[q + c] : 0.0;
+                            xj[c] = rec[c];
+                            uj[c] = rec[DIM + c];
                         }
                     } else {
                         const size_t q = (size_t)(a.nbr[t] - 1) * DIM;
@@ -646,7 +647,8 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     FracState &F = ctx->frac;
     StepArgs a;
     a.n = ctx->n; a.ps = F.ps; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
-    a.roff = F.roff.p; a.runs = F.runs.p; a.gslot = F.gslot.p; a.gcap = F.gcap;
+    a.perm = F.perm.p; a.roff = F.roff.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
+    a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
     a.anybroke = F.anybroke.p; a.step = step; a.u = u; a.uold = uold; a.out = out;
     a.sc = F.sc; a.tension_only = F.tension_only; a.damage = (F.damage_on && F.sc > 0.0) ? 1 : 0;
@@ -706,10 +708,15 @@ int launch_guard_final(pxg_ctx *ctx, int step)
     return PXG_OK;
 }
 
-// Builds the LDS staging tables of the step kernel from the families: per
-// step block the sorted distinct neighbour ids, compressed into runs of
-// consecutive ids, and per bond its LDS slot.  Blocks whose neighbourhood
-// exceeds the slot budget read their neighbours from global memory.
+// Builds the LDS staging tables of the step kernel.  Nodes are ordered by
+// tiles of TX x TY x TZ cells (cell = floor(x / dx_char); the tile order is
+// lexicographic z, y, x, and cells within a tile follow z, y, x), and step
+// block b owns the npb nodes perm[b*npb ...]: on a structured grid every
+// block is one compact 2 x 4 x 4 tile, whose horizon union holds about 14
+// rows per node where a 32-node row would need 31.  Per block the sorted
+// distinct neighbours form the row list ulist[roff[b] ..], and every bond
+// records the LDS slot of its neighbour.  Blocks whose neighbourhood exceeds
+// the slot budget gather from global memory instead (bovf).
 int build_step_gather(pxg_ctx *ctx)
 {
     FracState &F = ctx->frac;
@@ -720,36 +727,62 @@ int build_step_gather(pxg_ctx *ctx)
     F.npb = npb;
     if (!staged) {
         F.gcap = 0;
-        F.roff.release(); F.runs.release(); F.gslot.release();
+        F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
         return PXG_OK;
     }
-    // slot budget: 640 rows (30 KB of u and x) in 3D, 512 (16 KB) in 2D
-    const int capmax = (dim == 3) ? 640 : 512;
+    constexpr int TX = 2, TY = 4, TZ = 4;
+    const int capmax = 640;   // rows per block: 640 x 48 B = 30 KB of LDS
     const int nblk = (int)gridn(n, npb);
     std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
     PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream));
     if (nb > 0)
         PXG_HIP(hipMemcpyAsync(nbr.data(), ctx->nbr.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, ctx->stream));
     PXG_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<unsigned short> slot((size_t)(nb > 0 ? nb : 1), GSLOT_GLOBAL);
-    std::vector<std::vector<int4>> bruns((size_t)nblk);
-    std::vector<int> bcount((size_t)nblk, 0);
+    // tile order of the nodes
+    const double *X = ctx->hcoords.data();
+    double lo[3] = {X[0], X[1], X[2]};
+    for (int i = 1; i < n; i++)
+        for (int a = 0; a < 3; a++) lo[a] = std::min(lo[a], X[3 * (size_t)i + a]);
+    const double h = std::max(ctx->dxyz[0], std::max(ctx->dxyz[1], ctx->dxyz[2]));
+    PXG_CHECK(h > 0.0, PXG_ERR_STATE, "build_step_gather: zero grid spacing");
+    struct Key { long long k0, k1; int id; };
+    std::vector<Key> keys((size_t)n);
+    for (int i = 0; i < n; i++) {
+        long long cc[3];
+        for (int a = 0; a < 3; a++) cc[a] = (long long)std::floor((X[3 * (size_t)i + a] - lo[a]) / h + 0.5);
+        const long long t[3] = {cc[0] / TX, cc[1] / TY, cc[2] / TZ};
+        keys[(size_t)i].k0 = (t[2] << 42) | (t[1] << 21) | t[0];
+        keys[(size_t)i].k1 = ((cc[2] % TZ) << 42) | ((cc[1] % TY) << 21) | (cc[0] % TX);
+        keys[(size_t)i].id = i;
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+        return a.k0 != b.k0 ? a.k0 < b.k0 : (a.k1 != b.k1 ? a.k1 < b.k1 : a.id < b.id);
+    });
+    std::vector<int> perm((size_t)n);
+    for (int q = 0; q < n; q++) perm[(size_t)q] = keys[(size_t)q].id;
+    std::vector<unsigned short> slot((size_t)(nb > 0 ? nb : 1), 0);
+    std::vector<unsigned char> ovf((size_t)nblk, 0);
+    std::vector<std::vector<int>> brows((size_t)nblk);
     auto work = [&](int first, int step) {
         std::vector<int> ids;
         for (int b = first; b < nblk; b += step) {
-            const long long t0 = off[(size_t)b * npb], t1 = off[std::min<size_t>((size_t)n, (size_t)(b + 1) * npb)];
-            ids.assign(nbr.begin() + t0, nbr.begin() + t1);
+            const int q1 = std::min(n, (b + 1) * npb);
+            ids.clear();
+            for (int q = b * npb; q < q1; q++) {
+                const int i = perm[(size_t)q];
+                ids.insert(ids.end(), nbr.begin() + off[(size_t)i], nbr.begin() + off[(size_t)i + 1]);
+            }
             std::sort(ids.begin(), ids.end());
             ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
-            if ((int)ids.size() > capmax) continue;           // global gathers for this block
-            bcount[b] = (int)ids.size();
-            std::vector<int4> &rr = bruns[b];
-            for (size_t q = 0; q < ids.size(); q++) {
-                if (q == 0 || ids[q] != ids[q - 1] + 1) rr.push_back(make_int4(ids[q] - 1, 0, (int)q, 0));
-                rr.back().y++;
+            if ((int)ids.size() > capmax) { ovf[(size_t)b] = 1; continue; }   // global gathers
+            for (int q = b * npb; q < q1; q++) {
+                const int i = perm[(size_t)q];
+                for (int t = off[(size_t)i]; t < off[(size_t)i + 1]; t++)
+                    slot[(size_t)t] = (unsigned short)(std::lower_bound(ids.begin(), ids.end(), nbr[(size_t)t]) - ids.begin());
             }
-            for (long long t = t0; t < t1; t++)
-                slot[(size_t)t] = (unsigned short)(std::lower_bound(ids.begin(), ids.end(), nbr[(size_t)t]) - ids.begin());
+            std::vector<int> &rr = brows[(size_t)b];
+            rr.resize(ids.size());
+            for (size_t q = 0; q < ids.size(); q++) rr[q] = ids[q] - 1;
         }
     };
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -759,19 +792,24 @@ int build_step_gather(pxg_ctx *ctx)
     std::vector<int> roff((size_t)nblk + 1, 0);
     int gcap = 0;
     for (int b = 0; b < nblk; b++) {
-        roff[(size_t)b + 1] = roff[(size_t)b] + (int)bruns[b].size();
-        gcap = std::max(gcap, bcount[b]);
+        roff[(size_t)b + 1] = roff[(size_t)b] + (int)brows[(size_t)b].size();
+        gcap = std::max(gcap, (int)brows[(size_t)b].size());
     }
-    std::vector<int4> runs((size_t)std::max(1, roff[(size_t)nblk]));
-    for (int b = 0; b < nblk; b++) std::copy(bruns[b].begin(), bruns[b].end(), runs.begin() + roff[(size_t)b]);
+    std::vector<int> ulist((size_t)std::max(1, roff[(size_t)nblk]));
+    for (int b = 0; b < nblk; b++)
+        std::copy(brows[(size_t)b].begin(), brows[(size_t)b].end(), ulist.begin() + roff[(size_t)b]);
+    PXG_HIP(F.perm.alloc(perm.size()));
     PXG_HIP(F.roff.alloc(roff.size()));
-    PXG_HIP(F.runs.alloc(runs.size()));
+    PXG_HIP(F.ulist.alloc(ulist.size()));
     PXG_HIP(F.gslot.alloc(slot.size()));
-    PXG_HIP(hipMemcpyAsync(F.roff.p, roff.data(), sizeof(int) * roff.size(), hipMemcpyHostToDevice, ctx->stream));
-    PXG_HIP(hipMemcpyAsync(F.runs.p, runs.data(), sizeof(int4) * runs.size(), hipMemcpyHostToDevice, ctx->stream));
-    PXG_HIP(hipMemcpyAsync(F.gslot.p, slot.data(), sizeof(unsigned short) * slot.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
-    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    PXG_HIP(F.bovf.alloc(ovf.size()));
+    hipStream_t st = ctx->stream;
+    PXG_HIP(hipMemcpyAsync(F.perm.p, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.roff.p, roff.data(), sizeof(int) * roff.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.ulist.p, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.gslot.p, slot.data(), sizeof(unsigned short) * slot.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.bovf.p, ovf.data(), ovf.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipStreamSynchronize(st));
     F.gcap = std::max(gcap, 1);
     return PXG_OK;
 }
