@@ -245,6 +245,104 @@ def block_case(problems, pxg, steps=100):
     return out
 
 
+def assembly_case(problems, pxg, reps=20):
+    """Config 1 (Poisson, 100x100) and config 3 (Cahn-Hilliard, 256x256, one
+    Picard iteration) CSR assembly on the device with inputs and outputs
+    resident in HBM: nnz/s, beside the reference's own CPU assembly (oracle/_ref)
+    of the same system when that build is present."""
+    import torch
+    out = {}
+    ctx, cloud, delta, f = problems.poisson_config1()
+    n, nnz = cloud.n, ctx.nnz
+    X = cloud.coords.reshape(-1, 3)
+    u_h = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])
+    u = torch.from_numpy(u_h).cuda()
+    fd = torch.from_numpy(np.ascontiguousarray(f)).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def poisson():
+        ctx.assemble_poisson_dev(1.0, u.data_ptr(), k.data_ptr(), r.data_ptr(), f_ptr=fd.data_ptr())
+
+    poisson()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        poisson()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    ctx.close()
+    ctx, cloud, delta, u0 = problems.cahnhilliard_config3()
+    n, nnz = cloud.n, ctx.nnz
+    u1 = u0.copy()
+    u1[1::2] = 0.01 * np.random.RandomState(3).rand(n)
+    ud, uod = torch.from_numpy(u1).cuda(), torch.from_numpy(u0).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(2 * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    kappa = 1e-2 * (1.0 / 256) ** 2
+
+    def ch():
+        ctx.assemble_ch_dev(kappa, 3.0, 1.0, 1e-3, ud.data_ptr(), uod.data_ptr(), k.data_ptr(), r.data_ptr())
+
+    ch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ch()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    ctx.close()
+    try:
+        out["cpu_baseline"] = cpu_assembly_baseline(u_h, u0, u1, kappa)
+    except Exception as e:       # reported, never required
+        out["cpu_baseline"] = {"value": None, "sample": "failed: %s" % e}
+    return out
+
+
+def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
+    """The reference's own OpenMP assembly (PDSystem::formResidualAndJacobianParallel
+    through oracle/_ref) of the same two systems on this host, one call each."""
+    from oracle import refpy
+    if not refpy.available():
+        return {"value": None, "sample": "reference build not present"}
+    res = {}
+    with quiet_stdout():
+        d = {"Mesh": refpy.mesh_block(2, 100, 100), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+             "DOFs": ["u"], "ElmtSystem": {"p": {"type": "poisson", "params": {"sigma": 1.0, "f": 0.0}}},
+             "BCSystem": {w: {"type": "dirichlet", "phygroup": w + "nodes_ghost", "dofs": ["u"], "value": 0.0}
+                          for w in ("left", "right", "bottom", "top")},
+             "JobSystem": {"type": "static", "assemble": "openmp"}}
+        c = refpy.Case(d)
+        c.assemble(u_poisson, u_poisson, 0.0, parallel=True)
+        t0 = time.perf_counter()
+        c.assemble(u_poisson, u_poisson, 0.0, parallel=True)
+        ms = (time.perf_counter() - t0) * 1e3
+        res["poisson_100x100"] = {"nnz": c.nnz, "ms_per_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3)}
+        c.close()
+        d = {"Mesh": refpy.mesh_block(2, 256, 256), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+             "DOFs": ["c", "mu"],
+             "ElmtSystem": {"ch": {"type": "cahnhilliard", "params": {"chi": 3.0, "kappa": kappa, "M": 1.0}}},
+             "BCSystem": {w: {"type": "neumann", "phygroup": w + "nodes_ghost", "dofs": ["c", "mu"],
+                              "value": [0.0, 0.0]} for w in ("left", "right", "bottom", "top")},
+             "JobSystem": {"type": "transient", "assemble": "openmp"},
+             "TimeStepping": {"dt": 1e-3, "total_time": 2e-2}}
+        c = refpy.Case(d)
+        c.assemble(u1_ch, u0_ch, 1e-3, parallel=True)
+        t0 = time.perf_counter()
+        c.assemble(u1_ch, u0_ch, 1e-3, parallel=True)
+        ms = (time.perf_counter() - t0) * 1e3
+        res["cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3)}
+        c.close()
+    res["kind"] = "reference"
+    res["cores"] = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    res["sample"] = "one assembly call of each system after a warm-up call (OpenMP assembler)"
+    return res
+
+
 def profiled_traffic(kernel_case, prec):
     """HBM bytes per launch of a step kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json), or None."""
@@ -298,6 +396,7 @@ def main():
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
     ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
     ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
+    ap.add_argument("--only-assembly", action="store_true", help="CSR assembly only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--stream-ceiling", action="store_true")
     args = ap.parse_args()
@@ -323,6 +422,9 @@ def main():
         return 0
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_case(problems, pxg)}))
+        return 0
+    if args.only_assembly:
+        print(json.dumps({"assembly": assembly_case(problems, pxg)}))
         return 0
     res = {}
     for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
@@ -398,6 +500,8 @@ def main():
         out["adjoint"] = adjoint_case(problems, pxg)
         log("3D block (160x80x40)")
         out["block3d"] = block_case(problems, pxg)
+        log("assembly (configs 1 and 3)")
+        out["assembly"] = assembly_case(problems, pxg)
     print(json.dumps(out))
     return 0
 

@@ -191,6 +191,24 @@ typedef struct {
 } pxg_bc;
 int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs);
 
+/* ---- BC row elimination of the implicit CSR system -----------------------
+ * Replaces BCSystem::applyBCs (BCSystem.cpp:212-233) -> DirichletBC::
+ * applyAtTime (DirichletBC.cpp:176-230), NeumannBC::apply (NeumannBC.cpp:
+ * 127-151) with bcRowScale (BCBase.h:33-39), which the reference runs on the
+ * host after every assembly.  pxg_csr_set_bcs resolves the list (same
+ * pxg_bc entries as the explicit presets; components index the CSR dofs;
+ * the box applies to Dirichlet entries only, as in the reference) against the
+ * current CSR pattern; call it again after pxg_build_csr.  Errors
+ * (PXG_ERR_ARG): a Neumann node without a mirror bulk node, a constrained row
+ * without a stored diagonal, a mirror column outside the pattern.
+ * pxg_csr_apply_bcs then rewrites the constrained rows of kvals and rhs at
+ * time t with the state u (n*ndof): host arrays, or device arrays with _dev. */
+int pxg_csr_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs);
+int pxg_csr_bc_rows(pxg_ctx *ctx, int *count);
+int pxg_csr_apply_bcs(pxg_ctx *ctx, double t, const double *u, double *kvals, double *rhs);
+int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *kvals_dev,
+                          double *rhs_dev);
+
 /* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
  * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):
  * damage with u, then health-gated Navier force, accel = (L+b)/rho.

@@ -126,6 +126,15 @@ typedef struct {
 } pxo_bc;
 void pxo_bc_preset(int n, int ndof, const double *coords, const int *mirror,
                    int nbc, const pxo_bc *bcs, double *u, double t);
+/* BC row elimination of the implicit system (BCSystem::applyBCs), applied
+ * in list order on CSR values K (0-based rowptr/cols, n*ndof rows) and the
+ * right-hand side R at time t with the state U.  Returns 0, or -1 when a
+ * constrained row has no stored diagonal, -2 when the mirror column is not
+ * stored, -3 for a Neumann node without a mirror bulk node, -4 for a node id
+ * or component out of range (the reference's exits). */
+int pxo_bc_rows(int n, int ndof, const double *coords, const int *mirror,
+                int nbc, const pxo_bc *bcs, double t, const double *U,
+                const int *rowptr, const int *cols, double *K, double *R);
 
 /* ---- a19: central-difference driver ------------------------------------- */
 /* Step schedule of TimeStepping::solveExplicitDynamics: returns the number of

@@ -255,6 +255,17 @@ def bc_preset(m, ndof, bcs, u, t):
                         C.c_double(t))
 
 
+def bc_rows(m, ndof, bcs, t, U, rowptr, cols, K, R):
+    """BC row elimination on copies of K / R; returns (rc, K', R')."""
+    arr, keep = bcs
+    K = np.ascontiguousarray(K, float).copy(); R = np.ascontiguousarray(R, float).copy()
+    rc = lib().pxo_bc_rows(m.n, ndof, dp(m.coords), ip(m.mirror), len(keep), arr, C.c_double(t),
+                           dp(np.ascontiguousarray(U, float)),
+                           ip(np.ascontiguousarray(rowptr, np.int32)),
+                           ip(np.ascontiguousarray(cols, np.int32)), dp(K), dp(R))
+    return rc, K, R
+
+
 def schedule(dt, total):
     L = lib()
     n = L.pxo_schedule(C.c_double(dt), C.c_double(total), 0, None, None)

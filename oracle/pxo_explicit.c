@@ -189,6 +189,91 @@ void pxo_bc_preset(int n, int ndof, const double *coords, const int *mirror,
     }
 }
 
+/* CSR slot of (row, col) in a sorted row, -1 if not stored
+ * (SparseMatrix::findEntry, SparseMatrix.cpp:187-214) */
+static int csr_find(const int *rowptr, const int *cols, int row, int col)
+{
+    int lo = rowptr[row], hi = rowptr[row + 1] - 1;
+    while (lo <= hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (cols[mid] == col) return mid;
+        if (cols[mid] < col) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+/* bcRowScale (BCBase.h:33-39): the stored diagonal, else the row's largest
+ * |entry| (SparseMatrix::getRowMaxAbs, SparseMatrix.cpp:317-331), else 1 */
+static double bc_row_scale(const int *rowptr, const double *K, int row, int dslot)
+{
+    double scale = K[dslot];
+    if (fabs(scale) < 1.0e-100) {
+        double m = 0.0;
+        for (int k = rowptr[row]; k < rowptr[row + 1]; k++) {
+            const double a = fabs(K[k]);
+            if (a > m) m = a;
+        }
+        scale = m;
+    }
+    if (fabs(scale) < 1.0e-100) scale = 1.0;
+    return scale;
+}
+
+/* BCSystem::applyBCs (BCSystem.cpp:212-233) in list order:
+ *   Dirichlet (DirichletBC::applyAtTime, DirichletBC.cpp:176-230): box
+ *     filter; row g zeroed with diagonal s; with a mirror bulk node b and not
+ *     direct, K(g,b) = s and R(g) = s (2 g(t) - U(g) - U(b)); otherwise
+ *     R(g) = s (g(t) - U(g)); g(t) = value + velocity t.
+ *   Neumann (NeumannBC::apply, NeumannBC.cpp:127-151): no box; row g zeroed
+ *     with diagonal s, K(g,b) = -s, R(g) = s (U(b) - U(g)).
+ * s = bcRowScale of the row as it stands when the entry reaches it
+ * (SparseMatrix::zeroRowAndSetDiagonal SparseMatrix.cpp:333-350, insertValue
+ * SparseMatrix.cpp:216-233). */
+int pxo_bc_rows(int n, int ndof, const double *coords, const int *mirror,
+                int nbc, const pxo_bc *bcs, double t, const double *U,
+                const int *rowptr, const int *cols, double *K, double *R)
+{
+    for (int e = 0; e < nbc; e++) {
+        const pxo_bc *bc = bcs + e;
+        for (int q = 0; q < bc->nnodes; q++) {
+            const int id = bc->nodes[q];
+            if (id < 1 || id > n) return -4;
+            const int b1 = mirror ? mirror[id - 1] : 0;
+            if (bc->type == 0 && (b1 < 1 || b1 > n)) return -3;
+            if (bc->type == 1 && bc->has_box && !inside_box(bc, coords, id)) continue;
+            for (int k = 0; k < bc->ncomp; k++) {
+                const int c = bc->comp[k];
+                if (c < 0 || c >= ndof) return -4;
+                const int g = (id - 1) * ndof + c;
+                const int dslot = csr_find(rowptr, cols, g, g);
+                if (dslot < 0) return -1;
+                const double scale = bc_row_scale(rowptr, K, g, dslot);
+                for (int s = rowptr[g]; s < rowptr[g + 1]; s++) K[s] = 0.0;
+                K[dslot] = scale;
+                if (bc->type == 0) {
+                    const int bd = (b1 - 1) * ndof + c;
+                    const int ps = csr_find(rowptr, cols, g, bd);
+                    if (ps < 0) return -2;
+                    K[ps] = -scale;
+                    R[g] = scale * (U[bd] - U[g]);
+                } else {
+                    const double value = bc->value[k] + bc->velocity[k] * t;
+                    if (b1 >= 1 && !bc->direct) {
+                        const int bd = (b1 - 1) * ndof + c;
+                        const int ps = csr_find(rowptr, cols, g, bd);
+                        if (ps < 0) return -2;
+                        K[ps] = scale;
+                        R[g] = scale * (2.0 * value - U[g] - U[bd]);
+                    } else {
+                        R[g] = scale * (value - U[g]);
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 /* Step schedule of TimeStepping::solveExplicitDynamics (TimeStepping.cpp:497-510) */
 int pxo_schedule(double dt, double total, int cap, double *dts, double *ts)
 {

@@ -176,9 +176,10 @@ void ref_preset(void *h, double *u, double t)
     std::copy(U.getDataPtr(), U.getDataPtr() + U.getSize(), u);
 }
 
-// One formResidualAndJacobian at (u, uold) with Info.Dt = dt (serial path).
-void ref_assemble(void *h, const double *u, const double *uold, double dt,
-                  double *kvals, double *rhs)
+// One formResidualAndJacobian at (u, uold) with Info.Dt = dt: the serial
+// path, or with `parallel` the OpenMP one (FormResidualAndJacobianParallel.cpp).
+static void assemble_any(void *h, const double *u, const double *uold, double dt,
+                         double *kvals, double *rhs, bool parallel)
 {
     auto &p = static_cast<RefCase *>(h)->prob;
     const int n = p.m_U.getSize();
@@ -187,9 +188,43 @@ void ref_assemble(void *h, const double *u, const double *uold, double dt,
     std::copy(uold, uold + n, Uo.getDataPtr());
     LocalElmtInfo info;
     info.Dt = dt; info.T = dt; info.Timestep = 1;
-    p.m_PDSystem.formResidualAndJacobian(p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, info,
-                                         U, Uo, p.m_Matrix, p.m_RHS);
+    if (parallel)
+        p.m_PDSystem.formResidualAndJacobianParallel(p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, info,
+                                                     U, Uo, p.m_Matrix, p.m_RHS);
+    else
+        p.m_PDSystem.formResidualAndJacobian(p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, info,
+                                             U, Uo, p.m_Matrix, p.m_RHS);
     const int nnz = p.m_Matrix.getNNZNum();
+    std::copy(p.m_Matrix.getCSRValuesPtr(), p.m_Matrix.getCSRValuesPtr() + nnz, kvals);
+    std::copy(p.m_RHS.getDataPtr(), p.m_RHS.getDataPtr() + n, rhs);
+}
+
+void ref_assemble(void *h, const double *u, const double *uold, double dt, double *kvals, double *rhs)
+{
+    assemble_any(h, u, uold, dt, kvals, rhs, false);
+}
+
+void ref_assemble_parallel(void *h, const double *u, const double *uold, double dt, double *kvals,
+                           double *rhs)
+{
+    assemble_any(h, u, uold, dt, kvals, rhs, true);
+}
+
+// BCSystem::applyBCs (BCSystem.cpp:212-233) on caller K values / RHS at
+// time t with the state u: the row elimination of the implicit solve.
+void ref_apply_bcs(void *h, const double *u, double t, double *kvals, double *rhs)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    const int n = p.m_U.getSize();
+    const int nnz = p.m_Matrix.getNNZNum();
+    VectorXd U(n);
+    std::copy(u, u + n, U.getDataPtr());
+    std::copy(kvals, kvals + nnz, p.m_Matrix.getCSRValuesPtr());
+    std::copy(rhs, rhs + n, p.m_RHS.getDataPtr());
+    LocalElmtInfo info;
+    info.T = t;
+    p.m_BCSystem.applyBCs(p.m_PDMesh, p.m_Operators, ndof, U, U, info, p.m_Matrix, p.m_RHS);
     std::copy(p.m_Matrix.getCSRValuesPtr(), p.m_Matrix.getCSRValuesPtr() + nnz, kvals);
     std::copy(p.m_RHS.getDataPtr(), p.m_RHS.getDataPtr() + n, rhs);
 }

@@ -29,7 +29,8 @@ def lib():
         _LIB.ref_open.restype = C.c_void_p
         _LIB.ref_open.argtypes = [C.c_char_p]
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
-                  "ref_initial_state", "ref_preset", "ref_assemble", "ref_ch_geometry",
+                  "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
+                  "ref_apply_bcs", "ref_ch_geometry",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -119,10 +120,18 @@ class Case:
         lib().ref_preset(self.h, _dp(u), C.c_double(t))
         return u
 
-    def assemble(self, u, uold, dt):
+    def assemble(self, u, uold, dt, parallel=False):
+        """formResidualAndJacobian (serial) or its OpenMP twin with `parallel`."""
         k = np.zeros(self.nnz); r = np.zeros(self.n * self.ndof)
-        lib().ref_assemble(self.h, _dp(np.ascontiguousarray(u, float)),
-                           _dp(np.ascontiguousarray(uold, float)), C.c_double(dt), _dp(k), _dp(r))
+        f = lib().ref_assemble_parallel if parallel else lib().ref_assemble
+        f(self.h, _dp(np.ascontiguousarray(u, float)), _dp(np.ascontiguousarray(uold, float)),
+          C.c_double(dt), _dp(k), _dp(r))
+        return k, r
+
+    def apply_bcs(self, u, t, k, r):
+        """BCSystem::applyBCs (row elimination) on copies of K values and RHS."""
+        k = np.ascontiguousarray(k, float).copy(); r = np.ascontiguousarray(r, float).copy()
+        lib().ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
         return k, r
 
     def ch_geometry(self):

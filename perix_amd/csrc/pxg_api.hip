@@ -40,6 +40,8 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->rowptr); rel(c->cols); rel(c->slot); rel(c->diag); rel(c->mslot); rel(c->lap);
     rel(c->rev); rel(c->isghost); rel(c->mnode); rel(c->su); rel(c->suo); rel(c->sk);
     rel(c->sr); rel(c->sf);
+    rel(c->csrbc.row); rel(c->csrbc.dslot); rel(c->csrbc.pslot); rel(c->csrbc.pdof);
+    rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);
     FracState &F = c->frac;
     rel(F.P64); rel(F.P32); rel(F.health); rel(F.break_step); rel(F.u[0]); rel(F.u[1]);
     rel(F.perm); rel(F.roff); rel(F.ulist); rel(F.gslot); rel(F.bovf); rel(F.ghist64); rel(F.bc.prog_of);

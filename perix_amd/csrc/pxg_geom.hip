@@ -542,6 +542,7 @@ extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
     PXG_CHECK(ndof >= 1 && ndof <= 4, PXG_ERR_ARG, "pxg_build_csr: ndof must be 1..4");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_build_csr: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
+    ctx->csrbc.set = 0;                 // a BC row program refers to the old pattern
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const long long rows = (long long)n * ndof;

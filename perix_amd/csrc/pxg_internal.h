@@ -82,6 +82,20 @@ struct BcNodeProgram {
     dbuf<int> inv_q;
 };
 
+// BC row program of the implicit CSR system: one entry per constrained row
+// (the last BC entry that reaches the row), sorted by row
+struct CsrBcProgram {
+    dbuf<int> row;           // constrained dof row
+    dbuf<int> dslot;         // CSR index of the row's diagonal
+    dbuf<int> pslot;         // CSR index of (row, partner dof), -1 none
+    dbuf<int> pdof;          // partner dof (mirror bulk node, same component)
+    dbuf<signed char> kind;  // BCROW_DIRECT, BCROW_REFLECT, BCROW_MIRROR
+    dbuf<double> val0, vel;  // Dirichlet g(t) = val0 + vel t
+    int count = 0;
+    int set = 0;             // program built for the current CSR pattern
+};
+enum { BCROW_DIRECT = 0, BCROW_REFLECT = 1, BCROW_MIRROR = 2 };
+
 // adjoint workspace, kept between calls and reallocated on size changes
 struct AdjWork {
     dbuf<double> cks;        // 2 * ncheckpoints * N*dim: (u^{k-1}, u^k) pairs
@@ -208,6 +222,7 @@ struct pxg_ctx {
     pxg::dbuf<unsigned char> isghost;
     pxg::dbuf<double> mnode;
     int ch_ready = 0;
+    pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
     // scratch for host-pointer entry points
     pxg::dbuf<double> su, suo, sk, sr, sf;
     // ---- explicit fracture ----

@@ -167,7 +167,8 @@ def block_config5(precision=pxg.FP32C, nsteps=1000, **kw):
 
 
 def poisson_config1(nx=100, ny=100, sigma=1.0, factor=3.015, device=0, ctx=None):
-    """Config 1: unit square, sigma = 1, manufactured f = 2 pi^2 sin(pi x) sin(pi y)."""
+    """Config 1: unit square, sigma = 1, manufactured f = 2 pi^2 sin(pi x) sin(pi y),
+    BC rows of poisson_bcs resolved on the context."""
     g = StructuredGrid(2, nx, ny)
     cloud = build_structured_cloud(g)
     delta = horizon_radius(g, factor)
@@ -177,16 +178,30 @@ def poisson_config1(nx=100, ny=100, sigma=1.0, factor=3.015, device=0, ctx=None)
     ctx.build_families(delta)
     ctx.build_operators(delta, float(max(cloud.dxyz)))
     ctx.build_csr(1)
+    ctx.csr_set_bcs(poisson_bcs(cloud))
     x = cloud.coords[0::3]
     y = cloud.coords[1::3]
     f = 2.0 * math.pi ** 2 * sigma * np.sin(math.pi * x) * np.sin(math.pi * y)
     return ctx, cloud, delta, f
 
 
+def poisson_bcs(cloud):
+    """Config 1 BC list: Dirichlet u = 0 by reflection on the four ghost walls."""
+    return [{"type": "dirichlet", "nodes": cloud.group(w), "comps": [0], "value": [0.0]}
+            for w in ("left", "right", "bottom", "top")]
+
+
+def cahnhilliard_bcs(cloud):
+    """Config 3 BC list: Neumann mirrors of c and mu on the four ghost walls."""
+    return [{"type": "neumann", "nodes": cloud.group(w), "comps": [0, 1]}
+            for w in ("left", "right", "bottom", "top")]
+
+
 def cahnhilliard_config3(nx=256, ny=256, seed=12345, factor=3.015, device=0, ctx=None):
     """Config 3: 256 x 256 unit square, c0 = 0.5 + U(-0.01, 0.01) by RandomIC
     over all PD nodes in id order (ICSystem.cpp:57-69), mu0 = 0, Neumann
-    mirrors of c and mu on all walls."""
+    mirrors of c and mu on all walls (presets applied to the returned state,
+    BC rows of cahnhilliard_bcs resolved on the context)."""
     g = StructuredGrid(2, nx, ny)
     cloud = build_structured_cloud(g)
     delta = horizon_radius(g, factor)
@@ -197,6 +212,7 @@ def cahnhilliard_config3(nx=256, ny=256, seed=12345, factor=3.015, device=0, ctx
     ctx.build_operators(delta, float(max(cloud.dxyz)))
     ctx.build_csr(2)
     ctx.ch_setup()
+    ctx.csr_set_bcs(cahnhilliard_bcs(cloud))
     u = np.zeros(2 * cloud.n)
     pxg.random_ic(seed, 0.49, 0.51, np.arange(1, cloud.n + 1), 2, [1], u)
     for w in ("left", "right", "bottom", "top"):

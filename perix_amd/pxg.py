@@ -51,7 +51,8 @@ SYMBOLS = [
     "pxg_set_cracks", "pxg_get_seed", "pxg_build_csr", "pxg_get_csr", "pxg_build_operators",
     "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
-    "pxg_frac_set_bcs", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
+    "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
+    "pxg_csr_apply_bcs_dev", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_loss",
 ]
@@ -89,6 +90,9 @@ def load():
                                           C.c_void_p]
         L.pxg_frac_setup.argtypes = [C.c_void_p, C.POINTER(FracParams), C.c_int, D, C.c_double,
                                      C.c_int]
+        L.pxg_csr_apply_bcs.argtypes = [C.c_void_p, C.c_double, D, D, D]
+        L.pxg_csr_apply_bcs_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]
         L.pxg_frac_init.argtypes = [C.c_void_p, D, D, C.c_double]
         L.pxg_frac_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, I]
         L.pxg_frac_adjoint.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -290,6 +294,27 @@ class Context:
     def assemble_ch_dev(self, kappa, chi, M0, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
         _check(load().pxg_assemble_ch_dev(self.h, kappa, chi, M0, dt, u_ptr, uold_ptr, k_ptr,
                                           r_ptr))
+
+    # -- BC row elimination of the implicit system -------------------------------
+    def csr_set_bcs(self, specs):
+        """Resolves a BC list (make_bcs specs, components index the CSR dofs)
+        against the current CSR pattern; returns the number of constrained rows."""
+        arr, n, keep = make_bcs(specs)
+        _check(load().pxg_csr_set_bcs(self.h, n, arr))
+        cnt = C.c_int()
+        _check(load().pxg_csr_bc_rows(self.h, C.byref(cnt)))
+        return cnt.value
+
+    def csr_apply_bcs(self, t, u, k, r):
+        """BCSystem::applyBCs on copies of the host arrays k (nnz), r (rows)."""
+        k = np.ascontiguousarray(k, float).copy()
+        r = np.ascontiguousarray(r, float).copy()
+        _check(load().pxg_csr_apply_bcs(self.h, t, _dp(np.ascontiguousarray(u, float)), _dp(k),
+                                        _dp(r)))
+        return k, r
+
+    def csr_apply_bcs_dev(self, t, u_ptr, k_ptr, r_ptr):
+        _check(load().pxg_csr_apply_bcs_dev(self.h, t, u_ptr, k_ptr, r_ptr))
 
     # -- a15-a20 ---------------------------------------------------------------
     def frac_setup(self, params, precision=FP64, node_E=None, nu=0.25, plane_stress=False):

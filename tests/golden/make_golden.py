@@ -18,6 +18,9 @@ Fixtures (all small, .npz):
   explicit_3d.npz   16x8x4 block, one notch, impactor, 200 FP64 steps
   ch_2d.npz         16x16 Cahn-Hilliard: RandomIC + Neumann presets, CSR K and
                     residual for a perturbed state (Picard-frozen mobility)
+  bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
+                    Poisson (20x12) / Cahn-Hilliard (16x16) system under an
+                    overlapping Dirichlet/Neumann list, with zeroed diagonals
 """
 import os
 import sys
@@ -129,6 +132,86 @@ def ch_fixture(name, nx):
     c.close()
 
 
+# BC lists of the row-elimination fixtures: (name, type, group, dofs, value,
+# velocity, direct, box) in deck order; overlapping entries on one row
+# exercise the list order
+BCROW_SPECS = {
+    "poisson": (["u"], [
+        ("l", "dirichlet", "left", ["u"], [0.3], [0.5], False, None),
+        ("r", "dirichlet", "right", ["u"], [-0.4], None, True, {"ymin": 0.1, "ymax": 0.45}),
+        ("b", "neumann", "bottom", ["u"], [0.0], None, False, None),
+        ("t", "dirichlet", "top", ["u"], [0.25], None, False, {"xmin": 0.2, "xmax": 0.7}),
+        ("t2", "neumann", "top", ["u"], [0.0], None, False, None),
+        ("t3", "dirichlet", "top", ["u"], [0.6], [-1.5], False, {"xmin": 0.5, "xmax": 0.9}),
+    ]),
+    "ch": (["c", "mu"], [
+        # the reference rejects Dirichlet rows on Cahn-Hilliard ghosts
+        # (BCSystem.cpp:61-112): Neumann rows only, per component
+        ("lm", "neumann", "left", ["mu"], [0.0], None, False, None),
+        ("lc", "neumann", "left", ["c"], [0.0], None, False, None),
+        ("b", "neumann", "bottom", ["c", "mu"], [0.0, 0.0], None, False, None),
+        ("rc", "neumann", "right", ["c"], [0.0], None, False, None),
+        ("t", "neumann", "top", ["mu"], [0.0], None, False, None),
+        ("t2", "neumann", "top", ["c", "mu"], [0.0, 0.0], None, False, None),
+    ]),
+}
+
+
+def bcrows_fixture(name, kind):
+    """K, R of one assembly, then BCSystem::applyBCs at t = 0.2.  Some
+    constrained rows get a zero diagonal or are zeroed whole beforehand, so
+    that the row-maximum and unit fallbacks of the row scale are taken."""
+    dofs, specs = BCROW_SPECS[kind]
+    if kind == "poisson":
+        d = deck_poisson(2, 20, 12, 1, (1.0, 0.6, 1.0))
+    else:
+        d = {"Mesh": refpy.mesh_block(2, 16, 16),
+             "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2}, "DOFs": dofs,
+             "ElmtSystem": {"ch": {"type": "cahnhilliard",
+                                   "params": {"chi": 3.0, "kappa": 1e-2 / 256, "M": 1.0}}},
+             "JobSystem": {"type": "transient", "assemble": "serial"},
+             "TimeStepping": {"dt": 1e-3, "total_time": 2e-2}}
+    d["BCSystem"] = {}
+    for nm, typ, grp, dd, val, vel, direct, box in specs:
+        e = {"type": typ, "phygroup": grp + "nodes_ghost", "dofs": dd,
+             "value": val if len(val) > 1 else val[0]}
+        if vel is not None:
+            e["velocity"] = vel if len(vel) > 1 else vel[0]
+        if direct:
+            e["direct"] = True
+        if box is not None:
+            e["box"] = box
+        d["BCSystem"][nm] = e
+    c = refpy.Case(d)
+    coords, vols, mirror = c.mesh()
+    rowptr, cols = c.csr()
+    rs = np.random.RandomState(5)
+    u = rs.rand(c.n * c.ndof)
+    k, r = c.assemble(u, u, 1e-3 if kind == "ch" else 0.0)
+    groups = {grp: c.group(grp + "nodes_ghost") for _, _, grp, *_ in specs}
+    # zero the diagonal of a few constrained rows and a whole row of others
+    rows = sorted({(i - 1) * c.ndof for g in groups.values() for i in g})
+    pick = rs.choice(len(rows), 8, replace=False)
+    for q in pick[:4]:
+        g = rows[q]
+        k[rowptr[g] + np.searchsorted(cols[rowptr[g]:rowptr[g + 1]], g)] = 0.0
+    for q in pick[4:]:
+        g = rows[q]
+        k[rowptr[g]:rowptr[g + 1]] = 0.0
+    t = 0.2
+    k2, r2 = c.apply_bcs(u, t, k, r)
+    spec_arr = np.array([(nm, typ, grp, ",".join(dd), ",".join(map(repr, val)),
+                          ",".join(map(repr, vel)) if vel else "", int(direct),
+                          repr(sorted(box.items())) if box else "")
+                         for nm, typ, grp, dd, val, vel, direct, box in specs])
+    np.savez_compressed(os.path.join(HERE, name), kind=kind, ndof=c.ndof, dofs=np.array(dofs),
+                        n=c.n, coords=coords, vols=vols, mirror=mirror, delta=c.delta,
+                        dxyz=c.dxyz(), rowptr=rowptr, cols=cols, u=u, t=t,
+                        K=k, R=r, K_bc=k2, R_bc=r2, specs=spec_arr,
+                        **{"group_" + g: ids for g, ids in groups.items()})
+    c.close()
+
+
 def main():
     if not refpy.available():
         raise SystemExit("oracle/_ref/libperix_ref.so is missing: run `make -C oracle/ref` first")
@@ -140,6 +223,8 @@ def main():
     explicit_fixture("explicit_3d.npz", 3, 16, 8, 4, (0.016, 0.008, 0.004),
                      [(0.008, 0.0, 0.008, 0.0008)], [0.004, 0.006, -1.0, 1.0, -1.0, 1.0], 200)
     ch_fixture("ch_2d.npz", 16)
+    bcrows_fixture("bcrows_poisson_2d.npz", "poisson")
+    bcrows_fixture("bcrows_ch_2d.npz", "ch")
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print("%-20s %8d bytes" % (f, os.path.getsize(os.path.join(HERE, f))))

@@ -97,3 +97,37 @@ def test_cahn_hilliard_assembly_and_random_ic():
     k, r = pxo.assemble_ch(m, lap, rev, isg, mn, float(z["kappa"]), float(z["chi"]), float(z["dt"]),
                            rp, cols, z["u1"], z["u0"])
     assert same(k, z["K"]) and same(r, z["R"]), "Cahn-Hilliard Jacobian / residual"
+
+
+@pytest.mark.parametrize("name", ["bcrows_poisson_2d.npz", "bcrows_ch_2d.npz"])
+def test_bc_row_elimination(name):
+    """BCSystem::applyBCs: the oracle's sequential restatement reproduces the
+    reference's K and RHS bit for bit, including rows hit by several entries
+    and rows whose scale falls back to the row maximum or to 1."""
+    from conftest import bcrow_specs
+    z = load(name)
+
+    class M:
+        n = int(z["n"]); coords = z["coords"]; mirror = z["mirror"]
+    bcs = pxo.make_bcs(bcrow_specs(z))
+    rc, k, r = pxo.bc_rows(M, int(z["ndof"]), bcs, float(z["t"]), z["u"], z["rowptr"], z["cols"],
+                           z["K"], z["R"])
+    assert rc == 0
+    assert same(k, z["K_bc"]) and same(r, z["R_bc"])
+    # the fixture exercises every scale rule: the stored diagonal, the row
+    # maximum (zeroed diagonal) and 1 (zeroed row)
+    rp, cols, K = z["rowptr"], z["cols"], z["K"]
+    kinds = set()
+    for g in range(len(rp) - 1):
+        s, e = rp[g], rp[g + 1]
+        d = s + int(np.searchsorted(cols[s:e], g))
+        if np.array_equal(z["K_bc"][s:e], K[s:e]):
+            continue
+        scale = z["K_bc"][d]
+        if scale == K[d] and K[d] != 0.0:
+            kinds.add("diag")
+        elif scale == np.abs(K[s:e]).max() and scale != 0.0:
+            kinds.add("rowmax")
+        elif scale == 1.0 and not K[s:e].any():
+            kinds.add("unit")
+    assert kinds == {"diag", "rowmax", "unit"}

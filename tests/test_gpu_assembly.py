@@ -100,3 +100,35 @@ def test_random_ic_mt19937():
     pxg.random_ic(5489, 0.0, 1.0, np.arange(1, 10001), 1, [1], vals)
     bits = int(round(vals[-1] * 9007199254740992.0))
     assert bits == (9981545732273789042 >> 11)
+
+
+def test_device_resident_assembly_matches_host_entry_points():
+    """pxg_assemble_poisson_dev / pxg_assemble_ch_dev (inputs and outputs in
+    device memory) give the same bits as the host-pointer entry points."""
+    import torch
+    ctx, cloud, delta, f = problems.poisson_config1(nx=40, ny=40)
+    u = np.random.RandomState(2).rand(cloud.n)
+    k, r = ctx.assemble_poisson(1.0, u, f_node=f)
+    ud, fd = torch.from_numpy(u).cuda(), torch.from_numpy(np.ascontiguousarray(f)).cuda()
+    kd = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rd = torch.empty(cloud.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.assemble_poisson_dev(1.0, ud.data_ptr(), kd.data_ptr(), rd.data_ptr(), f_ptr=fd.data_ptr())
+    ctx.sync()
+    assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
+    assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
+    ctx.close()
+    ctx, cloud, delta, u0 = problems.cahnhilliard_config3(nx=32, ny=32)
+    u1 = u0.copy()
+    u1[1::2] = 0.05 * np.random.RandomState(4).rand(cloud.n)
+    kappa = 1e-2 / 32 ** 2
+    k, r = ctx.assemble_ch(kappa, 3.0, 1.0, 1e-3, u1, u0)
+    ud, uod = torch.from_numpy(u1).cuda(), torch.from_numpy(u0).cuda()
+    kd = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rd = torch.empty(2 * cloud.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.assemble_ch_dev(kappa, 3.0, 1.0, 1e-3, ud.data_ptr(), uod.data_ptr(), kd.data_ptr(), rd.data_ptr())
+    ctx.sync()
+    assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
+    assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
+    ctx.close()
