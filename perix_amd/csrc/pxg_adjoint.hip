@@ -233,7 +233,9 @@ static int launch_adj_rev(pxg_ctx *ctx, int k, const double *u, const double *wb
 {
     FracState &F = ctx->frac;
     AdjArgs a;
-    a.n = ctx->n; a.k = k; a.ps = F.ps; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.bs = F.break_step.p;
+    // break steps in CSR order (bond_state_csr refreshed the copy at the start)
+    a.n = ctx->n; a.k = k; a.ps = F.ps; a.off = ctx->off.p; a.nbr = ctx->nbr.p;
+    a.bs = F.bsh.p ? F.bs_csr.p : F.break_step.p;
     a.Phat = F.Phat.p; a.PhatT = F.PhatT.p; a.u = u; a.wbar = wbar; a.ew = ew;
     a.lam = lam; a.lamp = lamp; a.Ebar = Ebar; a.cf = cf;
     constexpr int npb = 4 * (64 / AdjCfg<DIM>::S);
@@ -349,7 +351,10 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
     if (nsens) PXG_HIP(hipMemcpyAsync(A.ds.p, sens, sizeof(int) * nsens, hipMemcpyHostToDevice, st));
     std::vector<double> tk(ncks, 0.0);
     double *uK = nullptr;
-    int rc = adj_forward(ctx, u0, v0, dt, K, ckpt, A.cks.p, &uK, tk.data());
+    const int *bs_csr = nullptr;
+    int rc = bond_state_csr(ctx, nullptr, &bs_csr);   // the recorded break steps, CSR order
+    if (rc) return rc;
+    rc = adj_forward(ctx, u0, v0, dt, K, ckpt, A.cks.p, &uK, tk.data());
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss, dim3(1), dim3(1), 0, st, nsens, A.ds.p, dim, comp, uK, A.dJ.p);
     PXG_HIP(hipMemsetAsync(A.lam.p, 0, sizeof(double) * nd, st));

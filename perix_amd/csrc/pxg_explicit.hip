@@ -84,7 +84,8 @@ __global__ void __launch_bounds__(256) k_bc_preset(int count, int dim, const int
 // rounded up to a multiple of 64 so that every component starts aligned.
 __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double *ops,
                            const double *nodeE, double C11, double C12, double C66, double nu,
-                           int plane_stress, long long ps, double *P64, float *P32, double *Phat)
+                           int plane_stress, long long ps, const int *bsh, long long pst, double *P64,
+                           float *P32, double *Phat)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -138,34 +139,58 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
             q[2] = (h12 + h66) * Gxy;
             nc = 3;
         }
+        const long long tp = t + (bsh ? bsh[i] : 0);   // operator layout position
         for (int c = 0; c < nc; c++) {
-            if (P64) P64[(size_t)c * ps + t] = p[c];
-            if (P32) P32[(size_t)c * ps + t] = (float)p[c];
+            if (P64) P64[(size_t)c * pst + tp] = p[c];
+            if (P32) P32[(size_t)c * pst + tp] = (float)p[c];
             if (Phat) Phat[(size_t)c * ps + t] = q[c];
         }
     }
 }
 
-__global__ void k_health_init(long long nb, const unsigned char *seed, unsigned char *health, int *bs)
+// Bond state (health, break steps) lives in the step kernel's stream
+// layout: CSR bond t of node i at t + bsh[i] (bsh null: CSR order).  These
+// per-node kernels convert at the API boundary; they are not on the step path.
+__global__ void k_health_init(int n, const int *off, const int *bsh, const unsigned char *seed,
+                              unsigned char *health, int *bs)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb) return;
-    health[t] = seed[t];
-    bs[t] = seed[t] ? INT_MAX : -1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sh = bsh ? bsh[i] : 0;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        health[t + sh] = seed[t];
+        bs[t + sh] = seed[t] ? INT_MAX : -1;
+    }
 }
 
-__global__ void k_health_from_double(long long nb, const double *h, unsigned char *health)
+__global__ void k_health_from_double(int n, const int *off, const int *bsh, const double *h,
+                                     unsigned char *health)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb) return;
-    health[t] = (h[t] > 0.0) ? 1 : 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sh = bsh ? bsh[i] : 0;
+    for (int t = off[i]; t < off[i + 1]; t++) health[t + sh] = (h[t] > 0.0) ? 1 : 0;
 }
 
-__global__ void k_health_to_double(long long nb, const unsigned char *health, double *h)
+__global__ void k_health_to_double(int n, const int *off, const int *bsh, const unsigned char *health,
+                                   double *h)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb) return;
-    h[t] = health[t] ? 1.0 : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sh = bsh ? bsh[i] : 0;
+    for (int t = off[i]; t < off[i + 1]; t++) h[t] = health[t + sh] ? 1.0 : 0.0;
+}
+
+__global__ void k_bonds_to_csr(int n, const int *off, const int *bsh, const unsigned char *health,
+                               const int *bs, unsigned char *hc, int *bc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sh = bsh[i];
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        if (hc) hc[t] = health[t + sh];
+        if (bc) bc[t] = bs[t + sh];
+    }
 }
 
 __global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
@@ -193,8 +218,27 @@ int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev)
     FracState &F = ctx->frac;
     hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
                        ctx->nop, ctx->off.p, ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress,
-                       F.ps, F.P64.p, (float *)nullptr, (double *)nullptr);
+                       F.ps, F.bsh.p, F.pst, F.P64.p, (float *)nullptr, (double *)nullptr);
     PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break_step)
+{
+    FracState &F = ctx->frac;
+    if (!F.bsh.p) {
+        if (health) *health = F.health.p;
+        if (break_step) *break_step = F.break_step.p;
+        return PXG_OK;
+    }
+    if (health) PXG_HIP(F.health_csr.alloc(ctx->nb));
+    if (break_step) PXG_HIP(F.bs_csr.alloc(ctx->nb));
+    hipLaunchKernelGGL(k_bonds_to_csr, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->off.p,
+                       F.bsh.p, F.health.p, F.break_step.p, health ? F.health_csr.p : nullptr,
+                       break_step ? F.bs_csr.p : nullptr);
+    PXG_HIP(hipGetLastError());
+    if (health) *health = F.health_csr.p;
+    if (break_step) *break_step = F.bs_csr.p;
     return PXG_OK;
 }
 
@@ -240,11 +284,15 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     } else {
         F.nodeE.release(); F.Phat.release(); F.rev.release();
     }
-    if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)F.ps * nc)); F.P32.release(); }
-    else { PXG_HIP(F.P32.alloc((size_t)F.ps * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)F.ps * nc)); }
+    {
+        const int rc = build_step_gather(ctx);   // node tiles and the operator layout
+        if (rc) return rc;
+    }
+    if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)F.pst * nc)); F.P32.release(); }
+    else { PXG_HIP(F.P32.alloc((size_t)F.pst * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)F.pst * nc)); }
     hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->dim, ctx->nop,
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
-                       plane_stress, F.ps, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
+                       plane_stress, F.ps, F.bsh.p, F.pst, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
     if (node_E) {
         const int rc = launch_transpose_ops(ctx);
@@ -252,14 +300,11 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     } else {
         F.PhatT.release();
     }
-    PXG_HIP(F.health.alloc(nb));
-    PXG_HIP(F.break_step.alloc(nb));
-    {
-        const int rc = build_step_gather(ctx);
-        if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_health_init, dim3(gridn(nb, 256)), dim3(256), 0, st, nb, ctx->seed.p, F.health.p,
-                       F.break_step.p);
+    PXG_HIP(F.health.alloc((size_t)F.pst));
+    PXG_HIP(F.break_step.alloc((size_t)F.pst));
+    F.health_csr.release(); F.bs_csr.release();
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, F.bsh.p,
+                       ctx->seed.p, F.health.p, F.break_step.p);
     for (int k = 0; k < 2; k++) PXG_HIP(F.u[k].alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.accel.alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.preset_mask.alloc(ctx->n));
@@ -397,8 +442,8 @@ extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inou
     if (health_inout && !F.health_uploaded) {
         PXG_HIP(ctx->sk.alloc(ctx->nb));
         PXG_HIP(hipMemcpyAsync(ctx->sk.p, health_inout, sizeof(double) * ctx->nb, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_health_from_double, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, ctx->sk.p,
-                           F.health.p);
+        hipLaunchKernelGGL(k_health_from_double, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p,
+                           F.bsh.p, ctx->sk.p, F.health.p);
         F.health_uploaded = 1;
     }
     PXG_HIP(hipMemcpyAsync(F.u[F.cur].p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
@@ -411,8 +456,8 @@ extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inou
     PXG_HIP(hipStreamSynchronize(st));
     if (broke && health_inout) {
         PXG_HIP(ctx->sk.alloc(ctx->nb));
-        hipLaunchKernelGGL(k_health_to_double, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, F.health.p,
-                           ctx->sk.p);
+        hipLaunchKernelGGL(k_health_to_double, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p,
+                           F.bsh.p, F.health.p, ctx->sk.p);
         PXG_HIP(hipMemcpyAsync(health_inout, ctx->sk.p, sizeof(double) * ctx->nb, hipMemcpyDeviceToHost, st));
         PXG_HIP(hipStreamSynchronize(st));
     }
@@ -431,8 +476,8 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     const size_t nd = (size_t)ctx->n * ctx->dim;
     F.cur = 0;
     // a new run starts from the seeded bond health
-    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, ctx->seed.p,
-                       F.health.p, F.break_step.p);
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, F.bsh.p,
+                       ctx->seed.p, F.health.p, F.break_step.p);
     PXG_HIP(hipGetLastError());
     PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
@@ -524,8 +569,14 @@ extern "C" int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold, unsigne
     const size_t nd = (size_t)ctx->n * ctx->dim;
     if (u) PXG_HIP(hipMemcpyAsync(u, F.u[F.cur].p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
     if (uold) PXG_HIP(hipMemcpyAsync(uold, F.u[F.cur ^ 1].p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
-    if (health) PXG_HIP(hipMemcpyAsync(health, F.health.p, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
-    if (break_step) PXG_HIP(hipMemcpyAsync(break_step, F.break_step.p, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    const unsigned char *hc = nullptr;
+    const int *bc = nullptr;
+    if (health || break_step) {
+        const int rc = bond_state_csr(ctx, health ? &hc : nullptr, break_step ? &bc : nullptr);
+        if (rc) return rc;
+    }
+    if (health) PXG_HIP(hipMemcpyAsync(health, hc, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    if (break_step) PXG_HIP(hipMemcpyAsync(break_step, bc, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
     return PXG_OK;
 }

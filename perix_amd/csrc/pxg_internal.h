@@ -112,7 +112,12 @@ struct FracState {
     int tension_only = 1, damage_on = 1;
     // per-bond force operators P (CSR bond order): 3 (2D) or 6 (3D) components
     // component-major: component c of bond t at [c*ps + t]
-    long long ps = 0;        // component stride: nb rounded up to a multiple of 64
+    long long ps = 0;        // CSR-order component stride: nb rounded up to a multiple of 64
+    // P64 / P32 (and gslot) use the tiled bond layout of the step blocks when
+    // the step kernel stages rows: CSR bond t of node i at t + bsh[i], with
+    // component stride pst; without staging bsh is empty and pst = ps
+    long long pst = 0;
+    dbuf<int> bsh;
     dbuf<double> P64;
     dbuf<float> P32;
     // LDS neighbour staging of the step kernel: step block b owns the npb
@@ -127,8 +132,10 @@ struct FracState {
     dbuf<int> ulist;
     dbuf<unsigned short> gslot;
     dbuf<unsigned char> bovf;
-    dbuf<unsigned char> health;   // nb
-    dbuf<int> break_step;         // nb (recorded break step, INT_MAX intact)
+    dbuf<unsigned char> health;   // pst, stream layout (see bsh)
+    dbuf<int> break_step;         // pst: recorded break step, INT_MAX intact
+    dbuf<unsigned char> health_csr;   // nb: CSR-order copies (tiled layout only)
+    dbuf<int> bs_csr;
     dbuf<double> u[2];            // ping-pong displacement buffers (N*dim)
     int cur = 0;                  // u[cur] = u^n, u[cur^1] = u^{n-1}
     dbuf<unsigned char> preset_mask;   // N: node is a preset target
@@ -177,6 +184,9 @@ int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uol
 int launch_guard_final(pxg_ctx *ctx, int step);
 int build_step_gather(pxg_ctx *ctx);
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
+// device pointers to the bond health / break steps in CSR order (a gathered
+// copy when the stream layout is tiled); either output may be NULL
+int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break_step);
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
 int launch_transpose_ops(pxg_ctx *ctx);
 

@@ -52,8 +52,11 @@ namespace pxg {
 struct StepArgs {
     int n;
     int nblk;                  // logical blocks (grid is rounded up to 8*)
-    long long ps;              // component stride of the operator arrays
+    long long pst;             // component stride of the operator arrays
     const int *off;
+    const int *bsh;            // tiled bond layout: the per-bond streams (operators,
+                               // slots, health, break steps) of CSR bond t of node i
+                               // sit at t + bsh[i] (null: CSR order)
     const int *nbr;
     const int *perm;           // LDS staging: nodes of step block b at perm[b*npb ..]
     const int *roff;           // rows of block b: ulist[roff[b] .. roff[b+1])
@@ -291,8 +294,11 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     const bool nvalid = gi < a.n;
     const int i = (STAGE && nvalid) ? a.perm[gi] : gi;
     const bool govf = STAGE && a.bovf[blk];
-    int b0 = 0, b1 = 0;
+    // [b0, b1): the node's bonds in the stream layout (tiled when staged;
+    // nbr stays in CSR order, at t - sh)
+    int b0 = 0, b1 = 0, sh = 0;
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
+    if (STAGE && nvalid) { sh = a.bsh[i]; b0 += sh; b1 += sh; }
     // BC program of this node (fused path): own entry, entries it mirrors
     int bq = -1, bp0 = 0, bp1 = 0;
     if constexpr (FUSED) {
@@ -312,8 +318,8 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     const double *P64c[F32 ? 1 : NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        if constexpr (F32) P32c[c] = a.P32 + c * a.ps;
-        else P64c[c] = a.P64 + c * a.ps;
+        if constexpr (F32) P32c[c] = a.P32 + c * a.pst;
+        else P64c[c] = a.P64 + c * a.pst;
     }
     auto load_group = [&](unsigned t0, GD &d) {
         if constexpr (STAGE) vload(a.gslot + t0, d.sv);
@@ -343,16 +349,33 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     if (!STAGE && tfirst < b1) load_group((unsigned)tfirst, d);
 
     if constexpr (STAGE) {
-        // stage the block's neighbour rows as LDS records {x_j, u_j}; the
-        // rows are sorted node ids, so consecutive lanes read runs of
-        // consecutive nodes (coalesced)
+        // stage the block's neighbour rows as LDS records {x_j, u_j}: a row
+        // per thread, two rows per thread and round, and every load of a
+        // round is issued before its LDS writes (the row ids are sorted node
+        // ids, so neighbouring lanes read neighbouring nodes)
         const int r0 = a.roff[blk];
-        const int len = (a.roff[blk + 1] - r0) * (2 * DIM);
+        const int nrows = a.roff[blk + 1] - r0;
         const int *rows = a.ulist + r0;
-        for (int e = threadIdx.x; e < len; e += 256) {
-            const int q = e / (2 * DIM), part = e - q * (2 * DIM);
-            const size_t j = (size_t)rows[q] * DIM;
-            s_rec[e] = (part < DIM) ? (live_dmg ? a.xp[j + part] : 0.0) : a.u[j + part - DIM];
+        for (int q0 = 0; q0 < nrows; q0 += 512) {
+            const int qa = q0 + (int)threadIdx.x, qb = qa + 256;
+            const bool ha = qa < nrows, hb = qb < nrows;
+            const size_t ja = ha ? (size_t)rows[qa] * DIM : 0, jb = hb ? (size_t)rows[qb] * DIM : 0;
+            double ra[2 * DIM], rb[2 * DIM];
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                ra[c] = (ha && live_dmg) ? a.xp[ja + c] : 0.0;
+                ra[DIM + c] = ha ? a.u[ja + c] : 0.0;
+                rb[c] = (hb && live_dmg) ? a.xp[jb + c] : 0.0;
+                rb[DIM + c] = hb ? a.u[jb + c] : 0.0;
+            }
+            if (ha) {
+#pragma unroll
+                for (int c = 0; c < 2 * DIM; c++) s_rec[qa * (2 * DIM) + c] = ra[c];
+            }
+            if (hb) {
+#pragma unroll
+                for (int c = 0; c < 2 * DIM; c++) s_rec[qb * (2 * DIM) + c] = rb[c];
+            }
         }
         __syncthreads();
     }
@@ -396,7 +419,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                             uj[c] = rec[DIM + c];
                         }
                     } else {
-                        const size_t q = (size_t)(a.nbr[t] - 1) * DIM;
+                        const size_t q = (size_t)(a.nbr[t - sh] - 1) * DIM;
 #pragma unroll
                         for (int c = 0; c < DIM; c++) {
                             uj[c] = a.u[q + c];
@@ -646,7 +669,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
 {
     FracState &F = ctx->frac;
     StepArgs a;
-    a.n = ctx->n; a.ps = F.ps; a.off = ctx->off.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
     a.perm = F.perm.p; a.roff = F.roff.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
     a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
@@ -716,7 +739,9 @@ int launch_guard_final(pxg_ctx *ctx, int step)
 // rows per node where a 32-node row would need 31.  Per block the sorted
 // distinct neighbours form the row list ulist[roff[b] ..], and every bond
 // records the LDS slot of its neighbour.  Blocks whose neighbourhood exceeds
-// the slot budget gather from global memory instead (bovf).
+// the slot budget gather from global memory instead (bovf).  The per-bond
+// operator and slot streams are stored in the same tile order (bsh), so each
+// block reads one contiguous range of them.
 int build_step_gather(pxg_ctx *ctx)
 {
     FracState &F = ctx->frac;
@@ -727,7 +752,9 @@ int build_step_gather(pxg_ctx *ctx)
     F.npb = npb;
     if (!staged) {
         F.gcap = 0;
+        F.pst = F.ps;
         F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
+        F.bsh.release();
         return PXG_OK;
     }
     constexpr int TX = 2, TY = 4, TZ = 4;
@@ -760,7 +787,25 @@ int build_step_gather(pxg_ctx *ctx)
     });
     std::vector<int> perm((size_t)n);
     for (int q = 0; q < n; q++) perm[(size_t)q] = keys[(size_t)q].id;
-    std::vector<unsigned short> slot((size_t)(nb > 0 ? nb : 1), 0);
+    // tiled bond layout of the per-bond streams: the nodes' bond runs in
+    // perm order, so a block's streams are one contiguous range, and every
+    // run starts on a 128-byte line of the FP32 operator arrays (32 bonds;
+    // one round of the node's 8 lanes reads exactly one line per array).
+    // Runs that shared a line with the next node made the line be fetched
+    // twice, once per node, because the many streams in flight evict it
+    // from L2 in between.
+    constexpr int kBondAlign = 32;
+    std::vector<int> bsh((size_t)n);
+    long long pos = 0;
+    for (int q = 0; q < n; q++) {
+        const int i = perm[(size_t)q];
+        pos = (pos + kBondAlign - 1) / kBondAlign * kBondAlign;
+        bsh[(size_t)i] = (int)(pos - off[(size_t)i]);
+        pos += off[(size_t)i + 1] - off[(size_t)i];
+    }
+    PXG_CHECK(pos < INT_MAX - 64, PXG_ERR_ARG, "build_step_gather: bond layout exceeds 2^31 entries");
+    F.pst = (pos + 63) / 64 * 64;
+    std::vector<unsigned short> slot((size_t)std::max(F.pst, 1LL), 0);
     std::vector<unsigned char> ovf((size_t)nblk, 0);
     std::vector<std::vector<int>> brows((size_t)nblk);
     auto work = [&](int first, int step) {
@@ -778,7 +823,8 @@ int build_step_gather(pxg_ctx *ctx)
             for (int q = b * npb; q < q1; q++) {
                 const int i = perm[(size_t)q];
                 for (int t = off[(size_t)i]; t < off[(size_t)i + 1]; t++)
-                    slot[(size_t)t] = (unsigned short)(std::lower_bound(ids.begin(), ids.end(), nbr[(size_t)t]) - ids.begin());
+                    slot[(size_t)(t + bsh[(size_t)i])] =
+                        (unsigned short)(std::lower_bound(ids.begin(), ids.end(), nbr[(size_t)t]) - ids.begin());
             }
             std::vector<int> &rr = brows[(size_t)b];
             rr.resize(ids.size());
@@ -803,7 +849,9 @@ int build_step_gather(pxg_ctx *ctx)
     PXG_HIP(F.ulist.alloc(ulist.size()));
     PXG_HIP(F.gslot.alloc(slot.size()));
     PXG_HIP(F.bovf.alloc(ovf.size()));
+    PXG_HIP(F.bsh.alloc(bsh.size()));
     hipStream_t st = ctx->stream;
+    PXG_HIP(hipMemcpyAsync(F.bsh.p, bsh.data(), sizeof(int) * bsh.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.perm.p, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.roff.p, roff.data(), sizeof(int) * roff.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.ulist.p, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice, st));

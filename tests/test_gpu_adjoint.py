@@ -65,3 +65,43 @@ def test_adjoint_vs_fd_and_oracle(oracle):
     assert abs(J - Jr) <= 1e-12 * abs(Jr)
     assert rel <= 1e-10
     ctx.close()
+
+
+def test_adjoint_3d_vs_fd_and_oracle(oracle):
+    """The 3D block path (tiled bond layout, break steps gathered back to CSR
+    order for the reverse sweep) against finite differences and the oracle."""
+    rs = np.random.RandomState(5)
+    hi = (0.016, 0.008, 0.004)
+    kw = dict(hi=hi, sc=0.003, cracks=[(0.008, 0.0, 0.008, 0.002)],
+              box=(0.004, 0.008, 1.0, -1.0, 1.0, -1.0), nsteps=40)
+    probe = problems.explicit_plate(3, 16, 8, 4, **kw)
+    n, nbulk = probe.cloud.n, probe.cloud.nbulk
+    probe.ctx.close()
+    E = problems.E_STEEL * (1.0 + 0.05 * rs.standard_normal(n))
+    s = problems.explicit_plate(3, 16, 8, 4, node_E=E, **kw)
+    sens = rs.choice(nbulk, 16, replace=False).astype(np.int32)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    ctx.frac_run(s.dt, s.total)
+    _, _, h, bs = ctx.frac_state()
+    print("3D adjoint case: %d broken bonds" % int((h == 0).sum()))
+    J, g = ctx.frac_adjoint(s.u0, s.dt, s.nsteps, sens, 1, ckpt=8)
+    worst = 0.0
+    for d in range(4):
+        v = rs.standard_normal(n) * E
+        Jp = ctx.frac_loss(s.u0, s.dt, s.nsteps, sens, 1, node_E=E + 1e-6 * v)
+        Jm = ctx.frac_loss(s.u0, s.dt, s.nsteps, sens, 1, node_E=E - 1e-6 * v)
+        fd = (Jp - Jm) / 2e-6
+        worst = max(worst, abs(fd - float(g @ v)) / max(abs(float(g @ v)), 1e-300))
+    print("3D adjoint vs FD worst rel %.3e" % worst)
+    assert worst <= 1e-4
+    cl = s.cloud
+    m = oracle.Mesh(oracle.make_grid(3, cl.grid.nx, cl.grid.ny, cl.grid.nz, hi=cl.grid.hi))
+    ops, _ = ctx.operators()
+    opc = oracle.frac_opcache(m, ops)
+    p = oracle.frac_params(3, problems.E_STEEL, problems.NU, problems.RHO, s.params.sc)
+    Jr, gr = oracle.frac_adjoint(m, p, opc, oracle.make_bcs(s.bcs), s.dt, s.nsteps, s.u0, bs, E, sens, 1)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print("3D adjoint vs oracle rel-L2 %.3e" % rel)
+    assert abs(J - Jr) <= 1e-12 * abs(Jr) and rel <= 1e-10
+    ctx.close()
