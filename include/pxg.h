@@ -228,9 +228,14 @@ int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout,
 int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, double dt);
 int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_max,
                  int *steps_done);
+/* Stream layout of the step kernel (no reference counterpart; reporting):
+ * info[0] bond slots per operator component, info[1] 1 when the per-bond
+ * streams follow the tile order of the step blocks, info[2] shared FP32
+ * operator runs, info[3] nodes reading a shared run (FP32C). */
+int pxg_frac_layout(pxg_ctx *ctx, long long *info);
 /* Downloads u^n, u^{n-1} (n*dim), health (nb, 0/1 bytes) and break steps
- * (nb ints: -1 seeded, 0 broken at t=0, k during step k, INT32_MAX intact);
- * any pointer may be NULL. */
+ * (nb ints: -1 seeded, 0 broken at t=0, k during step k, INT32_MAX intact)
+ * in CSR bond order; any pointer may be NULL. */
 int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold,
                        unsigned char *health, int *break_step);
 /* Device pointer of the displacement u^n and u^{n-1} (valid until the next

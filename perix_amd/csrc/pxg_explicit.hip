@@ -13,9 +13,12 @@
 // This file holds k_bc_preset, the per-bond operator build, the health
 // bookkeeping and the host side of the explicit loop.
 #include "pxg_internal.h"
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <unordered_map>
+#include <vector>
 
 namespace pxg {
 
@@ -193,6 +196,53 @@ __global__ void k_bonds_to_csr(int n, const int *off, const int *bsh, const unsi
     }
 }
 
+// Shared FP32 operator runs.  The order pattern of a family: a hash of its
+// length and of its neighbour offsets in family order, in units of the grid
+// spacing (rounded), so nodes on a regular lattice whose families visit the
+// same offsets in the same order share a hash.
+__global__ void k_pattern_hash(int n, int dim, const int *off, const int *nbr, const double *xp, double inv_h,
+                               unsigned long long *hash)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long h = 1469598103934665603ull ^ (unsigned long long)(off[i + 1] - off[i]);
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        const int j = nbr[t] - 1;
+        for (int c = 0; c < dim; c++) {
+            const long long q = llrint((xp[(size_t)j * dim + c] - xp[(size_t)i * dim + c]) * inv_h);
+            h = (h ^ (unsigned long long)(q & 0xffff)) * 1099511628211ull;
+        }
+    }
+    hash[i] = h;
+}
+
+// A node with candidate representative r shares r's operator run when the
+// families have the same length and every FP64 operator of i is within
+// tol x (the largest |operator| of i) of r's, bond by bond in family order
+// (NaN never passes).  pofs[i] then moves i's operator loads onto r's run.
+__global__ void k_class_check(int n, int nc, const int *off, const int *bsh, const int *rep, long long ps,
+                              const double *P, double tol, int *pofs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pofs[i] = 0;
+    const int r = rep[i];
+    if (r < 0 || r == i) return;
+    const int len = off[i + 1] - off[i];
+    if (off[r + 1] - off[r] != len) return;
+    double mx = 0.0;
+    for (int c = 0; c < nc; c++)
+        for (int k = 0; k < len; k++) {
+            const double a = fabs(P[(size_t)c * ps + off[i] + k]);
+            mx = a > mx ? a : mx;
+        }
+    const double lim = tol * mx;
+    for (int c = 0; c < nc; c++)
+        for (int k = 0; k < len; k++)
+            if (!(fabs(P[(size_t)c * ps + off[i] + k] - P[(size_t)c * ps + off[r] + k]) <= lim)) return;
+    pofs[i] = (off[r] + bsh[r]) - (off[i] + bsh[i]);
+}
+
 __global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,6 +289,66 @@ int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break
     PXG_HIP(hipGetLastError());
     if (health) *health = F.health_csr.p;
     if (break_step) *break_step = F.bs_csr.p;
+    return PXG_OK;
+}
+
+// FP32C with the tiled stream layout: nodes of one family order pattern
+// whose operators agree to kShareTol read one representative's FP32
+// operator run (F.pofs).  On a structured lattice most interior nodes fall
+// into a few hundred patterns, and their operators differ from the
+// representative's only by the rounding of the coordinates (about 1e-14
+// relative), far below the FP32 rounding of the operators themselves.
+constexpr double kShareTol = 1e-12;
+
+static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
+{
+    FracState &F = ctx->frac;
+    F.pofs.release();
+    F.nshared = 0;
+    F.nclasses = 0;
+    if (!F.bsh.p || F.precision != PXG_FP32C) return PXG_OK;
+    const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
+    hipStream_t st = ctx->stream;
+    dbuf<double> P;                     // FP64 operators, CSR order
+    PXG_HIP(P.alloc((size_t)F.ps * nc));
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(n, 128)), dim3(128), 0, st, n, ctx->dim, ctx->nop, ctx->off.p,
+                       ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress, F.ps, (const int *)nullptr,
+                       F.ps, P.p, (float *)nullptr, (double *)nullptr);
+    PXG_HIP(hipGetLastError());
+    dbuf<unsigned long long> hash;
+    PXG_HIP(hash.alloc(n));
+    const double h = std::max(ctx->dxyz[0], std::max(ctx->dxyz[1], ctx->dxyz[2]));
+    PXG_CHECK(h > 0.0, PXG_ERR_STATE, "build_op_classes: zero grid spacing");
+    hipLaunchKernelGGL(k_pattern_hash, dim3(gridn(n, 128)), dim3(128), 0, st, n, ctx->dim, ctx->off.p,
+                       ctx->nbr.p, ctx->xp.p, 1.0 / h, hash.p);
+    PXG_HIP(hipGetLastError());
+    std::vector<unsigned long long> hh((size_t)n);
+    PXG_HIP(hipMemcpyAsync(hh.data(), hash.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    // candidate representative: the first node of each pattern
+    std::unordered_map<unsigned long long, int> first;
+    first.reserve((size_t)n / 8 + 16);
+    std::vector<int> rep((size_t)n, -1);
+    for (int i = 0; i < n; i++) {
+        auto it = first.emplace(hh[(size_t)i], i).first;
+        rep[(size_t)i] = it->second;
+    }
+    dbuf<int> drep;
+    PXG_HIP(drep.alloc(n));
+    PXG_HIP(F.pofs.alloc(n));
+    PXG_HIP(hipMemcpyAsync(drep.p, rep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_class_check, dim3(gridn(n, 128)), dim3(128), 0, st, n, nc, ctx->off.p, F.bsh.p, drep.p,
+                       F.ps, P.p, kShareTol, F.pofs.p);
+    PXG_HIP(hipGetLastError());
+    std::vector<int> pofs((size_t)n);
+    PXG_HIP(hipMemcpyAsync(pofs.data(), F.pofs.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    std::vector<char> used((size_t)n, 0);
+    for (int i = 0; i < n; i++)
+        if (pofs[(size_t)i] != 0) {
+            F.nshared++;
+            if (!used[(size_t)rep[(size_t)i]]) { used[(size_t)rep[(size_t)i]] = 1; F.nclasses++; }
+        }
     return PXG_OK;
 }
 
@@ -294,6 +404,10 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
                        plane_stress, F.ps, F.bsh.p, F.pst, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
+    {
+        const int rc = build_op_classes(ctx, node_E ? F.nodeE.p : nullptr);
+        if (rc) return rc;
+    }
     if (node_E) {
         const int rc = launch_transpose_ops(ctx);
         if (rc) return rc;
@@ -578,6 +692,17 @@ extern "C" int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold, unsigne
     if (health) PXG_HIP(hipMemcpyAsync(health, hc, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
     if (break_step) PXG_HIP(hipMemcpyAsync(break_step, bc, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_layout(pxg_ctx *ctx, long long *info)
+{
+    PXG_CHECK(ctx && ctx->frac.ready && info, PXG_ERR_ARG, "pxg_frac_layout: bad arguments");
+    const FracState &F = ctx->frac;
+    info[0] = F.pst;
+    info[1] = F.bsh.p ? 1 : 0;
+    info[2] = F.nclasses;
+    info[3] = F.nshared;
     return PXG_OK;
 }
 

@@ -118,6 +118,11 @@ struct FracState {
     // component stride pst; without staging bsh is empty and pst = ps
     long long pst = 0;
     dbuf<int> bsh;
+    // FP32C shared operator runs: node i reads its FP32 operators at bond
+    // position + pofs[i] (the run of a representative node whose FP64
+    // operators agree to 1e-12 relative); empty when nothing is shared
+    dbuf<int> pofs;
+    long long nshared = 0, nclasses = 0;
     dbuf<double> P64;
     dbuf<float> P32;
     // LDS neighbour staging of the step kernel: step block b owns the npb

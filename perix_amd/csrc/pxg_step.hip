@@ -57,6 +57,8 @@ struct StepArgs {
     const int *bsh;            // tiled bond layout: the per-bond streams (operators,
                                // slots, health, break steps) of CSR bond t of node i
                                // sit at t + bsh[i] (null: CSR order)
+    const int *pofs;           // FP32C: node i's FP32 operators at stream position
+                               // + pofs[i] (a shared run; null: none shared)
     const int *nbr;
     const int *perm;           // LDS staging: nodes of step block b at perm[b*npb ..]
     const int *roff;           // rows of block b: ulist[roff[b] .. roff[b+1])
@@ -299,6 +301,10 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     int b0 = 0, b1 = 0, sh = 0;
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
     if (STAGE && nvalid) { sh = a.bsh[i]; b0 += sh; b1 += sh; }
+    int po = 0;                // operator run offset (shared runs, FP32C)
+    if constexpr (STAGE && F32) {
+        if (nvalid && a.pofs) po = a.pofs[i];
+    }
     // BC program of this node (fused path): own entry, entries it mirrors
     int bq = -1, bp0 = 0, bp1 = 0;
     if constexpr (FUSED) {
@@ -337,7 +343,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            if constexpr (F32) vload(P32c[c] + t0, d.p32[c]);
+            if constexpr (F32) vload(P32c[c] + (unsigned)((int)t0 + po), d.p32[c]);
             else vload(P64c[c] + t0, d.p64[c]);
         }
     };
@@ -669,7 +675,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
 {
     FracState &F = ctx->frac;
     StepArgs a;
-    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.pofs = F.pofs.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
     a.perm = F.perm.p; a.roff = F.roff.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
     a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;

@@ -53,7 +53,7 @@ SYMBOLS = [
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
     "pxg_csr_apply_bcs_dev", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
-    "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
+    "pxg_frac_layout", "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_loss",
 ]
 
@@ -332,6 +332,12 @@ class Context:
         _check(load().pxg_frac_accel(self.h, _dp(np.ascontiguousarray(u, float)), _dp(health),
                                      _dp(a), C.byref(ab)))
         return a, ab.value
+
+    def frac_layout(self):
+        """(bond slots per component, tiled, shared runs, nodes on shared runs)."""
+        info = (C.c_longlong * 4)()
+        _check(load().pxg_frac_layout(self.h, info))
+        return tuple(int(v) for v in info)
 
     def frac_init(self, u0, dt, v0=None):
         v = None if v0 is None else np.ascontiguousarray(v0, float)
