@@ -86,7 +86,7 @@ def timed_run(ctx, s, steps, warmup, profile=False):
 
 def plate_case(problems, pxg, precision, steps):
     t0 = time.perf_counter()
-    s = problems.plate_config2(precision=precision, nsteps=steps)
+    s = problems.plate_400x200(precision=precision, nsteps=steps)
     s.ctx.sync()
     return s, time.perf_counter() - t0
 
@@ -165,7 +165,7 @@ def cpu_reference_baseline(problems, n_nodes, dt, budget_s=12.0):
 
 
 def adjoint_case(problems, pxg, nsteps=500, nsens=64, ndirs=20, seed=2024):
-    """Config 4: 200 x 100 plate (dx = 1 mm), E_i = 191 GPa (1 + 0.05 N(0,1)),
+    """Adjoint of the 200 x 100 plate (dx = 1 mm), E_i = 191 GPa (1 + 0.05 N(0,1)),
     J = sum over 64 sensors of u_y^2 at step 500, checkpoints every 50 steps."""
     rs = np.random.RandomState(seed)
     probe = problems.explicit_plate(2, 200, 100, nsteps=nsteps)
@@ -209,13 +209,55 @@ def adjoint_case(problems, pxg, nsteps=500, nsens=64, ndirs=20, seed=2024):
             "fd_max_rel_err_20dirs": worst, "J": J, "J_forward": J0}
 
 
+def block_adjoint_case(problems, pxg, nsteps=200, nsens=64, ckpt=20, seed=2025):
+    """Adjoint of the 160 x 80 x 40 block over 200 steps, J = sum over 64 sensors
+    of u_y^2, gradient w.r.t. the uniform body force (3 scalars) from one
+    reverse sweep with checkpoints every 20 steps (2 x 10 states of 13 MB),
+    checked per component against central differences."""
+    rs = np.random.RandomState(seed)
+    probe = problems.block_160x80x40(precision=pxg.FP64, nsteps=nsteps)
+    n, nbulk = probe.cloud.n, probe.cloud.nbulk
+    probe.ctx.close()
+    E = np.full(n, problems.E_STEEL)
+    s = problems.block_160x80x40(precision=pxg.FP64, nsteps=nsteps, node_E=E)
+    sens = rs.choice(nbulk, nsens, replace=False).astype(np.int32)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    ctx.frac_run(s.dt, s.total)          # records break steps
+    ctx.sync()
+    ctx.frac_adjoint_ex(s.u0, s.dt, nsteps, sens, 1, ckpt=ckpt, grad_E=False)   # warm
+    t0 = time.perf_counter()
+    ctx.frac_init(s.u0, s.dt)
+    ctx.frac_run(s.dt, s.total)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, _, gb = ctx.frac_adjoint_ex(s.u0, s.dt, nsteps, sens, 1, ckpt=ckpt, grad_E=False)
+    t_adj = time.perf_counter() - t0
+    worst = 0.0
+    hb = 1e3 * problems.RHO
+    for a in range(3):
+        b = np.zeros(3); b[a] = hb
+        ctx.frac_set_body(b)
+        jp = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1)
+        ctx.frac_set_body(-b)
+        jm = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1)
+        fd = (jp - jm) / (2 * hb)
+        worst = max(worst, abs(fd - gb[a]) / max(abs(fd), 1e-300))
+    ctx.close()
+    return {"workload": "160x80x40 block, 200 steps, 64 sensors, dJ/d(body force), checkpoint every 20",
+            "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+            "adjoint_over_forward": t_adj / t_fwd, "dJ_db": [float(v) for v in gb],
+            "fd_max_rel_err_3comps": worst, "J": J}
+
+
 def block_case(problems, pxg, steps=100):
-    """Config 5: 3D block 160 x 80 x 40 (556,800 PD nodes), FP32C and FP64."""
+    """3D block 160 x 80 x 40 (556,800 PD nodes), FP32C and FP64."""
     out = {}
     states = {}
     for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
         t0 = time.perf_counter()
-        s = problems.block_config5(precision=prec, nsteps=steps)
+        s = problems.block_160x80x40(precision=prec, nsteps=steps)
         s.ctx.sync()
         setup = time.perf_counter() - t0
         sec, done, ms, launches = timed_run(s.ctx, s, steps, 5, profile=True)
@@ -246,13 +288,13 @@ def block_case(problems, pxg, steps=100):
 
 
 def assembly_case(problems, pxg, reps=20):
-    """Config 1 (Poisson, 100x100) and config 3 (Cahn-Hilliard, 256x256, one
+    """Poisson (100x100) and Cahn-Hilliard (256x256, one
     Picard iteration) CSR assembly on the device with inputs and outputs
     resident in HBM: nnz/s, beside the reference's own CPU assembly (oracle/_ref)
     of the same system when that build is present."""
     import torch
     out = {}
-    ctx, cloud, delta, f = problems.poisson_config1()
+    ctx, cloud, delta, f = problems.poisson_square()
     n, nnz = cloud.n, ctx.nnz
     X = cloud.coords.reshape(-1, 3)
     u_h = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])
@@ -274,7 +316,7 @@ def assembly_case(problems, pxg, reps=20):
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
     ctx.close()
-    ctx, cloud, delta, u0 = problems.cahnhilliard_config3()
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square()
     n, nnz = cloud.n, ctx.nnz
     u1 = u0.copy()
     u1[1::2] = 0.01 * np.random.RandomState(3).rand(n)
@@ -421,7 +463,8 @@ def main():
         print(json.dumps({"block3d": block_case(problems, pxg, steps=K)}))
         return 0
     if args.only_adjoint:
-        print(json.dumps({"adjoint": adjoint_case(problems, pxg)}))
+        print(json.dumps({"adjoint": adjoint_case(problems, pxg),
+                          "adjoint_block3d_body_force": block_adjoint_case(problems, pxg)}))
         return 0
     if args.only_assembly:
         print(json.dumps({"assembly": assembly_case(problems, pxg)}))
@@ -487,7 +530,7 @@ def main():
     }
     if not args.no_cpu and not args.only_plate:
         log("cpu baseline")
-        s0 = problems.plate_config2(precision=pxg.FP64, nsteps=K)
+        s0 = problems.plate_400x200(precision=pxg.FP64, nsteps=K)
         dt = s0.dt
         s0.ctx.close()
         try:
@@ -498,9 +541,11 @@ def main():
     if not args.no_extra and not args.only_plate:
         log("adjoint (200x100, 500 steps)")
         out["adjoint"] = adjoint_case(problems, pxg)
+        log("adjoint (160x80x40 block, body force, 200 steps)")
+        out["adjoint_block3d_body_force"] = block_adjoint_case(problems, pxg)
         log("3D block (160x80x40)")
         out["block3d"] = block_case(problems, pxg)
-        log("assembly (configs 1 and 3)")
+        log("assembly (Poisson 100x100, Cahn-Hilliard 256x256)")
         out["assembly"] = assembly_case(problems, pxg)
     print(json.dumps(out))
     return 0

@@ -228,6 +228,13 @@ int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout,
 int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, double dt);
 int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_max,
                  int *steps_done);
+/* Nodal output projections of the current state (ExplicitPDDOFracElement::
+ * computeNodalProjection, ExplicitPDDOFracElement.cpp:314-405, called by the
+ * reference's output writer): which = 0 damage, 1 von Mises (n values),
+ * 2 stress, 3 strain (9 per node, row-major 3x3).  out: host (n or 9n
+ * doubles); _dev: a device pointer of that size. */
+int pxg_frac_project(pxg_ctx *ctx, int which, double *out);
+int pxg_frac_project_dev(pxg_ctx *ctx, int which, double *out_dev);
 /* Stream layout of the step kernel (no reference counterpart; reporting):
  * info[0] bond slots per operator component, info[1] 1 when the per-bond
  * streams follow the tile order of the step blocks, info[2] shared FP32
@@ -256,6 +263,16 @@ int pxg_frac_kernel_time(pxg_ctx *ctx, double *ms_total, int *launches);
 int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0,
                      double dt, int nsteps, int ckpt, int nsens,
                      const int *sens, int comp, double *J_out, double *grad);
+/* The same sweep with the gradient with respect to the uniform body force b
+ * of pxg_frac_params as well: grad_b[dim] = dJ/db (b enters every
+ * central-difference value and u^{-1} through a = (L + b)/rho).  grad (n)
+ * and grad_b may each be NULL. */
+int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double *v0,
+                        double dt, int nsteps, int ckpt, int nsens,
+                        const int *sens, int comp, double *J_out, double *grad,
+                        double *grad_b);
+/* Replaces the body force b (dim values) of pxg_frac_setup. */
+int pxg_frac_set_body(pxg_ctx *ctx, const double *b);
 /* Frozen-health forward loss of the same map (finite-difference checks);
  * node_E (optional, host, n) replaces the per-node modulus for this call. */
 int pxg_frac_loss(pxg_ctx *ctx, const double *u0, const double *v0, double dt,

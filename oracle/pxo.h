@@ -136,6 +136,14 @@ int pxo_bc_rows(int n, int ndof, const double *coords, const int *mirror,
                 int nbc, const pxo_bc *bcs, double t, const double *U,
                 const int *rowptr, const int *cols, double *K, double *R);
 
+/* Nodal output projections of the explicit element
+ * (ExplicitPDDOFracElement::computeNodalProjection): which = 0 damage,
+ * 1 von Mises (1 value per node), 2 stress, 3 strain (9 per node, row-major
+ * 3x3).  opc is the op cache (nops per bond), health per bond (0/1). */
+void pxo_frac_project(int n, int dim, int plane_stress, double C12, double C66,
+                      const int *off, const int *nbr, const double *opc,
+                      const double *health, const double *u, int which, double *out);
+
 /* ---- a19: central-difference driver ------------------------------------- */
 /* Step schedule of TimeStepping::solveExplicitDynamics: returns the number of
  * steps; fills dts[k], ts[k] (tNew) when non-NULL (capacity cap). */
@@ -161,6 +169,8 @@ void pxo_random_ic(uint64_t seed, double lo, double hi, int nnodes,
  * driver when no step is clamped).
  * Loss J = sum_s u^N[sens_s*dim+comp]^2.  Health frozen from break_step.
  * node_E = per-node Young's modulus (required; C constants of node i follow E_i).
+ * grad_b (optional, dim values) receives dJ/db for the uniform body force b:
+ * b enters every central-difference value and u^{-1} through a = (L + b)/rho.
  * grad_e receives dJ/dE_i. Returns J. */
 double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
                         const int *off, const int *nbr, const double *opc,
@@ -168,7 +178,7 @@ double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
                         double dt, int nsteps, const double *u0,
                         const double *v0, const int *break_step,
                         const double *node_E, int nsens, const int *sens,
-                        int comp, double *grad_e, int nthreads);
+                        int comp, double *grad_e, double *grad_b, int nthreads);
 /* Forward of the same frozen-health map (for finite differences). */
 double pxo_frac_loss_frozen(int n, const pxo_frac *p, const double *coords,
                             const int *off, const int *nbr, const double *opc,

@@ -255,6 +255,21 @@ def bc_preset(m, ndof, bcs, u, t):
                         C.c_double(t))
 
 
+PROJECTIONS = ("damage", "vonMises", "stress", "strain")
+
+
+def frac_project(m, p, opc, health, u, name):
+    """Nodal projection `name` of the explicit element (n x 1 or n x 9)."""
+    which = PROJECTIONS.index(name)
+    nc = 9 if which >= 2 else 1
+    out = np.zeros(m.n * nc)
+    lib().pxo_frac_project(m.n, m.dim, int(p.plane_stress), C.c_double(p.C12), C.c_double(p.C66),
+                           ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(opc)),
+                           dp(np.ascontiguousarray(health, float)), dp(np.ascontiguousarray(u, float)),
+                           which, dp(out))
+    return out.reshape(m.n, nc)
+
+
 def bc_rows(m, ndof, bcs, t, U, rowptr, cols, K, R):
     """BC row elimination on copies of K / R; returns (rc, K', R')."""
     arr, keep = bcs
@@ -288,9 +303,11 @@ def explicit_run(m, p, opc, bcs, dt, total, u, uold, health, v0=None, node_E=Non
 
 
 def frac_adjoint(m, p, opc, bcs, dt, nsteps, u0, break_step, node_E, sens, comp, v0=None,
-                 nthreads=0):
+                 nthreads=0, want_b=False):
+    """(J, dJ/dE) or, with want_b, (J, dJ/dE, dJ/db)."""
     arr, keep = bcs
     g = np.zeros(m.n)
+    gb = np.zeros(3)
     sens = np.ascontiguousarray(sens, np.int32)
     v = np.ascontiguousarray(v0, float) if v0 is not None else None
     J = lib().pxo_frac_adjoint(m.n, C.byref(p), dp(m.coords), ip(m.off), ip(m.nbr),
@@ -298,8 +315,8 @@ def frac_adjoint(m, p, opc, bcs, dt, nsteps, u0, break_step, node_E, sens, comp,
                                C.c_double(dt), nsteps, dp(np.ascontiguousarray(u0, float)),
                                dp(v), ip(np.ascontiguousarray(break_step, np.int32)),
                                dp(np.ascontiguousarray(node_E, float)), len(sens), ip(sens),
-                               comp, dp(g), nthreads)
-    return J, g
+                               comp, dp(g), dp(gb), nthreads)
+    return (J, g, gb[:m.dim]) if want_b else (J, g)
 
 
 def frac_loss_frozen(m, p, opc, bcs, dt, nsteps, u0, break_step, node_E, sens, comp,

@@ -189,6 +189,71 @@ void pxo_bc_preset(int n, int ndof, const double *coords, const int *mirror,
     }
 }
 
+/* ExplicitPDDOFracElement::computeNodalProjection (ExplicitPDDOFracElement.cpp:
+ * 324-405): damage = 1 - (sum of bond health)/|H_i|; otherwise the
+ * displacement gradient gu[a][c] = sum_j (u_j - u_i)_a G_c(i,j) over the whole
+ * family in family order (health is not consulted), eps = sym(gu),
+ * sigma = C12 tr(eps) I + 2 C66 eps with sigma_zz = C12 tr(eps) under 2D plane
+ * strain and 0 under plane stress, von Mises from sigma. */
+void pxo_frac_project(int n, int dim, int plane_stress, double C12, double C66,
+                      const int *off, const int *nbr, const double *opc,
+                      const double *health, const double *u, int which, double *out)
+{
+    const int nops = (dim == 3) ? 9 : 5;
+    const int nc = (which >= 2) ? 9 : 1;
+    for (int i = 0; i < n; i++) {
+        double *o = out + (size_t)i * nc;
+        const int b = off[i], nb = off[i + 1] - off[i];
+        if (which == 0) {
+            if (nb <= 0) { o[0] = 0.0; continue; }
+            double live = 0.0;
+            for (int t = 0; t < nb; t++) live += health[b + t];
+            o[0] = 1.0 - live / (double)nb;
+            continue;
+        }
+        const int rowI = i * dim;
+        double gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < nb; t++) {
+            const int rowJ = (nbr[b + t] - 1) * dim;
+            const double *op = opc + (size_t)(b + t) * nops;
+            const double du0 = u[rowJ] - u[rowI], du1 = u[rowJ + 1] - u[rowI + 1];
+            if (dim >= 3) {
+                const double du2 = u[rowJ + 2] - u[rowI + 2];
+                const double Gx = op[0], Gy = op[1], Gz = op[2];
+                gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[2] += du0 * Gz;
+                gu[3] += du1 * Gx; gu[4] += du1 * Gy; gu[5] += du1 * Gz;
+                gu[6] += du2 * Gx; gu[7] += du2 * Gy; gu[8] += du2 * Gz;
+            } else {
+                const double Gx = op[0], Gy = op[1];
+                gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[3] += du1 * Gx; gu[4] += du1 * Gy;
+            }
+        }
+        double eps[6] = {0, 0, 0, 0, 0, 0};
+        if (dim >= 3) {
+            eps[0] = gu[0]; eps[1] = gu[4]; eps[2] = gu[8];
+            eps[3] = 0.5 * (gu[1] + gu[3]); eps[4] = 0.5 * (gu[2] + gu[6]); eps[5] = 0.5 * (gu[5] + gu[7]);
+        } else {
+            eps[0] = gu[0]; eps[1] = gu[4]; eps[3] = 0.5 * (gu[1] + gu[3]);
+        }
+        const double lam = C12, mu = C66;
+        const double tr = eps[0] + eps[1] + eps[2];
+        double sig[6];
+        sig[0] = lam * tr + 2.0 * mu * eps[0]; sig[1] = lam * tr + 2.0 * mu * eps[1];
+        sig[2] = (dim >= 3) ? (lam * tr + 2.0 * mu * eps[2]) : (!plane_stress ? lam * tr : 0.0);
+        sig[3] = 2.0 * mu * eps[3]; sig[4] = 2.0 * mu * eps[4]; sig[5] = 2.0 * mu * eps[5];
+        if (which == 1) {
+            const double sm = (sig[0] + sig[1] + sig[2]) / 3.0;
+            const double a = sig[0] - sm, c = sig[1] - sm, d = sig[2] - sm;
+            o[0] = sqrt(1.5 * (a * a + c * c + d * d + 2.0 * (sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5])));
+            continue;
+        }
+        const double *v = (which == 2) ? sig : eps;
+        o[0] = v[0]; o[1] = v[3]; o[2] = v[4];
+        o[3] = v[3]; o[4] = v[1]; o[5] = v[5];
+        o[6] = v[4]; o[7] = v[5]; o[8] = v[2];
+    }
+}
+
 /* CSR slot of (row, col) in a sorted row, -1 if not stored
  * (SparseMatrix::findEntry, SparseMatrix.cpp:187-214) */
 static int csr_find(const int *rowptr, const int *cols, int row, int col)
@@ -629,7 +694,7 @@ double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
                         double dt, int nsteps, const double *u0,
                         const double *v0, const int *break_step,
                         const double *node_E, int nsens, const int *sens,
-                        int comp, double *grad_e, int nthreads)
+                        int comp, double *grad_e, double *grad_b, int nthreads)
 {
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -648,6 +713,7 @@ double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
     double *wb = malloc(sizeof(double) * nd), *ku = malloc(sizeof(double) * nd);
     double *z = malloc(sizeof(double) * nd), *ew = malloc(sizeof(double) * nd);
     memset(grad_e, 0, sizeof(double) * (size_t)n);
+    double gb[3] = {0.0, 0.0, 0.0};
     const double *uK = hist + (size_t)(K + 1) * nd;
     for (int q = 0; q < nsens; q++) {
         const size_t idx = (size_t)sens[q] * dim + comp;
@@ -663,8 +729,10 @@ double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
             double s = 0.0;
             for (int a = 0; a < dim; a++) s += ku[(size_t)i * dim + a] * wb[(size_t)i * dim + a];
             grad_e[i] += c * s;
-            for (int a = 0; a < dim; a++)
+            for (int a = 0; a < dim; a++) {
                 ew[(size_t)i * dim + a] = node_E[i] * wb[(size_t)i * dim + a];
+                gb[a] += c * wb[(size_t)i * dim + a];   /* u^k = ... + c (L + b) */
+            }
         }
         khat_transpose(n, dim, off, nbr, opc, break_step, k, Cu, ew, z);
         for (size_t q = 0; q < nd; q++) {
@@ -677,9 +745,14 @@ double pxo_frac_adjoint(int n, const pxo_frac *p, const double *coords,
     khat_apply(n, dim, off, nbr, opc, break_step, 0, Cu, hist + nd, ku, nthreads);
     for (int i = 0; i < n; i++) {
         double s = 0.0;
-        for (int a = 0; a < dim; a++) s += ku[(size_t)i * dim + a] * lp[(size_t)i * dim + a];
+        for (int a = 0; a < dim; a++) {
+            s += ku[(size_t)i * dim + a] * lp[(size_t)i * dim + a];
+            gb[a] += 0.5 * c * lp[(size_t)i * dim + a];   /* u^{-1} = ... + (c/2)(L + b) */
+        }
         grad_e[i] += 0.5 * c * s;
     }
+    if (grad_b)
+        for (int a = 0; a < dim; a++) grad_b[a] = gb[a];
     free(hist); free(lc); free(lp); free(wb); free(ku); free(z); free(ew);
     return J;
 }

@@ -268,6 +268,25 @@ int ref_explicit(void *h, double *u, double *uold, double *health, double *opc,
     return ok ? p.m_TimeStepping.m_LastSteps : -p.m_TimeStepping.m_LastSteps - 1;
 }
 
+// ExplicitPDDOFracElement::computeNodalProjection (ExplicitPDDOFracElement.cpp:
+// 324-405) of every node at the current state (after ref_explicit): `name` one
+// of damage / vonMises (1 value per node) or stress / strain (9 per node).
+void ref_projection(void *h, const char *name, double *out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    const int n = p.m_PDMesh.getNodesNum();
+    const std::string nm(name);
+    const size_t nc = (nm == "stress" || nm == "strain") ? 9 : 1;
+    std::vector<double> o(nc);
+    auto *e = p.m_ElmtSystem.m_Elements[0].Element.get();
+    for (int i = 1; i <= n; i++) {
+        std::fill(o.begin(), o.end(), 0.0);
+        e->computeNodalProjection(nm, p.m_PDMesh, p.m_Operators, p.m_U, i, ndof, o);
+        std::copy(o.begin(), o.end(), out + (size_t)(i - 1) * nc);
+    }
+}
+
 // Elastic constants of the explicit element (C11, C12, C66, rho, sc).
 void ref_frac_consts(void *h, double *out)
 {

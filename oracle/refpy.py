@@ -30,7 +30,7 @@ def lib():
         _LIB.ref_open.argtypes = [C.c_char_p]
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
                   "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
-                  "ref_apply_bcs", "ref_ch_geometry",
+                  "ref_apply_bcs", "ref_projection", "ref_ch_geometry",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -133,6 +133,13 @@ class Case:
         k = np.ascontiguousarray(k, float).copy(); r = np.ascontiguousarray(r, float).copy()
         lib().ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
         return k, r
+
+    def projection(self, name):
+        """computeNodalProjection of every node (n x 1 or n x 9)."""
+        nc = 9 if name in ("stress", "strain") else 1
+        out = np.zeros(self.n * nc)
+        lib().ref_projection(self.h, name.encode(), _dp(out))
+        return out.reshape(self.n, nc)
 
     def ch_geometry(self):
         lap = np.zeros(self.nb); rev = np.zeros(self.nb, np.int32); mn = np.zeros(self.n)

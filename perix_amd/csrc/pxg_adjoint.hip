@@ -24,31 +24,81 @@
 // are checkpointed every `ckpt` steps and each segment is recomputed forward
 // (one fused, frozen-health launch per step) before its reverse sweep.
 #include "pxg_internal.h"
+#include <algorithm>
 #include <climits>
 #include <cmath>
+#include <vector>
 
 namespace pxg {
 
 // wbar = S^T lam for the merged BC program, and ew = E .* wbar
-__global__ void k_preset_T(int n, int dim, const double *lam, const int *prog_of, const int *bnode,
-                           const int *bmode, const int *inv_off, const int *inv_q, const double *E,
-                           double *wbar, double *ew)
+// per-block sums of w over the block's nodes, per component, added to the
+// block's own slot part[blockIdx.x * 3 + c] (a fixed summation order, no
+// atomics: the body-force gradient is reproducible).  256 threads.
+__device__ __forceinline__ void block_sum3(const double (&w)[3], int dim, double *part)
+{
+    __shared__ double red[4][3];
+    double v[3] = {w[0], w[1], w[2]};
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] += __shfl_xor(v[c], o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int c = 0; c < 3; c++) red[wv][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < dim; c++)
+            part[blockIdx.x * 3 + c] += ((red[0][c] + red[1][c]) + (red[2][c] + red[3][c]));
+}
+
+__global__ void __launch_bounds__(256) k_preset_T(int n, int dim, const double *lam, const int *prog_of,
+                                                  const int *bnode, const int *bmode, const int *inv_off,
+                                                  const int *inv_q, const double *E, double *wbar, double *ew,
+                                                  double *bpart)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    const int q = prog_of[m];
-    for (int c = 0; c < dim; c++) {
-        double v = lam[(size_t)m * dim + c];
-        if (q >= 0 && bmode[q * 3 + c] != 0) v = 0.0;
-        for (int p = inv_off[m]; p < inv_off[m + 1]; p++) {
-            const int qq = inv_q[p];
-            const int md = bmode[qq * 3 + c];
-            const int g = bnode[qq];
-            if (md == 1) v += lam[(size_t)g * dim + c];
-            else if (md == 2) v += -lam[(size_t)g * dim + c];
+    double wsum[3] = {0.0, 0.0, 0.0};
+    if (m < n) {
+        const int q = prog_of[m];
+        for (int c = 0; c < dim; c++) {
+            double v = lam[(size_t)m * dim + c];
+            if (q >= 0 && bmode[q * 3 + c] != 0) v = 0.0;
+            for (int p = inv_off[m]; p < inv_off[m + 1]; p++) {
+                const int qq = inv_q[p];
+                const int md = bmode[qq * 3 + c];
+                const int g = bnode[qq];
+                if (md == 1) v += lam[(size_t)g * dim + c];
+                else if (md == 2) v += -lam[(size_t)g * dim + c];
+            }
+            wbar[(size_t)m * dim + c] = v;
+            ew[(size_t)m * dim + c] = E[m] * v;
+            wsum[c] = v;
         }
-        wbar[(size_t)m * dim + c] = v;
-        ew[(size_t)m * dim + c] = E[m] * v;
+    }
+    // the body force enters every central-difference value: dJ/db += c S^T lam
+    if (bpart) block_sum3(wsum, dim, bpart);
+}
+
+// per-block sums of ubar^{-1} (the u^{-1} term of dJ/db)
+__global__ void __launch_bounds__(256) k_body_last(int n, int dim, const double *lamp, double *bpart)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double w[3] = {0.0, 0.0, 0.0};
+    if (m < n)
+        for (int c = 0; c < dim; c++) w[c] = lamp[(size_t)m * dim + c];
+    block_sum3(w, dim, bpart);
+}
+
+// dJ/db = c sum_k S^T lam_k + (c/2) ubar^{-1}, block partials in block order
+__global__ void k_body_final(int nblk, int dim, const double *part, const double *part_last, double c,
+                             double *gb)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int a = 0; a < dim; a++) {
+        double s = 0.0, l = 0.0;
+        for (int b = 0; b < nblk; b++) { s += part[b * 3 + a]; l += part_last[b * 3 + a]; }
+        gb[a] = c * s + 0.5 * c * l;
     }
 }
 
@@ -85,6 +135,12 @@ struct AdjArgs {
     const double *wbar, *ew;    // S^T lam_k and E .* wbar
     double *lam, *lamp, *Ebar;
     double cf;
+    // without the E-gradient: the family sums sp_k = sum_{j intact at k} Phat_ij
+    // kept per node (spg) and grown as k decreases by the bonds that break at
+    // step k + 1 (brk_t[cur[i] ..], sorted by break step, descending)
+    double *spg;
+    int *cur;
+    const int *brk_off, *brk_t;
 };
 
 template <int DIM> struct AdjCfg;
@@ -95,9 +151,10 @@ template <> struct AdjCfg<3> { static constexpr int S = 16, G = 2; };
 // family in aligned groups of G bonds with vector loads of ids, break steps
 // and both operators; the per-lane FP64 partial sums are combined by a lane
 // butterfly (a fixed order, so results are reproducible).
-template <int DIM, bool LAST>
+template <int DIM, bool LAST, bool EGRAD = true>
 __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
 {
+    static_assert(EGRAD || !LAST, "the u^{-1} term is part of the E-gradient");
     constexpr int S = AdjCfg<DIM>::S, G = AdjCfg<DIM>::G;
     constexpr int NPW = 64 / S, W = S * G;
     constexpr int NC = (DIM == 3) ? 6 : 3;
@@ -109,7 +166,7 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
     double ui[DIM];
 #pragma unroll
-    for (int c = 0; c < DIM; c++) ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
+    for (int c = 0; c < DIM; c++) ui[c] = (EGRAD && nvalid) ? a.u[(size_t)i * DIM + c] : 0.0;
     double ku[DIM], zo[DIM], sp[NC];
 #pragma unroll
     for (int c = 0; c < DIM; c++) { ku[c] = 0.0; zo[c] = 0.0; }
@@ -125,8 +182,12 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
             jv[0] = j2.x; jv[1] = j2.y; bv[0] = b2.x; bv[1] = b2.y;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const double2 q = *reinterpret_cast<const double2 *>(a.Phat + c * a.ps + u0);
-                p[c][0] = q.x; p[c][1] = q.y;
+                if constexpr (EGRAD) {
+                    const double2 q = *reinterpret_cast<const double2 *>(a.Phat + c * a.ps + u0);
+                    p[c][0] = q.x; p[c][1] = q.y;
+                } else {
+                    p[c][0] = 0.0; p[c][1] = 0.0;
+                }
                 if constexpr (!LAST) {
                     const double2 r = *reinterpret_cast<const double2 *>(a.PhatT + c * a.ps + u0);
                     pt[c][0] = r.x; pt[c][1] = r.y;
@@ -138,17 +199,22 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
             const int t = t0 + g;
             if (t < b0 || t >= b1 || bv[g] <= a.k) continue;
             const size_t jo = (size_t)(jv[g] - 1) * DIM;
-            double du[DIM], pc[NC], o[DIM];
-#pragma unroll
-            for (int c = 0; c < DIM; c++) du[c] = a.u[jo + c] - ui[c];
+            double pc[NC], o[DIM];
 #pragma unroll
             for (int c = 0; c < NC; c++) pc[c] = p[c][g];
-            pmul<DIM>(pc, du, o);
+            if constexpr (EGRAD) {
+                double du[DIM];
 #pragma unroll
-            for (int c = 0; c < DIM; c++) ku[c] += o[c];
+                for (int c = 0; c < DIM; c++) du[c] = a.u[jo + c] - ui[c];
+                pmul<DIM>(pc, du, o);
+#pragma unroll
+                for (int c = 0; c < DIM; c++) ku[c] += o[c];
+            }
             if constexpr (!LAST) {
+                if constexpr (EGRAD) {
 #pragma unroll
-                for (int c = 0; c < NC; c++) sp[c] += pc[c];
+                    for (int c = 0; c < NC; c++) sp[c] += pc[c];
+                }
                 double ej[DIM], ptc[NC];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) ej[c] = a.ew[jo + c];
@@ -164,19 +230,40 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
     for (int m = 1; m < S; m <<= 1) {
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
-            ku[c] += __shfl_xor(ku[c], m);
+            if constexpr (EGRAD) ku[c] += __shfl_xor(ku[c], m);
             if constexpr (!LAST) zo[c] += __shfl_xor(zo[c], m);
         }
-        if constexpr (!LAST) {
+        if constexpr (!LAST && EGRAD) {
 #pragma unroll
             for (int c = 0; c < NC; c++) sp[c] += __shfl_xor(sp[c], m);
         }
     }
     if (!nvalid || sl >= DIM) return;
+    if constexpr (!EGRAD) {
+        // sp_k = sp_{k+1} + Phat of the node's bonds that break at step k+1
+        // (every component lane reads the same entries; lane 0 writes back)
+        const size_t base = (size_t)i * NC;
+#pragma unroll
+        for (int c = 0; c < NC; c++) sp[c] = a.spg[base + c];
+        int q = a.cur[i];
+        const int qe = a.brk_off[i + 1];
+        while (q < qe) {
+            const int t = a.brk_t[q];
+            if (a.bs[t] != a.k + 1) break;
+#pragma unroll
+            for (int c = 0; c < NC; c++) sp[c] += a.Phat[c * a.ps + t];
+            q++;
+        }
+        if (sl == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) a.spg[base + c] = sp[c];
+            a.cur[i] = q;
+        }
+    }
     double wi[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; c++) wi[c] = a.wbar[(size_t)i * DIM + c];
-    if (sl == 0) {
+    if (EGRAD && sl == 0) {
         double s = 0.0;
 #pragma unroll
         for (int c = 0; c < DIM; c++) s += ku[c] * wi[c];
@@ -195,6 +282,51 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
         a.lamp[q] = -wb;
         a.lam[q] = np;
     }
+}
+
+// broken-bond lists for the family sums without the E-gradient: per node the
+// bonds with a break step in [1, K], sorted by break step (descending)
+__global__ void k_brk_count(int n, const int *off, const int *bs, int K, int *cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    for (int t = off[i]; t < off[i + 1]; t++) c += (bs[t] >= 1 && bs[t] <= K) ? 1 : 0;
+    cnt[i] = c;
+}
+
+__global__ void k_brk_fill(int n, const int *off, const int *bs, int K, const int *brk_off, int *brk_t)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int q = brk_off[i];
+    const int q0 = q;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        if (!(bs[t] >= 1 && bs[t] <= K)) continue;
+        int p = q++;                    // insertion: descending break step, then bond order
+        while (p > q0 && bs[brk_t[p - 1]] < bs[t]) { brk_t[p] = brk_t[p - 1]; p--; }
+        brk_t[p] = t;
+    }
+}
+
+// sp_K[i] = sum over the bonds intact at step K of Phat (family order)
+template <int NC>
+__global__ void k_sp_init(int n, const int *off, const int *bs, int K, const double *Phat, long long ps,
+                          const int *brk_off, double *spg, int *cur)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) s[c] = 0.0;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        if (bs[t] <= K) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++) s[c] += Phat[c * ps + t];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) spg[(size_t)i * NC + c] = s[c];
+    cur[i] = brk_off[i];
 }
 
 __global__ void k_seed_lambda(int nsens, const int *sens, int dim, int comp, const double *uK, double *lam)
@@ -227,7 +359,7 @@ int launch_transpose_ops(pxg_ctx *ctx)
     return PXG_OK;
 }
 
-template <int DIM, bool LAST>
+template <int DIM, bool LAST, bool EGRAD = true>
 static int launch_adj_rev(pxg_ctx *ctx, int k, const double *u, const double *wbar, const double *ew,
                           double *lam, double *lamp, double *Ebar, double cf)
 {
@@ -238,8 +370,9 @@ static int launch_adj_rev(pxg_ctx *ctx, int k, const double *u, const double *wb
     a.bs = F.bsh.p ? F.bs_csr.p : F.break_step.p;
     a.Phat = F.Phat.p; a.PhatT = F.PhatT.p; a.u = u; a.wbar = wbar; a.ew = ew;
     a.lam = lam; a.lamp = lamp; a.Ebar = Ebar; a.cf = cf;
+    a.spg = F.adj.spg.p; a.cur = F.adj.cur.p; a.brk_off = F.adj.brk_off.p; a.brk_t = F.adj.brk_t.p;
     constexpr int npb = 4 * (64 / AdjCfg<DIM>::S);
-    hipLaunchKernelGGL((k_adj_rev<DIM, LAST>), dim3(gridn2(ctx->n, npb)), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_adj_rev<DIM, LAST, EGRAD>), dim3(gridn2(ctx->n, npb)), dim3(256), 0, ctx->stream, a);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
@@ -326,13 +459,28 @@ extern "C" int pxg_frac_loss(pxg_ctx *ctx, const double *u0, const double *v0, d
     return PXG_OK;
 }
 
+extern "C" int pxg_frac_set_body(pxg_ctx *ctx, const double *b)
+{
+    PXG_CHECK(ctx && ctx->frac.ready && b, PXG_ERR_ARG, "pxg_frac_set_body: bad arguments");
+    for (int c = 0; c < 3; c++) ctx->frac.b[c] = (c < ctx->dim) ? b[c] : 0.0;
+    return PXG_OK;
+}
+
 extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0, double dt, int nsteps,
                                 int ckpt, int nsens, const int *sens, int comp, double *J_out, double *grad)
+{
+    PXG_CHECK(grad, PXG_ERR_ARG, "pxg_frac_adjoint: bad arguments");
+    return pxg_frac_adjoint_ex(ctx, u0, v0, dt, nsteps, ckpt, nsens, sens, comp, J_out, grad, nullptr);
+}
+
+extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double *v0, double dt, int nsteps,
+                                   int ckpt, int nsens, const int *sens, int comp, double *J_out,
+                                   double *grad, double *grad_b)
 {
     PXG_CHECK(ctx && ctx->frac.ready && ctx->frac.has_nodeE, PXG_ERR_STATE,
               "pxg_frac_adjoint: needs pxg_frac_setup with a per-node E (and a recorded forward run)");
     PXG_CHECK(u0 && dt > 0.0 && nsteps >= 1 && ckpt >= 1 && nsens >= 0 && (nsens == 0 || sens) &&
-              comp >= 0 && comp < ctx->dim && grad, PXG_ERR_ARG, "pxg_frac_adjoint: bad arguments");
+              comp >= 0 && comp < ctx->dim, PXG_ERR_ARG, "pxg_frac_adjoint: bad arguments");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_adjoint: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
@@ -341,20 +489,27 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
     const int n = ctx->n, dim = ctx->dim, K = nsteps;
     const size_t nd = (size_t)n * dim;
     const int ncks = (K - 1) / ckpt + 1;
-    // workspace kept by the context between calls (allocated on size changes)
-    PXG_HIP(A.cks.alloc((size_t)2 * ncks * nd));
-    PXG_HIP(A.seg.alloc((size_t)(ckpt + 2) * nd));
+    // workspace kept by the context between calls (allocated on size changes);
+    // without the E-gradient the reverse sweep reads no states: no checkpoints
+    if (grad) {
+        PXG_HIP(A.cks.alloc((size_t)2 * ncks * nd));
+        PXG_HIP(A.seg.alloc((size_t)(ckpt + 2) * nd));
+    }
     PXG_HIP(A.lam.alloc(nd)); PXG_HIP(A.lamp.alloc(nd));
     PXG_HIP(A.wbar.alloc(nd)); PXG_HIP(A.ew.alloc(nd));
     PXG_HIP(A.Ebar.alloc(n)); PXG_HIP(A.dJ.alloc(1));
     PXG_HIP(A.ds.alloc(nsens > 0 ? nsens : 1));
+    const int nblk = (int)gridn2(n, 256);
+    PXG_HIP(A.bpart.alloc((size_t)6 * nblk));     // step partials, then the u^{-1} partials
+    PXG_HIP(A.gb.alloc(3));
+    PXG_HIP(hipMemsetAsync(A.bpart.p, 0, sizeof(double) * 6 * nblk, st));
     if (nsens) PXG_HIP(hipMemcpyAsync(A.ds.p, sens, sizeof(int) * nsens, hipMemcpyHostToDevice, st));
     std::vector<double> tk(ncks, 0.0);
     double *uK = nullptr;
     const int *bs_csr = nullptr;
     int rc = bond_state_csr(ctx, nullptr, &bs_csr);   // the recorded break steps, CSR order
     if (rc) return rc;
-    rc = adj_forward(ctx, u0, v0, dt, K, ckpt, A.cks.p, &uK, tk.data());
+    rc = adj_forward(ctx, u0, v0, dt, K, ckpt, grad ? A.cks.p : nullptr, &uK, tk.data());
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss, dim3(1), dim3(1), 0, st, nsens, A.ds.p, dim, comp, uK, A.dJ.p);
     PXG_HIP(hipMemsetAsync(A.lam.p, 0, sizeof(double) * nd, st));
@@ -362,6 +517,52 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
     PXG_HIP(hipMemsetAsync(A.Ebar.p, 0, sizeof(double) * n, st));
     hipLaunchKernelGGL(k_seed_lambda, dim3(1), dim3(1), 0, st, nsens, A.ds.p, dim, comp, uK, A.lam.p);
     const double c = dt * dt / F.rho;
+    if (!grad) {
+        // body-force gradient alone: the lambda recursion needs the family sums
+        // of the intact operators, kept per node and grown by the bonds that
+        // break at step k+1 as k decreases
+        const int nc = (dim == 3) ? 6 : 3;
+        PXG_HIP(A.cur.alloc(n));
+        PXG_HIP(A.brk_off.alloc((size_t)n + 1));
+        PXG_HIP(A.spg.alloc((size_t)n * nc));
+        hipLaunchKernelGGL(k_brk_count, dim3(gridn2(n, 256)), dim3(256), 0, st, n, ctx->off.p, bs_csr, K, A.cur.p);
+        PXG_HIP(hipGetLastError());
+        std::vector<int> cnt((size_t)n), boff((size_t)n + 1, 0);
+        PXG_HIP(hipMemcpyAsync(cnt.data(), A.cur.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        PXG_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < n; i++) boff[(size_t)i + 1] = boff[(size_t)i] + cnt[(size_t)i];
+        PXG_HIP(A.brk_t.alloc((size_t)std::max(boff[(size_t)n], 1)));
+        PXG_HIP(hipMemcpyAsync(A.brk_off.p, boff.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_brk_fill, dim3(gridn2(n, 256)), dim3(256), 0, st, n, ctx->off.p, bs_csr, K,
+                           A.brk_off.p, A.brk_t.p);
+        if (dim == 3)
+            hipLaunchKernelGGL(k_sp_init<6>, dim3(gridn2(n, 256)), dim3(256), 0, st, n, ctx->off.p, bs_csr, K,
+                               F.Phat.p, F.ps, A.brk_off.p, A.spg.p, A.cur.p);
+        else
+            hipLaunchKernelGGL(k_sp_init<3>, dim3(gridn2(n, 256)), dim3(256), 0, st, n, ctx->off.p, bs_csr, K,
+                               F.Phat.p, F.ps, A.brk_off.p, A.spg.p, A.cur.p);
+        PXG_HIP(hipGetLastError());
+        for (int k = K; k >= 1; k--) {
+            hipLaunchKernelGGL(k_preset_T, dim3(gridn2(n, 256)), dim3(256), 0, st, n, dim, A.lam.p,
+                               F.bc.prog_of.p, F.bc.node.p, F.bc.mode.p, F.bc.inv_off.p, F.bc.inv_q.p,
+                               F.nodeE.p, A.wbar.p, A.ew.p, grad_b ? A.bpart.p : nullptr);
+            PXG_HIP(hipGetLastError());
+            rc = (dim == 2)
+                     ? launch_adj_rev<2, false, false>(ctx, k, nullptr, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, nullptr, c)
+                     : launch_adj_rev<3, false, false>(ctx, k, nullptr, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, nullptr, c);
+            if (rc) return rc;
+        }
+        if (grad_b) {
+            hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, A.bpart.p + 3 * nblk);
+            hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk, dim, A.bpart.p, A.bpart.p + 3 * nblk,
+                               c, A.gb.p);
+            PXG_HIP(hipGetLastError());
+            PXG_HIP(hipMemcpyAsync(grad_b, A.gb.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));
+        }
+        if (J_out) PXG_HIP(hipMemcpyAsync(J_out, A.dJ.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        PXG_HIP(hipStreamSynchronize(st));
+        return PXG_OK;
+    }
     for (int cidx = ncks - 1; cidx >= 0; cidx--) {
         const int k0 = cidx * ckpt;                 // segment covers steps k0+1 .. k1
         const int k1 = (k0 + ckpt < K) ? k0 + ckpt : K;
@@ -373,7 +574,7 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
             const double *ukm1 = A.seg.p + (size_t)(k - k0) * nd;     // u^{k-1}
             hipLaunchKernelGGL(k_preset_T, dim3(gridn2(n, 256)), dim3(256), 0, st, n, dim, A.lam.p,
                                F.bc.prog_of.p, F.bc.node.p, F.bc.mode.p, F.bc.inv_off.p, F.bc.inv_q.p,
-                               F.nodeE.p, A.wbar.p, A.ew.p);
+                               F.nodeE.p, A.wbar.p, A.ew.p, grad_b ? A.bpart.p : nullptr);
             PXG_HIP(hipGetLastError());
             rc = (dim == 2) ? launch_adj_rev<2, false>(ctx, k, ukm1, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, A.Ebar.p, c)
                             : launch_adj_rev<3, false>(ctx, k, ukm1, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, A.Ebar.p, c);
@@ -384,7 +585,14 @@ extern "C" int pxg_frac_adjoint(pxg_ctx *ctx, const double *u0, const double *v0
     rc = (dim == 2) ? launch_adj_rev<2, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, A.lam.p, A.lamp.p, A.Ebar.p, 0.5 * c)
                     : launch_adj_rev<3, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, A.lam.p, A.lamp.p, A.Ebar.p, 0.5 * c);
     if (rc) return rc;
-    PXG_HIP(hipMemcpyAsync(grad, A.Ebar.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (grad_b) {
+        hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, A.bpart.p + 3 * nblk);
+        hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk, dim, A.bpart.p, A.bpart.p + 3 * nblk, c,
+                           A.gb.p);
+        PXG_HIP(hipGetLastError());
+        PXG_HIP(hipMemcpyAsync(grad_b, A.gb.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));
+    }
+    if (grad) PXG_HIP(hipMemcpyAsync(grad, A.Ebar.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     if (J_out) PXG_HIP(hipMemcpyAsync(J_out, A.dJ.p, sizeof(double), hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
     return PXG_OK;

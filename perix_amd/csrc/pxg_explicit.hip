@@ -243,6 +243,71 @@ __global__ void k_class_check(int n, int nc, const int *off, const int *bsh, con
     pofs[i] = (off[r] + bsh[r]) - (off[i] + bsh[i]);
 }
 
+// Nodal output projections of the explicit element, one thread per node
+// (ExplicitPDDOFracElement::computeNodalProjection, ExplicitPDDOFracElement.cpp:
+// 324-405): which = 0 damage = 1 - (intact bonds)/|H_i|; otherwise the
+// displacement gradient sum_j (u_j - u_i) (x) G(i,j) over the whole family in
+// family order (first-derivative g-weights, health not consulted), the small
+// strain, the Navier stress (sigma_zz = C12 tr(eps) in 2D plane strain, 0 in
+// plane stress), von Mises (1), stress (2) or strain (3) as row-major 3x3.
+__global__ void k_frac_project(int n, int dim, int nop, int which, int plane_stress, double C12, double C66,
+                               const int *off, const int *nbr, const double *ops, const unsigned char *health,
+                               const int *bsh, const double *u, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nc = (which >= 2) ? 9 : 1;
+    double *o = out + (size_t)i * nc;
+    const int b = off[i], nb = off[i + 1] - off[i];
+    if (which == 0) {
+        if (nb <= 0) { o[0] = 0.0; return; }
+        const int sh = bsh ? bsh[i] : 0;
+        double live = 0.0;
+        for (int t = b; t < b + nb; t++) live += health[t + sh] ? 1.0 : 0.0;
+        o[0] = 1.0 - live / (double)nb;
+        return;
+    }
+    const size_t rowI = (size_t)i * dim;
+    double gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = b; t < b + nb; t++) {
+        const size_t rowJ = (size_t)(nbr[t] - 1) * dim;
+        const double *op = ops + (size_t)t * nop;
+        const double du0 = u[rowJ] - u[rowI], du1 = u[rowJ + 1] - u[rowI + 1];
+        if (dim >= 3) {
+            const double du2 = u[rowJ + 2] - u[rowI + 2];
+            const double Gx = op[0], Gy = op[1], Gz = op[2];
+            gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[2] += du0 * Gz;
+            gu[3] += du1 * Gx; gu[4] += du1 * Gy; gu[5] += du1 * Gz;
+            gu[6] += du2 * Gx; gu[7] += du2 * Gy; gu[8] += du2 * Gz;
+        } else {
+            const double Gx = op[0], Gy = op[1];
+            gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[3] += du1 * Gx; gu[4] += du1 * Gy;
+        }
+    }
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    if (dim >= 3) {
+        eps[0] = gu[0]; eps[1] = gu[4]; eps[2] = gu[8];
+        eps[3] = 0.5 * (gu[1] + gu[3]); eps[4] = 0.5 * (gu[2] + gu[6]); eps[5] = 0.5 * (gu[5] + gu[7]);
+    } else {
+        eps[0] = gu[0]; eps[1] = gu[4]; eps[3] = 0.5 * (gu[1] + gu[3]);
+    }
+    const double tr = eps[0] + eps[1] + eps[2];
+    double sig[6];
+    sig[0] = C12 * tr + 2.0 * C66 * eps[0]; sig[1] = C12 * tr + 2.0 * C66 * eps[1];
+    sig[2] = (dim >= 3) ? (C12 * tr + 2.0 * C66 * eps[2]) : (!plane_stress ? C12 * tr : 0.0);
+    sig[3] = 2.0 * C66 * eps[3]; sig[4] = 2.0 * C66 * eps[4]; sig[5] = 2.0 * C66 * eps[5];
+    if (which == 1) {
+        const double sm = (sig[0] + sig[1] + sig[2]) / 3.0;
+        const double a = sig[0] - sm, c = sig[1] - sm, d = sig[2] - sm;
+        o[0] = sqrt(1.5 * (a * a + c * c + d * d + 2.0 * (sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5])));
+        return;
+    }
+    const double *v = (which == 2) ? sig : eps;
+    o[0] = v[0]; o[1] = v[3]; o[2] = v[4];
+    o[3] = v[3]; o[4] = v[1]; o[5] = v[5];
+    o[6] = v[4]; o[7] = v[5]; o[8] = v[2];
+}
+
 __global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,7 +371,7 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     F.pofs.release();
     F.nshared = 0;
     F.nclasses = 0;
-    if (!F.bsh.p || F.precision != PXG_FP32C) return PXG_OK;
+    if (!F.perm.p || F.precision != PXG_FP32C) return PXG_OK;   // tiled (staged) layout only
     const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
     hipStream_t st = ctx->stream;
     dbuf<double> P;                     // FP64 operators, CSR order
@@ -416,6 +481,8 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     }
     PXG_HIP(F.health.alloc((size_t)F.pst));
     PXG_HIP(F.break_step.alloc((size_t)F.pst));
+    // layout padding: dead in every mode (health 0, a break step below any step)
+    PXG_HIP(hipMemsetAsync(F.break_step.p, 0x80, sizeof(int) * (size_t)F.pst, st));
     F.health_csr.release(); F.bs_csr.release();
     hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, F.bsh.p,
                        ctx->seed.p, F.health.p, F.break_step.p);
@@ -692,6 +759,34 @@ extern "C" int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold, unsigne
     if (health) PXG_HIP(hipMemcpyAsync(health, hc, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
     if (break_step) PXG_HIP(hipMemcpyAsync(break_step, bc, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_project_dev(pxg_ctx *ctx, int which, double *out_dev)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_project: no explicit state");
+    PXG_CHECK(which >= 0 && which <= 3 && out_dev, PXG_ERR_ARG, "pxg_frac_project: bad arguments");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_project: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const FracState &F = ctx->frac;
+    hipLaunchKernelGGL(k_frac_project, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
+                       ctx->nop, which, F.plane_stress, F.C12, F.C66, ctx->off.p, ctx->nbr.p, ctx->ops.p,
+                       F.health.p, F.bsh.p, F.u[F.cur].p, out_dev);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_project(pxg_ctx *ctx, int which, double *out)
+{
+    PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_project: no explicit state");
+    PXG_CHECK(which >= 0 && which <= 3 && out, PXG_ERR_ARG, "pxg_frac_project: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const size_t cnt = (size_t)ctx->n * (which >= 2 ? 9 : 1);
+    PXG_HIP(ctx->sf.alloc(cnt));
+    int rc = pxg_frac_project_dev(ctx, which, ctx->sf.p);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(out, ctx->sf.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->stream));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
     return PXG_OK;
 }
 

@@ -101,6 +101,9 @@ struct AdjWork {
     dbuf<double> cks;        // 2 * ncheckpoints * N*dim: (u^{k-1}, u^k) pairs
     dbuf<double> seg;        // (ckpt + 2) * N*dim: one recomputed segment
     dbuf<double> lam, lamp, wbar, ew, Ebar, dJ, eo;
+    dbuf<double> bpart, gb;  // body-force gradient: per-block partials, result
+    dbuf<double> spg;        // family sums of intact Phat (sweeps without the E-gradient)
+    dbuf<int> cur, brk_off, brk_t;
     dbuf<int> ds;            // sensor node ids
 };
 

@@ -118,7 +118,7 @@ def initial_state(cloud, dim, specs):
 def explicit_plate(dim=2, nx=400, ny=200, nz=1, lo=(0.0, 0.0, 0.0), hi=(0.2, 0.1, 0.04),
                    sc=0.01, cracks=None, box=None, vel=16.5, nsteps=2000, cfl=0.9,
                    precision=pxg.FP64, node_E=None, factor=3.015, device=0, ctx=None):
-    """Explicit PDDO fracture plate/block (configs 2, 4 and 5 of the benchmark set).
+    """Explicit PDDO fracture plate (2D) or block (3D) with edge notches and an impactor patch.
 
     Defaults are the 2D plate: 0.2 m x 0.1 m, dx = 0.5 mm, two 50 mm edge
     notches at x = 50 and 150 mm from y = 0 (force-only cracks), a 16.5 m/s
@@ -150,13 +150,13 @@ def explicit_plate(dim=2, nx=400, ny=200, nz=1, lo=(0.0, 0.0, 0.0), hi=(0.2, 0.1
     return FracSetup(ctx, cloud, delta, p, specs, dt, total, nsteps, cracks, G0, u0)
 
 
-def plate_config2(precision=pxg.FP64, nsteps=2000, **kw):
-    """Config 2: 400 x 200 plate (80,000 cells, 81,200 PD nodes)."""
+def plate_400x200(precision=pxg.FP64, nsteps=2000, **kw):
+    """The 400 x 200 plate (80,000 cells, 81,200 PD nodes)."""
     return explicit_plate(2, 400, 200, precision=precision, nsteps=nsteps, **kw)
 
 
-def block_config5(precision=pxg.FP32C, nsteps=1000, **kw):
-    """Config 5: 160 x 80 x 40 block, dx = 1 mm (512,000 cells, 556,800 PD
+def block_160x80x40(precision=pxg.FP32C, nsteps=1000, **kw):
+    """The 160 x 80 x 40 block, dx = 1 mm (512,000 cells, 556,800 PD
     nodes).  An 8 mm deep through-thickness notch at x = 80 mm from y = 0
     (a force-only crack segment, extruded over z as in PDMesh.cpp:125-137)
     and a 16.5 m/s direct velocity on the bottom-face patch
@@ -166,8 +166,8 @@ def block_config5(precision=pxg.FP32C, nsteps=1000, **kw):
                           box=(0.04, 0.06, 1.0, -1.0, 1.0, -1.0), nsteps=nsteps, **kw)
 
 
-def poisson_config1(nx=100, ny=100, sigma=1.0, factor=3.015, device=0, ctx=None):
-    """Config 1: unit square, sigma = 1, manufactured f = 2 pi^2 sin(pi x) sin(pi y),
+def poisson_square(nx=100, ny=100, sigma=1.0, factor=3.015, device=0, ctx=None):
+    """Poisson on the unit square, sigma = 1, manufactured f = 2 pi^2 sin(pi x) sin(pi y),
     BC rows of poisson_bcs resolved on the context."""
     g = StructuredGrid(2, nx, ny)
     cloud = build_structured_cloud(g)
@@ -186,19 +186,19 @@ def poisson_config1(nx=100, ny=100, sigma=1.0, factor=3.015, device=0, ctx=None)
 
 
 def poisson_bcs(cloud):
-    """Config 1 BC list: Dirichlet u = 0 by reflection on the four ghost walls."""
+    """Poisson BC list: Dirichlet u = 0 by reflection on the four ghost walls."""
     return [{"type": "dirichlet", "nodes": cloud.group(w), "comps": [0], "value": [0.0]}
             for w in ("left", "right", "bottom", "top")]
 
 
 def cahnhilliard_bcs(cloud):
-    """Config 3 BC list: Neumann mirrors of c and mu on the four ghost walls."""
+    """Cahn-Hilliard BC list: Neumann mirrors of c and mu on the four ghost walls."""
     return [{"type": "neumann", "nodes": cloud.group(w), "comps": [0, 1]}
             for w in ("left", "right", "bottom", "top")]
 
 
-def cahnhilliard_config3(nx=256, ny=256, seed=12345, factor=3.015, device=0, ctx=None):
-    """Config 3: 256 x 256 unit square, c0 = 0.5 + U(-0.01, 0.01) by RandomIC
+def cahnhilliard_square(nx=256, ny=256, seed=12345, factor=3.015, device=0, ctx=None):
+    """Cahn-Hilliard on the 256 x 256 unit square, c0 = 0.5 + U(-0.01, 0.01) by RandomIC
     over all PD nodes in id order (ICSystem.cpp:57-69), mu0 = 0, Neumann
     mirrors of c and mu on all walls (presets applied to the returned state,
     BC rows of cahnhilliard_bcs resolved on the context)."""

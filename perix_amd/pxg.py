@@ -53,8 +53,8 @@ SYMBOLS = [
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
     "pxg_csr_apply_bcs_dev", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
-    "pxg_frac_layout", "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
-    "pxg_frac_adjoint", "pxg_frac_loss",
+    "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
+    "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
 ]
 
 
@@ -70,6 +70,14 @@ def load():
         if not os.path.exists(p):
             raise PxgError(-99, "libpxg.so not built (run __graft_entry__.build() or "
                                 "python perix_amd/build.py)")
+        # PyTorch-ROCm ships its own HIP runtime under the same soname.  When
+        # libpxg is loaded first, torch's later import brings a second runtime
+        # into the process and that one finds no GPU; with torch loaded first,
+        # libpxg binds to torch's runtime and both share the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(p)
         L.pxg_last_error.restype = C.c_char_p
         L.pxg_horizon_radius.restype = C.c_double
@@ -99,6 +107,11 @@ def load():
                                        I, C.c_int, D, D]
         L.pxg_frac_loss.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, D, C.c_int, I,
                                     C.c_int, D]
+        L.pxg_frac_adjoint_ex.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, C.c_int, C.c_int,
+                                          I, C.c_int, D, D, D]
+        L.pxg_frac_set_body.argtypes = [C.c_void_p, D]
+        L.pxg_frac_project.argtypes = [C.c_void_p, C.c_int, D]
+        L.pxg_frac_project_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.pxg_frac_state_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]
         _LIB = L
@@ -333,6 +346,16 @@ class Context:
                                      _dp(a), C.byref(ab)))
         return a, ab.value
 
+    PROJECTIONS = ("damage", "vonMises", "stress", "strain")
+
+    def frac_project(self, name):
+        """Nodal projection `name` of the current state (n x 1 or n x 9)."""
+        which = self.PROJECTIONS.index(name)
+        nc = 9 if which >= 2 else 1
+        out = np.zeros(self.n * nc)
+        _check(load().pxg_frac_project(self.h, which, _dp(out)))
+        return out.reshape(self.n, nc)
+
     def frac_layout(self):
         """(bond slots per component, tiled, shared runs, nodes on shared runs)."""
         info = (C.c_longlong * 4)()
@@ -387,6 +410,21 @@ class Context:
                                        nsteps, ckpt, len(sens), _ip(sens), comp, C.byref(J),
                                        _dp(g)))
         return J.value, g
+
+    def frac_adjoint_ex(self, u0, dt, nsteps, sens, comp, ckpt=50, v0=None, grad_E=True):
+        """(J, dJ/dE or None, dJ/db) in one reverse sweep."""
+        g = np.zeros(self.n) if grad_E else None
+        gb = np.zeros(self.dim)
+        J = C.c_double()
+        sens = np.ascontiguousarray(sens, np.int32)
+        v = None if v0 is None else np.ascontiguousarray(v0, float)
+        _check(load().pxg_frac_adjoint_ex(self.h, _dp(np.ascontiguousarray(u0, float)), _dp(v), dt,
+                                          nsteps, ckpt, len(sens), _ip(sens), comp, C.byref(J),
+                                          _dp(g), _dp(gb)))
+        return J.value, g, gb
+
+    def frac_set_body(self, b):
+        _check(load().pxg_frac_set_body(self.h, _dp(np.ascontiguousarray(b, float))))
 
     def frac_loss(self, u0, dt, nsteps, sens, comp, node_E=None, v0=None):
         J = C.c_double()

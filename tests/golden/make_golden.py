@@ -14,7 +14,8 @@ Fixtures (all small, .npz):
                     g-weights, CSR, Poisson K/residual for a seeded field
   mesh_fam_3d.npz   6x5x4 grid on [0,0.6]x[0,0.5]x[0,0.4]: same, 10-term basis
   explicit_2d.npz   40x20 plate, two force-only notches, 16.5 m/s impactor,
-                    400 FP64 steps: final u, u_prev, bond health
+                    400 FP64 steps: final u, u_prev, bond health and the
+                    element's damage / von Mises / stress / strain projections
   explicit_3d.npz   16x8x4 block, one notch, impactor, 200 FP64 steps
   ch_2d.npz         16x16 Cahn-Hilliard: RandomIC + Neumann presets, CSR K and
                     residual for a perturbed state (Picard-frozen mobility)
@@ -101,11 +102,12 @@ def explicit_fixture(name, dim, nx, ny, nz, hi, cracks, box, nsteps, sc_target=0
     total = nsteps * dt
     c = refpy.Case(deck_frac(dim, nx, ny, nz, hi, G0, dt, total, cracks, box))
     steps, u, uold, health, opc, sc = c.explicit()
+    proj = {"proj_" + nm: c.projection(nm) for nm in ("damage", "vonMises", "stress", "strain")}
     np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
                         hi=np.array(hi, float), cracks=np.array(cracks, float),
                         box=np.array(box, float), E=E_STEEL, nu=NU, rho=RHO, G0=G0, sc=sc, dt=dt,
                         total=total, nsteps=nsteps, steps=steps, u=u, uold=uold,
-                        health=health.astype(np.uint8))
+                        health=health.astype(np.uint8), **proj)
     c.close()
 
 

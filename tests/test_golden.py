@@ -77,6 +77,9 @@ def test_explicit_trajectory(name):
     assert steps == int(z["steps"])
     assert same(u, z["u"]) and same(uo, z["uold"]), "FP64 displacement history"
     assert np.array_equal((h > 0).astype(np.uint8), z["health"]), "broken-bond set"
+    # the element's output projections of the final state
+    for nm in pxo.PROJECTIONS:
+        assert same(pxo.frac_project(m, p, opc, h, u, nm), z["proj_" + nm]), nm
 
 
 def test_cahn_hilliard_assembly_and_random_ic():

@@ -64,7 +64,37 @@ def test_adjoint_vs_fd_and_oracle(oracle):
     print("adjoint vs oracle rel-L2 %.3e (J %.6e vs %.6e)" % (rel, J, Jr))
     assert abs(J - Jr) <= 1e-12 * abs(Jr)
     assert rel <= 1e-10
+    _check_body_gradient(ctx, s, sens, oracle, m, p, opc, bs, E)
     ctx.close()
+
+
+def _check_body_gradient(ctx, s, sens, oracle, m, p, opc, bs, E):
+    """dJ/db of the uniform body force from the same reverse sweep: central
+    differences per component (exact to rounding: J is quadratic in b at
+    frozen health) and the oracle's gradient."""
+    dim = ctx.dim
+    J, g, gb = ctx.frac_adjoint_ex(s.u0, s.dt, s.nsteps, sens, 1, ckpt=8)
+    _, g2 = ctx.frac_adjoint(s.u0, s.dt, s.nsteps, sens, 1, ckpt=8)
+    assert np.array_equal(g, g2)
+    # the sweep without the E-gradient (no states, incremental family sums)
+    J3, g3, gb3 = ctx.frac_adjoint_ex(s.u0, s.dt, s.nsteps, sens, 1, ckpt=8, grad_E=False)
+    assert g3 is None and J3 == J
+    print("dJ/db with / without the E-gradient: %s / %s" % (gb, gb3))
+    assert np.abs(gb3 - gb).max() <= 1e-12 * np.abs(gb).max()
+    hb = 1e3 * problems.RHO
+    for a in range(dim):
+        b = np.zeros(dim); b[a] = hb
+        ctx.frac_set_body(b)
+        jp = ctx.frac_loss(s.u0, s.dt, s.nsteps, sens, 1)
+        ctx.frac_set_body(-b)
+        jm = ctx.frac_loss(s.u0, s.dt, s.nsteps, sens, 1)
+        fd = (jp - jm) / (2 * hb)
+        print("dJ/db[%d] adjoint %.6e fd %.6e" % (a, gb[a], fd))
+        assert abs(fd - gb[a]) <= 1e-6 * max(abs(fd), abs(gb[a]))
+    ctx.frac_set_body(np.zeros(dim))
+    _, _, gbr = oracle.frac_adjoint(m, p, opc, oracle.make_bcs(s.bcs), s.dt, s.nsteps, s.u0, bs, E, sens, 1,
+                                    want_b=True)
+    assert np.abs(gb - gbr).max() <= 1e-10 * np.abs(gbr).max()
 
 
 def test_adjoint_3d_vs_fd_and_oracle(oracle):
@@ -104,4 +134,5 @@ def test_adjoint_3d_vs_fd_and_oracle(oracle):
     rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
     print("3D adjoint vs oracle rel-L2 %.3e" % rel)
     assert abs(J - Jr) <= 1e-12 * abs(Jr) and rel <= 1e-10
+    _check_body_gradient(ctx, s, sens, oracle, m, p, opc, bs, E)
     ctx.close()

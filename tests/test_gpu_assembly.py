@@ -19,7 +19,7 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("nx", [20, 100])
 def test_poisson_assembly(oracle, nx):
-    ctx, cloud, delta, f = problems.poisson_config1(nx=nx, ny=nx)
+    ctx, cloud, delta, f = problems.poisson_square(nx=nx, ny=nx)
     m = oracle.Mesh(oracle.make_grid(2, nx, nx))
     rp, cols = ctx.csr()
     rrp, rcols = oracle.csr_pattern(m, 1)
@@ -52,7 +52,7 @@ def test_poisson_manufactured_residual(oracle):
     """The exact field sin(pi x) sin(pi y) leaves an O(dx^2) interior residual."""
     errs = []
     for nx in (40, 80):
-        ctx, cloud, delta, f = problems.poisson_config1(nx=nx, ny=nx)
+        ctx, cloud, delta, f = problems.poisson_square(nx=nx, ny=nx)
         u = np.sin(math.pi * cloud.coords[0::3]) * np.sin(math.pi * cloud.coords[1::3])
         # manufactured: sigma Lap u + f = 0 with f = 2 pi^2 u
         k, r = ctx.assemble_poisson(1.0, u, f_node=f)
@@ -66,7 +66,7 @@ def test_poisson_manufactured_residual(oracle):
 
 @pytest.mark.parametrize("nx", [24, 64])
 def test_cahnhilliard_assembly(oracle, nx):
-    ctx, cloud, delta, u = problems.cahnhilliard_config3(nx=nx, ny=nx)
+    ctx, cloud, delta, u = problems.cahnhilliard_square(nx=nx, ny=nx)
     m = oracle.Mesh(oracle.make_grid(2, nx, nx))
     ops, _ = ctx.operators()
     lap, rev, isg = ctx.ch_geometry()
@@ -106,7 +106,7 @@ def test_device_resident_assembly_matches_host_entry_points():
     """pxg_assemble_poisson_dev / pxg_assemble_ch_dev (inputs and outputs in
     device memory) give the same bits as the host-pointer entry points."""
     import torch
-    ctx, cloud, delta, f = problems.poisson_config1(nx=40, ny=40)
+    ctx, cloud, delta, f = problems.poisson_square(nx=40, ny=40)
     u = np.random.RandomState(2).rand(cloud.n)
     k, r = ctx.assemble_poisson(1.0, u, f_node=f)
     ud, fd = torch.from_numpy(u).cuda(), torch.from_numpy(np.ascontiguousarray(f)).cuda()
@@ -118,7 +118,7 @@ def test_device_resident_assembly_matches_host_entry_points():
     assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
     assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
     ctx.close()
-    ctx, cloud, delta, u0 = problems.cahnhilliard_config3(nx=32, ny=32)
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=32, ny=32)
     u1 = u0.copy()
     u1[1::2] = 0.05 * np.random.RandomState(4).rand(cloud.n)
     kappa = 1e-2 / 32 ** 2

@@ -76,12 +76,12 @@ def _random_list(cloud, rs, ndof):
 def test_bc_rows_vs_oracle_and_properties(kind, nx):
     rs = np.random.RandomState(nx + (7 if kind == "ch" else 0))
     if kind == "poisson":
-        ctx, cloud, _, f = problems.poisson_config1(nx=nx, ny=nx)
+        ctx, cloud, _, f = problems.poisson_square(nx=nx, ny=nx)
         ndof = 1
         u = rs.standard_normal(cloud.n)
         k, r = ctx.assemble_poisson(1.0, u, f_node=f)
     else:
-        ctx, cloud, _, u0 = problems.cahnhilliard_config3(nx=nx, ny=nx)
+        ctx, cloud, _, u0 = problems.cahnhilliard_square(nx=nx, ny=nx)
         ndof = 2
         u = u0.copy()
         u[1::2] = 0.1 * rs.standard_normal(cloud.n)
@@ -120,7 +120,7 @@ def test_bc_rows_vs_oracle_and_properties(kind, nx):
 
 
 def test_bc_rows_errors():
-    ctx, cloud, _, _ = problems.poisson_config1(nx=20, ny=20)
+    ctx, cloud, _, _ = problems.poisson_square(nx=20, ny=20)
     bulk = np.array([1, 2, 3], np.int32)          # bulk nodes have no mirror
     with pytest.raises(pxg.PxgError):
         ctx.csr_set_bcs([{"type": "neumann", "nodes": bulk, "comps": [0]}])

@@ -12,8 +12,8 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     (2, 20, 12, 1, (1.0, 0.6, 1.0)),
-    (2, 100, 100, 1, (1.0, 1.0, 1.0)),       # config 1 grid
-    (2, 400, 200, 1, (0.2, 0.1, 1.0)),       # config 2 grid
+    (2, 100, 100, 1, (1.0, 1.0, 1.0)),       # Poisson square
+    (2, 400, 200, 1, (0.2, 0.1, 1.0)),       # fracture plate
     (3, 7, 6, 5, (0.7, 0.6, 0.5)),
     (3, 24, 12, 6, (0.024, 0.012, 0.006)),
 ]

@@ -62,7 +62,7 @@ def test_families_csr_ops_poisson_vs_reference(name):
 
 
 @pytest.mark.parametrize("name", ["explicit_2d.npz", "explicit_3d.npz"])
-def test_explicit_fp64_vs_reference(name):
+def test_explicit_fp64_vs_reference(name, oracle):
     print("fixture", name, flush=True)
     z = load(name)
     dim = int(z["dim"])
@@ -81,6 +81,21 @@ def test_explicit_fp64_vs_reference(name):
     print("broken %d, mismatches %d, rel-L2 %.3e" % (int((h == 0).sum()), mism, rel))
     assert mism == 0
     assert rel <= 1e-9
+    # output projections of the final state: bit for bit against the oracle
+    # on the device's own state and g-weights, and within the trajectory's
+    # tolerance against the reference's projections
+    m = oracle.Mesh(oracle.make_grid(dim, nx, ny, nz, hi=tuple(float(v) for v in z["hi"])))
+    ops, _ = s.ctx.operators()
+    opc = oracle.frac_opcache(m, ops)
+    p = oracle.frac_params(dim, float(z["E"]), float(z["nu"]), float(z["rho"]), s.params.sc)
+    for nm in oracle.PROJECTIONS:
+        d = s.ctx.frac_project(nm)
+        o = oracle.frac_project(m, p, opc, h.astype(float), u, nm)
+        assert np.array_equal(d.view(np.uint64), o.view(np.uint64)), nm
+        ref = z["proj_" + nm]
+        err = np.abs(d - ref).max() / max(np.abs(ref).max(), 1e-300)
+        print("projection %s: max rel to the reference %.2e" % (nm, err))
+        assert err <= 1e-7 if nm != "damage" else np.array_equal(d, ref)
     s.ctx.close()
 
 

@@ -327,3 +327,17 @@ def test_adjoint_matches_finite_differences():
         fd = (jp - jm) / 2.0
         ad = float(g @ hE)
         assert abs(fd - ad) <= 1e-4 * max(abs(fd), abs(ad)), (fd, ad)
+    # dJ/db of the uniform body force: central differences per component
+    # (J is quadratic in b at frozen health, so they are exact to rounding)
+    _, _, gb = pxo.frac_adjoint(m, p, opc, bcs, dt, nsteps, u0, bs, nodeE, sens, 1, want_b=True)
+    for a in range(2):
+        hb = 1e3 * rho                       # 1000 m/s^2 of body acceleration
+        bp = [0.0, 0.0, 0.0]; bp[a] = hb
+        bm = [0.0, 0.0, 0.0]; bm[a] = -hb
+        jp = pxo.frac_loss_frozen(m, pxo.frac_params(2, E, nu, rho, sc, body=bp), opc, bcs, dt,
+                                  nsteps, u0, bs, nodeE, sens, 1)
+        jm = pxo.frac_loss_frozen(m, pxo.frac_params(2, E, nu, rho, sc, body=bm), opc, bcs, dt,
+                                  nsteps, u0, bs, nodeE, sens, 1)
+        fd = (jp - jm) / (2 * hb)
+        assert abs(fd - gb[a]) <= 1e-6 * max(abs(fd), abs(gb[a])), (a, fd, gb[a])
+    assert abs(gb[1]) > 0
