@@ -275,6 +275,14 @@ def block_case(problems, pxg, steps=100):
                                   "traffic": profiled_traffic("frac_step_block160x80x40", name),
                                   "kernel": "k_frac_step<3,%s>" % name,
                                   "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms}}
+        pst, tiled, nclasses, nshared = s.ctx.frac_layout()
+        out[name]["layout"] = {"bond_slots": pst, "tiled": bool(tiled), "shared_operator_runs": nclasses,
+                               "nodes_on_shared_runs": nshared}
+        if name == "fp32c":
+            out[name]["roofline"]["note"] = (
+                "achieved prices the launch at the model's per-bond bytes (id, 6 FP32 operators, health); "
+                "the kernel reads the operators of nodes on shared runs from L2, so its measured HBM "
+                "traffic (traffic) is well below that figure")
         out["nodes"] = n
         out["bonds"] = nb
         s.ctx.close()
@@ -315,6 +323,7 @@ def assembly_case(problems, pxg, reps=20):
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["poisson_100x100"].update(bc_rows_timing(ctx, u, k, r, poisson, reps))
     ctx.close()
     ctx, cloud, delta, u0 = problems.cahnhilliard_square()
     n, nnz = cloud.n, ctx.nnz
@@ -337,12 +346,35 @@ def assembly_case(problems, pxg, reps=20):
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["cahnhilliard_256x256"].update(bc_rows_timing(ctx, ud, k, r, ch, reps))
     ctx.close()
     try:
         out["cpu_baseline"] = cpu_assembly_baseline(u_h, u0, u1, kappa)
     except Exception as e:       # reported, never required
         out["cpu_baseline"] = {"value": None, "sample": "failed: %s" % e}
     return out
+
+
+def bc_rows_timing(ctx, u, k, r, assemble, reps):
+    """BC row elimination (BCSystem::applyBCs) of the assembled system on the
+    device, alone and with the assembly (the whole linear-system build of one
+    Newton/Picard iteration), inputs and outputs resident in HBM."""
+    nrows = ctx.csr_set_bcs_count
+    assemble()
+    ctx.csr_apply_bcs_dev(0.0, u.data_ptr(), k.data_ptr(), r.data_ptr())
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.csr_apply_bcs_dev(0.0, u.data_ptr(), k.data_ptr(), r.data_ptr())
+    ctx.sync()
+    ms_bc = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        assemble()
+        ctx.csr_apply_bcs_dev(0.0, u.data_ptr(), k.data_ptr(), r.data_ptr())
+    ctx.sync()
+    ms_sys = (time.perf_counter() - t0) * 1e3 / reps
+    return {"bc_rows": nrows, "ms_bc_rows": ms_bc, "ms_assembly_plus_bc_rows": ms_sys}
 
 
 def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
@@ -361,9 +393,11 @@ def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
         c = refpy.Case(d)
         c.assemble(u_poisson, u_poisson, 0.0, parallel=True)
         t0 = time.perf_counter()
-        c.assemble(u_poisson, u_poisson, 0.0, parallel=True)
+        k, r = c.assemble(u_poisson, u_poisson, 0.0, parallel=True)
         ms = (time.perf_counter() - t0) * 1e3
-        res["poisson_100x100"] = {"nnz": c.nnz, "ms_per_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3)}
+        c.apply_bcs(u_poisson, 0.0, k, r)
+        res["poisson_100x100"] = {"nnz": c.nnz, "ms_per_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3),
+                                  "ms_bc_rows": c.last_bc_seconds() * 1e3}
         c.close()
         d = {"Mesh": refpy.mesh_block(2, 256, 256), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
              "DOFs": ["c", "mu"],
@@ -375,13 +409,16 @@ def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
         c = refpy.Case(d)
         c.assemble(u1_ch, u0_ch, 1e-3, parallel=True)
         t0 = time.perf_counter()
-        c.assemble(u1_ch, u0_ch, 1e-3, parallel=True)
+        k, r = c.assemble(u1_ch, u0_ch, 1e-3, parallel=True)
         ms = (time.perf_counter() - t0) * 1e3
-        res["cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3)}
+        c.apply_bcs(u1_ch, 0.0, k, r)
+        res["cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3),
+                                       "ms_bc_rows": c.last_bc_seconds() * 1e3}
         c.close()
     res["kind"] = "reference"
     res["cores"] = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    res["sample"] = "one assembly call of each system after a warm-up call (OpenMP assembler)"
+    res["sample"] = ("one assembly call of each system after a warm-up call (OpenMP assembler), "
+                     "then one BCSystem::applyBCs (serial, timed alone inside the harness)")
     return res
 
 

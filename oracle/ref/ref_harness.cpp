@@ -44,6 +44,11 @@
 #undef protected
 
 namespace {
+double &last_bc_seconds()
+{
+    static double s = 0.0;
+    return s;
+}
 struct RefCase {
     PDProblem prob;
     std::string deck;
@@ -224,10 +229,16 @@ void ref_apply_bcs(void *h, const double *u, double t, double *kvals, double *rh
     std::copy(rhs, rhs + n, p.m_RHS.getDataPtr());
     LocalElmtInfo info;
     info.T = t;
+    const auto t0 = std::chrono::steady_clock::now();
     p.m_BCSystem.applyBCs(p.m_PDMesh, p.m_Operators, ndof, U, U, info, p.m_Matrix, p.m_RHS);
+    const auto t1 = std::chrono::steady_clock::now();
+    last_bc_seconds() = std::chrono::duration<double>(t1 - t0).count();
     std::copy(p.m_Matrix.getCSRValuesPtr(), p.m_Matrix.getCSRValuesPtr() + nnz, kvals);
     std::copy(p.m_RHS.getDataPtr(), p.m_RHS.getDataPtr() + n, rhs);
 }
+
+// wall time of the applyBCs call alone in the last ref_apply_bcs
+double ref_last_bc_seconds() { return last_bc_seconds(); }
 
 // CH element geometry (valid after one ref_assemble): lapbond, rev
 void ref_ch_geometry(void *h, double *lapbond, int *rev, double *mnode)

@@ -34,6 +34,7 @@ def lib():
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
+        _LIB.ref_last_bc_seconds.restype = C.c_double
         _LIB.ref_explicit_timed.argtypes = [C.c_void_p, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), D]
         _LIB.ref_group.restype = C.c_int
@@ -133,6 +134,10 @@ class Case:
         k = np.ascontiguousarray(k, float).copy(); r = np.ascontiguousarray(r, float).copy()
         lib().ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
         return k, r
+
+    def last_bc_seconds(self):
+        """Wall time of the applyBCs call alone in the last apply_bcs."""
+        return lib().ref_last_bc_seconds()
 
     def projection(self, name):
         """computeNodalProjection of every node (n x 1 or n x 9)."""

@@ -53,7 +53,8 @@ SYMBOLS = [
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
     "pxg_csr_apply_bcs_dev", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
-    "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state", "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
+    "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
+    "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
 ]
 
@@ -316,6 +317,7 @@ class Context:
         _check(load().pxg_csr_set_bcs(self.h, n, arr))
         cnt = C.c_int()
         _check(load().pxg_csr_bc_rows(self.h, C.byref(cnt)))
+        self.csr_set_bcs_count = cnt.value
         return cnt.value
 
     def csr_apply_bcs(self, t, u, k, r):
