@@ -371,7 +371,7 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     F.pofs.release();
     F.nshared = 0;
     F.nclasses = 0;
-    if (!F.perm.p || F.precision != PXG_FP32C) return PXG_OK;   // tiled (staged) layout only
+    if (!F.bsh.p || F.precision != PXG_FP32C) return PXG_OK;
     const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
     hipStream_t st = ctx->stream;
     dbuf<double> P;                     // FP64 operators, CSR order

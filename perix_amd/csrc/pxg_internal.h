@@ -125,6 +125,7 @@ struct FracState {
     // position + pofs[i] (the run of a representative node whose FP64
     // operators agree to 1e-12 relative); empty when nothing is shared
     dbuf<int> pofs;
+    dbuf<int> nbr_t;              // unstaged (2D): neighbour ids in the bond layout
     long long nshared = 0, nclasses = 0;
     dbuf<double> P64;
     dbuf<float> P32;

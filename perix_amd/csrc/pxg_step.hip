@@ -16,7 +16,9 @@
 //    saturates the texture units long before HBM.  2D (latency-bound, in the
 //    Infinity Cache) gathers straight from global memory over consecutive
 //    node blocks.
-//  * A node's bonds [off[i], off[i+1]) are contiguous; the lanes walk them in
+//  * The per-bond streams use the kernel's own bond layout (bsh: line-aligned
+//    node runs; tile order in 3D, node order for 2D FP32C, CSR order for 2D
+//    FP64), where a node's bonds are contiguous; the lanes walk them in
 //    groups of G consecutive bonds aligned to G, so one G-wide vector load
 //    fetches a group's LDS slots (or ids), its health bytes and each operator
 //    component (all bond arrays are allocated with zeroed slack and the
@@ -57,6 +59,7 @@ struct StepArgs {
     const int *bsh;            // tiled bond layout: the per-bond streams (operators,
                                // slots, health, break steps) of CSR bond t of node i
                                // sit at t + bsh[i] (null: CSR order)
+    const int *nbr_t;          // unstaged path: neighbour ids (1-based) in the bond layout
     const int *pofs;           // FP32C: node i's FP32 operators at stream position
                                // + pofs[i] (a shared run; null: none shared)
     const int *nbr;
@@ -124,8 +127,11 @@ template <int DIM, bool F32> struct StepCfg;
 // S lanes per node (S divides 64), G bonds per lane group, STAGE: neighbour
 // rows staged in LDS.  The 2D plate is latency-bound (its working set lives
 // in the Infinity Cache) and runs faster gathering straight from global
-// memory; the 3D block saturates the texture units without staging.
-template <> struct StepCfg<2, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = true; };
+// memory; the 3D block saturates the texture units without staging.  2D
+// FP32C gives a node 4 lanes (16 nodes per wave: the 81,200-node plate fits
+// the GPU in one round of resident waves); 2D FP64 keeps 8 (its LDS family
+// sums and registers favour fewer nodes per wave).
+template <> struct StepCfg<2, true> { static constexpr int S = 4, G = 4; static constexpr bool STAGE = false, PF2 = true; };
 template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = false; };
 template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = true, PF2 = false; };
 template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; static constexpr bool STAGE = true, PF2 = false; };
@@ -300,9 +306,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     // nbr stays in CSR order, at t - sh)
     int b0 = 0, b1 = 0, sh = 0;
     if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
-    if (STAGE && nvalid) { sh = a.bsh[i]; b0 += sh; b1 += sh; }
+    if (nvalid && (STAGE || a.bsh)) { sh = a.bsh[i]; b0 += sh; b1 += sh; }
     int po = 0;                // operator run offset (shared runs, FP32C)
-    if constexpr (STAGE && F32) {
+    if constexpr (F32) {
         if (nvalid && a.pofs) po = a.pofs[i];
     }
     // BC program of this node (fused path): own entry, entries it mirrors
@@ -329,7 +335,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     }
     auto load_group = [&](unsigned t0, GD &d) {
         if constexpr (STAGE) vload(a.gslot + t0, d.sv);
-        else vload(a.nbr + t0, d.jv);
+        else vload((a.bsh ? a.nbr_t : a.nbr) + t0, d.jv);
         if (a.frozen) {
             int bs[G];
             vload(a.break_step + t0, bs);
@@ -659,13 +665,22 @@ constexpr size_t step_force_rows()
     return FF ? 0 : (size_t)4 * (64 / StepCfg<D, FF>::S) * (StepCfg<D, FF>::G * StepCfg<D, FF>::S * D + 1);
 }
 
-// nodes per step block; both precisions tile nodes alike, so one LDS
-// neighbour table serves either
+// nodes per step block of the staged (3D) path; both precisions tile nodes
+// alike there, so one LDS neighbour table serves either
 template <int D>
 constexpr int step_npb()
 {
-    static_assert(StepCfg<D, true>::S == StepCfg<D, false>::S, "precisions must share the node tiling");
+    // staged blocks are node tiles that both precisions share
+    static_assert(!StepCfg<D, true>::STAGE || StepCfg<D, true>::S == StepCfg<D, false>::S,
+                  "precisions must share the node tiling");
     return 4 * (64 / StepCfg<D, true>::S);
+}
+
+// nodes per block of one launched configuration
+static int launch_npb(int dim, bool f32)
+{
+    if (dim == 3) return f32 ? 4 * (64 / StepCfg<3, true>::S) : 4 * (64 / StepCfg<3, false>::S);
+    return f32 ? 4 * (64 / StepCfg<2, true>::S) : 4 * (64 / StepCfg<2, false>::S);
 }
 
 
@@ -675,7 +690,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
 {
     FracState &F = ctx->frac;
     StepArgs a;
-    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.pofs = F.pofs.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.pofs = F.pofs.p; a.nbr_t = F.nbr_t.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
     a.perm = F.perm.p; a.roff = F.roff.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
     a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
@@ -696,7 +711,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     PXG_CHECK(!fused || ((mode == MODE_CD || mode == MODE_VAR) && a.prog_of && a.inv_off),
               PXG_ERR_STATE, "launch_step_fused: CD/VAR steps with a BC program only (pxg_frac_set_bcs)");
     hipStream_t st = ctx->stream;
-    a.nblk = (int)gridn(ctx->n, F.npb);
+    a.nblk = (int)gridn(ctx->n, launch_npb(ctx->dim, f32));
     const dim3 grid(8 * gridn(a.nblk, 8));
 #define PXG_LAUNCH(D, FF, M, FU)                                                                    \
     hipLaunchKernelGGL((k_frac_step<D, FF, M, FU>), grid, dim3(256),                                  \
@@ -748,6 +763,10 @@ int launch_guard_final(pxg_ctx *ctx, int step)
 // the slot budget gather from global memory instead (bovf).  The per-bond
 // operator and slot streams are stored in the same tile order (bsh), so each
 // block reads one contiguous range of them.
+// node runs of the per-bond streams start on a 128-byte line of the FP32
+// operator arrays (32 bonds)
+constexpr int kBondAlign = 32;
+
 int build_step_gather(pxg_ctx *ctx)
 {
     FracState &F = ctx->frac;
@@ -756,13 +775,46 @@ int build_step_gather(pxg_ctx *ctx)
     const int npb = (dim == 3) ? step_npb<3>() : step_npb<2>();
     const bool staged = (dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
     F.npb = npb;
-    if (!staged) {
+    if (!staged && F.precision != PXG_FP32C) {
+        // FP64 reads every bond's own operators: CSR order (line-aligned runs
+        // only add padding reads to its latency-bound 2D path)
         F.gcap = 0;
         F.pst = F.ps;
         F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
-        F.bsh.release();
+        F.bsh.release(); F.nbr_t.release();
         return PXG_OK;
     }
+    if (!staged) {
+        // FP32C: node order kept; the per-bond streams (and a copy of the
+        // neighbour ids) in line-aligned runs, as in the staged layout, so
+        // that the operator runs can be shared
+        F.gcap = 0;
+        F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
+        std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
+        PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream));
+        if (nb > 0)
+            PXG_HIP(hipMemcpyAsync(nbr.data(), ctx->nbr.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, ctx->stream));
+        PXG_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<int> bsh((size_t)n);
+        long long pos = 0;
+        for (int i = 0; i < n; i++) {
+            pos = (pos + kBondAlign - 1) / kBondAlign * kBondAlign;
+            bsh[(size_t)i] = (int)(pos - off[(size_t)i]);
+            pos += off[(size_t)i + 1] - off[(size_t)i];
+        }
+        PXG_CHECK(pos < INT_MAX - 64, PXG_ERR_ARG, "build_step_gather: bond layout exceeds 2^31 entries");
+        F.pst = (pos + 63) / 64 * 64;
+        std::vector<int> nt((size_t)std::max(F.pst, 1LL), 0);
+        for (int i = 0; i < n; i++)
+            for (int t = off[(size_t)i]; t < off[(size_t)i + 1]; t++) nt[(size_t)(t + bsh[(size_t)i])] = nbr[(size_t)t];
+        PXG_HIP(F.bsh.alloc(bsh.size()));
+        PXG_HIP(F.nbr_t.alloc(nt.size()));
+        PXG_HIP(hipMemcpyAsync(F.bsh.p, bsh.data(), sizeof(int) * bsh.size(), hipMemcpyHostToDevice, ctx->stream));
+        PXG_HIP(hipMemcpyAsync(F.nbr_t.p, nt.data(), sizeof(int) * nt.size(), hipMemcpyHostToDevice, ctx->stream));
+        PXG_HIP(hipStreamSynchronize(ctx->stream));
+        return PXG_OK;
+    }
+    F.nbr_t.release();
     constexpr int TX = 2, TY = 4, TZ = 4;
     const int capmax = 640;   // rows per block: 640 x 48 B = 30 KB of LDS
     const int nblk = (int)gridn(n, npb);
@@ -800,7 +852,6 @@ int build_step_gather(pxg_ctx *ctx)
     // Runs that shared a line with the next node made the line be fetched
     // twice, once per node, because the many streams in flight evict it
     // from L2 in between.
-    constexpr int kBondAlign = 32;
     std::vector<int> bsh((size_t)n);
     long long pos = 0;
     for (int q = 0; q < n; q++) {
