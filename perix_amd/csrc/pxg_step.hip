@@ -322,6 +322,19 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
         xi[c] = (nvalid && live_dmg) ? a.xp[(size_t)i * DIM + c] : 0.0;
     }
+    // unstaged (latency-bound) path: the update's own inputs are loaded up
+    // front so that their latency hides behind the family loop (when out
+    // aliases uold, the lane reads its own entry before writing it; preset
+    // targets written by other lanes do not use theirs).  The staged path
+    // loads them at the update: holding them costs it occupancy.
+    double uo_pre = 0.0;
+    int m_pre = 0;
+    if (!STAGE && nvalid && sl < DIM) {
+        if constexpr (MODE != MODE_ACCEL) uo_pre = a.uold[(size_t)i * DIM + sl];
+        if constexpr (FUSED) {
+            if (bq >= 0) m_pre = a.bmode[bq * 3 + sl];
+        }
+    }
     const int A = b0 & ~(G - 1);
     int broke = 0;
 
@@ -594,19 +607,22 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         // preset mode of this component (fused path): 0 none, 1 copy and
         // 2 reflection (written by the mirror source's lane), 3 direct
         const int q = bq;
-        int m = 0;
-        if constexpr (FUSED) {
+        int m = m_pre;
+        if constexpr (STAGE && FUSED) {
             if (q >= 0) m = a.bmode[q * 3 + c];
+        }
+        if constexpr (STAGE) {
+            if (MODE != MODE_ACCEL) uo_pre = a.uold[idx];
         }
         double v = 0.0;
         if (m == 0) {
             const double bb = (c == 0) ? a.b0 : (c == 1 ? a.b1 : a.b2);
             const double acc = (Lown + bb) / a.rho;
             const double uu = pick(ui, c);
-            if (MODE == MODE_CD) v = 2.0 * uu - a.uold[idx] + a.c1 * acc;
-            else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - a.uold[idx]) + a.c2 * acc;
+            if (MODE == MODE_CD) v = 2.0 * uu - uo_pre + a.c1 * acc;
+            else if (MODE == MODE_VAR) v = uu + a.c1 * (uu - uo_pre) + a.c2 * acc;
             else if (MODE == MODE_ACCEL) v = acc;
-            else v = uu - a.c1 * a.uold[idx] + a.c2 * acc;
+            else v = uu - a.c1 * uo_pre + a.c2 * acc;
             a.out[idx] = v;
             if (FUSED || ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i])) fold(v);
         } else if (m == 3) {
