@@ -135,6 +135,19 @@ int pxg_assemble_poisson_dev(pxg_ctx *ctx, double sigma, const double *f_node_de
                              double f_const, const double *u_dev,
                              double *kvals_dev, double *rhs_dev);
 
+/* ---- remaining elements: diffusion ---------------------------------------
+ * DiffusionElement (DiffusionElement.cpp:28-62) through the same assembler:
+ * backward Euler, K = -dF/dU with bond coeff = -D lap_ij and nodal
+ * R = (u - uold)/dt - f, K = 1/dt.  f_node (optional, n) generalises the
+ * scalar f_const.  Host pointers: u, uold [n], kvals[nnz], rhs[n]; _dev:
+ * device pointers. */
+int pxg_assemble_diffusion(pxg_ctx *ctx, double D, const double *f_node, double f_const,
+                           double dt, const double *u, const double *uold, double *kvals,
+                           double *rhs);
+int pxg_assemble_diffusion_dev(pxg_ctx *ctx, double D, const double *f_node_dev,
+                               double f_const, double dt, const double *u_dev,
+                               const double *uold_dev, double *kvals_dev, double *rhs_dev);
+
 /* ---- a12-a13: Cahn-Hilliard ---------------------------------------------
  * Replaces perix_cuda::assembleCahnHilliard (CudaAssemble.h:139-142) and the
  * host-built geometry of CahnHilliardElement::preprocessIteration

@@ -75,6 +75,12 @@ void pxo_assemble_poisson(int n, int dim, int nop, const int *off, const int *nb
                           const double *ops, const int *rowptr, const int *cols,
                           double sigma, const double *f_node, double f_const,
                           const double *u, double *kvals, double *rhs);
+/* DiffusionElement (DiffusionElement.cpp:28-62) through the same assembler:
+ * bond coeff = -D lap_ij, nodal R = (u_i - uold_i)/dt - f, nodal K = 1/dt. */
+void pxo_assemble_diffusion(int n, int dim, int nop, const int *off, const int *nbr,
+                            const double *ops, const int *rowptr, const int *cols,
+                            double D, double f, double dt, const double *u,
+                            const double *uold, double *kvals, double *rhs);
 
 /* ---- a12/a13: Cahn-Hilliard -------------------------------------------- */
 void pxo_ch_geometry(int n, int dim, int nop, const double *coords,

@@ -159,6 +159,18 @@ def assemble_poisson(m, ops, rowptr, cols, sigma, u, f_node=None, f_const=0.0):
     return k, r
 
 
+def assemble_diffusion(m, ops, rowptr, cols, D, f, dt, u, uold):
+    """DiffusionElement through the CSR assembler: (K values, residual)."""
+    nop = ops.shape[1] if ops.ndim == 2 else (9 if m.dim == 3 else 5)
+    k = np.zeros(int(rowptr[-1])); r = np.zeros(m.n)
+    lib().pxo_assemble_diffusion(m.n, m.dim, nop, ip(m.off), ip(m.nbr),
+                                 dp(np.ascontiguousarray(ops)), ip(np.ascontiguousarray(rowptr, np.int32)),
+                                 ip(np.ascontiguousarray(cols, np.int32)), C.c_double(D), C.c_double(f),
+                                 C.c_double(dt), dp(np.ascontiguousarray(u, float)),
+                                 dp(np.ascontiguousarray(uold, float)), dp(k), dp(r))
+    return k, r
+
+
 def ch_geometry(m, ops):
     L = lib()
     lap = np.zeros(m.nb); rev = np.zeros(m.nb, np.int32); isg = np.zeros(m.n, np.uint8)

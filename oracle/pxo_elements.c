@@ -48,6 +48,32 @@ void pxo_assemble_poisson(int n, int dim, int nop, const int *off, const int *nb
     }
 }
 
+/* DiffusionElement::computeBondResidualAndJacobian / computeNodalResidual-
+ * AndJacobian (DiffusionElement.cpp:28-62) in the assembler of
+ * FormResidualAndJacobian.cpp:25-132: bond coeff = -D * lap_ij,
+ * R_i += coeff (u_j - u_i), K_II = -coeff, K_IJ = coeff; nodal
+ * R = (u_i - uold_i)/dt - f, K = 1/dt; K = -dF/dU. */
+void pxo_assemble_diffusion(int n, int dim, int nop, const int *off, const int *nbr,
+                            const double *ops, const int *rowptr, const int *cols,
+                            double D, double f, double dt, const double *u,
+                            const double *uold, double *kvals, double *rhs)
+{
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[n]);
+    memset(rhs, 0, sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int d = find_slot(rowptr, cols, i, i);
+        rhs[i] += (u[i] - uold[i]) / dt - f;
+        kvals[d] += -(1.0 / dt);
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const double coeff = -D * lap_of(ops + (size_t)t * nop, dim);
+            rhs[i] += coeff * (u[j] - u[i]);
+            kvals[d] += -(-coeff);
+            kvals[find_slot(rowptr, cols, i, j)] += -coeff;
+        }
+    }
+}
+
 /* CahnHilliardElement::preprocessIteration one-off geometry
  * (CahnHilliardElement.cpp:93-153). */
 void pxo_ch_geometry(int n, int dim, int nop, const double *coords,

@@ -38,9 +38,14 @@ __global__ void k_lap(long long nb, int dim, int nop, const double *ops, double 
     lap[t] = pd_lap(ops + (size_t)t * nop, dim);
 }
 
+// Poisson (PoissonElement.cpp:26-58: coeff = sigma lap, nodal R = f, K = 0)
+// and, with uold, diffusion (DiffusionElement.cpp:28-62: coeff = -D lap,
+// nodal R = (u_i - uold_i)/dt - f, K = 1/dt) through the assembler's
+// accumulation order: nodal term first, then the bonds in family order.
 __global__ void __launch_bounds__(256) k_poisson(int n, const int *off, const int *nbr, const double *lap,
                                                  const int *slot, const int *diag, const int *mslot,
                                                  double sigma, const double *fnode, double fconst,
+                                                 double dt, const double *uold,
                                                  const double *u, double *kvals, double *rhs)
 {
     __shared__ int s_off[AT + 1];
@@ -56,8 +61,14 @@ __global__ void __launch_bounds__(256) k_poisson(int n, const int *off, const in
     double kd = 0.0, r = 0.0;
     if (tid < nt) {
         const int i = i0 + tid;
-        r = r + (fnode ? fnode[i] : fconst);   // nodal R = f
-        kd = kd + (-0.0);                      // nodal K = 0
+        const double fv = fnode ? fnode[i] : fconst;
+        if (uold) {
+            r = r + ((s_u[tid] - uold[i]) / dt - fv);   // nodal R = (u - uold)/dt - f
+            kd = kd + (-(1.0 / dt));                   // nodal K = 1/dt
+        } else {
+            r = r + fv;                                // nodal R = f
+            kd = kd + (-0.0);                          // nodal K = 0
+        }
         if (mslot[i] >= 0) kvals[mslot[i]] = 0.0;
     }
     for (int cs = bt0; cs < bt1; cs += ACH) {
@@ -254,7 +265,7 @@ extern "C" int pxg_assemble_poisson_dev(pxg_ctx *ctx, double sigma, const double
     if (rc) return rc;
     hipLaunchKernelGGL(k_poisson, dim3(gridb(ctx->n, AT)), dim3(256), 0, ctx->stream, ctx->n, ctx->off.p,
                        ctx->nbr.p, ctx->lap.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, sigma, f_node_dev,
-                       f_const, u_dev, kvals_dev, rhs_dev);
+                       f_const, 0.0, (const double *)nullptr, u_dev, kvals_dev, rhs_dev);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
@@ -275,6 +286,51 @@ extern "C" int pxg_assemble_poisson(pxg_ctx *ctx, double sigma, const double *f_
     }
     int rc = pxg_assemble_poisson_dev(ctx, sigma, f_node ? ctx->sf.p : nullptr, f_const, ctx->su.p, ctx->sk.p,
                                       ctx->sr.p);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_diffusion_dev(pxg_ctx *ctx, double D, const double *f_node_dev, double f_const,
+                                          double dt, const double *u_dev, const double *uold_dev,
+                                          double *kvals_dev, double *rhs_dev)
+{
+    PXG_CHECK(ctx && ctx->nop > 0 && ctx->nnz > 0 && ctx->csr_ndof == 1, PXG_ERR_STATE,
+              "pxg_assemble_diffusion: needs operators and a 1-dof CSR pattern");
+    PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev && dt > 0.0, PXG_ERR_ARG,
+              "pxg_assemble_diffusion: bad arguments (the element is transient-only, dt > 0)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_diffusion: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_lap_poisson(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_poisson, dim3(gridb(ctx->n, AT)), dim3(256), 0, ctx->stream, ctx->n, ctx->off.p,
+                       ctx->nbr.p, ctx->lap.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, -D, f_node_dev,
+                       f_const, dt, uold_dev, u_dev, kvals_dev, rhs_dev);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_diffusion(pxg_ctx *ctx, double D, const double *f_node, double f_const, double dt,
+                                      const double *u, const double *uold, double *kvals, double *rhs)
+{
+    PXG_CHECK(ctx && ctx->nnz > 0 && u && uold && kvals && rhs, PXG_ERR_ARG,
+              "pxg_assemble_diffusion: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PXG_HIP(ctx->su.alloc(ctx->n));
+    PXG_HIP(ctx->suo.alloc(ctx->n));
+    PXG_HIP(ctx->sk.alloc(ctx->nnz));
+    PXG_HIP(ctx->sr.alloc(ctx->n));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
+    if (f_node) {
+        PXG_HIP(ctx->sf.alloc(ctx->n));
+        PXG_HIP(hipMemcpyAsync(ctx->sf.p, f_node, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
+    }
+    int rc = pxg_assemble_diffusion_dev(ctx, D, f_node ? ctx->sf.p : nullptr, f_const, dt, ctx->su.p, ctx->suo.p,
+                                        ctx->sk.p, ctx->sr.p);
     if (rc) return rc;
     PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, st));

@@ -19,6 +19,8 @@ Fixtures (all small, .npz):
   explicit_3d.npz   16x8x4 block, one notch, impactor, 200 FP64 steps
   ch_2d.npz         16x16 Cahn-Hilliard: RandomIC + Neumann presets, CSR K and
                     residual for a perturbed state (Picard-frozen mobility)
+  diffusion_*.npz   DiffusionElement assembly (D = 0.7, f = 0.3, dt = 2e-3) of
+                    the mesh_fam grids for a seeded (u, u_old)
   bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
                     Poisson (20x12) / Cahn-Hilliard (16x16) system under an
                     overlapping Dirichlet/Neumann list, with zeroed diagonals
@@ -134,6 +136,25 @@ def ch_fixture(name, nx):
     c.close()
 
 
+def diffusion_fixture(name, dim, nx, ny, nz, hi):
+    """DiffusionElement assembly (backward Euler) for a seeded (u, uold)."""
+    d = {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "DOFs": ["c"],
+         "ElmtSystem": {"d": {"type": "diffusion", "params": {"D": 0.7, "f": 0.3}}},
+         "BCSystem": {"l": {"type": "dirichlet", "phygroup": "leftnodes_ghost", "dofs": ["c"],
+                            "value": 0.0}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 2e-3, "total_time": 2e-2}}
+    c = refpy.Case(d)
+    rs = np.random.RandomState(9)
+    u = rs.rand(c.n); uold = u + 0.01 * rs.standard_normal(c.n)
+    k, r = c.assemble(u, uold, 2e-3)
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), D=0.7, f=0.3, dt=2e-3, u=u, uold=uold, K=k, R=r)
+    c.close()
+
+
 # BC lists of the row-elimination fixtures: (name, type, group, dofs, value,
 # velocity, direct, box) in deck order; overlapping entries on one row
 # exercise the list order
@@ -225,6 +246,8 @@ def main():
     explicit_fixture("explicit_3d.npz", 3, 16, 8, 4, (0.016, 0.008, 0.004),
                      [(0.008, 0.0, 0.008, 0.0008)], [0.004, 0.006, -1.0, 1.0, -1.0, 1.0], 200)
     ch_fixture("ch_2d.npz", 16)
+    diffusion_fixture("diffusion_2d.npz", 2, 20, 12, 1, (1.0, 0.6, 1.0))
+    diffusion_fixture("diffusion_3d.npz", 3, 6, 5, 4, (0.6, 0.5, 0.4))
     bcrows_fixture("bcrows_poisson_2d.npz", "poisson")
     bcrows_fixture("bcrows_ch_2d.npz", "ch")
     for f in sorted(os.listdir(HERE)):

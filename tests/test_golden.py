@@ -134,3 +134,15 @@ def test_bc_row_elimination(name):
         elif scale == 1.0 and not K[s:e].any():
             kinds.add("unit")
     assert kinds == {"diag", "rowmax", "unit"}
+
+
+@pytest.mark.parametrize("name", ["diffusion_2d.npz", "diffusion_3d.npz"])
+def test_diffusion_assembly(name):
+    """DiffusionElement through the assembler: K and the residual bit for bit."""
+    z = load(name)
+    m = mesh_of(dict(z, factor=3.015))
+    ops, _ = pxo.operators(m)
+    rp, cols = pxo.csr_pattern(m, 1)
+    k, r = pxo.assemble_diffusion(m, ops, rp, cols, float(z["D"]), float(z["f"]), float(z["dt"]),
+                                  z["u"], z["uold"])
+    assert same(k, z["K"]) and same(r, z["R"])

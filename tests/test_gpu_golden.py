@@ -120,3 +120,36 @@ def test_cahn_hilliard_vs_reference():
     assert np.abs(k - z["K"]).max() <= 1e-12 * kscale
     assert np.abs(r - z["R"]).max() <= 1e-12 * max(kscale, np.abs(z["R"]).max())
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["diffusion_2d.npz", "diffusion_3d.npz"])
+def test_diffusion_vs_reference(name, oracle):
+    """DiffusionElement assembly: bit for bit against the oracle on the
+    device's g-weights, within 1e-12 of the term scale against the reference."""
+    z = load(name)
+    ctx, cloud, delta = device_mesh(z)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(1)
+    D, f, dt = float(z["D"]), float(z["f"]), float(z["dt"])
+    k, r = ctx.assemble_diffusion(D, dt, z["u"], z["uold"], f_const=f)
+    g = cloud.grid
+    m = oracle.Mesh(oracle.make_grid(g.dim, g.nx, g.ny, g.nz, hi=g.hi))
+    ops, _ = ctx.operators()
+    rp, cols = ctx.csr()
+    ok, orr = oracle.assemble_diffusion(m, ops, rp, cols, D, f, dt, z["u"], z["uold"])
+    assert np.array_equal(k.view(np.uint64), ok.view(np.uint64))
+    assert np.array_equal(r.view(np.uint64), orr.view(np.uint64))
+    kscale = np.abs(z["K"]).max()
+    assert np.abs(k - z["K"]).max() <= 1e-12 * kscale
+    assert np.abs(r - z["R"]).max() <= 1e-12 * (kscale + np.abs(z["R"]).max())
+    # device pointers give the same bits
+    import torch
+    ud, uod = torch.from_numpy(np.ascontiguousarray(z["u"])).cuda(), torch.from_numpy(np.ascontiguousarray(z["uold"])).cuda()
+    kd = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rd = torch.empty(cloud.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.assemble_diffusion_dev(D, dt, ud.data_ptr(), uod.data_ptr(), kd.data_ptr(), rd.data_ptr(), f_const=f)
+    ctx.sync()
+    assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
+    assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
+    ctx.close()
