@@ -275,6 +275,20 @@ def block_case(problems, pxg, steps=100):
                                   "traffic": profiled_traffic("frac_step_block160x80x40", name),
                                   "kernel": "k_frac_step<3,%s>" % name,
                                   "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms}}
+        if name == "fp64":
+            # output projections of the final state (device, n or 9n doubles)
+            import torch
+            proj = {}
+            buf = torch.empty(9 * n, dtype=torch.float64, device="cuda")
+            for which, pname in enumerate(("damage", "vonMises", "stress", "strain")):
+                s.ctx.frac_project_dev(which, buf.data_ptr())
+                s.ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    s.ctx.frac_project_dev(which, buf.data_ptr())
+                s.ctx.sync()
+                proj[pname + "_ms"] = (time.perf_counter() - t0) * 1e3 / 5
+            out["projections"] = proj
         pst, tiled, nclasses, nshared = s.ctx.frac_layout()
         out[name]["layout"] = {"bond_slots": pst, "tiled": bool(tiled), "shared_operator_runs": nclasses,
                                "nodes_on_shared_runs": nshared}
@@ -324,6 +338,28 @@ def assembly_case(problems, pxg, reps=20):
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
     out["poisson_100x100"].update(bc_rows_timing(ctx, u, k, r, poisson, reps))
+    ctx.close()
+    # diffusion (backward Euler) on the same 100x100 square
+    ctx, cloud, delta, f = problems.poisson_square()
+    n, nnz = cloud.n, ctx.nnz
+    uo = torch.from_numpy(u_h * 0.99).cuda()
+    u = torch.from_numpy(u_h).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def diffusion():
+        ctx.assemble_diffusion_dev(0.7, 1e-3, u.data_ptr(), uo.data_ptr(), k.data_ptr(), r.data_ptr(),
+                                   f_const=0.3)
+
+    diffusion()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        diffusion()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["diffusion_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
     ctx.close()
     ctx, cloud, delta, u0 = problems.cahnhilliard_square()
     n, nnz = cloud.n, ctx.nnz

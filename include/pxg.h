@@ -222,6 +222,32 @@ int pxg_csr_apply_bcs(pxg_ctx *ctx, double t, const double *u, double *kvals, do
 int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *kvals_dev,
                           double *rhs_dev);
 
+/* ---- remaining elements: implicit dynamic fracture ------------------------
+ * PDDODynamicFracElement (pddo_dynamic_frac): backward-Euler elastodynamics
+ * on the strong-form PDDO operators with critical-stretch bond failure.
+ * Requires order-2 operators and pxg_build_csr(ctx, dim, ...); p gives the
+ * elastic constants, rho, the body force and the critical stretch
+ * (FractureCriterion.h:30-43 evaluated by the caller), v0 the initial
+ * velocity (dim values, may be NULL) and uold0 (n*dim host) the initial
+ * displacement history; health starts from the crack seed
+ * (PDDODynamicFracElement.cpp:112-147).
+ * pxg_dynfrac_commit: start of a new time step on the converged u_old
+ * (PDDODynamicFracElement.cpp:148-204): vel = (uold - uprev)/dt_prev,
+ * uprev = uold, bond failure from u_old.
+ * pxg_assemble_dynfrac: K = -dF/dU and the residual
+ * (PDDODynamicFracElement.cpp:207-297) for (u, uold) at step size dt.
+ * Host pointers, or device pointers with _dev. */
+int pxg_dynfrac_setup(pxg_ctx *ctx, const pxg_frac_params *p, const double *v0,
+                      const double *uold0);
+int pxg_dynfrac_commit(pxg_ctx *ctx, const double *uold, double dt_prev);
+int pxg_dynfrac_commit_dev(pxg_ctx *ctx, const double *uold_dev, double dt_prev);
+int pxg_assemble_dynfrac(pxg_ctx *ctx, double dt, const double *u, const double *uold,
+                         double *kvals, double *rhs);
+int pxg_assemble_dynfrac_dev(pxg_ctx *ctx, double dt, const double *u_dev,
+                             const double *uold_dev, double *kvals_dev, double *rhs_dev);
+/* bond health (nb bytes, CSR order) and committed velocity (n*dim); either may be NULL */
+int pxg_dynfrac_get_state(pxg_ctx *ctx, unsigned char *health, double *vel);
+
 /* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
  * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):
  * damage with u, then health-gated Navier force, accel = (L+b)/rho.

@@ -150,6 +150,21 @@ void pxo_frac_project(int n, int dim, int plane_stress, double C12, double C66,
                       const int *off, const int *nbr, const double *opc,
                       const double *health, const double *u, int which, double *out);
 
+/* PDDODynamicFracElement (implicit backward-Euler dynamic fracture).
+ * Commit of a new time step on the converged u_old (preprocessIteration,
+ * PDDODynamicFracElement.cpp:148-204): vel = (uold - uprev)/dt_prev,
+ * uprev = uold, then irreversible bond failure from uold.  Assembly
+ * (PDDODynamicFracElement.cpp:207-297 through FormResidualAndJacobian.cpp:
+ * 25-132) with bond health mu (0/1 per bond, family order). */
+void pxo_dynfrac_commit(int n, int dim, const double *coords, const int *off, const int *nbr,
+                        double sc, int tension_only, int damage_on, double dt_prev,
+                        const double *uold, double *uprev, double *vel, double *health);
+void pxo_assemble_dynfrac(int n, int dim, int nop, const int *off, const int *nbr,
+                          const double *ops, const int *rowptr, const int *cols,
+                          double C11, double C12, double C66, double rho, const double *b,
+                          double dt, const double *vel, const double *health,
+                          const double *u, const double *uold, double *kvals, double *rhs);
+
 /* ---- a19: central-difference driver ------------------------------------- */
 /* Step schedule of TimeStepping::solveExplicitDynamics: returns the number of
  * steps; fills dts[k], ts[k] (tNew) when non-NULL (capacity cap). */

@@ -159,6 +159,26 @@ def assemble_poisson(m, ops, rowptr, cols, sigma, u, f_node=None, f_const=0.0):
     return k, r
 
 
+def dynfrac_commit(m, sc, tension_only, damage_on, dt_prev, uold, uprev, vel, health):
+    """Per-step commit of the dynamic fracture element (in place)."""
+    lib().pxo_dynfrac_commit(m.n, m.dim, dp(m.coords), ip(m.off), ip(m.nbr), C.c_double(sc),
+                             int(tension_only), int(damage_on), C.c_double(dt_prev),
+                             dp(np.ascontiguousarray(uold, float)), dp(uprev), dp(vel), dp(health))
+
+
+def assemble_dynfrac(m, ops, rowptr, cols, C11, C12, C66, rho, b, dt, vel, health, u, uold):
+    """Dynamic fracture element through the CSR assembler: (K values, residual)."""
+    k = np.zeros(int(rowptr[-1])); r = np.zeros(m.n * m.dim)
+    lib().pxo_assemble_dynfrac(m.n, m.dim, ops.shape[1], ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(ops)),
+                               ip(np.ascontiguousarray(rowptr, np.int32)), ip(np.ascontiguousarray(cols, np.int32)),
+                               C.c_double(C11), C.c_double(C12), C.c_double(C66), C.c_double(rho),
+                               dp(np.ascontiguousarray(b, float)), C.c_double(dt),
+                               dp(np.ascontiguousarray(vel, float)), dp(np.ascontiguousarray(health, float)),
+                               dp(np.ascontiguousarray(u, float)), dp(np.ascontiguousarray(uold, float)),
+                               dp(k), dp(r))
+    return k, r
+
+
 def assemble_diffusion(m, ops, rowptr, cols, D, f, dt, u, uold):
     """DiffusionElement through the CSR assembler: (K values, residual)."""
     nop = ops.shape[1] if ops.ndim == 2 else (9 if m.dim == 3 else 5)

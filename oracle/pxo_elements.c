@@ -74,6 +74,105 @@ void pxo_assemble_diffusion(int n, int dim, int nop, const int *off, const int *
     }
 }
 
+/* PDDODynamicFracElement::preprocessIteration, per-step commit
+ * (PDDODynamicFracElement.cpp:148-204) */
+void pxo_dynfrac_commit(int n, int dim, const double *coords, const int *off, const int *nbr,
+                        double sc, int tension_only, int damage_on, double dt_prev,
+                        const double *uold, double *uprev, double *vel, double *health)
+{
+    const size_t nd = (size_t)n * dim;
+    for (size_t k = 0; k < nd; k++) {
+        vel[k] = (uold[k] - uprev[k]) / dt_prev;
+        uprev[k] = uold[k];
+    }
+    if (!(damage_on && sc > 0.0)) return;
+    for (int i = 0; i < n; i++) {
+        const double xi0 = coords[3 * i], yi0 = coords[3 * i + 1];
+        const double zi0 = (dim >= 3) ? coords[3 * i + 2] : 0.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            if (health[t] <= 0.0) continue;
+            const int j = nbr[t] - 1;
+            double xi[3], eta[3];
+            xi[0] = coords[3 * j] - xi0;
+            xi[1] = coords[3 * j + 1] - yi0;
+            xi[2] = (dim >= 3) ? coords[3 * j + 2] - zi0 : 0.0;
+            eta[0] = uold[j * dim] - uold[i * dim];
+            eta[1] = uold[j * dim + 1] - uold[i * dim + 1];
+            eta[2] = (dim >= 3) ? uold[j * dim + 2] - uold[i * dim + 2] : 0.0;
+            const double r = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+            if (r <= 0.0) continue;
+            const double yx = xi[0] + eta[0], yy = xi[1] + eta[1], yz = xi[2] + eta[2];
+            const double yr = sqrt(yx * yx + yy * yy + yz * yz);
+            const double s = (yr - r) / r;
+            if (tension_only ? (s > sc) : (fabs(s) > sc)) health[t] = 0.0;
+        }
+    }
+}
+
+/* PDDODynamicFracElement::computeBondResidualAndJacobian / computeNodal-
+ * ResidualAndJacobian (PDDODynamicFracElement.cpp:207-297) in the assembler
+ * of FormResidualAndJacobian.cpp:25-132: nodal R = b - c0 (u - uold - v dt),
+ * K = -c0 I with c0 = rho/dt^2; per intact bond the Navier block
+ * P = mu C:G(i,j), R = P (u_j - u_i), K_II = -P, K_IJ = P; K = -dF/dU. */
+void pxo_assemble_dynfrac(int n, int dim, int nop, const int *off, const int *nbr,
+                          const double *ops, const int *rowptr, const int *cols,
+                          double C11, double C12, double C66, double rho, const double *b,
+                          double dt, const double *vel, const double *health,
+                          const double *u, const double *uold, double *kvals, double *rhs)
+{
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[n * dim]);
+    memset(rhs, 0, sizeof(double) * (size_t)n * dim);
+    const double c0 = rho / (dt * dt);
+    for (int i = 0; i < n; i++) {
+        const int ri = i * dim;
+        for (int a = 0; a < dim; a++) {
+            const double v = vel[ri + a];
+            rhs[ri + a] += b[a] - c0 * (u[ri + a] - uold[ri + a] - v * dt);
+            for (int bb = 0; bb < dim; bb++)
+                kvals[find_slot(rowptr, cols, ri + a, ri + bb)] += -((a == bb) ? -c0 : 0.0);
+        }
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1, rj = j * dim;
+            const double *op = ops + (size_t)t * nop;
+            const double mu = health[t];
+            double R[3] = {0, 0, 0}, P[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            if (mu > 0.0) {
+                if (dim >= 3) {
+                    const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+                    const double aa = C12 + C66;
+                    const double p11 = mu * (C11 * Gxx + C66 * (Gyy + Gzz));
+                    const double p22 = mu * (C11 * Gyy + C66 * (Gxx + Gzz));
+                    const double p33 = mu * (C11 * Gzz + C66 * (Gxx + Gyy));
+                    const double p12 = mu * aa * Gxy, p13 = mu * aa * Gxz, p23 = mu * aa * Gyz;
+                    const double dux = u[rj] - u[ri], duy = u[rj + 1] - u[ri + 1], duz = u[rj + 2] - u[ri + 2];
+                    R[0] = p11 * dux + p12 * duy + p13 * duz;
+                    R[1] = p12 * dux + p22 * duy + p23 * duz;
+                    R[2] = p13 * dux + p23 * duy + p33 * duz;
+                    P[0][0] = p11; P[0][1] = p12; P[0][2] = p13;
+                    P[1][0] = p12; P[1][1] = p22; P[1][2] = p23;
+                    P[2][0] = p13; P[2][1] = p23; P[2][2] = p33;
+                } else {
+                    const double Gxx = op[2], Gxy = op[3], Gyy = op[4];
+                    const double p11 = mu * (C11 * Gxx + C66 * Gyy);
+                    const double p22 = mu * (C66 * Gxx + C11 * Gyy);
+                    const double pxy = mu * (C12 + C66) * Gxy;
+                    const double dux = u[rj] - u[ri], duy = u[rj + 1] - u[ri + 1];
+                    R[0] = p11 * dux + pxy * duy;
+                    R[1] = pxy * dux + p22 * duy;
+                    P[0][0] = p11; P[0][1] = pxy; P[1][0] = pxy; P[1][1] = p22;
+                }
+            }
+            for (int a = 0; a < dim; a++) {
+                rhs[ri + a] += R[a];
+                for (int bb = 0; bb < dim; bb++) {
+                    kvals[find_slot(rowptr, cols, ri + a, ri + bb)] += -(-P[a][bb]);
+                    kvals[find_slot(rowptr, cols, ri + a, rj + bb)] += -P[a][bb];
+                }
+            }
+        }
+    }
+}
+
 /* CahnHilliardElement::preprocessIteration one-off geometry
  * (CahnHilliardElement.cpp:93-153). */
 void pxo_ch_geometry(int n, int dim, int nop, const double *coords,

@@ -40,6 +40,7 @@
 #include "ElmtSystem/ExplicitPDDOFracElement.h"
 #include "ElmtSystem/CahnHilliardElement.h"
 #include "ElmtSystem/PoissonElement.h"
+#include "ElmtSystem/PDDODynamicFracElement.h"
 #undef private
 #undef protected
 
@@ -239,6 +240,48 @@ void ref_apply_bcs(void *h, const double *u, double t, double *kvals, double *rh
 
 // wall time of the applyBCs call alone in the last ref_apply_bcs
 double ref_last_bc_seconds() { return last_bc_seconds(); }
+
+// One serial formResidualAndJacobian as time step `timestep` (Info.Dt = dt,
+// Info.T = timestep*dt): time-split elements commit their per-step state
+// when the step number changes (PDDODynamicFracElement::preprocessIteration).
+void ref_assemble_step(void *h, const double *u, const double *uold, double dt, int timestep, double *kvals,
+                       double *rhs)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int n = p.m_U.getSize();
+    VectorXd U(n), Uo(n);
+    std::copy(u, u + n, U.getDataPtr());
+    std::copy(uold, uold + n, Uo.getDataPtr());
+    LocalElmtInfo info;
+    info.Dt = dt; info.T = dt * timestep; info.Timestep = timestep;
+    p.m_PDSystem.formResidualAndJacobian(p.m_PDMesh, p.m_Operators, p.m_ElmtSystem, info, U, Uo, p.m_Matrix,
+                                         p.m_RHS);
+    const int nnz = p.m_Matrix.getNNZNum();
+    std::copy(p.m_Matrix.getCSRValuesPtr(), p.m_Matrix.getCSRValuesPtr() + nnz, kvals);
+    std::copy(p.m_RHS.getDataPtr(), p.m_RHS.getDataPtr() + n, rhs);
+}
+
+// PDDODynamicFracElement state: bond health (family order) and the committed
+// velocity; the element's body force and initial velocity setters.
+void ref_dynfrac_state(void *h, double *health, double *vel)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<PDDODynamicFracElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    long long q = 0;
+    for (const auto &row : e->getBondHealth())
+        for (double v : row) health[q++] = v;
+    std::copy(e->getVel().begin(), e->getVel().end(), vel);
+}
+
+void ref_dynfrac_set(void *h, const double *b, const double *v0)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<PDDODynamicFracElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    e->setBodyForce(b[0], b[1], b[2]);
+    e->setInitialVelocity(v0[0], v0[1], v0[2]);
+}
 
 // CH element geometry (valid after one ref_assemble): lapbond, rev
 void ref_ch_geometry(void *h, double *lapbond, int *rev, double *mnode)

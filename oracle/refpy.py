@@ -30,7 +30,8 @@ def lib():
         _LIB.ref_open.argtypes = [C.c_char_p]
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
                   "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
-                  "ref_apply_bcs", "ref_projection", "ref_ch_geometry",
+                  "ref_apply_bcs", "ref_projection", "ref_ch_geometry", "ref_assemble_step",
+                  "ref_dynfrac_state", "ref_dynfrac_set",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -145,6 +146,22 @@ class Case:
         out = np.zeros(self.n * nc)
         lib().ref_projection(self.h, name.encode(), _dp(out))
         return out.reshape(self.n, nc)
+
+    def assemble_step(self, u, uold, dt, timestep):
+        """formResidualAndJacobian as time step `timestep` (per-step commits)."""
+        k = np.zeros(self.nnz); r = np.zeros(self.n * self.ndof)
+        lib().ref_assemble_step(self.h, _dp(np.ascontiguousarray(u, float)),
+                                _dp(np.ascontiguousarray(uold, float)), C.c_double(dt), C.c_int(timestep),
+                                _dp(k), _dp(r))
+        return k, r
+
+    def dynfrac_state(self):
+        h = np.zeros(self.nb); v = np.zeros(self.n * self.ndof)
+        lib().ref_dynfrac_state(self.h, _dp(h), _dp(v))
+        return h, v
+
+    def dynfrac_set(self, b, v0):
+        lib().ref_dynfrac_set(self.h, _dp(np.ascontiguousarray(b, float)), _dp(np.ascontiguousarray(v0, float)))
 
     def ch_geometry(self):
         lap = np.zeros(self.nb); rev = np.zeros(self.nb, np.int32); mn = np.zeros(self.n)

@@ -96,6 +96,16 @@ struct CsrBcProgram {
 };
 enum { BCROW_DIRECT = 0, BCROW_REFLECT = 1, BCROW_MIRROR = 2 };
 
+// implicit dynamic fracture element state (PDDODynamicFracElement): bond
+// health (CSR order), committed velocity and displacement history
+struct DynFracState {
+    int ready = 0;
+    double C11 = 0, C12 = 0, C66 = 0, rho = 1, b[3] = {0, 0, 0}, sc = 0;
+    int tension_only = 1, damage_on = 1;
+    dbuf<unsigned char> health;
+    dbuf<double> vel, uprev;
+};
+
 // adjoint workspace, kept between calls and reallocated on size changes
 struct AdjWork {
     dbuf<double> cks;        // 2 * ncheckpoints * N*dim: (u^{k-1}, u^k) pairs
@@ -242,6 +252,7 @@ struct pxg_ctx {
     pxg::dbuf<double> mnode;
     int ch_ready = 0;
     pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
+    pxg::DynFracState dyn;          // implicit dynamic fracture element
     // scratch for host-pointer entry points
     pxg::dbuf<double> su, suo, sk, sr, sf;
     // ---- explicit fracture ----

@@ -53,7 +53,8 @@ SYMBOLS = [
     "pxg_assemble_diffusion", "pxg_assemble_diffusion_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
-    "pxg_csr_apply_bcs_dev", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
+    "pxg_csr_apply_bcs_dev", "pxg_dynfrac_setup", "pxg_dynfrac_commit", "pxg_dynfrac_commit_dev",
+    "pxg_assemble_dynfrac", "pxg_assemble_dynfrac_dev", "pxg_dynfrac_get_state", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
     "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
@@ -109,6 +110,13 @@ def load():
         L.pxg_csr_apply_bcs_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                                             C.c_void_p]
         L.pxg_frac_init.argtypes = [C.c_void_p, D, D, C.c_double]
+        L.pxg_dynfrac_setup.argtypes = [C.c_void_p, C.POINTER(FracParams), D, D]
+        L.pxg_dynfrac_commit.argtypes = [C.c_void_p, D, C.c_double]
+        L.pxg_dynfrac_commit_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+        L.pxg_assemble_dynfrac.argtypes = [C.c_void_p, C.c_double, D, D, D, D]
+        L.pxg_assemble_dynfrac_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
+        L.pxg_dynfrac_get_state.argtypes = [C.c_void_p, U8, D]
         L.pxg_frac_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, I]
         L.pxg_frac_adjoint.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, C.c_int, C.c_int,
                                        I, C.c_int, D, D]
@@ -350,6 +358,31 @@ class Context:
     def csr_apply_bcs_dev(self, t, u_ptr, k_ptr, r_ptr):
         _check(load().pxg_csr_apply_bcs_dev(self.h, t, u_ptr, k_ptr, r_ptr))
 
+    # -- implicit dynamic fracture element ---------------------------------------
+    def dynfrac_setup(self, params, uold0, v0=None):
+        v = None if v0 is None else np.ascontiguousarray(list(v0) + [0.0] * (3 - len(v0)), float)
+        _check(load().pxg_dynfrac_setup(self.h, C.byref(params), _dp(v),
+                                        _dp(np.ascontiguousarray(uold0, float))))
+
+    def dynfrac_commit(self, uold, dt_prev):
+        _check(load().pxg_dynfrac_commit(self.h, _dp(np.ascontiguousarray(uold, float)), dt_prev))
+
+    def assemble_dynfrac(self, dt, u, uold):
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n * self.dim)
+        _check(load().pxg_assemble_dynfrac(self.h, dt, _dp(np.ascontiguousarray(u, float)),
+                                           _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_dynfrac_dev(self, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
+        _check(load().pxg_assemble_dynfrac_dev(self.h, dt, u_ptr, uold_ptr, k_ptr, r_ptr))
+
+    def dynfrac_state(self):
+        h = np.zeros(self.nb, np.uint8)
+        v = np.zeros(self.n * self.dim)
+        _check(load().pxg_dynfrac_get_state(self.h, _up(h), _dp(v)))
+        return h, v
+
     # -- a15-a20 ---------------------------------------------------------------
     def frac_setup(self, params, precision=FP64, node_E=None, nu=0.25, plane_stress=False):
         ne = None if node_E is None else np.ascontiguousarray(node_E, float)
@@ -376,6 +409,9 @@ class Context:
         out = np.zeros(self.n * nc)
         _check(load().pxg_frac_project(self.h, which, _dp(out)))
         return out.reshape(self.n, nc)
+
+    def frac_project_dev(self, which, out_ptr):
+        _check(load().pxg_frac_project_dev(self.h, which, out_ptr))
 
     def frac_layout(self):
         """(bond slots per component, tiled, shared runs, nodes on shared runs)."""

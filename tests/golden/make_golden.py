@@ -21,6 +21,8 @@ Fixtures (all small, .npz):
                     residual for a perturbed state (Picard-frozen mobility)
   diffusion_*.npz   DiffusionElement assembly (D = 0.7, f = 0.3, dt = 2e-3) of
                     the mesh_fam grids for a seeded (u, u_old)
+  dynfrac_*.npz     PDDODynamicFracElement: two backward-Euler steps (the
+                    initialising step and a committing step with bond failure)
   bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
                     Poisson (20x12) / Cahn-Hilliard (16x16) system under an
                     overlapping Dirichlet/Neumann list, with zeroed diagonals
@@ -155,6 +157,46 @@ def diffusion_fixture(name, dim, nx, ny, nz, hi):
     c.close()
 
 
+def dynfrac_fixture(name, dim, nx, ny, nz, hi, crack, amp):
+    """PDDODynamicFracElement: two backward-Euler steps through the assembler.
+    Step 1 initialises the element (velocity v0, force-only seeded health);
+    step 2 commits (velocity from u_old, bond failure on u_old) and assembles."""
+    dofs = ["ux", "uy"] + (["uz"] if dim == 3 else [])
+    G0 = 50.0
+    d = {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                        "Cracks": [{"x1": crack[0], "y1": crack[1], "x2": crack[2], "y2": crack[3]}]},
+         "DOFs": dofs,
+         "ElmtSystem": {"f": {"type": "pddo_dynamic_frac",
+                              "params": {"E": E_STEEL, "nu": NU, "rho": RHO, "G0": G0,
+                                         "tension_only": True, "damage": True}}},
+         "BCSystem": {"l": {"type": "dirichlet", "phygroup": "leftnodes_ghost", "dofs": dofs,
+                            "value": [0.0] * dim}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 1e-7, "total_time": 1e-6}}
+    if dim == 2:
+        d["ElmtSystem"]["f"]["params"]["state"] = "plane_strain"
+    c = refpy.Case(d)
+    b, v0, dt = [1e5, -2e5, 3e4], [0.3, -0.1, 0.2], 1e-7
+    c.dynfrac_set(b, v0)
+    rs = np.random.RandomState(1)
+    nd = c.n * dim
+    uold1 = 1e-6 * rs.standard_normal(nd); u1 = uold1 + 1e-7 * rs.standard_normal(nd)
+    k1, r1 = c.assemble_step(u1, uold1, dt, 1)
+    h1, v1 = c.dynfrac_state()
+    uold2 = uold1 + amp * rs.standard_normal(nd); u2 = uold2 + 1e-7 * rs.standard_normal(nd)
+    k2, r2 = c.assemble_step(u2, uold2, dt, 2)
+    h2, v2 = c.dynfrac_state()
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), crack=np.array(crack, float), E=E_STEEL, nu=NU, rho=RHO,
+                        G0=G0, b=np.array(b[:dim]), v0=np.array(v0[:dim]), dt=dt,
+                        uold1=uold1, u1=u1, K1=k1, R1=r1, health1=h1, vel1=v1,
+                        uold2=uold2, u2=u2, K2=k2, R2=r2, health2=h2, vel2=v2)
+    c.close()
+    return int((h1 == 0).sum()), int((h2 == 0).sum()), len(h2)
+
+
 # BC lists of the row-elimination fixtures: (name, type, group, dofs, value,
 # velocity, direct, box) in deck order; overlapping entries on one row
 # exercise the list order
@@ -248,6 +290,8 @@ def main():
     ch_fixture("ch_2d.npz", 16)
     diffusion_fixture("diffusion_2d.npz", 2, 20, 12, 1, (1.0, 0.6, 1.0))
     diffusion_fixture("diffusion_3d.npz", 3, 6, 5, 4, (0.6, 0.5, 0.4))
+    dynfrac_fixture("dynfrac_2d.npz", 2, 24, 12, 1, (0.024, 0.012, 1.0), (0.012, 0.0, 0.012, 0.005), 1.5e-7)
+    dynfrac_fixture("dynfrac_3d.npz", 3, 6, 4, 3, (0.006, 0.004, 0.003), (0.003, 0.0, 0.003, 0.002), 1e-7)
     bcrows_fixture("bcrows_poisson_2d.npz", "poisson")
     bcrows_fixture("bcrows_ch_2d.npz", "ch")
     for f in sorted(os.listdir(HERE)):

@@ -146,3 +146,33 @@ def test_diffusion_assembly(name):
     k, r = pxo.assemble_diffusion(m, ops, rp, cols, float(z["D"]), float(z["f"]), float(z["dt"]),
                                   z["u"], z["uold"])
     assert same(k, z["K"]) and same(r, z["R"])
+
+
+@pytest.mark.parametrize("name", ["dynfrac_2d.npz", "dynfrac_3d.npz"])
+def test_dynamic_fracture_element(name):
+    """PDDODynamicFracElement through the assembler over two steps: the
+    initialising step (seeded health, initial velocity) and a committing step
+    (velocity from u_old, bond failure on u_old); K, the residual, the
+    committed velocity and the bond health bit for bit."""
+    z = load(name)
+    dim = int(z["dim"])
+    m = mesh_of(dict(z, factor=3.015))
+    ops, _ = pxo.operators(m)
+    rp, cols = pxo.csr_pattern(m, dim)
+    E, nu, rho, dt = float(z["E"]), float(z["nu"]), float(z["rho"]), float(z["dt"])
+    C11, C12, C66 = pxo.elastic_constants(dim, False, E, nu)
+    sc = pxo.critical_stretch(float(z["G0"]), E, m.delta, dim, False)
+    health = pxo.seed_health(m, segs=[tuple(z["crack"])])
+    assert same(health, z["health1"]), "force-only seeded health"
+    vel = np.tile(z["v0"], m.n)
+    uprev = z["uold1"].copy()
+    assert same(vel, z["vel1"])
+    k, r = pxo.assemble_dynfrac(m, ops, rp, cols, C11, C12, C66, rho, z["b"], dt, vel, health,
+                                z["u1"], z["uold1"])
+    assert same(k, z["K1"]) and same(r, z["R1"]), "step 1"
+    pxo.dynfrac_commit(m, sc, True, True, dt, z["uold2"], uprev, vel, health)
+    assert same(vel, z["vel2"]) and same(health, z["health2"]), "commit"
+    assert (health == 0).sum() > (z["health1"] == 0).sum()
+    k, r = pxo.assemble_dynfrac(m, ops, rp, cols, C11, C12, C66, rho, z["b"], dt, vel, health,
+                                z["u2"], z["uold2"])
+    assert same(k, z["K2"]) and same(r, z["R2"]), "step 2"

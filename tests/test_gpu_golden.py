@@ -153,3 +153,52 @@ def test_diffusion_vs_reference(name, oracle):
     assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
     assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["dynfrac_2d.npz", "dynfrac_3d.npz"])
+def test_dynamic_fracture_vs_reference(name, oracle):
+    """PDDODynamicFracElement on the device over the initialising and a
+    committing step: health and velocity bit for bit against the reference,
+    K and the residual bit for bit against the oracle on the device's
+    g-weights and within 1e-12 of the term scale against the reference."""
+    z = load(name)
+    dim = int(z["dim"])
+    nx, ny, nz = (int(v) for v in z["grid"])
+    g = perix_amd.StructuredGrid(dim, nx, ny, nz, hi=tuple(float(v) for v in z["hi"]))
+    cloud = perix_amd.build_structured_cloud(g)
+    delta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+    ctx.set_cracks(segs=[tuple(float(v) for v in z["crack"])])
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(dim)
+    E, nu, rho, dt = float(z["E"]), float(z["nu"]), float(z["rho"]), float(z["dt"])
+    c11, c12, c66 = problems.elastic_constants(dim, False, E, nu)
+    sc = problems.critical_stretch(float(z["G0"]), E, delta, dim)
+    b = list(z["b"]) + [0.0] * (3 - dim)
+    ctx.dynfrac_setup(pxg.FracParams(dim, c11, c12, c66, rho, b[0], b[1], b[2], sc, 1, 1), z["uold1"],
+                      v0=list(z["v0"]))
+    h, v = ctx.dynfrac_state()
+    assert np.array_equal(h.astype(float), z["health1"]) and np.array_equal(v, z["vel1"])
+    m = oracle.Mesh(oracle.make_grid(dim, nx, ny, nz, hi=tuple(float(v) for v in z["hi"])))
+    ops, _ = ctx.operators()
+    rp, cols = ctx.csr()
+    for step in (1, 2):
+        if step == 2:
+            ctx.dynfrac_commit(z["uold2"], dt)
+            h, v = ctx.dynfrac_state()
+            assert np.array_equal(v.view(np.uint64), z["vel2"].view(np.uint64))
+            assert np.array_equal(h.astype(float), z["health2"])
+            print("%s: broken bonds %d -> %d" % (name, int((z["health1"] == 0).sum()), int((h == 0).sum())))
+        u, uold = z["u%d" % step], z["uold%d" % step]
+        k, r = ctx.assemble_dynfrac(dt, u, uold)
+        ok, orr = oracle.assemble_dynfrac(m, ops, rp, cols, c11, c12, c66, rho, z["b"], dt, v,
+                                          h.astype(float), u, uold)
+        assert np.array_equal(k.view(np.uint64), ok.view(np.uint64)), "K, step %d" % step
+        assert np.array_equal(r.view(np.uint64), orr.view(np.uint64)), "R, step %d" % step
+        K, R = z["K%d" % step], z["R%d" % step]
+        kscale = np.abs(K).max()
+        assert np.abs(k - K).max() <= 1e-12 * kscale
+        assert np.abs(r - R).max() <= 1e-12 * (np.abs(R).max() + kscale * np.abs(u).max())
+    ctx.close()
