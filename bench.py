@@ -320,6 +320,7 @@ def assembly_case(problems, pxg, reps=20):
     n, nnz = cloud.n, ctx.nnz
     X = cloud.coords.reshape(-1, 3)
     u_h = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])
+    u_h_poisson = u_h
     u = torch.from_numpy(u_h).cuda()
     fd = torch.from_numpy(np.ascontiguousarray(f)).cuda()
     k = torch.empty(nnz, dtype=torch.float64, device="cuda")
@@ -361,6 +362,43 @@ def assembly_case(problems, pxg, reps=20):
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["diffusion_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
     ctx.close()
+    # implicit dynamic fracture element on the 400x200 plate (2 dofs per node)
+    import perix_amd
+    g = perix_amd.StructuredGrid(2, 400, 200, hi=(0.2, 0.1, 1.0))
+    pc = perix_amd.build_structured_cloud(g)
+    pdelta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(pc.coords, pc.vols, pc.mirror, pc.dxyz, 2)
+    ctx.set_cracks(segs=[(0.05, 0.0, 0.05, 0.05), (0.15, 0.0, 0.15, 0.05)])
+    ctx.build_families(pdelta)
+    ctx.build_operators(pdelta, float(max(pc.dxyz)))
+    ctx.build_csr(2)
+    c11, c12, c66 = problems.elastic_constants(2, False, problems.E_STEEL, problems.NU)
+    G0 = problems.g0_for_stretch(0.01, problems.E_STEEL, pdelta, 2)
+    sc = problems.critical_stretch(G0, problems.E_STEEL, pdelta, 2)
+    nd, nnz = 2 * pc.n, ctx.nnz
+    rs = np.random.RandomState(4)
+    uold_h = 1e-7 * rs.standard_normal(nd)
+    u_h = uold_h + 1e-8 * rs.standard_normal(nd)
+    ctx.dynfrac_setup(pxg.FracParams(2, c11, c12, c66, problems.RHO, 0.0, 0.0, 0.0, sc, 1, 1), uold_h)
+    ud, uod = torch.from_numpy(u_h).cuda(), torch.from_numpy(uold_h).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(nd, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    dt_dyn = 1e-7
+
+    def dynfrac():
+        ctx.assemble_dynfrac_dev(dt_dyn, ud.data_ptr(), uod.data_ptr(), k.data_ptr(), r.data_ptr())
+
+    dynfrac()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dynfrac()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["dynamic_fracture_400x200"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    ctx.close()
     ctx, cloud, delta, u0 = problems.cahnhilliard_square()
     n, nnz = cloud.n, ctx.nnz
     u1 = u0.copy()
@@ -385,7 +423,7 @@ def assembly_case(problems, pxg, reps=20):
     out["cahnhilliard_256x256"].update(bc_rows_timing(ctx, ud, k, r, ch, reps))
     ctx.close()
     try:
-        out["cpu_baseline"] = cpu_assembly_baseline(u_h, u0, u1, kappa)
+        out["cpu_baseline"] = cpu_assembly_baseline(u_h_poisson, u0, u1, kappa, (u_h, uold_h, G0))
     except Exception as e:       # reported, never required
         out["cpu_baseline"] = {"value": None, "sample": "failed: %s" % e}
     return out
@@ -413,7 +451,7 @@ def bc_rows_timing(ctx, u, k, r, assemble, reps):
     return {"bc_rows": nrows, "ms_bc_rows": ms_bc, "ms_assembly_plus_bc_rows": ms_sys}
 
 
-def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
+def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa, dyn=None):
     """The reference's own OpenMP assembly (PDSystem::formResidualAndJacobianParallel
     through oracle/_ref) of the same two systems on this host, one call each."""
     from oracle import refpy
@@ -451,6 +489,29 @@ def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa):
         res["cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms, "nnz_per_s": c.nnz / (ms * 1e-3),
                                        "ms_bc_rows": c.last_bc_seconds() * 1e3}
         c.close()
+        if dyn is not None:
+            u_d, uold_d, G0 = dyn
+            d = {"Mesh": refpy.mesh_block(2, 400, 200, hi=(0.2, 0.1, 1.0)),
+                 "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+                 "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                                "Cracks": [{"x1": 0.05, "y1": 0.0, "x2": 0.05, "y2": 0.05},
+                                           {"x1": 0.15, "y1": 0.0, "x2": 0.15, "y2": 0.05}]},
+                 "DOFs": ["ux", "uy"],
+                 "ElmtSystem": {"f": {"type": "pddo_dynamic_frac", "params": {
+                     "E": 191e9, "nu": 0.25, "rho": 8000.0, "G0": G0, "state": "plane_strain",
+                     "tension_only": True, "damage": True}}},
+                 "BCSystem": {"l": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["ux", "uy"],
+                                    "value": [0.0, 0.0]}},
+                 "JobSystem": {"type": "transient", "assemble": "openmp"},
+                 "TimeStepping": {"dt": 1e-7, "total_time": 1e-6}}
+            c = refpy.Case(d)
+            c.assemble(u_d, uold_d, 1e-7, parallel=True)
+            t0 = time.perf_counter()
+            c.assemble(u_d, uold_d, 1e-7, parallel=True)
+            ms = (time.perf_counter() - t0) * 1e3
+            res["dynamic_fracture_400x200"] = {"nnz": c.nnz, "ms_per_assembly": ms,
+                                               "nnz_per_s": c.nnz / (ms * 1e-3)}
+            c.close()
     res["kind"] = "reference"
     res["cores"] = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     res["sample"] = ("one assembly call of each system after a warm-up call (OpenMP assembler), "
