@@ -310,8 +310,9 @@ def block_case(problems, pxg, steps=100):
 
 
 def assembly_case(problems, pxg, reps=20):
-    """Poisson (100x100) and Cahn-Hilliard (256x256, one
-    Picard iteration) CSR assembly on the device with inputs and outputs
+    """Poisson and diffusion (100x100), dynamic fracture (400x200 plate),
+    Cahn-Hilliard and stress-coupled Cahn-Hilliard (256x256, one Picard
+    iteration) CSR assembly on the device with inputs and outputs
     resident in HBM: nnz/s, beside the reference's own CPU assembly (oracle/_ref)
     of the same system when that build is present."""
     import torch
@@ -422,8 +423,61 @@ def assembly_case(problems, pxg, reps=20):
     out["cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
     out["cahnhilliard_256x256"].update(bc_rows_timing(ctx, ud, k, r, ch, reps))
     ctx.close()
+    # stress-coupled Cahn-Hilliard (c, mu, ux, uy), silicon parameters, 256x256
+    ctx, cloud, delta, us_h, usold_h = problems.stressch_square()
+    n, nnz = cloud.n, ctx.nnz
+    ud, uod = torch.from_numpy(us_h).cuda(), torch.from_numpy(usold_h).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(4 * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def stressch():
+        ctx.assemble_stressch_dev(1e-4, ud.data_ptr(), uod.data_ptr(), k.data_ptr(), r.data_ptr())
+
+    stressch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        stressch()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["stress_cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms,
+                                          "nnz_per_s": nnz / (ms * 1e-3)}
+    ctx.close()
+    # fracture variant (silicon_frac parameters, two edge notches), with the
+    # per-step commit timed on its own
+    ctx, cloud, delta, uf_h, ufold_h = problems.fsch_square()
+    n, nnz = cloud.n, ctx.nnz
+    ud, uod = torch.from_numpy(uf_h).cuda(), torch.from_numpy(ufold_h).cuda()
+    k = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    r = torch.empty(4 * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def fsch():
+        ctx.assemble_fsch_dev(1e-4, ud.data_ptr(), uod.data_ptr(), k.data_ptr(), r.data_ptr())
+
+    def fsch_commit():
+        ctx.fsch_commit_dev(uod.data_ptr(), 1e-4)
+
+    fsch()
+    fsch_commit()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fsch()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fsch_commit()
+    ctx.sync()
+    ms_c = (time.perf_counter() - t0) * 1e3 / reps
+    out["frac_stress_cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms,
+                                               "nnz_per_s": nnz / (ms * 1e-3), "ms_per_commit": ms_c}
+    ctx.close()
     try:
-        out["cpu_baseline"] = cpu_assembly_baseline(u_h_poisson, u0, u1, kappa, (u_h, uold_h, G0))
+        out["cpu_baseline"] = cpu_assembly_baseline(u_h_poisson, u0, u1, kappa, (u_h, uold_h, G0),
+                                                    (us_h, usold_h), (uf_h, ufold_h))
     except Exception as e:       # reported, never required
         out["cpu_baseline"] = {"value": None, "sample": "failed: %s" % e}
     return out
@@ -451,9 +505,9 @@ def bc_rows_timing(ctx, u, k, r, assemble, reps):
     return {"bc_rows": nrows, "ms_bc_rows": ms_bc, "ms_assembly_plus_bc_rows": ms_sys}
 
 
-def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa, dyn=None):
+def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa, dyn=None, stressch=None, fsch=None):
     """The reference's own OpenMP assembly (PDSystem::formResidualAndJacobianParallel
-    through oracle/_ref) of the same two systems on this host, one call each."""
+    through oracle/_ref) of the same systems on this host, one call each."""
     from oracle import refpy
     if not refpy.available():
         return {"value": None, "sample": "reference build not present"}
@@ -511,6 +565,47 @@ def cpu_assembly_baseline(u_poisson, u0_ch, u1_ch, kappa, dyn=None):
             ms = (time.perf_counter() - t0) * 1e3
             res["dynamic_fracture_400x200"] = {"nnz": c.nnz, "ms_per_assembly": ms,
                                                "nnz_per_s": c.nnz / (ms * 1e-3)}
+            c.close()
+        if stressch is not None:
+            from perix_amd.problems import SILICON
+            us, usold = stressch
+            d = {"Mesh": refpy.mesh_block(2, 256, 256), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+                 "DOFs": ["c", "mu", "ux", "uy"],
+                 "ElmtSystem": {"s": {"type": "stress_cahnhilliard",
+                                      "params": dict(SILICON, state="plane_stress")}},
+                 "BCSystem": {"n": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["c", "mu"],
+                                    "value": [0.0, 0.0]}},
+                 "JobSystem": {"type": "transient", "assemble": "openmp"},
+                 "TimeStepping": {"dt": 1e-4, "total_time": 1e-3}}
+            c = refpy.Case(d)
+            c.assemble(us, usold, 1e-4, parallel=True)
+            t0 = time.perf_counter()
+            c.assemble(us, usold, 1e-4, parallel=True)
+            ms = (time.perf_counter() - t0) * 1e3
+            res["stress_cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms,
+                                                  "nnz_per_s": c.nnz / (ms * 1e-3)}
+            c.close()
+        if fsch is not None:
+            from perix_amd.problems import SILICON_FRAC
+            uf, ufold = fsch
+            prm = dict(SILICON_FRAC, state="plane_stress")
+            d = {"Mesh": refpy.mesh_block(2, 256, 256), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+                 "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                                "Cracks": [{"x1": 0.25, "y1": 0.0, "x2": 0.25, "y2": 0.1},
+                                           {"x1": 0.75, "y1": 0.0, "x2": 0.75, "y2": 0.1}]},
+                 "DOFs": ["c", "mu", "ux", "uy"],
+                 "ElmtSystem": {"s": {"type": "frac_stress_cahnhilliard", "params": prm}},
+                 "BCSystem": {"n": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["c", "mu"],
+                                    "value": [0.0, 0.0]}},
+                 "JobSystem": {"type": "transient", "assemble": "openmp"},
+                 "TimeStepping": {"dt": 1e-4, "total_time": 1e-3}}
+            c = refpy.Case(d)
+            c.assemble(uf, ufold, 1e-4, parallel=True)
+            t0 = time.perf_counter()
+            c.assemble(uf, ufold, 1e-4, parallel=True)
+            ms = (time.perf_counter() - t0) * 1e3
+            res["frac_stress_cahnhilliard_256x256"] = {"nnz": c.nnz, "ms_per_picard_assembly": ms,
+                                                       "nnz_per_s": c.nnz / (ms * 1e-3)}
             c.close()
     res["kind"] = "reference"
     res["cores"] = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()

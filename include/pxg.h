@@ -105,7 +105,8 @@ int pxg_get_seed(pxg_ctx *ctx, unsigned char *seed);
 
 /* ---- a4: CSR pattern -----------------------------------------------------
  * Replaces SparseMatrix::initFromPDMesh + setup (SparseMatrix.cpp:73-185).
- * rowptr[n*ndof+1], cols[nnz] (0-based, sorted). */
+ * rowptr[n*ndof+1], cols[nnz] (0-based, sorted); 1 <= ndof <= 8 (the
+ * elements use 1, dim, 2 and dim+2). */
 int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out);
 int pxg_get_csr(pxg_ctx *ctx, int *rowptr, int *cols);
 
@@ -247,6 +248,74 @@ int pxg_assemble_dynfrac_dev(pxg_ctx *ctx, double dt, const double *u_dev,
                              const double *uold_dev, double *kvals_dev, double *rhs_dev);
 /* bond health (nb bytes, CSR order) and committed velocity (n*dim); either may be NULL */
 int pxg_dynfrac_get_state(pxg_ctx *ctx, unsigned char *health, double *vel);
+
+/* ---- remaining elements: stress-coupled Cahn-Hilliard ---------------------
+ * StressCahnHilliardElement (stress_cahnhilliard): species c, chemical
+ * potential mu and displacement u per node (dofs c, mu, ux, uy[, uz]);
+ * the element parameters of the input deck (ReadElmtSystemBlock.cpp:88-103).  plane_stress selects the 2D
+ * constants (ignored in 3D). */
+typedef struct pxg_stressch_params {
+    double E, nu;       /* Young's modulus, Poisson ratio */
+    double D;           /* mobility prefactor: M(c) = D c (1 - c) */
+    double Omega;       /* partial molar volume (chemo-mechanical coupling) */
+    double cref;        /* stress-free reference concentration */
+    double chi;         /* regular-solution interaction parameter */
+    double kappa;       /* gradient-energy coefficient */
+    int plane_stress;   /* 2D: 1 plane stress, 0 plane strain */
+} pxg_stressch_params;
+/* Requires order-2 operators and pxg_build_csr(ctx, dim + 2, ...); computes
+ * the elastic constants (recalcElasticConstants,
+ * StressCahnHilliardElement.cpp:80-128; PXG_ERR_ARG for a Poisson ratio the
+ * reference rejects) and the bond geometry of preprocessIteration
+ * (StressCahnHilliardElement.cpp:166-250). */
+int pxg_stressch_setup(pxg_ctx *ctx, const pxg_stressch_params *p);
+/* out[11] = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref */
+int pxg_stressch_constants(pxg_ctx *ctx, double *out);
+/* K = -dF/dU and the residual of the element through the assembler
+ * (StressCahnHilliardElement.cpp:253-467, FormResidualAndJacobian.cpp:25-132)
+ * for (u, uold) at step size dt > 0; u, uold n*(dim+2) interleaved per node.
+ * Host pointers, or device pointers with _dev. */
+int pxg_assemble_stressch(pxg_ctx *ctx, double dt, const double *u, const double *uold,
+                          double *kvals, double *rhs);
+int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *u_dev,
+                              const double *uold_dev, double *kvals_dev, double *rhs_dev);
+
+/* ---- remaining elements: fracture stress-coupled Cahn-Hilliard -------------
+ * FracStressCahnHilliardElement (frac_stress_cahnhilliard): the stress-coupled
+ * Cahn-Hilliard system plus irreversible bond-stretch failure that gates the
+ * mechanical bonds (deck parameters ReadElmtSystemBlock.cpp:105-131). */
+typedef struct pxg_fsch_params {
+    pxg_stressch_params base;   /* E, nu, D, Omega, cref, chi, kappa, plane_stress */
+    double rho;                 /* density; <= 0: quasi-static mechanics */
+    double sc;                  /* critical stretch (criticalStretchFromGc,
+                                   FractureCriterion.h:30-43, evaluated by the
+                                   caller); <= 0: no failure */
+    double residual_stiffness;  /* stiffness floor of a broken bond (the
+                                   reference's default is 1e-8) */
+    int tension_only, damage_on;
+} pxg_fsch_params;
+/* Requires order-2 operators and pxg_build_csr(ctx, dim + 2, ...); bond health
+ * starts from the crack seed, the velocity at zero and u_prev at the
+ * displacement dofs of uold0 (n*(dim+2) host; only read when rho > 0)
+ * (FracStressCahnHilliardElement.cpp:160-301). */
+int pxg_fsch_setup(pxg_ctx *ctx, const pxg_fsch_params *p, const double *uold0);
+/* out[14] = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho,
+ * residual stiffness, critical stretch */
+int pxg_fsch_constants(pxg_ctx *ctx, double *out);
+/* start of a new time step on the accepted u_old
+ * (FracStressCahnHilliardElement.cpp:303-398): with rho > 0 the velocity
+ * (u_old - u_prev)/dt_prev and u_prev = u_old; bond failure on the
+ * swelling-corrected stretch of u_old. */
+int pxg_fsch_commit(pxg_ctx *ctx, const double *uold, double dt_prev);
+int pxg_fsch_commit_dev(pxg_ctx *ctx, const double *uold_dev, double dt_prev);
+/* K = -dF/dU and the residual (FracStressCahnHilliardElement.cpp:400-561
+ * through FormResidualAndJacobian.cpp:25-132) for (u, uold) at step dt > 0. */
+int pxg_assemble_fsch(pxg_ctx *ctx, double dt, const double *u, const double *uold,
+                      double *kvals, double *rhs);
+int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_dev,
+                          const double *uold_dev, double *kvals_dev, double *rhs_dev);
+/* bond health (nb bytes, CSR order) and committed velocity (n*dim); either may be NULL */
+int pxg_fsch_get_state(pxg_ctx *ctx, unsigned char *health, double *vel);
 
 /* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
  * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):

@@ -88,6 +88,24 @@ void pxo_ch_geometry(int n, int dim, int nop, const double *coords,
                      const int *mirror, double *lapbond, int *rev,
                      unsigned char *isghost);
 void pxo_ch_mobility(int n, const double *u, double M0, double *mnode);
+int pxo_stressch_consts(int dim, int plane_stress, double E, double nu, double Omega, double *out);
+void pxo_assemble_stressch(int n, int dim, int nop, const int *off, const int *nbr,
+                           const double *ops, const int *rev, const double *lapbond,
+                           const unsigned char *isghost, const double *vol, const double *k,
+                           double dt, const int *rowptr, const int *cols,
+                           const double *u, const double *uold, double *kvals, double *rhs);
+void pxo_fsch_lapc(int n, int dim, int nop, const int *off, const int *nbr, const double *ops,
+                   const int *mirror, double *lapc);
+void pxo_fsch_commit(int n, int dim, const double *coords, const int *off, const int *nbr,
+                     double sc, int tension_only, int damage_on, double Omega, double cref,
+                     double rho, double dt_prev, const double *uold, double *uprev, double *vel,
+                     double *health);
+void pxo_assemble_fsch(int n, int dim, int nop, const int *off, const int *nbr,
+                       const double *ops, const int *rev, const double *lapc,
+                       const unsigned char *isghost, const double *vol, const double *k,
+                       double dt, const double *vel, const double *health,
+                       const int *rowptr, const int *cols, const double *u, const double *uold,
+                       double *kvals, double *rhs);
 void pxo_assemble_ch(int n, const int *off, const int *nbr, const int *rev,
                      const double *lapbond, const unsigned char *isghost,
                      const double *vol, const double *mnode,

@@ -215,6 +215,66 @@ def assemble_ch(m, lap, rev, isg, mnode, kappa, chi, dt, rowptr, cols, u, uold):
     return k, r
 
 
+def stressch_consts(dim, plane_stress, E, nu, Omega):
+    """C11, C12, C66, A, Kh, Ch of the stress-coupled Cahn-Hilliard element."""
+    out = np.zeros(6)
+    if lib().pxo_stressch_consts(dim, int(plane_stress), C.c_double(E), C.c_double(nu),
+                                 C.c_double(Omega), dp(out)) != 0:
+        raise ValueError("stress Cahn-Hilliard: Poisson ratio out of range")
+    return out
+
+
+def assemble_stressch(m, ops, lap, rev, isg, consts, dt, rowptr, cols, u, uold):
+    """StressCahnHilliardElement through the CSR assembler: (K values,
+    residual); consts = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref."""
+    nd = m.dim + 2
+    k = np.zeros(int(rowptr[-1])); r = np.zeros(nd * m.n)
+    kc = np.ascontiguousarray(consts, float)
+    assert kc.shape == (11,)
+    lib().pxo_assemble_stressch(m.n, m.dim, ops.shape[1], ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(ops)),
+                                ip(rev), dp(lap), up(isg), dp(m.vols), dp(kc), C.c_double(dt),
+                                ip(np.ascontiguousarray(rowptr, np.int32)), ip(np.ascontiguousarray(cols, np.int32)),
+                                dp(np.ascontiguousarray(u, float)), dp(np.ascontiguousarray(uold, float)),
+                                dp(k), dp(r))
+    return k, r
+
+
+def fsch_lapc(m, ops):
+    """Species Laplacian of the fracture stress-CH element: per bond, zero
+    on bonds to ghosts."""
+    lc = np.zeros(m.nb)
+    lib().pxo_fsch_lapc(m.n, m.dim, ops.shape[1], ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(ops)),
+                        ip(m.mirror), dp(lc))
+    return lc
+
+
+def fsch_commit(m, sc, tension_only, damage_on, Omega, cref, rho, dt_prev, uold, uprev, vel, health):
+    """Per-step commit, in place on uprev, vel (n*dim) and health (nb, 0/1)."""
+    for a in (uprev, vel, health):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().pxo_fsch_commit(m.n, m.dim, dp(m.coords), ip(m.off), ip(m.nbr), C.c_double(sc), int(tension_only),
+                          int(damage_on), C.c_double(Omega), C.c_double(cref), C.c_double(rho),
+                          C.c_double(dt_prev), dp(np.ascontiguousarray(uold, float)), dp(uprev), dp(vel),
+                          dp(health))
+
+
+def assemble_fsch(m, ops, lapc, rev, isg, consts, dt, vel, health, rowptr, cols, u, uold):
+    """FracStressCahnHilliardElement through the CSR assembler: (K values,
+    residual); consts = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref,
+    rho, residual stiffness."""
+    nd = m.dim + 2
+    k = np.zeros(int(rowptr[-1])); r = np.zeros(nd * m.n)
+    kc = np.ascontiguousarray(consts, float)
+    assert kc.shape == (13,)
+    lib().pxo_assemble_fsch(m.n, m.dim, ops.shape[1], ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(ops)),
+                            ip(rev), dp(lapc), up(isg), dp(m.vols), dp(kc), C.c_double(dt),
+                            dp(np.ascontiguousarray(vel, float)), dp(np.ascontiguousarray(health, float)),
+                            ip(np.ascontiguousarray(rowptr, np.int32)), ip(np.ascontiguousarray(cols, np.int32)),
+                            dp(np.ascontiguousarray(u, float)), dp(np.ascontiguousarray(uold, float)),
+                            dp(k), dp(r))
+    return k, r
+
+
 def elastic_constants(dim, plane_stress, E, nu):
     c = [C.c_double(), C.c_double(), C.c_double()]
     lib().pxo_elastic_constants(dim, int(plane_stress), C.c_double(E), C.c_double(nu),

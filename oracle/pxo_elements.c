@@ -1,9 +1,11 @@
 /*
- * pxo_elements.c -- oracle (test infrastructure): Poisson and Cahn-Hilliard
- * CSR assembly, restated from the reference's serial assembler.
+ * pxo_elements.c -- oracle (test infrastructure): Poisson, diffusion,
+ * dynamic fracture, Cahn-Hilliard and stress-coupled Cahn-Hilliard CSR
+ * assembly, restated from the reference's serial assembler.
  */
 #include "pxo.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* SparseMatrix::findEntry (SparseMatrix.cpp:187-197): binary search */
@@ -283,4 +285,344 @@ void pxo_assemble_ch(int n, const int *off, const int *nbr, const int *rev,
             kvals[find_slot(rowptr, cols, r1, j1)] += -0.0;
         }
     }
+}
+
+/* StressCahnHilliardElement::recalcElasticConstants
+ * (StressCahnHilliardElement.cpp:80-128): out = C11, C12, C66, A, Kh, Ch.
+ * Returns 0, or -1 for a Poisson ratio the reference rejects. */
+int pxo_stressch_consts(int dim, int plane_stress, double E, double nu, double Omega, double *out)
+{
+    for (int k = 0; k < 6; k++) out[k] = 0.0;
+    if (E <= 0.0) return 0;
+    if (dim < 3 && plane_stress) {
+        const double d = 1.0 - nu * nu;
+        if (d <= 0.0) return -1;
+        out[0] = E / d;
+        out[1] = E * nu / d;
+        out[2] = E / (2.0 * (1.0 + nu));
+        out[3] = E * Omega / (3.0 * (1.0 - nu));
+        out[4] = (out[0] + out[1]) / 3.0;
+        out[5] = -2.0 * out[3] / 3.0;
+        return 0;
+    }
+    const double d = (1.0 + nu) * (1.0 - 2.0 * nu);
+    if (d <= 0.0) return -1;
+    out[0] = E * (1.0 - nu) / d;
+    out[1] = E * nu / d;
+    out[2] = E / (2.0 * (1.0 + nu));
+    out[3] = E * Omega / (3.0 * (1.0 - 2.0 * nu));
+    out[4] = (out[0] + 2.0 * out[1]) / 3.0;
+    out[5] = -out[3];
+    return 0;
+}
+
+/* regLog (StressCahnHilliardElement.cpp:42-46): ln(c/(1-c)) and its
+ * derivative inside [delta, 1-delta], continued by the tangent line outside. */
+static void reg_log(double c, double delta, double *L, double *Lp)
+{
+    if (c < delta) {
+        const double Lpd = 1.0 / (delta * (1.0 - delta));
+        *L = log(delta / (1.0 - delta)) + Lpd * (c - delta);
+        *Lp = Lpd;
+    } else if (c > 1.0 - delta) {
+        const double cb = 1.0 - delta, Lpb = 1.0 / (cb * (1.0 - cb));
+        *L = log(cb / (1.0 - cb)) + Lpb * (c - cb);
+        *Lp = Lpb;
+    } else {
+        *L = log(c / (1.0 - c));
+        *Lp = 1.0 / (c * (1.0 - c));
+    }
+}
+
+/* StressCahnHilliardElement through the serial assembler
+ * (FormResidualAndJacobian.cpp:25-132): Picard mobility per node
+ * (StressCahnHilliardElement.cpp:253-278, only M enters the bond term), bond
+ * term (:282-428) with the lambda-corrected, ghost-excluded Laplacian of
+ * pxo_ch_geometry, nodal term (:431-467). ndof = dim + 2 (c, mu, u).
+ * k = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref. */
+void pxo_assemble_stressch(int n, int dim, int nop, const int *off, const int *nbr,
+                           const double *ops, const int *rev, const double *lapbond,
+                           const unsigned char *isghost, const double *vol, const double *k,
+                           double dt, const int *rowptr, const int *cols,
+                           const double *u, const double *uold, double *kvals, double *rhs)
+{
+    const int nd = dim + 2;
+    const double C11 = k[0], C12 = k[1], C66 = k[2], A = k[3], Kh = k[4], Ch = k[5];
+    const double D = k[6], kappa = k[7], chi = k[8], Omega = k[9], cref = k[10];
+    double *mn = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const double cc = clampc(u[(size_t)nd * i]);
+        mn[i] = D * cc * (1.0 - cc);
+    }
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[(size_t)nd * n]);
+    memset(rhs, 0, sizeof(double) * (size_t)n * nd);
+    double KL[5][5], R[5];
+    for (int i = 0; i < n; i++) {
+        const size_t ri = (size_t)nd * i;
+        /* nodal term */
+        const double c = u[ri], mu = u[ri + 1], cold = uold[ri];
+        double L, Lp;
+        reg_log(c, 1.0e-3, &L, &Lp);
+        const double fp = L + chi * (1.0 - 2.0 * c);
+        const double fpp = Lp - 2.0 * chi;
+        memset(KL, 0, sizeof KL);
+        memset(R, 0, sizeof R);
+        R[0] = (c - cold) / dt;
+        R[1] = mu - fp + Omega * Ch * (c - cref);
+        KL[0][0] = 1.0 / dt;
+        KL[1][0] = -fpp + Omega * Ch;
+        KL[1][1] = 1.0;
+        for (int a = 0; a < nd; a++) {
+            rhs[ri + a] += R[a];
+            for (int b = 0; b < nd; b++)
+                kvals[find_slot(rowptr, cols, (int)ri + a, (int)ri + b)] += -KL[a][b];
+        }
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const size_t rj = (size_t)nd * j;
+            const double *op = ops + (size_t)t * nop;
+            const int ghostJ = isghost[j];
+            const double g2ij = lapbond[t];
+            double coeff_c = 0.0;
+            if (!ghostJ) {
+                const double Mi = mn[i], Mj = mn[j], Vi = vol[i], Vj = vol[j];
+                const int rv = rev[t];
+                double wij;
+                if (rv >= 0 && Vi > 0.0) wij = 0.5 * (Mi * g2ij + (Vj / Vi) * Mj * lapbond[rv]);
+                else wij = Mi * g2ij;
+                coeff_c = -wij;
+            }
+            const double kl = ghostJ ? 0.0 : (kappa * g2ij);
+            double d[5];
+            for (int a = 0; a < nd; a++) d[a] = u[rj + a] - u[ri + a];
+            /* P = the bond's dR/du_j; K_II = -P, K_IJ = P */
+            memset(KL, 0, sizeof KL);
+            if (dim >= 3) {
+                const double Gx = op[0], Gy = op[1], Gz = op[2];
+                const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+                const double aa = C12 + C66;
+                const double cu[3] = {Omega * Kh * Gx, Omega * Kh * Gy, Omega * Kh * Gz};
+                const double p11 = C11 * Gxx + C66 * (Gyy + Gzz);
+                const double p22 = C11 * Gyy + C66 * (Gxx + Gzz);
+                const double p33 = C11 * Gzz + C66 * (Gxx + Gyy);
+                const double p12 = aa * Gxy, p13 = aa * Gxz, p23 = aa * Gyz;
+                const double cx = -A * Gx, cy = -A * Gy, cz = -A * Gz;
+                R[0] = coeff_c * d[1];
+                R[1] = kl * d[0] + cu[0] * d[2] + cu[1] * d[3] + cu[2] * d[4];
+                R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+                R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+                R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+                KL[0][1] = coeff_c;
+                KL[1][0] = kl; KL[1][2] = cu[0]; KL[1][3] = cu[1]; KL[1][4] = cu[2];
+                KL[2][0] = cx; KL[2][2] = p11; KL[2][3] = p12; KL[2][4] = p13;
+                KL[3][0] = cy; KL[3][2] = p12; KL[3][3] = p22; KL[3][4] = p23;
+                KL[4][0] = cz; KL[4][2] = p13; KL[4][3] = p23; KL[4][4] = p33;
+            } else {
+                const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+                const double cux = Omega * Kh * Gx, cuy = Omega * Kh * Gy;
+                const double p11 = C11 * Gxx + C66 * Gyy;
+                const double p22 = C66 * Gxx + C11 * Gyy;
+                const double pxy = (C12 + C66) * Gxy;
+                const double cx = -A * Gx, cy = -A * Gy;
+                R[0] = coeff_c * d[1];
+                R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+                R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+                R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+                KL[0][1] = coeff_c;
+                KL[1][0] = kl; KL[1][2] = cux; KL[1][3] = cuy;
+                KL[2][0] = cx; KL[2][2] = p11; KL[2][3] = pxy;
+                KL[3][0] = cy; KL[3][2] = pxy; KL[3][3] = p22;
+            }
+            for (int a = 0; a < nd; a++) {
+                rhs[ri + a] += R[a];
+                for (int b = 0; b < nd; b++) {
+                    kvals[find_slot(rowptr, cols, (int)(ri + a), (int)(ri + b))] += -(-KL[a][b]);
+                    kvals[find_slot(rowptr, cols, (int)(ri + a), (int)(rj + b))] += -KL[a][b];
+                }
+            }
+        }
+    }
+    free(mn);
+}
+
+/* FracStressCahnHilliardElement species Laplacian (FracStressCahnHilliard-
+ * Element.cpp:222-268): the full-family pdLaplacian per bond, zero on bonds
+ * to boundary ghosts, no consistency rescaling.  The reverse-bond slots and
+ * ghost flags are those of pxo_ch_geometry. */
+void pxo_fsch_lapc(int n, int dim, int nop, const int *off, const int *nbr, const double *ops,
+                   const int *mirror, double *lapc)
+{
+    for (int i = 0; i < n; i++)
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            lapc[t] = (mirror && mirror[j] > 0) ? 0.0 : lap_of(ops + (size_t)t * nop, dim);
+        }
+}
+
+/* Per-step history commit on the accepted u_old (FracStressCahnHilliard-
+ * Element.cpp:303-398): with rho > 0 the velocity (u_old - u_prev)/dt_prev
+ * and u_prev = u_old of the displacement dofs; then, with damage on and
+ * sc > 0, irreversible failure of every intact bond whose elastic stretch
+ * (geometric stretch minus the swelling (Omega/3)(c_bar - cref)) reaches sc,
+ * positive part only when tension_only.  Both ends of a bond see the same
+ * stretch, so the reciprocal break of the reference is the per-node test
+ * applied at the other end. */
+void pxo_fsch_commit(int n, int dim, const double *coords, const int *off, const int *nbr,
+                     double sc, int tension_only, int damage_on, double Omega, double cref,
+                     double rho, double dt_prev, const double *uold, double *uprev, double *vel,
+                     double *health)
+{
+    const int nd = dim + 2;
+    if (rho > 0.0)
+        for (int i = 0; i < n; i++)
+            for (int a = 0; a < dim; a++) {
+                const double ua = uold[(size_t)nd * i + 2 + a];
+                vel[(size_t)dim * i + a] = (ua - uprev[(size_t)dim * i + a]) / dt_prev;
+                uprev[(size_t)dim * i + a] = ua;
+            }
+    if (!(damage_on && sc > 0.0)) return;
+    for (int i = 0; i < n; i++) {
+        const size_t ri = (size_t)nd * i;
+        const double ci = uold[ri];
+        const double xi0 = coords[3 * i], yi0 = coords[3 * i + 1];
+        const double zi0 = (dim >= 3) ? coords[3 * i + 2] : 0.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            if (health[t] <= 0.0) continue;
+            const int j = nbr[t] - 1;
+            const size_t rj = (size_t)nd * j;
+            double xi[3], eta[3];
+            xi[0] = coords[3 * j] - xi0;
+            xi[1] = coords[3 * j + 1] - yi0;
+            xi[2] = (dim >= 3) ? coords[3 * j + 2] - zi0 : 0.0;
+            eta[0] = uold[rj + 2] - uold[ri + 2];
+            eta[1] = uold[rj + 3] - uold[ri + 3];
+            eta[2] = (dim >= 3) ? uold[rj + 4] - uold[ri + 4] : 0.0;
+            const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+            if (r2 <= 0.0) continue;
+            const double r = sqrt(r2);
+            const double rinv = 1.0 / r;
+            const double yx = xi[0] + eta[0], yy = xi[1] + eta[1], yz = xi[2] + eta[2];
+            const double sn = (sqrt(yx * yx + yy * yy + yz * yz) - r) * rinv;
+            const double cbar = 0.5 * (ci + uold[rj]);
+            const double s_el = sn - (Omega / 3.0) * (cbar - cref);
+            const double so = tension_only ? fmax(s_el, 0.0) : fabs(s_el);
+            if (so >= sc) health[t] = 0.0;
+        }
+    }
+}
+
+/* FracStressCahnHilliardElement through the serial assembler
+ * (FormResidualAndJacobian.cpp:25-132): bond term (FracStressCahnHilliard-
+ * Element.cpp:411-526: species flux and kappa Laplacian ungated, mechanics
+ * and the sigma_h coupling gated by 1 or the residual stiffness), nodal term
+ * (:528-561: clamped logarithm, optional inertia with the committed
+ * velocity).  k = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho,
+ * residual stiffness. */
+void pxo_assemble_fsch(int n, int dim, int nop, const int *off, const int *nbr,
+                       const double *ops, const int *rev, const double *lapc,
+                       const unsigned char *isghost, const double *vol, const double *k,
+                       double dt, const double *vel, const double *health,
+                       const int *rowptr, const int *cols, const double *u, const double *uold,
+                       double *kvals, double *rhs)
+{
+    const int nd = dim + 2;
+    const double C11 = k[0], C12 = k[1], C66 = k[2], A = k[3], Kh = k[4], Ch = k[5];
+    const double D = k[6], kappa = k[7], chi = k[8], Omega = k[9], cref = k[10];
+    const double rho = k[11], resid = k[12];
+    double *mn = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const double cc = clampc(u[(size_t)nd * i]);
+        mn[i] = D * cc * (1.0 - cc);
+    }
+    memset(kvals, 0, sizeof(double) * (size_t)rowptr[(size_t)nd * n]);
+    memset(rhs, 0, sizeof(double) * (size_t)n * nd);
+    double KL[5][5], R[5];
+    for (int i = 0; i < n; i++) {
+        const size_t ri = (size_t)nd * i;
+        const double c = u[ri], mu = u[ri + 1], cold = uold[ri];
+        const double cc = clampc(c);
+        const double fp = log(cc / (1.0 - cc)) + chi * (1.0 - 2.0 * cc);
+        const double fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * chi;
+        memset(KL, 0, sizeof KL);
+        memset(R, 0, sizeof R);
+        R[0] = (c - cold) / dt;
+        R[1] = mu - fp + Omega * Ch * (c - cref);
+        KL[0][0] = 1.0 / dt;
+        KL[1][0] = -fpp + Omega * Ch;
+        KL[1][1] = 1.0;
+        if (rho > 0.0) {
+            const double c0 = rho / (dt * dt);
+            for (int a = 0; a < dim; a++) {
+                const double v = vel[(size_t)dim * i + a];
+                R[2 + a] += -c0 * (u[ri + 2 + a] - uold[ri + 2 + a] - v * dt);
+                KL[2 + a][2 + a] += -c0;
+            }
+        }
+        for (int a = 0; a < nd; a++) {
+            rhs[ri + a] += R[a];
+            for (int b = 0; b < nd; b++)
+                kvals[find_slot(rowptr, cols, (int)ri + a, (int)ri + b)] += -KL[a][b];
+        }
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const int j = nbr[t] - 1;
+            const size_t rj = (size_t)nd * j;
+            const double *op = ops + (size_t)t * nop;
+            const double g = (health[t] > 0.0) ? 1.0 : resid;
+            double wf = 0.0, g2c = 0.0;
+            if (!isghost[j]) {
+                g2c = lapc[t];
+                const int rv = rev[t];
+                if (rv >= 0) wf = 0.5 * (mn[i] * g2c + (vol[j] / vol[i]) * mn[j] * lapc[rv]);
+                else wf = mn[i] * g2c;
+            }
+            const double coeff_c = -wf, kl = kappa * g2c;
+            double d[5];
+            for (int a = 0; a < nd; a++) d[a] = u[rj + a] - u[ri + a];
+            memset(KL, 0, sizeof KL);
+            if (dim >= 3) {
+                const double Gx = op[0], Gy = op[1], Gz = op[2];
+                const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+                const double aa = C12 + C66;
+                const double cux = g * Omega * Kh * Gx, cuy = g * Omega * Kh * Gy, cuz = g * Omega * Kh * Gz;
+                const double p11 = g * (C11 * Gxx + C66 * (Gyy + Gzz));
+                const double p22 = g * (C11 * Gyy + C66 * (Gxx + Gzz));
+                const double p33 = g * (C11 * Gzz + C66 * (Gxx + Gyy));
+                const double p12 = g * aa * Gxy, p13 = g * aa * Gxz, p23 = g * aa * Gyz;
+                const double cx = -g * A * Gx, cy = -g * A * Gy, cz = -g * A * Gz;
+                R[0] = coeff_c * d[1];
+                R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
+                R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+                R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+                R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+                KL[0][1] = coeff_c;
+                KL[1][0] = kl; KL[1][2] = cux; KL[1][3] = cuy; KL[1][4] = cuz;
+                KL[2][0] = cx; KL[2][2] = p11; KL[2][3] = p12; KL[2][4] = p13;
+                KL[3][0] = cy; KL[3][2] = p12; KL[3][3] = p22; KL[3][4] = p23;
+                KL[4][0] = cz; KL[4][2] = p13; KL[4][3] = p23; KL[4][4] = p33;
+            } else {
+                const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+                const double cux = g * Omega * Kh * Gx, cuy = g * Omega * Kh * Gy;
+                const double p11 = g * (C11 * Gxx + C66 * Gyy);
+                const double p22 = g * (C66 * Gxx + C11 * Gyy);
+                const double pxy = g * (C12 + C66) * Gxy;
+                const double cx = -g * A * Gx, cy = -g * A * Gy;
+                R[0] = coeff_c * d[1];
+                R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+                R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+                R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+                KL[0][1] = coeff_c;
+                KL[1][0] = kl; KL[1][2] = cux; KL[1][3] = cuy;
+                KL[2][0] = cx; KL[2][2] = p11; KL[2][3] = pxy;
+                KL[3][0] = cy; KL[3][2] = pxy; KL[3][3] = p22;
+            }
+            for (int a = 0; a < nd; a++) {
+                rhs[ri + a] += R[a];
+                for (int b = 0; b < nd; b++) {
+                    kvals[find_slot(rowptr, cols, (int)(ri + a), (int)(ri + b))] += -(-KL[a][b]);
+                    kvals[find_slot(rowptr, cols, (int)(ri + a), (int)(rj + b))] += -KL[a][b];
+                }
+            }
+        }
+    }
+    free(mn);
 }

@@ -41,6 +41,8 @@
 #include "ElmtSystem/CahnHilliardElement.h"
 #include "ElmtSystem/PoissonElement.h"
 #include "ElmtSystem/PDDODynamicFracElement.h"
+#include "ElmtSystem/StressCahnHilliardElement.h"
+#include "ElmtSystem/FracStressCahnHilliardElement.h"
 #undef private
 #undef protected
 
@@ -281,6 +283,38 @@ void ref_dynfrac_set(void *h, const double *b, const double *v0)
     if (!e) return;
     e->setBodyForce(b[0], b[1], b[2]);
     e->setInitialVelocity(v0[0], v0[1], v0[2]);
+}
+
+// StressCahnHilliardElement constants as the element holds them:
+// C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref
+void ref_stressch_consts(void *h, double *out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<StressCahnHilliardElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    const double v[11] = {e->m_C11, e->m_C12, e->m_C66, e->m_A, e->m_Kh, e->m_Ch,
+                          e->m_D, e->m_Kappa, e->m_Chi, e->m_Omega, e->m_CRef};
+    std::copy(v, v + 11, out);
+}
+
+// FracStressCahnHilliardElement state (valid after one ref_assemble_step):
+// bond health (CSR order), committed velocity (n*dim, zeros when rho <= 0),
+// the species Laplacian per bond, and k = C11, C12, C66, A, Kh, Ch, D, kappa,
+// chi, Omega, cref, rho, residual stiffness, critical stretch
+void ref_fsch_state(void *h, double *health, double *vel, double *lapc, double *k)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    auto *e = dynamic_cast<FracStressCahnHilliardElement *>(p.m_ElmtSystem.m_Elements[0].Element.get());
+    if (!e) return;
+    long long q = 0;
+    for (const auto &row : e->getBondHealth())
+        for (double v : row) health[q++] = v;
+    std::copy(e->getVel().begin(), e->getVel().end(), vel);
+    std::copy(e->getLapC().begin(), e->getLapC().end(), lapc);
+    const double v[14] = {e->m_C11, e->m_C12, e->m_C66, e->m_A, e->m_Kh, e->m_Ch, e->m_D, e->m_Kappa,
+                          e->m_Chi, e->m_Omega, e->m_CRef, e->m_Rho, e->m_ResidualStiffness,
+                          e->criticalStretch(p.m_PDMesh.getDataConstRef().HorizonRadius)};
+    std::copy(v, v + 14, k);
 }
 
 // CH element geometry (valid after one ref_assemble): lapbond, rev

@@ -31,7 +31,7 @@ def lib():
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
                   "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
                   "ref_apply_bcs", "ref_projection", "ref_ch_geometry", "ref_assemble_step",
-                  "ref_dynfrac_state", "ref_dynfrac_set",
+                  "ref_dynfrac_state", "ref_dynfrac_set", "ref_stressch_consts", "ref_fsch_state",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -162,6 +162,18 @@ class Case:
 
     def dynfrac_set(self, b, v0):
         lib().ref_dynfrac_set(self.h, _dp(np.ascontiguousarray(b, float)), _dp(np.ascontiguousarray(v0, float)))
+
+    def stressch_consts(self):
+        """C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref of the element."""
+        out = np.zeros(11)
+        lib().ref_stressch_consts(self.h, _dp(out))
+        return out
+
+    def fsch_state(self, dim):
+        """FracStressCahnHilliardElement: (health, vel, lapc, constants)."""
+        h = np.zeros(self.nb); v = np.zeros(self.n * dim); lc = np.zeros(self.nb); k = np.zeros(14)
+        lib().ref_fsch_state(self.h, _dp(h), _dp(v), _dp(lc), _dp(k))
+        return h, v, lc, k
 
     def ch_geometry(self):
         lap = np.zeros(self.nb); rev = np.zeros(self.nb, np.int32); mn = np.zeros(self.n)

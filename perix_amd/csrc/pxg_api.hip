@@ -43,6 +43,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->csrbc.row); rel(c->csrbc.dslot); rel(c->csrbc.pslot); rel(c->csrbc.pdof);
     rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);
     rel(c->dyn.health); rel(c->dyn.vel); rel(c->dyn.uprev);
+    rel(c->fsch.health); rel(c->fsch.lapc); rel(c->fsch.vel); rel(c->fsch.uprev);
     FracState &F = c->frac;
     rel(F.P64); rel(F.P32); rel(F.health); rel(F.break_step); rel(F.u[0]); rel(F.u[1]);
     rel(F.perm); rel(F.roff); rel(F.ulist); rel(F.gslot); rel(F.bovf); rel(F.ghist64); rel(F.bc.prog_of); rel(F.bsh); rel(F.pofs); rel(F.nbr_t); rel(F.health_csr); rel(F.bs_csr);

@@ -338,10 +338,8 @@ extern "C" int pxg_assemble_diffusion(pxg_ctx *ctx, double D, const double *f_no
     return PXG_OK;
 }
 
-extern "C" int pxg_ch_setup(pxg_ctx *ctx)
+int pxg::ch_geometry(pxg_ctx *ctx)
 {
-    PXG_CHECK(ctx && ctx->nop > 0 && ctx->nnz > 0 && ctx->csr_ndof == 2, PXG_ERR_STATE,
-              "pxg_ch_setup: needs operators and a 2-dof CSR pattern");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_ch_setup: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
     PXG_HIP(ctx->lap.alloc(ctx->nb));
@@ -355,6 +353,13 @@ extern "C" int pxg_ch_setup(pxg_ctx *ctx)
     PXG_HIP(hipStreamSynchronize(ctx->stream));
     ctx->ch_ready = 1;
     return PXG_OK;
+}
+
+extern "C" int pxg_ch_setup(pxg_ctx *ctx)
+{
+    PXG_CHECK(ctx && ctx->nop > 0 && ctx->nnz > 0 && ctx->csr_ndof == 2, PXG_ERR_STATE,
+              "pxg_ch_setup: needs operators and a 2-dof CSR pattern");
+    return ch_geometry(ctx);
 }
 
 extern "C" int pxg_ch_get_geometry(pxg_ctx *ctx, double *lapbond, int *rev, unsigned char *isghost)
@@ -374,6 +379,7 @@ extern "C" int pxg_assemble_ch_dev(pxg_ctx *ctx, double kappa, double chi, doubl
                                    double *rhs_dev)
 {
     PXG_CHECK(ctx && ctx->ch_ready, PXG_ERR_STATE, "pxg_assemble_ch: call pxg_ch_setup first");
+    PXG_CHECK(ctx->csr_ndof == 2, PXG_ERR_STATE, "pxg_assemble_ch: needs the 2-dof CSR pattern");
     PXG_CHECK(dt > 0.0, PXG_ERR_ARG, "pxg_assemble_ch: Cahn-Hilliard is transient-only (dt > 0)");
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_ch: bad arguments");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_ch: injected fault");

@@ -539,7 +539,7 @@ extern "C" int pxg_get_seed(pxg_ctx *ctx, unsigned char *seed)
 extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
 {
     PXG_CHECK(ctx && ctx->nb > 0, PXG_ERR_STATE, "pxg_build_csr: no families");
-    PXG_CHECK(ndof >= 1 && ndof <= 4, PXG_ERR_ARG, "pxg_build_csr: ndof must be 1..4");
+    PXG_CHECK(ndof >= 1 && ndof <= 8, PXG_ERR_ARG, "pxg_build_csr: ndof must be 1..8");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_build_csr: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
     ctx->csrbc.set = 0;                 // a BC row program refers to the old pattern

@@ -98,6 +98,25 @@ enum { BCROW_DIRECT = 0, BCROW_REFLECT = 1, BCROW_MIRROR = 2 };
 
 // implicit dynamic fracture element state (PDDODynamicFracElement): bond
 // health (CSR order), committed velocity and displacement history
+// stress-coupled Cahn-Hilliard element: C11, C12, C66, A, Kh, Ch, D, kappa,
+// chi, Omega, cref
+struct StressChState {
+    double k[11] = {0};
+    int ready = 0;
+};
+
+// fracture stress-coupled Cahn-Hilliard element: k = C11, C12, C66, A, Kh,
+// Ch, D, kappa, chi, Omega, cref, rho, residual stiffness
+struct FracSchState {
+    double k[13] = {0};
+    double sc = 0;
+    int tension_only = 1, damage_on = 1;
+    dbuf<unsigned char> health;   // nb, CSR order
+    dbuf<double> lapc;            // nb species Laplacian (0 on ghost bonds)
+    dbuf<double> vel, uprev;      // n*dim
+    int ready = 0;
+};
+
 struct DynFracState {
     int ready = 0;
     double C11 = 0, C12 = 0, C66 = 0, rho = 1, b[3] = {0, 0, 0}, sc = 0;
@@ -208,6 +227,9 @@ int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break_step);
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev);
 int launch_transpose_ops(pxg_ctx *ctx);
+// Cahn-Hilliard bond geometry (lambda-corrected Laplacian, reverse bonds,
+// ghost flags) into ctx->lap / rev / isghost; sets ch_ready
+int ch_geometry(pxg_ctx *ctx);
 
 } // namespace pxg
 
@@ -253,6 +275,8 @@ struct pxg_ctx {
     int ch_ready = 0;
     pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
     pxg::DynFracState dyn;          // implicit dynamic fracture element
+    pxg::StressChState sch;         // stress-coupled Cahn-Hilliard element
+    pxg::FracSchState fsch;         // fracture stress-coupled Cahn-Hilliard element
     // scratch for host-pointer entry points
     pxg::dbuf<double> su, suo, sk, sr, sf;
     // ---- explicit fracture ----

@@ -1,0 +1,661 @@
+// pxg_stressch.hip -- the stress-coupled Cahn-Hilliard elements
+// (StressCahnHilliardElement, FracStressCahnHilliardElement) on the device: CSR assembly of the
+// (c, mu, u) system with the conservative degenerate-mobility species flux,
+// the lambda-corrected ghost-excluded gradient-energy Laplacian, the
+// chemo-mechanical coupling and the PDDO Navier operator
+// (StressCahnHilliardElement.cpp:139-467 through the assembler of
+// FormResidualAndJacobian.cpp:25-132).
+//
+// One thread per node: the node's (dim+2)x(dim+2) block row is accumulated in
+// the reference's order (nodal term first, then the bonds in family order),
+// so K comes out bit for bit; the residual's mu rows carry ln(c/(1-c)),
+// whose device logarithm can differ from the host's in the last bit.  The
+// Picard mobility M(c) = D c (1-c) is evaluated from u at both ends of each
+// bond, as the reference's per-iteration cache holds it.
+//
+// The fracture variant adds irreversible bond failure on the elastic
+// (swelling-corrected) stretch, committed once per step on u_old, a health
+// gate with a residual-stiffness floor on the mechanical and sigma_h terms,
+// optional inertia with the committed velocity, and its own species
+// Laplacian (ghost bonds zeroed, no consistency rescaling).
+#include <algorithm>
+#include "pxg_internal.h"
+
+using namespace pxg;
+
+namespace {
+
+unsigned grids(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+struct SchConst {
+    double C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref;
+};
+
+__device__ __forceinline__ double sch_mobility(double c, double D)
+{
+    // mobility (StressCahnHilliardElement.cpp:61-64) with the 1e-8 clamp
+    const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
+    return D * cc * (1.0 - cc);
+}
+
+// regLog (StressCahnHilliardElement.cpp:42-46)
+__device__ __forceinline__ void sch_reglog(double c, double &L, double &Lp)
+{
+    const double delta = 1.0e-3;
+    if (c < delta) {
+        const double Lpd = 1.0 / (delta * (1.0 - delta));
+        L = log(delta / (1.0 - delta)) + Lpd * (c - delta);
+        Lp = Lpd;
+    } else if (c > 1.0 - delta) {
+        const double cb = 1.0 - delta, Lpb = 1.0 / (cb * (1.0 - cb));
+        L = log(cb / (1.0 - cb)) + Lpb * (c - cb);
+        Lp = Lpb;
+    } else {
+        L = log(c / (1.0 - c));
+        Lp = 1.0 / (c * (1.0 - c));
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(128) k_stressch(int n, int nop, const int *off, const int *nbr,
+                                                  const double *ops, const int *rev, const double *lap,
+                                                  const unsigned char *isghost, const double *vol, SchConst k,
+                                                  double dt, const int *rowptr, const int *slot,
+                                                  const int *diag, const int *mslot, const double *u,
+                                                  const double *uold, double *kvals, double *rhs)
+{
+    constexpr int ND = DIM + 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t ri = (size_t)i * ND;
+    const int rb = rowptr[ri];
+    int rowoff[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) rowoff[a] = rowptr[ri + a] - rb;
+    if (mslot[i] >= 0) {
+        // the ghost's mirror block is in the pattern but receives no term
+        const int m0 = mslot[i] - rb;
+#pragma unroll
+        for (int a = 0; a < ND; a++)
+#pragma unroll
+            for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + m0 + b] = 0.0;
+    }
+    double ui[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) ui[a] = u[ri + a];
+    // nodal term (StressCahnHilliardElement.cpp:449-466)
+    double r[ND], kd[ND][ND];
+    {
+        const double c = ui[0], mu = ui[1], cold = uold[ri];
+        double L, Lp;
+        sch_reglog(c, L, Lp);
+        const double fp = L + k.chi * (1.0 - 2.0 * c);
+        const double fpp = Lp - 2.0 * k.chi;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            r[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < ND; b++) kd[a][b] = 0.0 + (-0.0);
+        }
+        r[0] = 0.0 + (c - cold) / dt;
+        r[1] = 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
+        kd[0][0] = 0.0 + (-(1.0 / dt));
+        kd[1][0] = 0.0 + (-(-fpp + k.Omega * k.Ch));
+        kd[1][1] = 0.0 + (-1.0);
+    }
+    const double Mi = sch_mobility(ui[0], k.D), Vi = vol[i];
+    const double ocx = k.Omega * k.Kh;
+    const int d0 = diag[i] - rb;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        const int j = nbr[t] - 1;
+        const size_t rj = (size_t)j * ND;
+        const double *op = ops + (size_t)t * nop;
+        const bool ghostJ = isghost[j] != 0;
+        const double g2ij = lap[t];
+        double coeff_c = 0.0;
+        if (!ghostJ) {
+            const double Mj = sch_mobility(u[rj], k.D), Vj = vol[j];
+            const int rv = rev[t];
+            double wij;
+            if (rv >= 0 && Vi > 0.0) wij = 0.5 * (Mi * g2ij + (Vj / Vi) * Mj * lap[rv]);
+            else wij = Mi * g2ij;
+            coeff_c = -wij;
+        }
+        const double kl = ghostJ ? 0.0 : (k.kappa * g2ij);
+        double d[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) d[a] = u[rj + a] - ui[a];
+        // P = the bond's dR/du_j (StressCahnHilliardElement.cpp:334-427):
+        // K_II = -P, K_IJ = P
+        double P[ND][ND], R[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++)
+#pragma unroll
+            for (int b = 0; b < ND; b++) P[a][b] = 0.0;
+        if constexpr (DIM == 3) {
+            const double Gx = op[0], Gy = op[1], Gz = op[2];
+            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+            const double aa = k.C12 + k.C66;
+            const double cux = ocx * Gx, cuy = ocx * Gy, cuz = ocx * Gz;
+            const double p11 = k.C11 * Gxx + k.C66 * (Gyy + Gzz);
+            const double p22 = k.C11 * Gyy + k.C66 * (Gxx + Gzz);
+            const double p33 = k.C11 * Gzz + k.C66 * (Gxx + Gyy);
+            const double p12 = aa * Gxy, p13 = aa * Gxz, p23 = aa * Gyz;
+            const double cx = -k.A * Gx, cy = -k.A * Gy, cz = -k.A * Gz;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
+            R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+            R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+            R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
+            P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
+            P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
+        } else {
+            const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+            const double cux = ocx * Gx, cuy = ocx * Gy;
+            const double p11 = k.C11 * Gxx + k.C66 * Gyy;
+            const double p22 = k.C66 * Gxx + k.C11 * Gyy;
+            const double pxy = (k.C12 + k.C66) * Gxy;
+            const double cx = -k.A * Gx, cy = -k.A * Gy;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+            R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+            R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
+            P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+        }
+        // scatter (FormResidualAndJacobian.cpp:119-128)
+        const int s0 = slot[t] - rb;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            r[a] += R[a];
+#pragma unroll
+            for (int b = 0; b < ND; b++) {
+                kd[a][b] += -(-P[a][b]);
+                kvals[rb + rowoff[a] + s0 + b] = 0.0 + (-P[a][b]);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < ND; a++) {
+        rhs[ri + a] = r[a];
+#pragma unroll
+        for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + d0 + b] = kd[a][b];
+    }
+}
+
+// ---- fracture variant (FracStressCahnHilliardElement) ----------------------
+
+// species Laplacian: the full-family pdLaplacian, zero on bonds to ghosts
+// (FracStressCahnHilliardElement.cpp:238-268; no consistency rescaling)
+__global__ void k_fsch_lapc(long long nb, int dim, int nop, const int *nbr, const double *ops,
+                            const int *mirror, double *lapc)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const double *op = ops + (size_t)t * nop;
+    const double w = (dim >= 3) ? op[3] + op[6] + op[8] : op[2] + op[4];
+    lapc[t] = (mirror[nbr[t] - 1] > 0) ? 0.0 : w;
+}
+
+__global__ void k_fsch_init(long long nb, int n, int dim, const unsigned char *seed, const double *uold,
+                            unsigned char *health, double *uprev, double *vel)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb) health[t] = seed[t];
+    if (t < (long long)n * dim) {
+        vel[t] = 0.0;
+        if (uold) uprev[t] = uold[(t / dim) * (dim + 2) + 2 + t % dim];
+    }
+}
+
+// per-step commit on the accepted u_old (FracStressCahnHilliardElement.cpp:303-398)
+template <int DIM>
+__global__ void k_fsch_commit(int n, const double *coords, const int *off, const int *nbr, double sc,
+                              int tension_only, int damage_on, double Omega, double cref, double rho,
+                              double dt_prev, const double *uold, double *uprev, double *vel,
+                              unsigned char *health)
+{
+    constexpr int ND = DIM + 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t ri = (size_t)i * ND;
+    if (rho > 0.0) {
+        for (int a = 0; a < DIM; a++) {
+            const double ua = uold[ri + 2 + a];
+            vel[(size_t)i * DIM + a] = (ua - uprev[(size_t)i * DIM + a]) / dt_prev;
+            uprev[(size_t)i * DIM + a] = ua;
+        }
+    }
+    if (!(damage_on && sc > 0.0)) return;
+    // both ends of a bond compute the same stretch (negated xi and eta), so
+    // testing every bond from its own end reproduces the reference's
+    // reciprocal break
+    const double ci = uold[ri];
+    const double xi0 = coords[3 * (size_t)i], yi0 = coords[3 * (size_t)i + 1];
+    const double zi0 = (DIM >= 3) ? coords[3 * (size_t)i + 2] : 0.0;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        if (!health[t]) continue;
+        const int j = nbr[t] - 1;
+        const size_t rj = (size_t)j * ND;
+        double xi[3], eta[3];
+        xi[0] = coords[3 * (size_t)j] - xi0;
+        xi[1] = coords[3 * (size_t)j + 1] - yi0;
+        xi[2] = (DIM >= 3) ? coords[3 * (size_t)j + 2] - zi0 : 0.0;
+        eta[0] = uold[rj + 2] - uold[ri + 2];
+        eta[1] = uold[rj + 3] - uold[ri + 3];
+        eta[2] = (DIM >= 3) ? uold[rj + 4] - uold[ri + 4] : 0.0;
+        const double r2 = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+        if (r2 <= 0.0) continue;
+        const double r = sqrt(r2);
+        const double rinv = 1.0 / r;
+        const double yx = xi[0] + eta[0], yy = xi[1] + eta[1], yz = xi[2] + eta[2];
+        const double sn = (sqrt(yx * yx + yy * yy + yz * yz) - r) * rinv;
+        const double cbar = 0.5 * (ci + uold[rj]);
+        const double s_el = sn - (Omega / 3.0) * (cbar - cref);
+        const double so = tension_only ? fmax(s_el, 0.0) : fabs(s_el);
+        if (so >= sc) health[t] = 0;
+    }
+}
+
+struct FschConst {
+    double C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho, resid;
+};
+
+// assembly (FracStressCahnHilliardElement.cpp:411-561 through
+// FormResidualAndJacobian.cpp:25-132), one thread per node as k_stressch
+template <int DIM>
+__global__ void __launch_bounds__(128) k_fsch(int n, int nop, const int *off, const int *nbr,
+                                              const double *ops, const int *rev, const double *lapc,
+                                              const unsigned char *isghost, const double *vol, FschConst k,
+                                              double dt, const double *vel, const unsigned char *health,
+                                              const int *rowptr, const int *slot, const int *diag,
+                                              const int *mslot, const double *u, const double *uold,
+                                              double *kvals, double *rhs)
+{
+    constexpr int ND = DIM + 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t ri = (size_t)i * ND;
+    const int rb = rowptr[ri];
+    int rowoff[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) rowoff[a] = rowptr[ri + a] - rb;
+    if (mslot[i] >= 0) {
+        const int m0 = mslot[i] - rb;
+#pragma unroll
+        for (int a = 0; a < ND; a++)
+#pragma unroll
+            for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + m0 + b] = 0.0;
+    }
+    double ui[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) ui[a] = u[ri + a];
+    double r[ND], kd[ND][ND];
+    {
+        // nodal term with the clamped logarithm and optional inertia
+        const double c = ui[0], mu = ui[1], cold = uold[ri];
+        const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
+        const double fp = log(cc / (1.0 - cc)) + k.chi * (1.0 - 2.0 * cc);
+        const double fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * k.chi;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            r[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < ND; b++) kd[a][b] = 0.0 + (-0.0);
+        }
+        r[0] = 0.0 + (c - cold) / dt;
+        r[1] = 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
+        kd[0][0] = 0.0 + (-(1.0 / dt));
+        kd[1][0] = 0.0 + (-(-fpp + k.Omega * k.Ch));
+        kd[1][1] = 0.0 + (-1.0);
+        if (k.rho > 0.0) {
+            const double c0 = k.rho / (dt * dt);
+#pragma unroll
+            for (int a = 0; a < DIM; a++) {
+                const double v = vel[(size_t)i * DIM + a];
+                r[2 + a] = 0.0 + (0.0 + -c0 * (ui[2 + a] - uold[ri + 2 + a] - v * dt));
+                kd[2 + a][2 + a] = 0.0 + (-(0.0 + -c0));
+            }
+        }
+    }
+    const double Mi = sch_mobility(ui[0], k.D), Vi = vol[i];
+    const int d0 = diag[i] - rb;
+    for (int t = off[i]; t < off[i + 1]; t++) {
+        const int j = nbr[t] - 1;
+        const size_t rj = (size_t)j * ND;
+        const double *op = ops + (size_t)t * nop;
+        const double g = health[t] ? 1.0 : k.resid;
+        double wf = 0.0, g2c = 0.0;
+        if (!isghost[j]) {
+            g2c = lapc[t];
+            const int rv = rev[t];
+            if (rv >= 0) wf = 0.5 * (Mi * g2c + (vol[j] / Vi) * sch_mobility(u[rj], k.D) * lapc[rv]);
+            else wf = Mi * g2c;
+        }
+        const double coeff_c = -wf, kl = k.kappa * g2c;
+        double d[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) d[a] = u[rj + a] - ui[a];
+        double P[ND][ND], R[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++)
+#pragma unroll
+            for (int b = 0; b < ND; b++) P[a][b] = 0.0;
+        const double go = g * k.Omega;
+        if constexpr (DIM == 3) {
+            const double Gx = op[0], Gy = op[1], Gz = op[2];
+            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+            const double ga = g * (k.C12 + k.C66), gA = -g * k.A;
+            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy, cuz = go * k.Kh * Gz;
+            const double p11 = g * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
+            const double p22 = g * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
+            const double p33 = g * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
+            const double p12 = ga * Gxy, p13 = ga * Gxz, p23 = ga * Gyz;
+            const double cx = gA * Gx, cy = gA * Gy, cz = gA * Gz;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
+            R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+            R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+            R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
+            P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
+            P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
+        } else {
+            const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+            const double gA = -g * k.A;
+            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy;
+            const double p11 = g * (k.C11 * Gxx + k.C66 * Gyy);
+            const double p22 = g * (k.C66 * Gxx + k.C11 * Gyy);
+            const double pxy = g * (k.C12 + k.C66) * Gxy;
+            const double cx = gA * Gx, cy = gA * Gy;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+            R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+            R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
+            P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+        }
+        const int s0 = slot[t] - rb;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            r[a] += R[a];
+#pragma unroll
+            for (int b = 0; b < ND; b++) {
+                kd[a][b] += -(-P[a][b]);
+                kvals[rb + rowoff[a] + s0 + b] = 0.0 + (-P[a][b]);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < ND; a++) {
+        rhs[ri + a] = r[a];
+#pragma unroll
+        for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + d0 + b] = kd[a][b];
+    }
+}
+
+} // namespace
+
+// recalcElasticConstants (StressCahnHilliardElement.cpp:80-128)
+static int sch_constants(int dim, int plane_stress, double E, double nu, double Omega, double out[6])
+{
+    for (int q = 0; q < 6; q++) out[q] = 0.0;
+    if (E <= 0.0) return PXG_OK;
+    if (dim < 3 && plane_stress) {
+        const double d = 1.0 - nu * nu;
+        PXG_CHECK(d > 0.0, PXG_ERR_ARG, "pxg_stressch_setup: plane_stress needs nu^2 < 1");
+        out[0] = E / d;
+        out[1] = E * nu / d;
+        out[2] = E / (2.0 * (1.0 + nu));
+        out[3] = E * Omega / (3.0 * (1.0 - nu));
+        out[4] = (out[0] + out[1]) / 3.0;
+        out[5] = -2.0 * out[3] / 3.0;
+        return PXG_OK;
+    }
+    const double d = (1.0 + nu) * (1.0 - 2.0 * nu);
+    PXG_CHECK(d > 0.0, PXG_ERR_ARG, "pxg_stressch_setup: needs (1+nu)(1-2nu) > 0");
+    out[0] = E * (1.0 - nu) / d;
+    out[1] = E * nu / d;
+    out[2] = E / (2.0 * (1.0 + nu));
+    out[3] = E * Omega / (3.0 * (1.0 - 2.0 * nu));
+    out[4] = (out[0] + 2.0 * out[1]) / 3.0;
+    out[5] = -out[3];
+    return PXG_OK;
+}
+
+extern "C" int pxg_stressch_setup(pxg_ctx *ctx, const pxg_stressch_params *p)
+{
+    PXG_CHECK(ctx && p, PXG_ERR_ARG, "pxg_stressch_setup: bad arguments");
+    PXG_CHECK(ctx->nop > 0 && ctx->order >= 2, PXG_ERR_STATE,
+              "pxg_stressch_setup: needs order-2 operators (StressCahnHilliardElement.cpp:158-161)");
+    PXG_CHECK(ctx->nnz > 0 && ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE,
+              "pxg_stressch_setup: needs a CSR pattern with dim+2 dofs per node (c, mu, u)");
+    double e[6];
+    const int rc = sch_constants(ctx->dim, p->plane_stress, p->E, p->nu, p->Omega, e);
+    if (rc) return rc;
+    const int g = ch_geometry(ctx);
+    if (g) return g;
+    StressChState &S = ctx->sch;
+    const double v[11] = {e[0], e[1], e[2], e[3], e[4], e[5], p->D, p->kappa, p->chi, p->Omega, p->cref};
+    for (int q = 0; q < 11; q++) S.k[q] = v[q];
+    S.ready = 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_stressch_constants(pxg_ctx *ctx, double *out)
+{
+    PXG_CHECK(ctx && ctx->sch.ready && out, PXG_ERR_STATE, "pxg_stressch_constants: call pxg_stressch_setup first");
+    for (int q = 0; q < 11; q++) out[q] = ctx->sch.k[q];
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev,
+                                         double *kvals_dev, double *rhs_dev)
+{
+    PXG_CHECK(ctx && ctx->sch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_assemble_stressch: call pxg_stressch_setup first");
+    PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_assemble_stressch: CSR pattern changed");
+    PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_stressch: bad arguments");
+    PXG_CHECK(dt > 0.0, PXG_ERR_ARG,
+              "pxg_assemble_stressch: the element is transient-only (StressCahnHilliardElement.cpp:443-447)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_stressch: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const double *q = ctx->sch.k;
+    const SchConst k{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10]};
+    const unsigned g = grids(ctx->n, 128);
+    if (ctx->dim == 3)
+        hipLaunchKernelGGL(k_stressch<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p, ctx->vols.p, k, dt,
+                           ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
+                           rhs_dev);
+    else
+        hipLaunchKernelGGL(k_stressch<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p, ctx->vols.p, k, dt,
+                           ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
+                           rhs_dev);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_stressch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
+                                     double *rhs)
+{
+    PXG_CHECK(ctx && ctx->sch.ready, PXG_ERR_STATE, "pxg_assemble_stressch: call pxg_stressch_setup first");
+    PXG_CHECK(u && uold && kvals && rhs, PXG_ERR_ARG, "pxg_assemble_stressch: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)ctx->n * (ctx->dim + 2);
+    PXG_HIP(ctx->su.alloc(nd));
+    PXG_HIP(ctx->suo.alloc(nd));
+    PXG_HIP(ctx->sk.alloc(ctx->nnz));
+    PXG_HIP(ctx->sr.alloc(nd));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    const int rc = pxg_assemble_stressch_dev(ctx, dt, ctx->su.p, ctx->suo.p, ctx->sk.p, ctx->sr.p);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+// ---- fracture variant entry points -----------------------------------------
+
+extern "C" int pxg_fsch_setup(pxg_ctx *ctx, const pxg_fsch_params *p, const double *uold0)
+{
+    PXG_CHECK(ctx && p, PXG_ERR_ARG, "pxg_fsch_setup: bad arguments");
+    PXG_CHECK(ctx->nop > 0 && ctx->order >= 2, PXG_ERR_STATE,
+              "pxg_fsch_setup: needs order-2 operators (FracStressCahnHilliardElement.cpp:180-183)");
+    PXG_CHECK(ctx->nnz > 0 && ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE,
+              "pxg_fsch_setup: needs a CSR pattern with dim+2 dofs per node (c, mu, u)");
+    PXG_CHECK(!(p->rho > 0.0) || uold0, PXG_ERR_ARG, "pxg_fsch_setup: rho > 0 needs the initial u_old");
+    double e[6];
+    const int rc = sch_constants(ctx->dim, p->base.plane_stress, p->base.E, p->base.nu, p->base.Omega, e);
+    if (rc) return rc;
+    const int g = ch_geometry(ctx);   // reverse bonds and ghost flags
+    if (g) return g;
+    FracSchState &F = ctx->fsch;
+    hipStream_t st = ctx->stream;
+    const double v[13] = {e[0], e[1], e[2], e[3], e[4], e[5], p->base.D, p->base.kappa, p->base.chi,
+                          p->base.Omega, p->base.cref, p->rho, p->residual_stiffness};
+    for (int q = 0; q < 13; q++) F.k[q] = v[q];
+    F.sc = p->sc;
+    F.tension_only = p->tension_only;
+    F.damage_on = p->damage_on;
+    const size_t nd = (size_t)ctx->n * ctx->dim;
+    PXG_HIP(F.health.alloc((size_t)ctx->nb));
+    PXG_HIP(F.lapc.alloc((size_t)ctx->nb));
+    PXG_HIP(F.vel.alloc(nd));
+    PXG_HIP(F.uprev.alloc(nd));
+    const double *uo = nullptr;
+    if (p->rho > 0.0) {
+        const size_t nu = (size_t)ctx->n * (ctx->dim + 2);
+        PXG_HIP(ctx->suo.alloc(nu));
+        PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold0, sizeof(double) * nu, hipMemcpyHostToDevice, st));
+        uo = ctx->suo.p;
+    }
+    hipLaunchKernelGGL(k_fsch_lapc, dim3(grids(ctx->nb, 256)), dim3(256), 0, st, ctx->nb, ctx->dim, ctx->nop,
+                       ctx->nbr.p, ctx->ops.p, ctx->mirror.p, F.lapc.p);
+    const long long m = std::max((long long)ctx->nb, (long long)nd);
+    hipLaunchKernelGGL(k_fsch_init, dim3(grids(m, 256)), dim3(256), 0, st, ctx->nb, ctx->n, ctx->dim,
+                       ctx->seed.p, uo, F.health.p, F.uprev.p, F.vel.p);
+    PXG_HIP(hipGetLastError());
+    PXG_HIP(hipStreamSynchronize(st));
+    F.ready = 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_fsch_constants(pxg_ctx *ctx, double *out)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready && out, PXG_ERR_STATE, "pxg_fsch_constants: call pxg_fsch_setup first");
+    for (int q = 0; q < 13; q++) out[q] = ctx->fsch.k[q];
+    out[13] = ctx->fsch.sc;
+    return PXG_OK;
+}
+
+extern "C" int pxg_fsch_commit_dev(pxg_ctx *ctx, const double *uold_dev, double dt_prev)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_commit: call pxg_fsch_setup first");
+    PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_fsch_commit: CSR pattern changed");
+    PXG_CHECK(uold_dev && dt_prev > 0.0, PXG_ERR_ARG, "pxg_fsch_commit: bad arguments");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_commit: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    FracSchState &F = ctx->fsch;
+    const unsigned g = grids(ctx->n, 128);
+    if (ctx->dim == 3)
+        hipLaunchKernelGGL(k_fsch_commit<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->coords.p,
+                           ctx->off.p, ctx->nbr.p, F.sc, F.tension_only, F.damage_on, F.k[9], F.k[10], F.k[11],
+                           dt_prev, uold_dev, F.uprev.p, F.vel.p, F.health.p);
+    else
+        hipLaunchKernelGGL(k_fsch_commit<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->coords.p,
+                           ctx->off.p, ctx->nbr.p, F.sc, F.tension_only, F.damage_on, F.k[9], F.k[10], F.k[11],
+                           dt_prev, uold_dev, F.uprev.p, F.vel.p, F.health.p);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_fsch_commit(pxg_ctx *ctx, const double *uold, double dt_prev)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_commit: call pxg_fsch_setup first");
+    PXG_CHECK(uold, PXG_ERR_ARG, "pxg_fsch_commit: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const size_t nu = (size_t)ctx->n * (ctx->dim + 2);
+    PXG_HIP(ctx->suo.alloc(nu));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * nu, hipMemcpyHostToDevice, ctx->stream));
+    const int rc = pxg_fsch_commit_dev(ctx, ctx->suo.p, dt_prev);
+    if (rc) return rc;
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev,
+                                     double *kvals_dev, double *rhs_dev)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_assemble_fsch: call pxg_fsch_setup first");
+    PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_assemble_fsch: CSR pattern changed");
+    PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_fsch: bad arguments");
+    PXG_CHECK(dt > 0.0, PXG_ERR_ARG,
+              "pxg_assemble_fsch: the element is transient-only (FracStressCahnHilliardElement.cpp:536-540)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_fsch: injected fault");
+    PXG_HIP(hipSetDevice(ctx->device));
+    const FracSchState &F = ctx->fsch;
+    const double *q = F.k;
+    const FschConst k{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12]};
+    const unsigned g = grids(ctx->n, 128);
+    if (ctx->dim == 3)
+        hipLaunchKernelGGL(k_fsch<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, F.lapc.p, ctx->isghost.p, ctx->vols.p, k, dt,
+                           F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev,
+                           uold_dev, kvals_dev, rhs_dev);
+    else
+        hipLaunchKernelGGL(k_fsch<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, F.lapc.p, ctx->isghost.p, ctx->vols.p, k, dt,
+                           F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev,
+                           uold_dev, kvals_dev, rhs_dev);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_fsch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
+                                 double *rhs)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_assemble_fsch: call pxg_fsch_setup first");
+    PXG_CHECK(u && uold && kvals && rhs, PXG_ERR_ARG, "pxg_assemble_fsch: bad arguments");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nd = (size_t)ctx->n * (ctx->dim + 2);
+    PXG_HIP(ctx->su.alloc(nd));
+    PXG_HIP(ctx->suo.alloc(nd));
+    PXG_HIP(ctx->sk.alloc(ctx->nnz));
+    PXG_HIP(ctx->sr.alloc(nd));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    const int rc = pxg_assemble_fsch_dev(ctx, dt, ctx->su.p, ctx->suo.p, ctx->sk.p, ctx->sr.p);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_fsch_get_state(pxg_ctx *ctx, unsigned char *health, double *vel)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_get_state: call pxg_fsch_setup first");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const FracSchState &F = ctx->fsch;
+    if (health) PXG_HIP(hipMemcpyAsync(health, F.health.p, (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    if (vel)
+        PXG_HIP(hipMemcpyAsync(vel, F.vel.p, sizeof(double) * ctx->n * ctx->dim, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}

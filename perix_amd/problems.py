@@ -221,3 +221,75 @@ def cahnhilliard_square(nx=256, ny=256, seed=12345, factor=3.015, device=0, ctx=
             u[2 * (nid - 1)] = u[2 * b]
             u[2 * (nid - 1) + 1] = u[2 * b + 1]
     return ctx, cloud, delta, u
+
+
+# silicon anode parameters of the reference's stress_cahnhilliard example deck
+# (examples/silicon.json: plane stress, nondimensional units)
+SILICON = dict(E=87.9336, nu=0.29, D=1e-4, Omega=3.0, cref=0.0, chi=2.6, kappa=2.19834e-6)
+
+
+def stressch_square(nx=256, ny=256, seed=7, factor=3.015, device=0, ctx=None, params=None):
+    """Stress-coupled Cahn-Hilliard (c, mu, ux, uy) on the nx x ny unit square
+    with the silicon parameters in plane stress; returns the context (CSR with
+    4 dofs per node, element set up), the cloud and a seeded (u, uold) pair:
+    c = 0.3 + U(0, 0.4), mu = 0.05 N(0, 1), u = 1e-3 N(0, 1), uold with c
+    lowered by U(0, 0.01)."""
+    g = StructuredGrid(2, nx, ny)
+    cloud = build_structured_cloud(g)
+    delta = horizon_radius(g, factor)
+    if ctx is None:
+        ctx = pxg.Context(device)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, 2)
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(4)
+    p = dict(SILICON, **(params or {}))
+    ctx.stressch_setup(pxg.StressChParams(p["E"], p["nu"], p["D"], p["Omega"], p["cref"], p["chi"],
+                                          p["kappa"], 1))
+    rs = np.random.RandomState(seed)
+    n = cloud.n
+    u = np.zeros(4 * n)
+    u[0::4] = 0.3 + 0.4 * rs.rand(n)
+    u[1::4] = 0.05 * rs.standard_normal(n)
+    u[2::4] = 1e-3 * rs.standard_normal(n)
+    u[3::4] = 1e-3 * rs.standard_normal(n)
+    uold = u.copy()
+    uold[0::4] -= 0.01 * rs.rand(n)
+    return ctx, cloud, delta, u, uold
+
+
+# fracture variant: the reference's examples/silicon_frac.json element block
+SILICON_FRAC = dict(SILICON, D=0.03, rho=2.41817e-22, Gc=0.10662, tension_only=True, damage_on=False)
+
+
+def fsch_square(nx=256, ny=256, seed=7, factor=3.015, device=0, ctx=None, params=None):
+    """Fracture stress-coupled Cahn-Hilliard on the nx x ny unit square with
+    the silicon_frac element parameters (plane stress) and two force-only edge
+    notches (x = 0.25 and 0.75 from y = 0 to 0.1); the element is set up from
+    u_old and returned with a seeded (u, uold) pair as in stressch_square."""
+    g = StructuredGrid(2, nx, ny)
+    cloud = build_structured_cloud(g)
+    delta = horizon_radius(g, factor)
+    if ctx is None:
+        ctx = pxg.Context(device)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, 2)
+    ctx.set_cracks(segs=[(0.25, 0.0, 0.25, 0.1), (0.75, 0.0, 0.75, 0.1)])
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(4)
+    p = dict(SILICON_FRAC, **(params or {}))
+    rs = np.random.RandomState(seed)
+    n = cloud.n
+    u = np.zeros(4 * n)
+    u[0::4] = 0.3 + 0.4 * rs.rand(n)
+    u[1::4] = 0.05 * rs.standard_normal(n)
+    u[2::4] = 1e-3 * rs.standard_normal(n)
+    u[3::4] = 1e-3 * rs.standard_normal(n)
+    uold = u.copy()
+    uold[0::4] -= 0.01 * rs.rand(n)
+    base = pxg.StressChParams(p["E"], p["nu"], p["D"], p["Omega"], p["cref"], p["chi"], p["kappa"], 1)
+    sc = critical_stretch(p["Gc"], p["E"], delta, 2, plane_stress=True)
+    ctx.fsch_setup(pxg.FschParams(base, p["rho"], sc, p.get("residual_stiffness", 1e-8),
+                                  int(p["tension_only"]), int(p["damage_on"])), uold)
+    return ctx, cloud, delta, u, uold
+

@@ -36,6 +36,21 @@ class FracParams(C.Structure):
                 ("tension_only", C.c_int), ("damage_on", C.c_int)]
 
 
+class StressChParams(C.Structure):
+    """pxg_stressch_params: the stress_cahnhilliard deck parameters."""
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("D", C.c_double),
+                ("Omega", C.c_double), ("cref", C.c_double), ("chi", C.c_double),
+                ("kappa", C.c_double), ("plane_stress", C.c_int)]
+
+
+class FschParams(C.Structure):
+    """pxg_fsch_params: the frac_stress_cahnhilliard deck parameters, with
+    the critical stretch evaluated from Gc by the caller."""
+    _fields_ = [("base", StressChParams), ("rho", C.c_double), ("sc", C.c_double),
+                ("residual_stiffness", C.c_double), ("tension_only", C.c_int),
+                ("damage_on", C.c_int)]
+
+
 class BC(C.Structure):
     _fields_ = [("type", C.c_int), ("direct", C.c_int), ("ncomp", C.c_int),
                 ("comp", C.c_int * 3), ("value", C.c_double * 3),
@@ -54,7 +69,11 @@ SYMBOLS = [
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
     "pxg_csr_apply_bcs_dev", "pxg_dynfrac_setup", "pxg_dynfrac_commit", "pxg_dynfrac_commit_dev",
-    "pxg_assemble_dynfrac", "pxg_assemble_dynfrac_dev", "pxg_dynfrac_get_state", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
+    "pxg_assemble_dynfrac", "pxg_assemble_dynfrac_dev", "pxg_dynfrac_get_state",
+    "pxg_stressch_setup", "pxg_stressch_constants", "pxg_assemble_stressch", "pxg_assemble_stressch_dev",
+    "pxg_fsch_setup", "pxg_fsch_constants", "pxg_fsch_commit", "pxg_fsch_commit_dev",
+    "pxg_assemble_fsch", "pxg_assemble_fsch_dev", "pxg_fsch_get_state",
+    "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
     "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
@@ -117,6 +136,19 @@ def load():
         L.pxg_assemble_dynfrac_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]
         L.pxg_dynfrac_get_state.argtypes = [C.c_void_p, U8, D]
+        L.pxg_fsch_setup.argtypes = [C.c_void_p, C.POINTER(FschParams), D]
+        L.pxg_fsch_constants.argtypes = [C.c_void_p, D]
+        L.pxg_fsch_commit.argtypes = [C.c_void_p, D, C.c_double]
+        L.pxg_fsch_commit_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+        L.pxg_assemble_fsch.argtypes = [C.c_void_p, C.c_double, D, D, D, D]
+        L.pxg_assemble_fsch_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
+        L.pxg_fsch_get_state.argtypes = [C.c_void_p, U8, D]
+        L.pxg_stressch_setup.argtypes = [C.c_void_p, C.POINTER(StressChParams)]
+        L.pxg_stressch_constants.argtypes = [C.c_void_p, D]
+        L.pxg_assemble_stressch.argtypes = [C.c_void_p, C.c_double, D, D, D, D]
+        L.pxg_assemble_stressch_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p]
         L.pxg_frac_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, I]
         L.pxg_frac_adjoint.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int, C.c_int, C.c_int,
                                        I, C.c_int, D, D]
@@ -381,6 +413,60 @@ class Context:
         h = np.zeros(self.nb, np.uint8)
         v = np.zeros(self.n * self.dim)
         _check(load().pxg_dynfrac_get_state(self.h, _up(h), _dp(v)))
+        return h, v
+
+    # -- stress-coupled Cahn-Hilliard element -------------------------------------
+    def stressch_setup(self, params):
+        _check(load().pxg_stressch_setup(self.h, C.byref(params)))
+
+    def stressch_constants(self):
+        """C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref."""
+        out = np.zeros(11)
+        _check(load().pxg_stressch_constants(self.h, _dp(out)))
+        return out
+
+    def assemble_stressch(self, dt, u, uold):
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n * (self.dim + 2))
+        _check(load().pxg_assemble_stressch(self.h, dt, _dp(np.ascontiguousarray(u, float)),
+                                            _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_stressch_dev(self, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
+        _check(load().pxg_assemble_stressch_dev(self.h, dt, u_ptr, uold_ptr, k_ptr, r_ptr))
+
+    # -- fracture stress-coupled Cahn-Hilliard element ----------------------------
+    def fsch_setup(self, params, uold0=None):
+        uo = None if uold0 is None else np.ascontiguousarray(uold0, float)
+        _check(load().pxg_fsch_setup(self.h, C.byref(params), _dp(uo)))
+
+    def fsch_constants(self):
+        """C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho, residual
+        stiffness, critical stretch."""
+        out = np.zeros(14)
+        _check(load().pxg_fsch_constants(self.h, _dp(out)))
+        return out
+
+    def fsch_commit(self, uold, dt_prev):
+        _check(load().pxg_fsch_commit(self.h, _dp(np.ascontiguousarray(uold, float)), dt_prev))
+
+    def fsch_commit_dev(self, uold_ptr, dt_prev):
+        _check(load().pxg_fsch_commit_dev(self.h, uold_ptr, dt_prev))
+
+    def assemble_fsch(self, dt, u, uold):
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n * (self.dim + 2))
+        _check(load().pxg_assemble_fsch(self.h, dt, _dp(np.ascontiguousarray(u, float)),
+                                        _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_fsch_dev(self, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
+        _check(load().pxg_assemble_fsch_dev(self.h, dt, u_ptr, uold_ptr, k_ptr, r_ptr))
+
+    def fsch_state(self):
+        h = np.zeros(self.nb, np.uint8)
+        v = np.zeros(self.n * self.dim)
+        _check(load().pxg_fsch_get_state(self.h, _up(h), _dp(v)))
         return h, v
 
     # -- a15-a20 ---------------------------------------------------------------

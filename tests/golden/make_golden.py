@@ -23,6 +23,13 @@ Fixtures (all small, .npz):
                     the mesh_fam grids for a seeded (u, u_old)
   dynfrac_*.npz     PDDODynamicFracElement: two backward-Euler steps (the
                     initialising step and a committing step with bond failure)
+  stressch_*.npz    StressCahnHilliardElement assembly (c, mu, u): plane
+                    strain, plane stress and 3D, concentrations also outside
+                    the log barrier and the mobility clamp
+  fsch_*.npz        FracStressCahnHilliardElement over an initialising and a
+                    committing step (2D plane stress with inertia and a
+                    residual stiffness; 3D quasi-static, tension and
+                    compression failure)
   bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
                     Poisson (20x12) / Cahn-Hilliard (16x16) system under an
                     overlapping Dirichlet/Neumann list, with zeroed diagonals
@@ -197,6 +204,104 @@ def dynfrac_fixture(name, dim, nx, ny, nz, hi, crack, amp):
     return int((h1 == 0).sum()), int((h2 == 0).sum()), len(h2)
 
 
+def stressch_fixture(name, dim, nx, ny, nz, hi, state="plane_strain"):
+    """StressCahnHilliardElement assembly (c, mu, u) for a seeded state whose
+    concentrations also visit the regularised-log and mobility-clamp ranges."""
+    dofs = ["c", "mu", "ux", "uy"] + (["uz"] if dim == 3 else [])
+    params = {"E": 70e9, "nu": 0.3, "D": 1e-3, "Omega": 3.5e-6, "cref": 0.5, "chi": 2.5, "kappa": 2e-4}
+    if dim == 2:
+        params["state"] = state
+    d = {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "DOFs": dofs,
+         "ElmtSystem": {"s": {"type": "stress_cahnhilliard", "params": params}},
+         "BCSystem": {"l": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["c", "mu"],
+                            "value": [0.0, 0.0]}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 1e-3, "total_time": 1e-2}}
+    c = refpy.Case(d)
+    nd = dim + 2
+    rs = np.random.RandomState(6)
+    u = np.zeros(c.n * nd)
+    u[0::nd] = 0.2 + 0.6 * rs.rand(c.n)
+    pick = rs.choice(c.n, 12, replace=False)
+    u[pick[:3] * nd] = [-0.01, 5e-4, 1e-9]             # below the barrier / clamp
+    u[pick[3:6] * nd] = [1.02, 1 - 5e-4, 1 - 1e-9]     # above
+    u[1::nd] = 0.1 * rs.standard_normal(c.n)
+    for a in range(dim):
+        u[2 + a::nd] = 1e-6 * rs.standard_normal(c.n)
+    uold = u.copy()
+    uold[0::nd] -= 0.01 * rs.rand(c.n)
+    k, r = c.assemble(u, uold, 1e-3)
+    consts = c.stressch_consts()
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), state=state, consts=consts, dt=1e-3, u=u, uold=uold,
+                        K=k, R=r)
+    c.close()
+
+
+def fsch_fixture(name, dim, nx, ny, nz, hi, crack, rho, tension_only, resid=None, state="plane_stress"):
+    """FracStressCahnHilliardElement: two steps through the assembler (step
+    sizes 1e-3 then 5e-4).  Step 1 initialises the element (force-only seeded
+    health, zero velocity, u_prev = u_old); step 2 commits on u_old (velocity
+    when rho > 0, bond failure on the swelling-corrected stretch) and
+    assembles.  Concentrations also visit the mobility / log clamp range."""
+    from perix_amd import problems
+    nd = dim + 2
+    dofs = ["c", "mu", "ux", "uy"] + (["uz"] if dim == 3 else [])
+    E, nu, sc = 87.9336, 0.29, 2e-3
+    dx = hi[0] / nx
+    delta = 3.015 * dx
+    G0 = problems.g0_for_stretch(sc, E, delta, dim, plane_stress=(dim == 2 and state == "plane_stress"))
+    params = {"E": E, "nu": nu, "D": 1e-2, "Omega": 0.01, "cref": 0.5, "chi": 2.6, "kappa": 2e-4,
+              "rho": rho, "Gc": G0, "tension_only": tension_only, "damage_on": True}
+    if resid is not None:
+        params["residual_stiffness"] = resid
+    if dim == 2:
+        params["state"] = state
+    d = {"Mesh": refpy.mesh_block(dim, nx, ny, nz, hi=hi),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
+                        "Cracks": [{"x1": crack[0], "y1": crack[1], "x2": crack[2], "y2": crack[3]}]},
+         "DOFs": dofs,
+         "ElmtSystem": {"s": {"type": "frac_stress_cahnhilliard", "params": params}},
+         "BCSystem": {"l": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["c", "mu"],
+                            "value": [0.0, 0.0]}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 1e-3, "total_time": 1e-2}}
+    c = refpy.Case(d)
+    rs = np.random.RandomState(11)
+    n = c.n
+
+    def state_pair(amp_u):
+        u = np.zeros(n * nd)
+        u[0::nd] = 0.2 + 0.6 * rs.rand(n)
+        pick = rs.choice(n, 6, replace=False)
+        u[pick * nd] = [-0.02, 1e-9, 0.0, 1.0, 1 - 1e-9, 1.03]
+        u[1::nd] = 0.1 * rs.standard_normal(n)
+        for a in range(dim):
+            u[2 + a::nd] = amp_u * rs.standard_normal(n)
+        return u
+
+    uold1 = state_pair(1e-5 * dx)
+    u1 = uold1 + 1e-6 * rs.standard_normal(n * nd)
+    dt1, dt2 = 1e-3, 5e-4
+    k1, r1 = c.assemble_step(u1, uold1, dt1, 1)
+    h1, v1, lapc, kc = c.fsch_state(dim)
+    uold2 = state_pair(0.6 * sc * dx)
+    u2 = uold2 + 1e-6 * rs.standard_normal(n * nd)
+    k2, r2 = c.assemble_step(u2, uold2, dt2, 2)
+    h2, v2, _, _ = c.fsch_state(dim)
+    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+                        hi=np.array(hi, float), crack=np.array(crack, float), state=state,
+                        E=E, nu=nu, G0=G0, consts=kc, tension_only=int(tension_only),
+                        dt1=dt1, dt2=dt2, lapc=lapc,
+                        uold1=uold1, u1=u1, K1=k1, R1=r1, health1=h1, vel1=v1,
+                        uold2=uold2, u2=u2, K2=k2, R2=r2, health2=h2, vel2=v2)
+    c.close()
+    return int((h1 == 0).sum()), int((h2 == 0).sum()), len(h2)
+
+
 # BC lists of the row-elimination fixtures: (name, type, group, dofs, value,
 # velocity, direct, box) in deck order; overlapping entries on one row
 # exercise the list order
@@ -292,6 +397,13 @@ def main():
     diffusion_fixture("diffusion_3d.npz", 3, 6, 5, 4, (0.6, 0.5, 0.4))
     dynfrac_fixture("dynfrac_2d.npz", 2, 24, 12, 1, (0.024, 0.012, 1.0), (0.012, 0.0, 0.012, 0.005), 1.5e-7)
     dynfrac_fixture("dynfrac_3d.npz", 3, 6, 4, 3, (0.006, 0.004, 0.003), (0.003, 0.0, 0.003, 0.002), 1e-7)
+    stressch_fixture("stressch_2d.npz", 2, 14, 10, 1, (1.4, 1.0, 1.0))
+    stressch_fixture("stressch_2d_ps.npz", 2, 10, 8, 1, (1.0, 0.8, 1.0), state="plane_stress")
+    stressch_fixture("stressch_3d.npz", 3, 5, 4, 3, (0.5, 0.4, 0.3))
+    print("fsch_2d", fsch_fixture("fsch_2d.npz", 2, 14, 10, 1, (1.4, 1.0, 1.0), (0.7, 0.0, 0.7, 0.4),
+                                  rho=2.0, tension_only=True, resid=1e-3))
+    print("fsch_3d", fsch_fixture("fsch_3d.npz", 3, 6, 4, 3, (0.6, 0.4, 0.3), (0.3, 0.0, 0.3, 0.15),
+                                  rho=0.0, tension_only=False))
     bcrows_fixture("bcrows_poisson_2d.npz", "poisson")
     bcrows_fixture("bcrows_ch_2d.npz", "ch")
     for f in sorted(os.listdir(HERE)):

@@ -176,3 +176,68 @@ def test_dynamic_fracture_element(name):
     k, r = pxo.assemble_dynfrac(m, ops, rp, cols, C11, C12, C66, rho, z["b"], dt, vel, health,
                                 z["u2"], z["uold2"])
     assert same(k, z["K2"]) and same(r, z["R2"]), "step 2"
+
+
+@pytest.mark.parametrize("name", ["stressch_2d.npz", "stressch_2d_ps.npz", "stressch_3d.npz"])
+def test_stress_cahn_hilliard_assembly(name):
+    """StressCahnHilliardElement through the assembler (plane strain, plane
+    stress, 3D; concentrations also below and above the log barrier and the
+    mobility clamp): the element constants, K and the residual bit for bit."""
+    z = load(name)
+    dim = int(z["dim"])
+    m = mesh_of(dict(z, factor=3.015))
+    ops, _ = pxo.operators(m)
+    nd = dim + 2
+    rp, cols = pxo.csr_pattern(m, nd)
+    ref = z["consts"]
+    E, nu, Omega = 70e9, 0.3, float(ref[9])
+    kc = pxo.stressch_consts(dim, str(z["state"]) == "plane_stress", E, nu, Omega)
+    assert same(kc, ref[:6])
+    lap, rev, isg = pxo.ch_geometry(m, ops)
+    k, r = pxo.assemble_stressch(m, ops, lap, rev, isg, ref, float(z["dt"]), rp, cols, z["u"], z["uold"])
+    assert same(k, z["K"]) and same(r, z["R"])
+    # the barrier: outside [1e-3, 1 - 1e-3] the residual's nodal part is the
+    # tangent line of ln(c/(1-c)), so the mu rows stay finite for c <= 0, c >= 1
+    c = z["u"][0::nd]
+    assert (c < 0).any() and (c > 1).any() and np.isfinite(r).all()
+
+
+@pytest.mark.parametrize("name", ["fsch_2d.npz", "fsch_3d.npz"])
+def test_fracture_stress_cahn_hilliard_element(name):
+    """FracStressCahnHilliardElement through the assembler over an
+    initialising step and a committing step (2D plane stress with inertia
+    and a residual stiffness of 1e-3; 3D quasi-static with tension and
+    compression failure): constants, species Laplacian, committed velocity,
+    bond health, K and the residual bit for bit."""
+    z = load(name)
+    dim = int(z["dim"])
+    nd = dim + 2
+    m = mesh_of(dict(z, factor=3.015))
+    ops, _ = pxo.operators(m)
+    rp, cols = pxo.csr_pattern(m, nd)
+    kc = z["consts"]
+    ps = dim == 2 and str(z["state"]) == "plane_stress"
+    E, nu = float(z["E"]), float(z["nu"])
+    assert same(pxo.stressch_consts(dim, ps, E, nu, float(kc[9])), kc[:6])
+    sc = pxo.critical_stretch(float(z["G0"]), E, m.delta, dim, ps)
+    assert sc == kc[13]
+    lapc = pxo.fsch_lapc(m, ops)
+    assert same(lapc, z["lapc"])
+    _, rev, isg = pxo.ch_geometry(m, ops)
+    rho = float(kc[11])
+    health = pxo.seed_health(m, segs=[tuple(z["crack"])])
+    assert same(health, z["health1"]), "force-only seeded health"
+    vel = np.zeros(m.n * dim)
+    k, r = pxo.assemble_fsch(m, ops, lapc, rev, isg, kc[:13], float(z["dt1"]), vel, health, rp, cols,
+                             z["u1"], z["uold1"])
+    assert same(k, z["K1"]) and same(r, z["R1"]), "step 1"
+    uprev = np.ascontiguousarray(z["uold1"].reshape(-1, nd)[:, 2:].ravel())
+    pxo.fsch_commit(m, sc, bool(z["tension_only"]), True, float(kc[9]), float(kc[10]), rho, float(z["dt1"]),
+                    z["uold2"], uprev, vel, health)
+    if rho > 0:
+        assert same(vel, z["vel2"])
+    assert same(health, z["health2"]), "commit"
+    assert (health == 0).sum() > (z["health1"] == 0).sum()
+    k, r = pxo.assemble_fsch(m, ops, lapc, rev, isg, kc[:13], float(z["dt2"]), vel, health, rp, cols,
+                             z["u2"], z["uold2"])
+    assert same(k, z["K2"]) and same(r, z["R2"]), "step 2"

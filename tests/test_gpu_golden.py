@@ -202,3 +202,125 @@ def test_dynamic_fracture_vs_reference(name, oracle):
         assert np.abs(k - K).max() <= 1e-12 * kscale
         assert np.abs(r - R).max() <= 1e-12 * (np.abs(R).max() + kscale * np.abs(u).max())
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["stressch_2d.npz", "stressch_2d_ps.npz", "stressch_3d.npz"])
+def test_stress_cahn_hilliard_vs_reference(name, oracle):
+    """StressCahnHilliardElement on the device: the element constants bit for
+    bit against the reference; K and the c and u rows of the residual bit for
+    bit against the oracle on the device's g-weights and bond geometry (the mu
+    rows carry the device logarithm: within 1e-14 of their scale); every
+    dof's rows within 1e-11 of that dof's scale against the reference."""
+    z = load(name)
+    print("fixture", name, flush=True)
+    dim = int(z["dim"])
+    nd = dim + 2
+    ctx, cloud, delta = device_mesh(z)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(nd)
+    ref = z["consts"]
+    p = pxg.StressChParams(70e9, 0.3, float(ref[6]), float(ref[9]), float(ref[10]), float(ref[8]),
+                           float(ref[7]), int(str(z["state"]) == "plane_stress"))
+    ctx.stressch_setup(p)
+    kc = ctx.stressch_constants()
+    assert np.array_equal(kc.view(np.uint64), ref.view(np.uint64))
+    dt = float(z["dt"])
+    k, r = ctx.assemble_stressch(dt, z["u"], z["uold"])
+    g = cloud.grid
+    m = oracle.Mesh(oracle.make_grid(g.dim, g.nx, g.ny, g.nz, hi=g.hi))
+    ops, _ = ctx.operators()
+    rp, cols = ctx.csr()
+    lap, rev, isg = ctx.ch_geometry()
+    ok, orr = oracle.assemble_stressch(m, ops, lap, rev, isg, kc, dt, rp, cols, z["u"], z["uold"])
+    assert np.array_equal(k.view(np.uint64), ok.view(np.uint64))
+    mu_rows = np.zeros(r.size, bool)
+    mu_rows[1::nd] = True
+    assert np.array_equal(r[~mu_rows].view(np.uint64), orr[~mu_rows].view(np.uint64))
+    dmu = np.abs(r[mu_rows] - orr[mu_rows]).max()
+    print("mu rows: max |device - oracle| %.3e of scale %.3e" % (dmu, np.abs(orr[mu_rows]).max()))
+    assert dmu <= 1e-14 * np.abs(orr[mu_rows]).max()
+    row = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    for a in range(nd):
+        sel = (row % nd) == a
+        ks = np.abs(z["K"][sel]).max()
+        rs = np.abs(z["R"][a::nd]).max()
+        ek = np.abs(k[sel] - z["K"][sel]).max() / ks
+        er = np.abs(r[a::nd] - z["R"][a::nd]).max() / rs
+        print("dof %d: K rel %.2e, R rel %.2e" % (a, ek, er))
+        assert ek <= 1e-11 and er <= 1e-11
+    # device pointers give the same bits
+    import torch
+    ud = torch.from_numpy(np.ascontiguousarray(z["u"])).cuda()
+    uod = torch.from_numpy(np.ascontiguousarray(z["uold"])).cuda()
+    kd = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rd = torch.empty(nd * cloud.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ctx.assemble_stressch_dev(dt, ud.data_ptr(), uod.data_ptr(), kd.data_ptr(), rd.data_ptr())
+    ctx.sync()
+    assert np.array_equal(kd.cpu().numpy().view(np.uint64), k.view(np.uint64))
+    assert np.array_equal(rd.cpu().numpy().view(np.uint64), r.view(np.uint64))
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["fsch_2d.npz", "fsch_3d.npz"])
+def test_fracture_stress_cahn_hilliard_vs_reference(name, oracle):
+    """FracStressCahnHilliardElement on the device over the initialising and
+    a committing step: constants, health and velocity bit for bit against
+    the reference; K and the c and u rows of the residual bit for bit against
+    the oracle on the device's g-weights (mu rows within 1e-14 of their
+    scale: device logarithm); every dof's rows within 1e-11 of that dof's
+    scale against the reference."""
+    z = load(name)
+    print("fixture", name, flush=True)
+    dim = int(z["dim"])
+    nd = dim + 2
+    nx, ny, nz = (int(v) for v in z["grid"])
+    g = perix_amd.StructuredGrid(dim, nx, ny, nz, hi=tuple(float(v) for v in z["hi"]))
+    cloud = perix_amd.build_structured_cloud(g)
+    delta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+    ctx.set_cracks(segs=[tuple(float(v) for v in z["crack"])])
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(nd)
+    kc = z["consts"]
+    ps = dim == 2 and str(z["state"]) == "plane_stress"
+    E, nu = float(z["E"]), float(z["nu"])
+    sc = problems.critical_stretch(float(z["G0"]), E, delta, dim, ps)
+    base = pxg.StressChParams(E, nu, float(kc[6]), float(kc[9]), float(kc[10]), float(kc[8]), float(kc[7]), int(ps))
+    ctx.fsch_setup(pxg.FschParams(base, float(kc[11]), sc, float(kc[12]), int(z["tension_only"]), 1), z["uold1"])
+    assert np.array_equal(ctx.fsch_constants().view(np.uint64), kc.view(np.uint64))
+    h, v = ctx.fsch_state()
+    assert np.array_equal(h.astype(float), z["health1"]) and np.array_equal(v, z["vel1"])
+    m = oracle.Mesh(oracle.make_grid(dim, nx, ny, nz, hi=tuple(float(v) for v in z["hi"])))
+    ops, _ = ctx.operators()
+    rp, cols = ctx.csr()
+    lapc = oracle.fsch_lapc(m, ops)
+    _, rev, isg = ctx.ch_geometry()
+    row = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    dts = {1: float(z["dt1"]), 2: float(z["dt2"])}
+    for step in (1, 2):
+        if step == 2:
+            ctx.fsch_commit(z["uold2"], dts[1])
+            h, v = ctx.fsch_state()
+            assert np.array_equal(v.view(np.uint64), z["vel2"].view(np.uint64))
+            assert np.array_equal(h.astype(float), z["health2"])
+            print("%s: broken bonds %d -> %d" % (name, int((z["health1"] == 0).sum()), int((h == 0).sum())))
+        u, uold = z["u%d" % step], z["uold%d" % step]
+        k, r = ctx.assemble_fsch(dts[step], u, uold)
+        ok, orr = oracle.assemble_fsch(m, ops, lapc, rev, isg, kc[:13], dts[step], v, h.astype(float), rp, cols,
+                                       u, uold)
+        assert np.array_equal(k.view(np.uint64), ok.view(np.uint64)), "K, step %d" % step
+        mu_rows = np.zeros(r.size, bool)
+        mu_rows[1::nd] = True
+        assert np.array_equal(r[~mu_rows].view(np.uint64), orr[~mu_rows].view(np.uint64)), "R, step %d" % step
+        assert np.abs(r[mu_rows] - orr[mu_rows]).max() <= 1e-14 * np.abs(orr[mu_rows]).max()
+        K, R = z["K%d" % step], z["R%d" % step]
+        for a in range(nd):
+            sel = (row % nd) == a
+            ek = np.abs(k[sel] - K[sel]).max() / np.abs(K[sel]).max()
+            er = np.abs(r[a::nd] - R[a::nd]).max() / np.abs(R[a::nd]).max()
+            print("step %d dof %d: K rel %.2e, R rel %.2e" % (step, a, ek, er))
+            assert ek <= 1e-11 and er <= 1e-11
+    ctx.close()
