@@ -332,14 +332,16 @@ __device__ int row_nodes(int i, const int *off, const int *nbr, const int *mirro
 }
 
 __global__ void k_csr_count(const int *off, const int *nbr, const int *mirror, int n, int ndof,
-                            int *rowcount, int *toolong)
+                            int *rowcount, int *flags)
 {
+    // flags[0]: a family exceeds the row limit; flags[1]: most blocks in a row
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (off[i + 1] - off[i] > kMaxRow - 2) { atomicExch(toolong, 1); return; }
+    if (off[i + 1] - off[i] > kMaxRow - 2) { atomicExch(flags, 1); return; }
     int tmp[kMaxRow];
     const int u = row_nodes(i, off, nbr, mirror, n, tmp);
     for (int a = 0; a < ndof; a++) rowcount[(size_t)i * ndof + a] = u * ndof;
+    atomicMax(flags + 1, u);
 }
 
 __global__ void k_csr_fill(const int *off, const int *nbr, const int *mirror, int n, int ndof,
@@ -548,14 +550,15 @@ extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
     const long long rows = (long long)n * ndof;
     dbuf<int> rc, flag;
     PXG_HIP(rc.alloc(rows));
-    PXG_HIP(flag.alloc(1));
-    PXG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    PXG_HIP(flag.alloc(2));
+    PXG_HIP(hipMemsetAsync(flag.p, 0, 2 * sizeof(int), st));
     hipLaunchKernelGGL(k_csr_count, dim3(grid_of(n, 64)), dim3(64), 0, st, ctx->off.p, ctx->nbr.p,
                        ctx->mirror.p, n, ndof, rc.p, flag.p);
-    int tl = 0;
-    PXG_HIP(hipMemcpyAsync(&tl, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    int fl[2] = {0, 0};
+    PXG_HIP(hipMemcpyAsync(fl, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
-    PXG_CHECK(!tl, PXG_ERR_ARG, "pxg_build_csr: a family exceeds the 510-bond row limit");
+    PXG_CHECK(!fl[0], PXG_ERR_ARG, "pxg_build_csr: a family exceeds the 510-bond row limit");
+    ctx->csr_maxblk = fl[1];
     PXG_HIP(ctx->rowptr.alloc(rows + 1));
     long long nnz = 0;
     int r = scan_counts(st, rc.p, rows, ctx->rowptr.p, &nnz);

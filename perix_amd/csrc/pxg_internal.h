@@ -263,6 +263,7 @@ struct pxg_ctx {
     int ndegenerate = 0;
     // ---- implicit CSR ----
     int csr_ndof = 0;
+    int csr_maxblk = 0;             // most node blocks in one block row
     long long nnz = 0;
     pxg::dbuf<int> rowptr, cols;
     pxg::dbuf<int> slot;            // nb: CSR index of (row i*ndof, col j*ndof)

@@ -6,10 +6,11 @@
 // (StressCahnHilliardElement.cpp:139-467 through the assembler of
 // FormResidualAndJacobian.cpp:25-132).
 //
-// One thread per node: the node's (dim+2)x(dim+2) block row is accumulated in
-// the reference's order (nodal term first, then the bonds in family order),
-// so K comes out bit for bit; the residual's mu rows carry ln(c/(1-c)),
-// whose device logarithm can differ from the host's in the last bit.  The
+// One wavefront per node, lanes over its bonds (see k_sch): every sum of the
+// node's (dim+2)x(dim+2) diagonal block and residual is taken in the
+// reference's order (nodal term first, then the bonds in family order), so K
+// comes out bit for bit; the residual's mu rows carry ln(c/(1-c)), whose
+// device logarithm can differ from the host's in the last bit.  The
 // Picard mobility M(c) = D c (1-c) is evaluated from u at both ends of each
 // bond, as the reference's per-iteration cache holds it.
 //
@@ -26,10 +27,6 @@ using namespace pxg;
 namespace {
 
 unsigned grids(long long n, int b) { return (unsigned)((n + b - 1) / b); }
-
-struct SchConst {
-    double C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref;
-};
 
 __device__ __forceinline__ double sch_mobility(double c, double D)
 {
@@ -53,138 +50,6 @@ __device__ __forceinline__ void sch_reglog(double c, double &L, double &Lp)
     } else {
         L = log(c / (1.0 - c));
         Lp = 1.0 / (c * (1.0 - c));
-    }
-}
-
-template <int DIM>
-__global__ void __launch_bounds__(128) k_stressch(int n, int nop, const int *off, const int *nbr,
-                                                  const double *ops, const int *rev, const double *lap,
-                                                  const unsigned char *isghost, const double *vol, SchConst k,
-                                                  double dt, const int *rowptr, const int *slot,
-                                                  const int *diag, const int *mslot, const double *u,
-                                                  const double *uold, double *kvals, double *rhs)
-{
-    constexpr int ND = DIM + 2;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t ri = (size_t)i * ND;
-    const int rb = rowptr[ri];
-    int rowoff[ND];
-#pragma unroll
-    for (int a = 0; a < ND; a++) rowoff[a] = rowptr[ri + a] - rb;
-    if (mslot[i] >= 0) {
-        // the ghost's mirror block is in the pattern but receives no term
-        const int m0 = mslot[i] - rb;
-#pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-            for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + m0 + b] = 0.0;
-    }
-    double ui[ND];
-#pragma unroll
-    for (int a = 0; a < ND; a++) ui[a] = u[ri + a];
-    // nodal term (StressCahnHilliardElement.cpp:449-466)
-    double r[ND], kd[ND][ND];
-    {
-        const double c = ui[0], mu = ui[1], cold = uold[ri];
-        double L, Lp;
-        sch_reglog(c, L, Lp);
-        const double fp = L + k.chi * (1.0 - 2.0 * c);
-        const double fpp = Lp - 2.0 * k.chi;
-#pragma unroll
-        for (int a = 0; a < ND; a++) {
-            r[a] = 0.0;
-#pragma unroll
-            for (int b = 0; b < ND; b++) kd[a][b] = 0.0 + (-0.0);
-        }
-        r[0] = 0.0 + (c - cold) / dt;
-        r[1] = 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
-        kd[0][0] = 0.0 + (-(1.0 / dt));
-        kd[1][0] = 0.0 + (-(-fpp + k.Omega * k.Ch));
-        kd[1][1] = 0.0 + (-1.0);
-    }
-    const double Mi = sch_mobility(ui[0], k.D), Vi = vol[i];
-    const double ocx = k.Omega * k.Kh;
-    const int d0 = diag[i] - rb;
-    for (int t = off[i]; t < off[i + 1]; t++) {
-        const int j = nbr[t] - 1;
-        const size_t rj = (size_t)j * ND;
-        const double *op = ops + (size_t)t * nop;
-        const bool ghostJ = isghost[j] != 0;
-        const double g2ij = lap[t];
-        double coeff_c = 0.0;
-        if (!ghostJ) {
-            const double Mj = sch_mobility(u[rj], k.D), Vj = vol[j];
-            const int rv = rev[t];
-            double wij;
-            if (rv >= 0 && Vi > 0.0) wij = 0.5 * (Mi * g2ij + (Vj / Vi) * Mj * lap[rv]);
-            else wij = Mi * g2ij;
-            coeff_c = -wij;
-        }
-        const double kl = ghostJ ? 0.0 : (k.kappa * g2ij);
-        double d[ND];
-#pragma unroll
-        for (int a = 0; a < ND; a++) d[a] = u[rj + a] - ui[a];
-        // P = the bond's dR/du_j (StressCahnHilliardElement.cpp:334-427):
-        // K_II = -P, K_IJ = P
-        double P[ND][ND], R[ND];
-#pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-            for (int b = 0; b < ND; b++) P[a][b] = 0.0;
-        if constexpr (DIM == 3) {
-            const double Gx = op[0], Gy = op[1], Gz = op[2];
-            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
-            const double aa = k.C12 + k.C66;
-            const double cux = ocx * Gx, cuy = ocx * Gy, cuz = ocx * Gz;
-            const double p11 = k.C11 * Gxx + k.C66 * (Gyy + Gzz);
-            const double p22 = k.C11 * Gyy + k.C66 * (Gxx + Gzz);
-            const double p33 = k.C11 * Gzz + k.C66 * (Gxx + Gyy);
-            const double p12 = aa * Gxy, p13 = aa * Gxz, p23 = aa * Gyz;
-            const double cx = -k.A * Gx, cy = -k.A * Gy, cz = -k.A * Gz;
-            R[0] = coeff_c * d[1];
-            R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
-            R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
-            R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
-            R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
-            P[0][1] = coeff_c;
-            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
-            P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
-            P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
-            P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
-        } else {
-            const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
-            const double cux = ocx * Gx, cuy = ocx * Gy;
-            const double p11 = k.C11 * Gxx + k.C66 * Gyy;
-            const double p22 = k.C66 * Gxx + k.C11 * Gyy;
-            const double pxy = (k.C12 + k.C66) * Gxy;
-            const double cx = -k.A * Gx, cy = -k.A * Gy;
-            R[0] = coeff_c * d[1];
-            R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
-            R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
-            R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
-            P[0][1] = coeff_c;
-            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
-            P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
-            P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
-        }
-        // scatter (FormResidualAndJacobian.cpp:119-128)
-        const int s0 = slot[t] - rb;
-#pragma unroll
-        for (int a = 0; a < ND; a++) {
-            r[a] += R[a];
-#pragma unroll
-            for (int b = 0; b < ND; b++) {
-                kd[a][b] += -(-P[a][b]);
-                kvals[rb + rowoff[a] + s0 + b] = 0.0 + (-P[a][b]);
-            }
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < ND; a++) {
-        rhs[ri + a] = r[a];
-#pragma unroll
-        for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + d0 + b] = kd[a][b];
     }
 }
 
@@ -266,141 +131,378 @@ struct FschConst {
     double C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho, resid;
 };
 
-// assembly (FracStressCahnHilliardElement.cpp:411-561 through
-// FormResidualAndJacobian.cpp:25-132), one thread per node as k_stressch
+// everything an assembly launch reads and writes
+struct SchArgs {
+    int n, nop;
+    const int *off, *nbr;
+    const double *ops;
+    const int *rev;
+    const double *lap;               // lambda-corrected (stress CH) or species Laplacian (fracture)
+    const unsigned char *isghost;
+    const double *vol;
+    FschConst k;
+    double dt;
+    const double *vel;               // fracture variant: committed velocity
+    const unsigned char *health;     // fracture variant: bond health
+    const int *rowptr, *slot, *diag, *mslot;
+    const double *u, *uold;
+    double *kvals, *rhs;
+};
+
+// The structural nonzeros of a bond's dR/du_j block P, in staging order:
+// (0,1), then for rows a = 1..ND-1 the columns 0, 2, 3, ..., ND-1.  They are
+// the bond terms of the diagonal block; its other entries hold only the
+// nodal term.
+template <int ND>
+__device__ __forceinline__ constexpr int sch_nk() { return 1 + (ND - 1) * (ND - 1); }
+
+template <int ND>
+__device__ __forceinline__ void sch_entry(int e, int &a, int &b)
+{
+    if (e == 0) { a = 0; b = 1; return; }
+    const int q = (e - 1) % (ND - 1);
+    a = 1 + (e - 1) / (ND - 1);
+    b = (q == 0) ? 0 : q + 1;
+}
+
+template <int ND>
+__device__ __forceinline__ bool sch_staged(int a, int b) { return (a == 0 && b == 1) || (a >= 1 && b != 1); }
+
+template <int ND>
+__device__ __forceinline__ int sch_index(int a, int b) { return (a == 0) ? 0 : 1 + (a - 1) * (ND - 1) + (b == 0 ? 0 : b - 1); }
+
+// Node-level quantities of node i (the row owner) used by its bonds.
 template <int DIM>
-__global__ void __launch_bounds__(128) k_fsch(int n, int nop, const int *off, const int *nbr,
-                                              const double *ops, const int *rev, const double *lapc,
-                                              const unsigned char *isghost, const double *vol, FschConst k,
-                                              double dt, const double *vel, const unsigned char *health,
-                                              const int *rowptr, const int *slot, const int *diag,
-                                              const int *mslot, const double *u, const double *uold,
-                                              double *kvals, double *rhs)
+struct SchNode {
+    double ui[DIM + 2];
+    double Mi, Vi;
+};
+
+// One bond t of node i: P (StressCahnHilliardElement.cpp:282-428,
+// FracStressCahnHilliardElement.cpp:400-526) and the residual terms R.  The
+// fracture variant gates the mechanics and sigma_h coupling with g (1 or the
+// residual stiffness); with g = 1 every product rounds as the reference's
+// ungated forms.
+template <int DIM, bool FRAC>
+__device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<DIM> &nd, int t,
+                                         double (&P)[DIM + 2][DIM + 2], double (&R)[DIM + 2])
 {
     constexpr int ND = DIM + 2;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t ri = (size_t)i * ND;
-    const int rb = rowptr[ri];
+    const FschConst &k = A.k;
+    const int j = A.nbr[t] - 1;
+    const size_t rj = (size_t)j * ND;
+    const double *op = A.ops + (size_t)t * A.nop;
+    const bool ghostJ = A.isghost[j] != 0;
+    double g = 1.0, coeff_c = 0.0, kl = 0.0;
+    if constexpr (FRAC) {
+        g = A.health[t] ? 1.0 : k.resid;
+        double wf = 0.0, g2c = 0.0;
+        if (!ghostJ) {
+            g2c = A.lap[t];
+            const int rv = A.rev[t];
+            if (rv >= 0) wf = 0.5 * (nd.Mi * g2c + (A.vol[j] / nd.Vi) * sch_mobility(A.u[rj], k.D) * A.lap[rv]);
+            else wf = nd.Mi * g2c;
+        }
+        coeff_c = -wf;
+        kl = k.kappa * g2c;
+    } else {
+        const double g2ij = A.lap[t];
+        if (!ghostJ) {
+            const int rv = A.rev[t];
+            double wij;
+            if (rv >= 0 && nd.Vi > 0.0)
+                wij = 0.5 * (nd.Mi * g2ij + (A.vol[j] / nd.Vi) * sch_mobility(A.u[rj], k.D) * A.lap[rv]);
+            else wij = nd.Mi * g2ij;
+            coeff_c = -wij;
+            kl = k.kappa * g2ij;
+        }
+    }
+    double d[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) d[a] = A.u[rj + a] - nd.ui[a];
+#pragma unroll
+    for (int a = 0; a < ND; a++)
+#pragma unroll
+        for (int b = 0; b < ND; b++) P[a][b] = 0.0;
+    const double go = g * k.Omega, gA = -g * k.A;
+    if constexpr (DIM == 3) {
+        const double Gx = op[0], Gy = op[1], Gz = op[2];
+        const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+        const double ga = g * (k.C12 + k.C66);
+        const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy, cuz = go * k.Kh * Gz;
+        const double p11 = g * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
+        const double p22 = g * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
+        const double p33 = g * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
+        const double p12 = ga * Gxy, p13 = ga * Gxz, p23 = ga * Gyz;
+        const double cx = gA * Gx, cy = gA * Gy, cz = gA * Gz;
+        R[0] = coeff_c * d[1];
+        R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
+        R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+        R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+        R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+        P[0][1] = coeff_c;
+        P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
+        P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
+        P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
+        P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
+    } else {
+        const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+        const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy;
+        const double p11 = g * (k.C11 * Gxx + k.C66 * Gyy);
+        const double p22 = g * (k.C66 * Gxx + k.C11 * Gyy);
+        const double pxy = g * (k.C12 + k.C66) * Gxy;
+        const double cx = gA * Gx, cy = gA * Gy;
+        R[0] = coeff_c * d[1];
+        R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+        R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+        R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+        P[0][1] = coeff_c;
+        P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
+        P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
+        P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+    }
+}
+
+// The nodal term (StressCahnHilliardElement.cpp:449-466,
+// FracStressCahnHilliardElement.cpp:528-561) of sum e: staged entry e < NK
+// of the diagonal block, residual component e - NK otherwise, as the
+// assembler leaves it before the bonds: 0 + (-K_node) and 0 + R_node.
+template <int DIM, bool FRAC>
+__device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<DIM> &nd, size_t ri, int ic, int e)
+{
+    constexpr int ND = DIM + 2;
+    constexpr int NK = sch_nk<ND>();
+    const FschConst &k = A.k;
+    const double dt = A.dt;
+    const double c = nd.ui[0], mu = nd.ui[1], cold = A.uold[ri];
+    double fp, fpp;
+    if constexpr (FRAC) {
+        const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
+        fp = log(cc / (1.0 - cc)) + k.chi * (1.0 - 2.0 * cc);
+        fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * k.chi;
+    } else {
+        double L, Lp;
+        sch_reglog(c, L, Lp);
+        fp = L + k.chi * (1.0 - 2.0 * c);
+        fpp = Lp - 2.0 * k.chi;
+    }
+    const bool inertia = FRAC && k.rho > 0.0;
+    const double c0 = inertia ? k.rho / (dt * dt) : 0.0;
+    if (e < NK) {
+        int a, b;
+        sch_entry<ND>(e, a, b);
+        double kn = 0.0;
+        if (a == 1 && b == 0) kn = -fpp + k.Omega * k.Ch;
+        else if (inertia && a >= 2 && a == b) kn = 0.0 + -c0;
+        return 0.0 + (-kn);
+    }
+    const int a = e - NK;
+    if (a == 0) return 0.0 + (c - cold) / dt;
+    if (a == 1) return 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
+    if (inertia) {
+        const double v = A.vel[(size_t)ic * DIM + a - 2];
+        double ua = 0.0;                  // nd.ui[a] without a dynamic index
+#pragma unroll
+        for (int q = 2; q < ND; q++)
+            if (q == a) ua = nd.ui[q];
+        return 0.0 + (0.0 + -c0 * (ua - A.uold[ri + a] - v * dt));
+    }
+    return 0.0;
+}
+
+// the diagonal-block entries without bond terms: 0 + (-K_node)
+template <int ND>
+__device__ __forceinline__ double sch_nodal_only(int a, int b, double dt)
+{
+    const double kn = (a == 0 && b == 0) ? 1.0 / dt : (a == 1 && b == 1) ? 1.0 : 0.0;
+    return 0.0 + (-kn);
+}
+
+// ---- assembly: one wavefront per node, lanes over its bonds ---------------
+//
+// k_sch_rows (rows of at most MAXB node blocks): lane l computes bonds
+// l, l + 64, ... of the node and stages P's structural nonzeros and the
+// residual terms in LDS, with a map from the row's block positions to bonds.
+// One lane per diagonal-block entry / residual component then adds the
+// staged terms in family order onto the nodal term -- the reference's order,
+// so every value comes out bit for bit -- and finally the wave writes the
+// node's dim+2 CSR rows front to back, 64 consecutive values per store, the
+// K_IJ blocks as 0 + (-P) (FormResidualAndJacobian.cpp:119-128).
+//
+// k_sch (any row length): the same sums, in chunks of 64 bonds, with each
+// lane writing its bond's K_IJ block straight into the rows.
+constexpr int kSchWaves = 4;   // nodes per block of k_sch
+
+template <int DIM, bool FRAC, int MAXB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
+{
+    constexpr int ND = DIM + 2;
+    constexpr int NK = sch_nk<ND>();
+    constexpr int NS = NK + ND;
+    __shared__ double s_v[NW][NS][MAXB + 1];
+    __shared__ double s_d[NW][ND * ND];
+    __shared__ int s_inv[NW][MAXB];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * NW + w;
+    const bool active = i < A.n;
+    const int ic = active ? i : 0;   // reads of an idle wave stay in bounds
+    const size_t ri = (size_t)ic * ND;
+    const int r0 = A.rowptr[ri];
+    const int rowlen = A.rowptr[ri + 1] - r0;
     int rowoff[ND];
 #pragma unroll
-    for (int a = 0; a < ND; a++) rowoff[a] = rowptr[ri + a] - rb;
-    if (mslot[i] >= 0) {
-        const int m0 = mslot[i] - rb;
+    for (int a = 0; a < ND; a++) rowoff[a] = A.rowptr[ri + a] - r0;
+    SchNode<DIM> nd;
 #pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-            for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + m0 + b] = 0.0;
-    }
-    double ui[ND];
-#pragma unroll
-    for (int a = 0; a < ND; a++) ui[a] = u[ri + a];
-    double r[ND], kd[ND][ND];
-    {
-        // nodal term with the clamped logarithm and optional inertia
-        const double c = ui[0], mu = ui[1], cold = uold[ri];
-        const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
-        const double fp = log(cc / (1.0 - cc)) + k.chi * (1.0 - 2.0 * cc);
-        const double fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * k.chi;
-#pragma unroll
-        for (int a = 0; a < ND; a++) {
-            r[a] = 0.0;
-#pragma unroll
-            for (int b = 0; b < ND; b++) kd[a][b] = 0.0 + (-0.0);
-        }
-        r[0] = 0.0 + (c - cold) / dt;
-        r[1] = 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
-        kd[0][0] = 0.0 + (-(1.0 / dt));
-        kd[1][0] = 0.0 + (-(-fpp + k.Omega * k.Ch));
-        kd[1][1] = 0.0 + (-1.0);
-        if (k.rho > 0.0) {
-            const double c0 = k.rho / (dt * dt);
-#pragma unroll
-            for (int a = 0; a < DIM; a++) {
-                const double v = vel[(size_t)i * DIM + a];
-                r[2 + a] = 0.0 + (0.0 + -c0 * (ui[2 + a] - uold[ri + 2 + a] - v * dt));
-                kd[2 + a][2 + a] = 0.0 + (-(0.0 + -c0));
-            }
-        }
-    }
-    const double Mi = sch_mobility(ui[0], k.D), Vi = vol[i];
-    const int d0 = diag[i] - rb;
-    for (int t = off[i]; t < off[i + 1]; t++) {
-        const int j = nbr[t] - 1;
-        const size_t rj = (size_t)j * ND;
-        const double *op = ops + (size_t)t * nop;
-        const double g = health[t] ? 1.0 : k.resid;
-        double wf = 0.0, g2c = 0.0;
-        if (!isghost[j]) {
-            g2c = lapc[t];
-            const int rv = rev[t];
-            if (rv >= 0) wf = 0.5 * (Mi * g2c + (vol[j] / Vi) * sch_mobility(u[rj], k.D) * lapc[rv]);
-            else wf = Mi * g2c;
-        }
-        const double coeff_c = -wf, kl = k.kappa * g2c;
-        double d[ND];
-#pragma unroll
-        for (int a = 0; a < ND; a++) d[a] = u[rj + a] - ui[a];
+    for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
+    nd.Mi = sch_mobility(nd.ui[0], A.k.D);
+    nd.Vi = A.vol[ic];
+    const int t0 = active ? A.off[i] : 0, deg = active ? A.off[i + 1] - t0 : 0;
+    for (int tl = lane; tl < deg; tl += 64) {
+        const int t = t0 + tl;
         double P[ND][ND], R[ND];
+        sch_bond<DIM, FRAC>(A, nd, t, P, R);
 #pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-            for (int b = 0; b < ND; b++) P[a][b] = 0.0;
-        const double go = g * k.Omega;
-        if constexpr (DIM == 3) {
-            const double Gx = op[0], Gy = op[1], Gz = op[2];
-            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
-            const double ga = g * (k.C12 + k.C66), gA = -g * k.A;
-            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy, cuz = go * k.Kh * Gz;
-            const double p11 = g * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
-            const double p22 = g * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
-            const double p33 = g * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
-            const double p12 = ga * Gxy, p13 = ga * Gxz, p23 = ga * Gyz;
-            const double cx = gA * Gx, cy = gA * Gy, cz = gA * Gz;
-            R[0] = coeff_c * d[1];
-            R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
-            R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
-            R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
-            R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
-            P[0][1] = coeff_c;
-            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
-            P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
-            P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
-            P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
-        } else {
-            const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
-            const double gA = -g * k.A;
-            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy;
-            const double p11 = g * (k.C11 * Gxx + k.C66 * Gyy);
-            const double p22 = g * (k.C66 * Gxx + k.C11 * Gyy);
-            const double pxy = g * (k.C12 + k.C66) * Gxy;
-            const double cx = gA * Gx, cy = gA * Gy;
-            R[0] = coeff_c * d[1];
-            R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
-            R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
-            R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
-            P[0][1] = coeff_c;
-            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
-            P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
-            P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+        for (int e = 0; e < NK; e++) {
+            int a, b;
+            sch_entry<ND>(e, a, b);
+            s_v[w][e][tl] = P[a][b];
         }
-        const int s0 = slot[t] - rb;
 #pragma unroll
-        for (int a = 0; a < ND; a++) {
-            r[a] += R[a];
-#pragma unroll
-            for (int b = 0; b < ND; b++) {
-                kd[a][b] += -(-P[a][b]);
-                kvals[rb + rowoff[a] + s0 + b] = 0.0 + (-P[a][b]);
-            }
-        }
+        for (int a = 0; a < ND; a++) s_v[w][NK + a][tl] = R[a];
+        s_inv[w][(A.slot[t] - r0) / ND] = tl;
     }
+    if (active && lane == 0) {
+        s_inv[w][(A.diag[i] - r0) / ND] = -1;
+        if (A.mslot[i] >= 0) s_inv[w][(A.mslot[i] - r0) / ND] = -2;
+    }
+    __syncthreads();
+    if (lane < NS) {
+        double acc = sch_nodal<DIM, FRAC>(A, nd, ri, ic, lane);
+#pragma unroll 8
+        for (int q = 0; q < deg; q++) acc += s_v[w][lane][q];   // in order; the loads pipeline
+        if (lane < NK) {
+            int a, b;
+            sch_entry<ND>(lane, a, b);
+            s_d[w][a * ND + b] = acc;
+        } else if (active) {
+            A.rhs[ri + lane - NK] = acc;
+        }
+    } else if (lane < NS + ND * ND) {
+        const int q = lane - NS, a = q / ND, b = q % ND;
+        if (!sch_staged<ND>(a, b)) s_d[w][q] = sch_nodal_only<ND>(a, b, A.dt);
+    }
+    __syncthreads();
+    if (!active) return;
 #pragma unroll
     for (int a = 0; a < ND; a++) {
-        rhs[ri + a] = r[a];
-#pragma unroll
-        for (int b = 0; b < ND; b++) kvals[rb + rowoff[a] + d0 + b] = kd[a][b];
+        double *row = A.kvals + r0 + rowoff[a];
+        for (int q = lane; q < rowlen; q += 64) {
+            const int kb = q / ND, b = q - kb * ND;
+            const int src = s_inv[w][kb];
+            double v = 0.0;                               // mirror block, structural zeros
+            if (src >= 0) {
+                if (sch_staged<ND>(a, b)) v = 0.0 + (-s_v[w][sch_index<ND>(a, b)][src]);
+            } else if (src == -1) {
+                v = s_d[w][a * ND + b];
+            }
+            row[q] = v;
+        }
     }
+}
+
+template <int DIM, bool FRAC>
+__global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
+{
+    constexpr int ND = DIM + 2;
+    constexpr int NK = sch_nk<ND>();
+    constexpr int NS = NK + ND;
+    __shared__ double s_v[kSchWaves][NS][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * kSchWaves;
+    const int i = i0 + w;
+    const bool active = i < A.n;
+    const int ic = active ? i : 0;   // reads of an idle wave stay in bounds
+    // chunks of 64 bonds: the block runs the largest count of its nodes
+    int nch = 0;
+#pragma unroll
+    for (int q = 0; q < kSchWaves; q++)
+        if (i0 + q < A.n) nch = max(nch, (A.off[i0 + q + 1] - A.off[i0 + q] + 63) >> 6);
+    const size_t ri = (size_t)ic * ND;
+    const int rb = A.rowptr[ri];
+    int rowoff[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) rowoff[a] = A.rowptr[ri + a] - rb;
+    SchNode<DIM> nd;
+#pragma unroll
+    for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
+    nd.Mi = sch_mobility(nd.ui[0], A.k.D);
+    nd.Vi = A.vol[ic];
+    const int t0 = active ? A.off[i] : 0, t1 = active ? A.off[i + 1] : 0;
+    if (active && lane < ND * ND && A.mslot[i] >= 0) {
+        // the ghost's mirror block is in the pattern but receives no term
+        A.kvals[A.mslot[i] + rowoff[lane / ND] + lane % ND] = 0.0;
+    }
+    double acc = (lane < NS) ? sch_nodal<DIM, FRAC>(A, nd, ri, ic, lane) : 0.0;
+    for (int ch = 0; ch < nch; ch++) {
+        const int t = t0 + ch * 64 + lane;
+        if (t < t1) {
+            double P[ND][ND], R[ND];
+            sch_bond<DIM, FRAC>(A, nd, t, P, R);
+            const int s0 = A.slot[t];
+#pragma unroll
+            for (int a = 0; a < ND; a++)
+#pragma unroll
+                for (int b = 0; b < ND; b++) A.kvals[s0 + rowoff[a] + b] = 0.0 + (-P[a][b]);
+#pragma unroll
+            for (int e = 0; e < NK; e++) {
+                int a, b;
+                sch_entry<ND>(e, a, b);
+                s_v[w][e][lane] = P[a][b];
+            }
+#pragma unroll
+            for (int a = 0; a < ND; a++) s_v[w][NK + a][lane] = R[a];
+        }
+        __syncthreads();
+        if (lane < NS) {
+            const int m = min(64, t1 - (t0 + ch * 64));
+#pragma unroll 8
+            for (int q = 0; q < m; q++) acc += s_v[w][lane][q];   // in order; the loads pipeline
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const int d0 = A.diag[i];
+    if (lane < NK) {
+        int a, b;
+        sch_entry<ND>(lane, a, b);
+        A.kvals[d0 + rowoff[a] + b] = acc;
+    } else if (lane < NS) {
+        A.rhs[ri + lane - NK] = acc;
+    }
+    if (lane < ND * ND) {
+        const int a = lane / ND, b = lane % ND;
+        if (!sch_staged<ND>(a, b)) A.kvals[d0 + rowoff[a] + b] = sch_nodal_only<ND>(a, b, A.dt);
+    }
+}
+
+template <int DIM, bool FRAC>
+int launch_sch(hipStream_t st, int maxblk, const SchArgs &A)
+{
+    const int n = A.n;
+    if constexpr (DIM == 2) {
+        if (maxblk <= 64) {
+            hipLaunchKernelGGL((k_sch_rows<DIM, FRAC, 64, 4>), dim3(grids(n, 4)), dim3(256), 0, st, A);
+            PXG_HIP(hipGetLastError());
+            return PXG_OK;
+        }
+    }
+    if (maxblk <= 128)
+        hipLaunchKernelGGL((k_sch_rows<DIM, FRAC, 128, 2>), dim3(grids(n, 2)), dim3(128), 0, st, A);
+    else
+        hipLaunchKernelGGL((k_sch<DIM, FRAC>), dim3(grids(n, kSchWaves)), dim3(64 * kSchWaves), 0, st, A);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
 }
 
 } // namespace
@@ -470,20 +572,13 @@ extern "C" int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_stressch: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
     const double *q = ctx->sch.k;
-    const SchConst k{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10]};
-    const unsigned g = grids(ctx->n, 128);
-    if (ctx->dim == 3)
-        hipLaunchKernelGGL(k_stressch<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
-                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p, ctx->vols.p, k, dt,
-                           ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
-                           rhs_dev);
-    else
-        hipLaunchKernelGGL(k_stressch<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
-                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p, ctx->vols.p, k, dt,
-                           ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
-                           rhs_dev);
-    PXG_HIP(hipGetLastError());
-    return PXG_OK;
+    const SchArgs A{ctx->n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p,
+                    ctx->vols.p,
+                    FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], 0.0, 1.0}, dt,
+                    nullptr, nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
+                    kvals_dev, rhs_dev};
+    return ctx->dim == 3 ? launch_sch<3, false>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, false>(ctx->stream, ctx->csr_maxblk, A);
 }
 
 extern "C" int pxg_assemble_stressch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
@@ -609,20 +704,13 @@ extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_de
     PXG_HIP(hipSetDevice(ctx->device));
     const FracSchState &F = ctx->fsch;
     const double *q = F.k;
-    const FschConst k{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12]};
-    const unsigned g = grids(ctx->n, 128);
-    if (ctx->dim == 3)
-        hipLaunchKernelGGL(k_fsch<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
-                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, F.lapc.p, ctx->isghost.p, ctx->vols.p, k, dt,
-                           F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev,
-                           uold_dev, kvals_dev, rhs_dev);
-    else
-        hipLaunchKernelGGL(k_fsch<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
-                           ctx->nbr.p, ctx->ops.p, ctx->rev.p, F.lapc.p, ctx->isghost.p, ctx->vols.p, k, dt,
-                           F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev,
-                           uold_dev, kvals_dev, rhs_dev);
-    PXG_HIP(hipGetLastError());
-    return PXG_OK;
+    const SchArgs A{ctx->n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rev.p, F.lapc.p, ctx->isghost.p,
+                    ctx->vols.p,
+                    FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12]}, dt,
+                    F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
+                    kvals_dev, rhs_dev};
+    return ctx->dim == 3 ? launch_sch<3, true>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, true>(ctx->stream, ctx->csr_maxblk, A);
 }
 
 extern "C" int pxg_assemble_fsch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
