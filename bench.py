@@ -679,9 +679,8 @@ def main():
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU slab "
-              "decomposition is a design (see the sharding notes), not a measured path. "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: perix_amd runs on one GPU only; the multi-GPU slab "
+              "decomposition is a design, not a measured path. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import perix_amd

@@ -2,7 +2,7 @@
  * pxo.h -- CPU oracle for the PeriX PDDO hot path (TEST INFRASTRUCTURE ONLY).
  *
  * This is a plain-C FP64 restatement of the reference algorithms, used by the
- * test-suite as the parity checker and by bench.py as the timed CPU baseline.
+ * test-suite as the parity reference and by bench.py as the timed CPU baseline.
  * It is never linked into, loaded by, or called from the product library
  * (perix_amd/csrc); the product path fails loudly when its HIP library is
  * missing instead of falling back here.

@@ -1,7 +1,7 @@
 """ctypes bindings of the C oracle (libpxo.so) -- TEST INFRASTRUCTURE ONLY.
 
-Imported by tests/, by __graft_entry__.smoke() as the checker and by
-bench.py's cpu_baseline leg.  The product (perix_amd) never imports it.
+Imported by tests/, by the smoke test (to compare one device step against)
+and by bench.py's cpu_baseline leg.  The product (perix_amd) never imports it.
 Every wrapper returns numpy arrays; node ids inside neighbour lists stay
 1-based as in the reference.
 """

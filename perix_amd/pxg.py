@@ -90,8 +90,7 @@ def load():
     if _LIB is None:
         p = lib_path()
         if not os.path.exists(p):
-            raise PxgError(-99, "libpxg.so not built (run __graft_entry__.build() or "
-                                "python perix_amd/build.py)")
+            raise PxgError(-99, "libpxg.so not built (run python perix_amd/build.py)")
         # PyTorch-ROCm ships its own HIP runtime under the same soname.  When
         # libpxg is loaded first, torch's later import brings a second runtime
         # into the process and that one finds no GPU; with torch loaded first,
