@@ -317,6 +317,30 @@ int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_dev,
 /* bond health (nb bytes, CSR order) and committed velocity (n*dim); either may be NULL */
 int pxg_fsch_get_state(pxg_ctx *ctx, unsigned char *health, double *vel);
 
+/* Output projections of the chemo-mechanical elements
+ * (StressCahnHilliardElement::computeNodalProjection,
+ * StressCahnHilliardElement.cpp:481-658; FracStressCahnHilliardElement::
+ * computeNodalProjection, FracStressCahnHilliardElement.cpp:47-110, 575-672)
+ * for the state u (n*(dim+2)).  stress_cahnhilliard lists strain .. gradC,
+ * frac_stress_cahnhilliard strain .. sigmaH, gradC, damage, DamageI (its
+ * current bond health).  out holds pxg_sch_components(which) values per node
+ * (tensors row-major 3x3, 2D zero-padded as the reference pads them).
+ * Host pointers, or device pointers with _dev. */
+#define PXG_SCH_STRAIN 0
+#define PXG_SCH_STRESS 1
+#define PXG_SCH_VONMISES 2
+#define PXG_SCH_SIGMAH 3
+#define PXG_SCH_FPRIME 4
+#define PXG_SCH_FDOUBLEPRIME 5
+#define PXG_SCH_GRADC 6
+#define PXG_SCH_DAMAGE 7
+#define PXG_SCH_DAMAGEI 8
+int pxg_sch_components(int which);
+int pxg_stressch_project(pxg_ctx *ctx, int which, const double *u, double *out);
+int pxg_stressch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev);
+int pxg_fsch_project(pxg_ctx *ctx, int which, const double *u, double *out);
+int pxg_fsch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev);
+
 /* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
  * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):
  * damage with u, then health-gated Navier force, accel = (L+b)/rho.

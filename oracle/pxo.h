@@ -106,6 +106,9 @@ void pxo_assemble_fsch(int n, int dim, int nop, const int *off, const int *nbr,
                        double dt, const double *vel, const double *health,
                        const int *rowptr, const int *cols, const double *u, const double *uold,
                        double *kvals, double *rhs);
+void pxo_sch_project(int n, int dim, int nop, const int *off, const int *nbr, const double *ops,
+                     const double *k, int frac, int plane_strain, const double *health,
+                     const double *seed, const double *u, int which, double *out);
 void pxo_assemble_ch(int n, const int *off, const int *nbr, const int *rev,
                      const double *lapbond, const unsigned char *isghost,
                      const double *vol, const double *mnode,

@@ -275,6 +275,27 @@ def assemble_fsch(m, ops, lapc, rev, isg, consts, dt, vel, health, rowptr, cols,
     return k, r
 
 
+# projections of the stress-coupled Cahn-Hilliard elements: name -> (index, components)
+SCH_PROJECTIONS = {"strain": (0, 9), "stress": (1, 9), "vonMises": (2, 1), "sigmaH": (3, 1),
+                   "fPrime": (4, 1), "fDoublePrime": (5, 1), "gradC": (6, 3), "damage": (7, 1),
+                   "DamageI": (8, 1)}
+
+
+def sch_project(m, ops, consts, frac, plane_strain, u, name, health=None, seed=None):
+    """A stress-CH element's output projection (n x components); consts =
+    C11, C12, C66, A, cref, chi; health / seed (nb, 0/1) for damage, DamageI."""
+    which, nc = SCH_PROJECTIONS[name]
+    out = np.zeros(m.n * nc)
+    kc = np.ascontiguousarray(consts, float)
+    assert kc.shape == (6,)
+    h = None if health is None else np.ascontiguousarray(health, float)
+    sd = None if seed is None else np.ascontiguousarray(seed, float)
+    lib().pxo_sch_project(m.n, m.dim, ops.shape[1], ip(m.off), ip(m.nbr), dp(np.ascontiguousarray(ops)),
+                          dp(kc), int(frac), int(plane_strain), dp(h), dp(sd),
+                          dp(np.ascontiguousarray(u, float)), which, dp(out))
+    return out.reshape(m.n, nc)
+
+
 def elastic_constants(dim, plane_stress, E, nu):
     c = [C.c_double(), C.c_double(), C.c_double()]
     lib().pxo_elastic_constants(dim, int(plane_stress), C.c_double(E), C.c_double(nu),

@@ -626,3 +626,117 @@ void pxo_assemble_fsch(int n, int dim, int nop, const int *off, const int *nbr,
     }
     free(mn);
 }
+
+/* Output projections of the stress-coupled Cahn-Hilliard elements
+ * (StressCahnHilliardElement.cpp:481-658, FracStressCahnHilliardElement.cpp:
+ * 47-110, 575-672).  which: 0 strain, 1 stress, 2 vonMises, 3 sigmaH,
+ * 4 fPrime, 5 fDoublePrime, 6 gradC, 7 damage, 8 DamageI; out holds 9
+ * (tensors), 3 (gradC) or 1 value per node.  k = C11, C12, C66, A, cref,
+ * chi.  frac selects the fracture element (clamped logarithm; damage from
+ * health, DamageI from the bonds broken after the crack seed). */
+void pxo_sch_project(int n, int dim, int nop, const int *off, const int *nbr, const double *ops,
+                     const double *k, int frac, int plane_strain, const double *health,
+                     const double *seed, const double *u, int which, double *out)
+{
+    const int nd = dim + 2;
+    const double C11 = k[0], C12 = k[1], C66 = k[2], A = k[3], cref = k[4], chi = k[5];
+    const int nc = (which <= 1) ? 9 : (which == 6) ? 3 : 1;
+    for (int i = 0; i < n; i++) {
+        double *o = out + (size_t)nc * i;
+        for (int q = 0; q < nc; q++) o[q] = 0.0;
+        const size_t ri = (size_t)nd * i;
+        const double c = u[ri];
+        const int deg = off[i + 1] - off[i];
+        if (which == 4 || which == 5) {
+            double L, Lp;
+            if (frac) {
+                const double cc = clampc(c);
+                L = log(cc / (1.0 - cc));
+                Lp = 1.0 / (cc * (1.0 - cc));
+                o[0] = (which == 4) ? L + chi * (1.0 - 2.0 * cc) : Lp - 2.0 * chi;
+            } else {
+                reg_log(c, 1.0e-3, &L, &Lp);
+                o[0] = (which == 4) ? L + chi * (1.0 - 2.0 * c) : Lp - 2.0 * chi;
+            }
+            continue;
+        }
+        if (which == 7 || which == 8) {
+            int cnt = 0;
+            for (int t = off[i]; t < off[i + 1]; t++)
+                if (health[t] <= 0.0 && (which == 7 || seed[t] > 0.0)) cnt++;
+            double phi = deg > 0 ? (double)cnt / (double)deg : 0.0;
+            if (which == 8) phi = fmin(fmax(phi, 0.0), 1.0);
+            o[0] = phi;
+            continue;
+        }
+        if (which == 6) {
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+            for (int t = off[i]; t < off[i + 1]; t++) {
+                const double *op = ops + (size_t)t * nop;
+                const double dc = u[(size_t)nd * (nbr[t] - 1)] - c;
+                gx += dc * op[0];
+                gy += dc * op[1];
+                if (dim >= 3) gz += dc * op[2];
+            }
+            o[0] = gx; o[1] = gy; o[2] = gz;
+            continue;
+        }
+        const double dc = c - cref;
+        if (dim >= 3) {
+            double g[3][3] = {{0}};   /* g[a][b] = sum (u_a,j - u_a,i) G_b */
+            for (int t = off[i]; t < off[i + 1]; t++) {
+                const double *op = ops + (size_t)t * nop;
+                const size_t rj = (size_t)nd * (nbr[t] - 1);
+                for (int a = 0; a < 3; a++) {
+                    const double du = u[rj + 2 + a] - u[ri + 2 + a];
+                    for (int b = 0; b < 3; b++) g[a][b] += du * op[b];
+                }
+            }
+            const double exx = g[0][0], eyy = g[1][1], ezz = g[2][2];
+            const double exy = 0.5 * (g[0][1] + g[1][0]), exz = 0.5 * (g[0][2] + g[2][0]);
+            const double eyz = 0.5 * (g[1][2] + g[2][1]);
+            const double sxx = C11 * exx + C12 * (eyy + ezz) - A * dc;
+            const double syy = C11 * eyy + C12 * (exx + ezz) - A * dc;
+            const double szz = C11 * ezz + C12 * (exx + eyy) - A * dc;
+            const double sxy = 2.0 * C66 * exy, sxz = 2.0 * C66 * exz, syz = 2.0 * C66 * eyz;
+            if (which == 0) {
+                const double e9[9] = {exx, exy, exz, exy, eyy, eyz, exz, eyz, ezz};
+                for (int q = 0; q < 9; q++) o[q] = e9[q];
+            } else if (which == 1) {
+                const double s9[9] = {sxx, sxy, sxz, sxy, syy, syz, sxz, syz, szz};
+                for (int q = 0; q < 9; q++) o[q] = s9[q];
+            } else if (which == 3) {
+                o[0] = (sxx + syy + szz) / 3.0;
+            } else if (which == 2) {
+                const double a = sxx - syy, b = syy - szz, cc = szz - sxx;
+                o[0] = sqrt(0.5 * (a * a + b * b + cc * cc + 6.0 * (sxy * sxy + sxz * sxz + syz * syz)));
+            }
+            continue;
+        }
+        double dxx = 0.0, dxy = 0.0, dyx = 0.0, dyy = 0.0;
+        for (int t = off[i]; t < off[i + 1]; t++) {
+            const double *op = ops + (size_t)t * nop;
+            const size_t rj = (size_t)nd * (nbr[t] - 1);
+            const double dux = u[rj + 2] - u[ri + 2], duy = u[rj + 3] - u[ri + 3];
+            dxx += dux * op[0];
+            dxy += dux * op[1];
+            dyx += duy * op[0];
+            dyy += duy * op[1];
+        }
+        const double exx = dxx, eyy = dyy, exy = 0.5 * (dxy + dyx);
+        const double sxx = C11 * exx + C12 * eyy - A * dc;
+        const double syy = C12 * exx + C11 * eyy - A * dc;
+        const double sxy = 2.0 * C66 * exy;
+        const double szz = plane_strain ? (C12 * (exx + eyy) - A * dc) : 0.0;
+        if (which == 0) {
+            o[0] = exx; o[1] = exy; o[3] = exy; o[4] = eyy;
+        } else if (which == 1) {
+            o[0] = sxx; o[1] = sxy; o[3] = sxy; o[4] = syy; o[8] = szz;
+        } else if (which == 3) {
+            o[0] = (sxx + syy + szz) / 3.0;
+        } else if (which == 2) {
+            const double d1 = sxx - syy, d2 = syy - szz, d3 = szz - sxx;
+            o[0] = sqrt(0.5 * (d1 * d1 + d2 * d2 + d3 * d3 + 6.0 * sxy * sxy));
+        }
+    }
+}

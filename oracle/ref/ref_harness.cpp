@@ -375,6 +375,25 @@ void ref_projection(void *h, const char *name, double *out)
     }
 }
 
+// computeNodalProjection of every node on the state u (n*ndof), nc
+// components per node
+void ref_projection_u(void *h, const char *name, const double *u, int nc, double *out)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int ndof = p.m_ElmtSystem.getMaxDofsPerNode();
+    const int n = p.m_PDMesh.getNodesNum();
+    VectorXd U(n * ndof);
+    std::copy(u, u + (size_t)n * ndof, U.getDataPtr());
+    const std::string nm(name);
+    std::vector<double> o((size_t)nc);
+    auto *e = p.m_ElmtSystem.m_Elements[0].Element.get();
+    for (int i = 1; i <= n; i++) {
+        std::fill(o.begin(), o.end(), 0.0);
+        e->computeNodalProjection(nm, p.m_PDMesh, p.m_Operators, U, i, ndof, o);
+        std::copy(o.begin(), o.end(), out + (size_t)(i - 1) * nc);
+    }
+}
+
 // Elastic constants of the explicit element (C11, C12, C66, rho, sc).
 void ref_frac_consts(void *h, double *out)
 {

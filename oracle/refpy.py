@@ -31,7 +31,7 @@ def lib():
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
                   "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
                   "ref_apply_bcs", "ref_projection", "ref_ch_geometry", "ref_assemble_step",
-                  "ref_dynfrac_state", "ref_dynfrac_set", "ref_stressch_consts", "ref_fsch_state",
+                  "ref_dynfrac_state", "ref_dynfrac_set", "ref_stressch_consts", "ref_fsch_state", "ref_projection_u",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -145,6 +145,12 @@ class Case:
         nc = 9 if name in ("stress", "strain") else 1
         out = np.zeros(self.n * nc)
         lib().ref_projection(self.h, name.encode(), _dp(out))
+        return out.reshape(self.n, nc)
+
+    def projection_u(self, name, u, nc):
+        """computeNodalProjection of every node on the state u (n x nc)."""
+        out = np.zeros(self.n * nc)
+        lib().ref_projection_u(self.h, name.encode(), _dp(np.ascontiguousarray(u, float)), nc, _dp(out))
         return out.reshape(self.n, nc)
 
     def assemble_step(self, u, uold, dt, timestep):

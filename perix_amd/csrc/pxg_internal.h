@@ -102,6 +102,7 @@ enum { BCROW_DIRECT = 0, BCROW_REFLECT = 1, BCROW_MIRROR = 2 };
 // chi, Omega, cref
 struct StressChState {
     double k[11] = {0};
+    int plane_strain = 1;   // 2D stress state of the projections (s_zz)
     int ready = 0;
 };
 
@@ -111,6 +112,7 @@ struct FracSchState {
     double k[13] = {0};
     double sc = 0;
     int tension_only = 1, damage_on = 1;
+    int plane_strain = 1;         // 2D stress state of the projections (s_zz)
     dbuf<unsigned char> health;   // nb, CSR order
     dbuf<double> lapc;            // nb species Laplacian (0 on ghost bonds)
     dbuf<double> vel, uprev;      // n*dim

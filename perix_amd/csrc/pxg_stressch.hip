@@ -505,6 +505,127 @@ int launch_sch(hipStream_t st, int maxblk, const SchArgs &A)
     return PXG_OK;
 }
 
+// Output projections (StressCahnHilliardElement.cpp:481-658,
+// FracStressCahnHilliardElement.cpp:47-110, 575-672), one thread per node,
+// family sums in family order: which = PXG_SCH_* (0 strain, 1 stress,
+// 2 vonMises, 3 sigmaH, 4 fPrime, 5 fDoublePrime, 6 gradC, 7 damage,
+// 8 DamageI); 9, 3 or 1 values per node.
+template <int DIM>
+__global__ void k_sch_project(int n, int nop, const int *off, const int *nbr, const double *ops, double C11,
+                              double C12, double C66, double A, double cref, double chi, int frac,
+                              int plane_strain, const unsigned char *health, const unsigned char *seed,
+                              const double *u, int which, double *out)
+{
+    constexpr int ND = DIM + 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nc = (which <= 1) ? 9 : (which == 6) ? 3 : 1;
+    double *o = out + (size_t)nc * i;
+    const size_t ri = (size_t)i * ND;
+    const double c = u[ri];
+    const int t0 = off[i], t1 = off[i + 1], deg = t1 - t0;
+    double v[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] = 0.0;
+    if (which == 4 || which == 5) {
+        if (frac) {
+            const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
+            v[0] = (which == 4) ? log(cc / (1.0 - cc)) + chi * (1.0 - 2.0 * cc)
+                                : 1.0 / (cc * (1.0 - cc)) - 2.0 * chi;
+        } else {
+            double L, Lp;
+            sch_reglog(c, L, Lp);
+            v[0] = (which == 4) ? L + chi * (1.0 - 2.0 * c) : Lp - 2.0 * chi;
+        }
+    } else if (which == 7 || which == 8) {
+        // damage: broken share of the family; DamageI: share broken by the
+        // per-step commits (intact in the crack seed)
+        int cnt = 0;
+        for (int t = t0; t < t1; t++)
+            if (!health[t] && (which == 7 || seed[t])) cnt++;
+        double phi = deg > 0 ? (double)cnt / (double)deg : 0.0;
+        if (which == 8) phi = fmin(fmax(phi, 0.0), 1.0);
+        v[0] = phi;
+    } else if (which == 6) {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (int t = t0; t < t1; t++) {
+            const double *op = ops + (size_t)t * nop;
+            const double dc = u[(size_t)(nbr[t] - 1) * ND] - c;
+            gx += dc * op[0];
+            gy += dc * op[1];
+            if (DIM == 3) gz += dc * op[2];
+        }
+        v[0] = gx; v[1] = gy; v[2] = gz;
+    } else {
+        const double dc = c - cref;
+        if constexpr (DIM == 3) {
+            double g[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) g[a][b] = 0.0;
+            for (int t = t0; t < t1; t++) {
+                const double *op = ops + (size_t)t * nop;
+                const size_t rj = (size_t)(nbr[t] - 1) * ND;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const double du = u[rj + 2 + a] - u[ri + 2 + a];
+#pragma unroll
+                    for (int b = 0; b < 3; b++) g[a][b] += du * op[b];
+                }
+            }
+            const double exx = g[0][0], eyy = g[1][1], ezz = g[2][2];
+            const double exy = 0.5 * (g[0][1] + g[1][0]), exz = 0.5 * (g[0][2] + g[2][0]);
+            const double eyz = 0.5 * (g[1][2] + g[2][1]);
+            const double sxx = C11 * exx + C12 * (eyy + ezz) - A * dc;
+            const double syy = C11 * eyy + C12 * (exx + ezz) - A * dc;
+            const double szz = C11 * ezz + C12 * (exx + eyy) - A * dc;
+            const double sxy = 2.0 * C66 * exy, sxz = 2.0 * C66 * exz, syz = 2.0 * C66 * eyz;
+            if (which == 0) {
+                v[0] = exx; v[1] = exy; v[2] = exz; v[3] = exy; v[4] = eyy; v[5] = eyz;
+                v[6] = exz; v[7] = eyz; v[8] = ezz;
+            } else if (which == 1) {
+                v[0] = sxx; v[1] = sxy; v[2] = sxz; v[3] = sxy; v[4] = syy; v[5] = syz;
+                v[6] = sxz; v[7] = syz; v[8] = szz;
+            } else if (which == 3) {
+                v[0] = (sxx + syy + szz) / 3.0;
+            } else {
+                const double a = sxx - syy, b = syy - szz, cc = szz - sxx;
+                v[0] = sqrt(0.5 * (a * a + b * b + cc * cc + 6.0 * (sxy * sxy + sxz * sxz + syz * syz)));
+            }
+        } else {
+            double dxx = 0.0, dxy = 0.0, dyx = 0.0, dyy = 0.0;
+            for (int t = t0; t < t1; t++) {
+                const double *op = ops + (size_t)t * nop;
+                const size_t rj = (size_t)(nbr[t] - 1) * ND;
+                const double dux = u[rj + 2] - u[ri + 2], duy = u[rj + 3] - u[ri + 3];
+                dxx += dux * op[0];
+                dxy += dux * op[1];
+                dyx += duy * op[0];
+                dyy += duy * op[1];
+            }
+            const double exx = dxx, eyy = dyy, exy = 0.5 * (dxy + dyx);
+            const double sxx = C11 * exx + C12 * eyy - A * dc;
+            const double syy = C12 * exx + C11 * eyy - A * dc;
+            const double sxy = 2.0 * C66 * exy;
+            const double szz = plane_strain ? (C12 * (exx + eyy) - A * dc) : 0.0;
+            if (which == 0) {
+                v[0] = exx; v[1] = exy; v[3] = exy; v[4] = eyy;
+            } else if (which == 1) {
+                v[0] = sxx; v[1] = sxy; v[3] = sxy; v[4] = syy; v[8] = szz;
+            } else if (which == 3) {
+                v[0] = (sxx + syy + szz) / 3.0;
+            } else {
+                const double d1 = sxx - syy, d2 = syy - szz, d3 = szz - sxx;
+                v[0] = sqrt(0.5 * (d1 * d1 + d2 * d2 + d3 * d3 + 6.0 * sxy * sxy));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+        if (q < nc) o[q] = v[q];
+}
+
 } // namespace
 
 // recalcElasticConstants (StressCahnHilliardElement.cpp:80-128)
@@ -549,6 +670,7 @@ extern "C" int pxg_stressch_setup(pxg_ctx *ctx, const pxg_stressch_params *p)
     StressChState &S = ctx->sch;
     const double v[11] = {e[0], e[1], e[2], e[3], e[4], e[5], p->D, p->kappa, p->chi, p->Omega, p->cref};
     for (int q = 0; q < 11; q++) S.k[q] = v[q];
+    S.plane_strain = (ctx->dim >= 3 || !p->plane_stress) ? 1 : 0;
     S.ready = 1;
     return PXG_OK;
 }
@@ -624,6 +746,7 @@ extern "C" int pxg_fsch_setup(pxg_ctx *ctx, const pxg_fsch_params *p, const doub
                           p->base.Omega, p->base.cref, p->rho, p->residual_stiffness};
     for (int q = 0; q < 13; q++) F.k[q] = v[q];
     F.sc = p->sc;
+    F.plane_strain = (ctx->dim >= 3 || !p->base.plane_stress) ? 1 : 0;
     F.tension_only = p->tension_only;
     F.damage_on = p->damage_on;
     const size_t nd = (size_t)ctx->n * ctx->dim;
@@ -746,4 +869,90 @@ extern "C" int pxg_fsch_get_state(pxg_ctx *ctx, unsigned char *health, double *v
         PXG_HIP(hipMemcpyAsync(vel, F.vel.p, sizeof(double) * ctx->n * ctx->dim, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
     return PXG_OK;
+}
+
+// ---- output projections ----------------------------------------------------
+
+static int sch_project(pxg_ctx *ctx, bool frac, int which, const double *u_dev, double *out_dev)
+{
+    PXG_HIP(hipSetDevice(ctx->device));
+    const double *k = frac ? ctx->fsch.k : ctx->sch.k;
+    const int ps = frac ? ctx->fsch.plane_strain : ctx->sch.plane_strain;
+    const unsigned char *h = frac ? ctx->fsch.health.p : nullptr;
+    const unsigned g = grids(ctx->n, 128);
+    if (ctx->dim == 3)
+        hipLaunchKernelGGL(k_sch_project<3>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, k[0], k[1], k[2], k[3], k[10], k[8], (int)frac, ps, h,
+                           ctx->seed.p, u_dev, which, out_dev);
+    else
+        hipLaunchKernelGGL(k_sch_project<2>, dim3(g), dim3(128), 0, ctx->stream, ctx->n, ctx->nop, ctx->off.p,
+                           ctx->nbr.p, ctx->ops.p, k[0], k[1], k[2], k[3], k[10], k[8], (int)frac, ps, h,
+                           ctx->seed.p, u_dev, which, out_dev);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+static int sch_project_host(pxg_ctx *ctx, bool frac, int which, const double *u, double *out)
+{
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t nu = (size_t)ctx->n * (ctx->dim + 2);
+    const size_t no = (size_t)ctx->n * pxg_sch_components(which);
+    PXG_HIP(ctx->su.alloc(nu));
+    PXG_HIP(ctx->sr.alloc(no));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * nu, hipMemcpyHostToDevice, st));
+    const int rc = sch_project(ctx, frac, which, ctx->su.p, ctx->sr.p);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(out, ctx->sr.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_sch_components(int which)
+{
+    return (which == PXG_SCH_STRAIN || which == PXG_SCH_STRESS) ? 9 : (which == PXG_SCH_GRADC) ? 3 : 1;
+}
+
+// the names each element lists (getAvailableProjections)
+static bool stressch_has(int which) { return which >= PXG_SCH_STRAIN && which <= PXG_SCH_GRADC; }
+static bool fsch_has(int which)
+{
+    return (which >= PXG_SCH_STRAIN && which <= PXG_SCH_SIGMAH) || which == PXG_SCH_GRADC ||
+           which == PXG_SCH_DAMAGE || which == PXG_SCH_DAMAGEI;
+}
+
+extern "C" int pxg_stressch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev)
+{
+    PXG_CHECK(ctx && ctx->sch.ready, PXG_ERR_STATE, "pxg_stressch_project: call pxg_stressch_setup first");
+    PXG_CHECK(stressch_has(which) && u_dev && out_dev, PXG_ERR_ARG,
+              "pxg_stressch_project: bad arguments (which: strain .. gradC)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_stressch_project: injected fault");
+    return sch_project(ctx, false, which, u_dev, out_dev);
+}
+
+extern "C" int pxg_stressch_project(pxg_ctx *ctx, int which, const double *u, double *out)
+{
+    PXG_CHECK(ctx && ctx->sch.ready, PXG_ERR_STATE, "pxg_stressch_project: call pxg_stressch_setup first");
+    PXG_CHECK(stressch_has(which) && u && out, PXG_ERR_ARG,
+              "pxg_stressch_project: bad arguments (which: strain .. gradC)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_stressch_project: injected fault");
+    return sch_project_host(ctx, false, which, u, out);
+}
+
+extern "C" int pxg_fsch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_project: call pxg_fsch_setup first");
+    PXG_CHECK(fsch_has(which) && u_dev && out_dev, PXG_ERR_ARG,
+              "pxg_fsch_project: bad arguments (which: strain, stress, vonMises, sigmaH, gradC, damage, DamageI)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_project: injected fault");
+    return sch_project(ctx, true, which, u_dev, out_dev);
+}
+
+extern "C" int pxg_fsch_project(pxg_ctx *ctx, int which, const double *u, double *out)
+{
+    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_project: call pxg_fsch_setup first");
+    PXG_CHECK(fsch_has(which) && u && out, PXG_ERR_ARG,
+              "pxg_fsch_project: bad arguments (which: strain, stress, vonMises, sigmaH, gradC, damage, DamageI)");
+    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_project: injected fault");
+    return sch_project_host(ctx, true, which, u, out);
 }

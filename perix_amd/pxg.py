@@ -36,6 +36,11 @@ class FracParams(C.Structure):
                 ("tension_only", C.c_int), ("damage_on", C.c_int)]
 
 
+# PXG_SCH_* projection codes of include/pxg.h
+SCH_PROJECTIONS = {"strain": 0, "stress": 1, "vonMises": 2, "sigmaH": 3, "fPrime": 4, "fDoublePrime": 5,
+                   "gradC": 6, "damage": 7, "DamageI": 8}
+
+
 class StressChParams(C.Structure):
     """pxg_stressch_params: the stress_cahnhilliard deck parameters."""
     _fields_ = [("E", C.c_double), ("nu", C.c_double), ("D", C.c_double),
@@ -73,6 +78,8 @@ SYMBOLS = [
     "pxg_stressch_setup", "pxg_stressch_constants", "pxg_assemble_stressch", "pxg_assemble_stressch_dev",
     "pxg_fsch_setup", "pxg_fsch_constants", "pxg_fsch_commit", "pxg_fsch_commit_dev",
     "pxg_assemble_fsch", "pxg_assemble_fsch_dev", "pxg_fsch_get_state",
+    "pxg_sch_components", "pxg_stressch_project", "pxg_stressch_project_dev", "pxg_fsch_project",
+    "pxg_fsch_project_dev",
     "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
     "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
@@ -143,6 +150,9 @@ def load():
         L.pxg_assemble_fsch_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p]
         L.pxg_fsch_get_state.argtypes = [C.c_void_p, U8, D]
+        for nm in ("pxg_stressch_project", "pxg_fsch_project"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_int, D, D]
+            getattr(L, nm + "_dev").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.pxg_stressch_setup.argtypes = [C.c_void_p, C.POINTER(StressChParams)]
         L.pxg_stressch_constants.argtypes = [C.c_void_p, D]
         L.pxg_assemble_stressch.argtypes = [C.c_void_p, C.c_double, D, D, D, D]
@@ -434,6 +444,23 @@ class Context:
     def assemble_stressch_dev(self, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
         _check(load().pxg_assemble_stressch_dev(self.h, dt, u_ptr, uold_ptr, k_ptr, r_ptr))
 
+    def stressch_project(self, name, u):
+        """A projection of the stress-CH element for the state u (n x components)."""
+        return self._sch_project("pxg_stressch_project", name, u)
+
+    def stressch_project_dev(self, which, u_ptr, out_ptr):
+        _check(load().pxg_stressch_project_dev(self.h, which, u_ptr, out_ptr))
+
+    def fsch_project_dev(self, which, u_ptr, out_ptr):
+        _check(load().pxg_fsch_project_dev(self.h, which, u_ptr, out_ptr))
+
+    def _sch_project(self, fn, name, u):
+        which = SCH_PROJECTIONS[name]
+        nc = load().pxg_sch_components(which)
+        out = np.zeros(self.n * nc)
+        _check(getattr(load(), fn)(self.h, which, _dp(np.ascontiguousarray(u, float)), _dp(out)))
+        return out.reshape(self.n, nc)
+
     # -- fracture stress-coupled Cahn-Hilliard element ----------------------------
     def fsch_setup(self, params, uold0=None):
         uo = None if uold0 is None else np.ascontiguousarray(uold0, float)
@@ -461,6 +488,10 @@ class Context:
 
     def assemble_fsch_dev(self, dt, u_ptr, uold_ptr, k_ptr, r_ptr):
         _check(load().pxg_assemble_fsch_dev(self.h, dt, u_ptr, uold_ptr, k_ptr, r_ptr))
+
+    def fsch_project(self, name, u):
+        """A projection of the fracture stress-CH element for the state u."""
+        return self._sch_project("pxg_fsch_project", name, u)
 
     def fsch_state(self):
         h = np.zeros(self.nb, np.uint8)

@@ -25,11 +25,12 @@ Fixtures (all small, .npz):
                     initialising step and a committing step with bond failure)
   stressch_*.npz    StressCahnHilliardElement assembly (c, mu, u): plane
                     strain, plane stress and 3D, concentrations also outside
-                    the log barrier and the mobility clamp
+                    the log barrier and the mobility clamp, with the
+                    element's output projections of that state
   fsch_*.npz        FracStressCahnHilliardElement over an initialising and a
                     committing step (2D plane stress with inertia and a
                     residual stiffness; 3D quasi-static, tension and
-                    compression failure)
+                    compression failure), projections after the commit
   bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
                     Poisson (20x12) / Cahn-Hilliard (16x16) system under an
                     overlapping Dirichlet/Neumann list, with zeroed diagonals
@@ -204,6 +205,15 @@ def dynfrac_fixture(name, dim, nx, ny, nz, hi, crack, amp):
     return int((h1 == 0).sum()), int((h2 == 0).sum()), len(h2)
 
 
+# the chemo-mechanical elements' projections and their component counts
+# (getAvailableProjections, StressCahnHilliardElement.cpp:469-479,
+# FracStressCahnHilliardElement.cpp:563-573)
+SCH_PROJECTIONS = [("strain", 9), ("stress", 9), ("vonMises", 1), ("sigmaH", 1), ("fPrime", 1),
+                   ("fDoublePrime", 1), ("gradC", 3)]
+FSCH_PROJECTIONS = [("strain", 9), ("stress", 9), ("vonMises", 1), ("sigmaH", 1), ("gradC", 3),
+                    ("damage", 1), ("DamageI", 1)]
+
+
 def stressch_fixture(name, dim, nx, ny, nz, hi, state="plane_strain"):
     """StressCahnHilliardElement assembly (c, mu, u) for a seeded state whose
     concentrations also visit the regularised-log and mobility-clamp ranges."""
@@ -234,9 +244,10 @@ def stressch_fixture(name, dim, nx, ny, nz, hi, state="plane_strain"):
     uold[0::nd] -= 0.01 * rs.rand(c.n)
     k, r = c.assemble(u, uold, 1e-3)
     consts = c.stressch_consts()
+    proj = {"proj_" + nm: c.projection_u(nm, u, nc) for nm, nc in SCH_PROJECTIONS}
     np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
                         hi=np.array(hi, float), state=state, consts=consts, dt=1e-3, u=u, uold=uold,
-                        K=k, R=r)
+                        K=k, R=r, **proj)
     c.close()
 
 
@@ -292,7 +303,8 @@ def fsch_fixture(name, dim, nx, ny, nz, hi, crack, rho, tension_only, resid=None
     u2 = uold2 + 1e-6 * rs.standard_normal(n * nd)
     k2, r2 = c.assemble_step(u2, uold2, dt2, 2)
     h2, v2, _, _ = c.fsch_state(dim)
-    np.savez_compressed(os.path.join(HERE, name), dim=dim, grid=np.array([nx, ny, nz]),
+    proj = {"proj_" + nm: c.projection_u(nm, u2, nc) for nm, nc in FSCH_PROJECTIONS}
+    np.savez_compressed(os.path.join(HERE, name), **proj, dim=dim, grid=np.array([nx, ny, nz]),
                         hi=np.array(hi, float), crack=np.array(crack, float), state=state,
                         E=E, nu=nu, G0=G0, consts=kc, tension_only=int(tension_only),
                         dt1=dt1, dt2=dt2, lapc=lapc,

@@ -241,3 +241,28 @@ def test_fracture_stress_cahn_hilliard_element(name):
     k, r = pxo.assemble_fsch(m, ops, lapc, rev, isg, kc[:13], float(z["dt2"]), vel, health, rp, cols,
                              z["u2"], z["uold2"])
     assert same(k, z["K2"]) and same(r, z["R2"]), "step 2"
+
+
+@pytest.mark.parametrize("name", ["stressch_2d.npz", "stressch_2d_ps.npz", "stressch_3d.npz",
+                                  "fsch_2d.npz", "fsch_3d.npz"])
+def test_stress_cahn_hilliard_projections(name):
+    """The chemo-mechanical elements' computeNodalProjection (strain, stress,
+    von Mises, sigma_h, f', f'', grad c; damage and DamageI after the commit
+    of the fracture element) bit for bit."""
+    z = load(name)
+    dim = int(z["dim"])
+    m = mesh_of(dict(z, factor=3.015))
+    ops, _ = pxo.operators(m)
+    k = z["consts"]
+    consts = [k[0], k[1], k[2], k[3], k[10], k[8]]
+    frac = name.startswith("fsch")
+    plane_strain = dim == 3 or str(z["state"]) != "plane_stress"
+    u = z["u2"] if frac else z["u"]
+    names = [n[len("proj_"):] for n in z.files if n.startswith("proj_")]
+    assert len(names) == 7
+    for nm in names:
+        o = pxo.sch_project(m, ops, consts, frac, plane_strain, u, nm,
+                            health=z["health2"] if frac else None, seed=z["health1"] if frac else None)
+        assert same(o, z["proj_" + nm]), nm
+        if nm in ("damage", "DamageI"):
+            assert o.max() > 0, nm

@@ -324,3 +324,85 @@ def test_fracture_stress_cahn_hilliard_vs_reference(name, oracle):
             print("step %d dof %d: K rel %.2e, R rel %.2e" % (step, a, ek, er))
             assert ek <= 1e-11 and er <= 1e-11
     ctx.close()
+
+
+def _sch_device(z, frac):
+    """Device context set up as the fixture's element: (ctx, oracle mesh, ops,
+    device health or None)."""
+    dim = int(z["dim"])
+    nx, ny, nz = (int(v) for v in z["grid"])
+    hi = tuple(float(v) for v in z["hi"])
+    g = perix_amd.StructuredGrid(dim, nx, ny, nz, hi=hi)
+    cloud = perix_amd.build_structured_cloud(g)
+    delta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+    if frac:
+        ctx.set_cracks(segs=[tuple(float(v) for v in z["crack"])])
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(dim + 2)
+    kc = z["consts"]
+    ps = dim == 2 and str(z["state"]) == "plane_stress"
+    E = float(z["E"]) if frac else 70e9
+    nu = float(z["nu"]) if frac else 0.3
+    base = pxg.StressChParams(E, nu, float(kc[6]), float(kc[9]), float(kc[10]), float(kc[8]), float(kc[7]), int(ps))
+    h = None
+    if frac:
+        sc = problems.critical_stretch(float(z["G0"]), E, delta, dim, ps)
+        ctx.fsch_setup(pxg.FschParams(base, float(kc[11]), sc, float(kc[12]), int(z["tension_only"]), 1), z["uold1"])
+        ctx.fsch_commit(z["uold2"], float(z["dt1"]))
+        h, _ = ctx.fsch_state()
+    else:
+        ctx.stressch_setup(base)
+    m = _oracle_mesh(z)
+    ops, _ = ctx.operators()
+    return ctx, m, ops, h
+
+
+def _oracle_mesh(z):
+    from oracle import pxo
+    nx, ny, nz = (int(v) for v in z["grid"])
+    return pxo.Mesh(pxo.make_grid(int(z["dim"]), nx, ny, nz, hi=tuple(float(v) for v in z["hi"])))
+
+
+@pytest.mark.parametrize("name", ["stressch_2d.npz", "stressch_2d_ps.npz", "stressch_3d.npz",
+                                  "fsch_2d.npz", "fsch_3d.npz"])
+def test_stress_cahn_hilliard_projections_vs_reference(name, oracle):
+    """The chemo-mechanical elements' projections on the device: bit for bit
+    against the oracle on the device's g-weights and state (f' within 1e-14:
+    device logarithm), within 1e-9 of each field's scale against the
+    reference's computeNodalProjection (the g-weights differ by 1e-12)."""
+    z = load(name)
+    print("fixture", name, flush=True)
+    frac = name.startswith("fsch")
+    dim = int(z["dim"])
+    ctx, m, ops, h = _sch_device(z, frac)
+    k = z["consts"]
+    consts = [k[0], k[1], k[2], k[3], k[10], k[8]]
+    plane_strain = dim == 3 or str(z["state"]) != "plane_stress"
+    u = z["u2"] if frac else z["u"]
+    names = [n[len("proj_"):] for n in z.files if n.startswith("proj_")]
+    for nm in names:
+        d = ctx.fsch_project(nm, u) if frac else ctx.stressch_project(nm, u)
+        o = oracle.sch_project(m, ops, consts, frac, plane_strain, u, nm,
+                               health=None if h is None else h.astype(float),
+                               seed=z["health1"] if frac else None)
+        if nm == "fPrime":
+            assert np.abs(d - o).max() <= 1e-14 * np.abs(o).max(), nm
+        else:
+            assert np.array_equal(d.view(np.uint64), o.view(np.uint64)), nm
+        ref = z["proj_" + nm]
+        err = np.abs(d - ref).max() / max(np.abs(ref).max(), 1e-300)
+        print("%s: max rel to the reference %.2e" % (nm, err))
+        assert err <= 1e-9, nm
+    import torch
+    ud = torch.from_numpy(np.ascontiguousarray(u)).cuda()
+    od = torch.empty(m.n * 9, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    (ctx.fsch_project_dev if frac else ctx.stressch_project_dev)(pxg.SCH_PROJECTIONS["stress"], ud.data_ptr(),
+                                                                od.data_ptr())
+    ctx.sync()
+    host = ctx.fsch_project("stress", u) if frac else ctx.stressch_project("stress", u)
+    assert np.array_equal(od.cpu().numpy().reshape(-1, 9), host)
+    ctx.close()
