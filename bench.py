@@ -338,7 +338,8 @@ def assembly_case(problems, pxg, reps=20):
         poisson()
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["poisson_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3),
+                              "roofline": assembly_roofline("poisson", ctx, ms)}
     out["poisson_100x100"].update(bc_rows_timing(ctx, u, k, r, poisson, reps))
     ctx.close()
     # diffusion (backward Euler) on the same 100x100 square
@@ -361,7 +362,8 @@ def assembly_case(problems, pxg, reps=20):
         diffusion()
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    out["diffusion_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["diffusion_100x100"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3),
+                                "roofline": assembly_roofline("diffusion", ctx, ms)}
     ctx.close()
     # implicit dynamic fracture element on the 400x200 plate (2 dofs per node)
     import perix_amd
@@ -398,7 +400,8 @@ def assembly_case(problems, pxg, reps=20):
         dynfrac()
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    out["dynamic_fracture_400x200"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["dynamic_fracture_400x200"] = {"nnz": nnz, "ms_per_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3),
+                                       "roofline": assembly_roofline("dynfrac", ctx, ms)}
     ctx.close()
     ctx, cloud, delta, u0 = problems.cahnhilliard_square()
     n, nnz = cloud.n, ctx.nnz
@@ -420,7 +423,8 @@ def assembly_case(problems, pxg, reps=20):
         ch()
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    out["cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3)}
+    out["cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms, "nnz_per_s": nnz / (ms * 1e-3),
+                                   "roofline": assembly_roofline("ch", ctx, ms)}
     out["cahnhilliard_256x256"].update(bc_rows_timing(ctx, ud, k, r, ch, reps))
     ctx.close()
     # stress-coupled Cahn-Hilliard (c, mu, ux, uy), silicon parameters, 256x256
@@ -442,7 +446,8 @@ def assembly_case(problems, pxg, reps=20):
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     out["stress_cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms,
-                                          "nnz_per_s": nnz / (ms * 1e-3)}
+                                          "nnz_per_s": nnz / (ms * 1e-3),
+                                          "roofline": assembly_roofline("stressch", ctx, ms)}
     ctx.close()
     # fracture variant (silicon_frac parameters, two edge notches), with the
     # per-step commit timed on its own
@@ -473,7 +478,8 @@ def assembly_case(problems, pxg, reps=20):
     ctx.sync()
     ms_c = (time.perf_counter() - t0) * 1e3 / reps
     out["frac_stress_cahnhilliard_256x256"] = {"nnz": nnz, "ms_per_picard_assembly": ms,
-                                               "nnz_per_s": nnz / (ms * 1e-3), "ms_per_commit": ms_c}
+                                               "nnz_per_s": nnz / (ms * 1e-3), "ms_per_commit": ms_c,
+                                               "roofline": assembly_roofline("fsch", ctx, ms)}
     ctx.close()
     try:
         out["cpu_baseline"] = cpu_assembly_baseline(u_h_poisson, u0, u1, kappa, (u_h, uold_h, G0),
@@ -481,6 +487,40 @@ def assembly_case(problems, pxg, reps=20):
     except Exception as e:       # reported, never required
         out["cpu_baseline"] = {"value": None, "sample": "failed: %s" % e}
     return out
+
+
+def assembly_bytes(kind, n, nb, nnz, dim, nop):
+    """Algorithmic HBM bytes of one assembly launch: the K values and the
+    residual written once; per directed bond the element's bond data read
+    once (neighbour id and CSR slot 4 B each; the per-bond Laplacian 8 B and
+    reverse-bond index 4 B for the Laplacian elements; the operators nop x 8 B
+    for the Navier elements; the health byte for the fracture ones); per node
+    the state, old state and node data.  Neighbour gathers are not counted
+    again (each node's data is fetched once from HBM)."""
+    if kind in ("poisson", "diffusion"):
+        nd = 1
+        bond = 4 + 4 + 8
+        node = 8 + 8 + 12 + (8 if kind == "diffusion" else 0)
+    elif kind == "ch":
+        nd = 2
+        bond = 4 + 4 + 4 + 8
+        node = 2 * 8 + 8 + 8 + 8 + 1 + 12
+    elif kind == "dynfrac":
+        nd = dim
+        bond = 4 + 4 + 1 + nop * 8
+        node = 3 * dim * 8 + 12
+    else:   # stressch, fsch
+        nd = dim + 2
+        bond = 4 + 4 + 4 + 8 + nop * 8 + (1 if kind == "fsch" else 0)
+        node = nd * 8 + 8 + 8 + 1 + 12 + (dim * 8 if kind == "fsch" else 0)
+    return nnz * 8 + n * nd * 8 + nb * bond + n * node
+
+
+def assembly_roofline(kind, ctx, ms):
+    b = assembly_bytes(kind, ctx.n, ctx.nb, ctx.nnz, ctx.dim, 9 if ctx.dim == 3 else 5)
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": b, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "timing": "host clock over back-to-back launches (an upper bound on the kernel time)"}
 
 
 def bc_rows_timing(ctx, u, k, r, assemble, reps):

@@ -38,7 +38,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->coords); rel(c->xp); rel(c->vols); rel(c->mirror); rel(c->off); rel(c->nbr);
     rel(c->crack_seg); rel(c->crack_quad); rel(c->seed); rel(c->ops); rel(c->flags);
     rel(c->rowptr); rel(c->cols); rel(c->slot); rel(c->diag); rel(c->mslot); rel(c->lap);
-    rel(c->rev); rel(c->isghost); rel(c->mnode); rel(c->su); rel(c->suo); rel(c->sk);
+    rel(c->rev); rel(c->isghost); rel(c->su); rel(c->suo); rel(c->sk);
     rel(c->sr); rel(c->sf);
     rel(c->csrbc.row); rel(c->csrbc.dslot); rel(c->csrbc.pslot); rel(c->csrbc.pdof);
     rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);

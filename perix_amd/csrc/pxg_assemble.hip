@@ -1,5 +1,7 @@
 // pxg_assemble.hip -- CSR residual/Jacobian assembly for the Poisson and
-// Cahn-Hilliard elements.
+// diffusion elements, and the Cahn-Hilliard element's bond geometry and
+// entry points (its assembly runs on the block-element kernels of
+// pxg_stressch.hip, launch_ch_rows).
 //
 // Reference: PDSystem::formResidualAndJacobian (FormResidualAndJacobian.cpp:
 // 25-132) with PoissonElement (PoissonElement.cpp:26-58) and
@@ -7,7 +9,7 @@
 // assembleKernel / cahnHilliardKernel (CudaAssembleKernels.cu:291-398,
 // 572-649) are the behaviour spec.
 //
-// Rows are node-owned, so no atomics: a 256-thread block owns T nodes,
+// Poisson / diffusion rows are node-owned, so no atomics: a 256-thread block owns T nodes,
 // lanes stream the tile's bonds (contiguous CSR order), write the
 // off-diagonal blocks straight to their precomputed CSR slots (built once
 // with the pattern, replacing the per-entry binary search of
@@ -132,110 +134,6 @@ __global__ void k_ch_geometry(int n, int dim, int nop, const double *coords, con
     isghost[i] = (mirror[i] > 0) ? 1 : 0;
 }
 
-__device__ __forceinline__ double clampc(double c)
-{
-    return fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
-}
-
-__global__ void k_ch_mobility(int n, const double *u, double M0, double *mnode)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double cc = clampc(u[2 * (size_t)i]);
-    mnode[i] = M0 * cc * (1.0 - cc);
-}
-
-__global__ void __launch_bounds__(256) k_ch(int n, const int *off, const int *nbr, const int *rev,
-                                          const double *lap, const unsigned char *isghost,
-                                          const double *vol, const double *mnode, double kappa,
-                                          double chi, double dt, const int *rowptr, const int *slot,
-                                          const int *diag, const int *mslot, const double *u,
-                                          const double *uold, double *kvals, double *rhs)
-{
-    __shared__ int s_off[AT + 1];
-    __shared__ double s_c[ACH], s_m[ACH], s_rc[ACH], s_rm[ACH];
-    const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * AT;
-    const int nt = min(AT, n - i0);
-    if (tid <= nt) s_off[tid] = off[i0 + tid];
-    __syncthreads();
-    const int bt0 = s_off[0], bt1 = s_off[nt];
-    double k01 = 0.0, k10 = 0.0, k00 = 0.0, k11 = 0.0, rc = 0.0, rm = 0.0;
-    if (tid < nt) {
-        const int i = i0 + tid;
-        const double c = u[2 * (size_t)i], mu = u[2 * (size_t)i + 1], cold = uold[2 * (size_t)i];
-        const double cc = clampc(c);
-        const double fp = log(cc / (1.0 - cc)) + chi * (1.0 - 2.0 * cc);
-        const double fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * chi;
-        rc = rc + (c - cold) / dt;
-        rm = rm + (mu - fp);
-        k00 = k00 + (-(1.0 / dt));
-        k01 = k01 + (-0.0);
-        k10 = k10 + (-(-fpp));
-        k11 = k11 + (-1.0);
-        if (mslot[i] >= 0) {
-            const int m0 = mslot[i];
-            const int m1 = rowptr[2 * i + 1] + (m0 - rowptr[2 * i]);
-            kvals[m0] = 0.0; kvals[m0 + 1] = 0.0; kvals[m1] = 0.0; kvals[m1 + 1] = 0.0;
-        }
-    }
-    for (int cs = bt0; cs < bt1; cs += ACH) {
-        const int ce = min(bt1, cs + ACH);
-        for (int k = 0; k < AK; k++) {
-            const int lt = tid + k * 256, t = cs + lt;
-            if (t >= ce) continue;
-            int lo = 0, hi = nt - 1;
-            while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_off[mid] <= t) lo = mid; else hi = mid - 1; }
-            const int i = i0 + lo;
-            const int j = nbr[t] - 1;
-            const bool ghostJ = isghost[j] != 0;
-            const double g2ij = lap[t];
-            double coeff_c = 0.0;
-            if (!ghostJ) {
-                const double Mi = mnode[i], Mj = mnode[j], Vi = vol[i], Vj = vol[j];
-                const int rv = rev[t];
-                double wij;
-                if (rv >= 0 && Vi > 0.0) wij = 0.5 * (Mi * g2ij + (Vj / Vi) * Mj * lap[rv]);
-                else wij = Mi * g2ij;
-                coeff_c = -wij;
-            }
-            const double coeff_mu = ghostJ ? 0.0 : (kappa * g2ij);
-            const double dC = u[2 * (size_t)j] - u[2 * (size_t)i];
-            const double dMu = u[2 * (size_t)j + 1] - u[2 * (size_t)i + 1];
-            s_c[lt] = coeff_c; s_m[lt] = coeff_mu;
-            s_rc[lt] = coeff_c * dMu; s_rm[lt] = coeff_mu * dC;
-            const int p0 = slot[t];
-            const int p1 = rowptr[2 * i + 1] + (p0 - rowptr[2 * i]);
-            kvals[p0] = 0.0 + (-0.0);
-            kvals[p0 + 1] = 0.0 + (-coeff_c);
-            kvals[p1] = 0.0 + (-coeff_mu);
-            kvals[p1 + 1] = 0.0 + (-0.0);
-        }
-        __syncthreads();
-        if (tid < nt) {
-            const int tb = max(s_off[tid], cs), te = min(s_off[tid + 1], ce);
-            for (int t = tb; t < te; t++) {
-                const int lt = t - cs;
-                rc += s_rc[lt];
-                rm += s_rm[lt];
-                k00 += -0.0;
-                k01 += -(-s_c[lt]);
-                k10 += -(-s_m[lt]);
-                k11 += -0.0;
-            }
-        }
-        __syncthreads();
-    }
-    if (tid < nt) {
-        const int i = i0 + tid;
-        rhs[2 * (size_t)i] = rc;
-        rhs[2 * (size_t)i + 1] = rm;
-        const int d0 = diag[i];
-        const int d1 = rowptr[2 * i + 1] + (d0 - rowptr[2 * i]);
-        kvals[d0] = k00; kvals[d0 + 1] = k01; kvals[d1] = k10; kvals[d1 + 1] = k11;
-    }
-}
-
 } // namespace pxg
 
 using namespace pxg;
@@ -345,7 +243,6 @@ int pxg::ch_geometry(pxg_ctx *ctx)
     PXG_HIP(ctx->lap.alloc(ctx->nb));
     PXG_HIP(ctx->rev.alloc(ctx->nb));
     PXG_HIP(ctx->isghost.alloc(ctx->n));
-    PXG_HIP(ctx->mnode.alloc(ctx->n));
     hipLaunchKernelGGL(k_ch_geometry, dim3(gridb(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
                        ctx->nop, ctx->coords.p, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->mirror.p, ctx->lap.p,
                        ctx->rev.p, ctx->isghost.p);
@@ -384,13 +281,7 @@ extern "C" int pxg_assemble_ch_dev(pxg_ctx *ctx, double kappa, double chi, doubl
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_ch: bad arguments");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_ch: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
-    hipLaunchKernelGGL(k_ch_mobility, dim3(gridb(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n, u_dev, M0,
-                       ctx->mnode.p);
-    hipLaunchKernelGGL(k_ch, dim3(gridb(ctx->n, AT)), dim3(256), 0, ctx->stream, ctx->n, ctx->off.p, ctx->nbr.p,
-                       ctx->rev.p, ctx->lap.p, ctx->isghost.p, ctx->vols.p, ctx->mnode.p, kappa, chi, dt,
-                       ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev, rhs_dev);
-    PXG_HIP(hipGetLastError());
-    return PXG_OK;
+    return launch_ch_rows(ctx, kappa, chi, M0, dt, u_dev, uold_dev, kvals_dev, rhs_dev);
 }
 
 extern "C" int pxg_assemble_ch(pxg_ctx *ctx, double kappa, double chi, double M0, double dt, const double *u,

@@ -232,6 +232,9 @@ int launch_transpose_ops(pxg_ctx *ctx);
 // Cahn-Hilliard bond geometry (lambda-corrected Laplacian, reverse bonds,
 // ghost flags) into ctx->lap / rev / isghost; sets ch_ready
 int ch_geometry(pxg_ctx *ctx);
+// the Cahn-Hilliard assembly on the block-element kernels (pxg_stressch.hip)
+int launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, double dt, const double *u_dev,
+                   const double *uold_dev, double *kvals_dev, double *rhs_dev);
 
 } // namespace pxg
 
@@ -274,7 +277,6 @@ struct pxg_ctx {
     pxg::dbuf<double> lap;          // nb: pdLaplacian per bond (Poisson) / lambda-corrected (CH)
     pxg::dbuf<int> rev;             // nb
     pxg::dbuf<unsigned char> isghost;
-    pxg::dbuf<double> mnode;
     int ch_ready = 0;
     pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
     pxg::DynFracState dyn;          // implicit dynamic fracture element

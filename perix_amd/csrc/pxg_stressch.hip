@@ -149,6 +149,14 @@ struct SchArgs {
     double *kvals, *rhs;
 };
 
+// Element kinds sharing the block assembly: the stress-coupled
+// Cahn-Hilliard element, its fracture variant, and the plain Cahn-Hilliard
+// element (the species/chemical-potential part alone, dofs c and mu).
+enum { EL_SCH = 0, EL_FSCH = 1, EL_CH = 2 };
+
+template <int DIM, int EL>
+__device__ __forceinline__ constexpr int el_nd() { return EL == EL_CH ? 2 : DIM + 2; }
+
 // The structural nonzeros of a bond's dR/du_j block P, in staging order:
 // (0,1), then for rows a = 1..ND-1 the columns 0, 2, 3, ..., ND-1.  They are
 // the bond terms of the diagonal block; its other entries hold only the
@@ -172,29 +180,29 @@ template <int ND>
 __device__ __forceinline__ int sch_index(int a, int b) { return (a == 0) ? 0 : 1 + (a - 1) * (ND - 1) + (b == 0 ? 0 : b - 1); }
 
 // Node-level quantities of node i (the row owner) used by its bonds.
-template <int DIM>
+template <int ND>
 struct SchNode {
-    double ui[DIM + 2];
+    double ui[ND];
     double Mi, Vi;
 };
 
-// One bond t of node i: P (StressCahnHilliardElement.cpp:282-428,
-// FracStressCahnHilliardElement.cpp:400-526) and the residual terms R.  The
-// fracture variant gates the mechanics and sigma_h coupling with g (1 or the
-// residual stiffness); with g = 1 every product rounds as the reference's
-// ungated forms.
-template <int DIM, bool FRAC>
-__device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<DIM> &nd, int t,
-                                         double (&P)[DIM + 2][DIM + 2], double (&R)[DIM + 2])
+// One bond t of node i: P and the residual terms R
+// (StressCahnHilliardElement.cpp:282-428, FracStressCahnHilliardElement.cpp:
+// 400-526, CahnHilliardElement.cpp:178-254).  The fracture variant gates the
+// mechanics and sigma_h coupling with g (1 or the residual stiffness); with
+// g = 1 every product rounds as the reference's ungated forms.
+template <int DIM, int EL>
+__device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<DIM, EL>()> &nd, int t,
+                                         double (&P)[el_nd<DIM, EL>()][el_nd<DIM, EL>()],
+                                         double (&R)[el_nd<DIM, EL>()])
 {
-    constexpr int ND = DIM + 2;
+    constexpr int ND = el_nd<DIM, EL>();
     const FschConst &k = A.k;
     const int j = A.nbr[t] - 1;
     const size_t rj = (size_t)j * ND;
-    const double *op = A.ops + (size_t)t * A.nop;
     const bool ghostJ = A.isghost[j] != 0;
     double g = 1.0, coeff_c = 0.0, kl = 0.0;
-    if constexpr (FRAC) {
+    if constexpr (EL == EL_FSCH) {
         g = A.health[t] ? 1.0 : k.resid;
         double wf = 0.0, g2c = 0.0;
         if (!ghostJ) {
@@ -224,81 +232,94 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<DIM> &n
     for (int a = 0; a < ND; a++)
 #pragma unroll
         for (int b = 0; b < ND; b++) P[a][b] = 0.0;
-    const double go = g * k.Omega, gA = -g * k.A;
-    if constexpr (DIM == 3) {
-        const double Gx = op[0], Gy = op[1], Gz = op[2];
-        const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
-        const double ga = g * (k.C12 + k.C66);
-        const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy, cuz = go * k.Kh * Gz;
-        const double p11 = g * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
-        const double p22 = g * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
-        const double p33 = g * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
-        const double p12 = ga * Gxy, p13 = ga * Gxz, p23 = ga * Gyz;
-        const double cx = gA * Gx, cy = gA * Gy, cz = gA * Gz;
+    if constexpr (EL == EL_CH) {
         R[0] = coeff_c * d[1];
-        R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
-        R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
-        R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
-        R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+        R[1] = kl * d[0];
         P[0][1] = coeff_c;
-        P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
-        P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
-        P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
-        P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
+        P[1][0] = kl;
     } else {
-        const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
-        const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy;
-        const double p11 = g * (k.C11 * Gxx + k.C66 * Gyy);
-        const double p22 = g * (k.C66 * Gxx + k.C11 * Gyy);
-        const double pxy = g * (k.C12 + k.C66) * Gxy;
-        const double cx = gA * Gx, cy = gA * Gy;
-        R[0] = coeff_c * d[1];
-        R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
-        R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
-        R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
-        P[0][1] = coeff_c;
-        P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
-        P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
-        P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+        const double *op = A.ops + (size_t)t * A.nop;
+        const double go = g * k.Omega, gA = -g * k.A;
+        if constexpr (DIM == 3) {
+            const double Gx = op[0], Gy = op[1], Gz = op[2];
+            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+            const double ga = g * (k.C12 + k.C66);
+            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy, cuz = go * k.Kh * Gz;
+            const double p11 = g * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
+            const double p22 = g * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
+            const double p33 = g * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
+            const double p12 = ga * Gxy, p13 = ga * Gxz, p23 = ga * Gyz;
+            const double cx = gA * Gx, cy = gA * Gy, cz = gA * Gz;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3] + cuz * d[4];
+            R[2] = cx * d[0] + p11 * d[2] + p12 * d[3] + p13 * d[4];
+            R[3] = cy * d[0] + p12 * d[2] + p22 * d[3] + p23 * d[4];
+            R[4] = cz * d[0] + p13 * d[2] + p23 * d[3] + p33 * d[4];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy; P[1][4] = cuz;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = p12; P[2][4] = p13;
+            P[3][0] = cy; P[3][2] = p12; P[3][3] = p22; P[3][4] = p23;
+            P[4][0] = cz; P[4][2] = p13; P[4][3] = p23; P[4][4] = p33;
+        } else {
+            const double Gx = op[0], Gy = op[1], Gxx = op[2], Gxy = op[3], Gyy = op[4];
+            const double cux = go * k.Kh * Gx, cuy = go * k.Kh * Gy;
+            const double p11 = g * (k.C11 * Gxx + k.C66 * Gyy);
+            const double p22 = g * (k.C66 * Gxx + k.C11 * Gyy);
+            const double pxy = g * (k.C12 + k.C66) * Gxy;
+            const double cx = gA * Gx, cy = gA * Gy;
+            R[0] = coeff_c * d[1];
+            R[1] = kl * d[0] + cux * d[2] + cuy * d[3];
+            R[2] = cx * d[0] + p11 * d[2] + pxy * d[3];
+            R[3] = cy * d[0] + pxy * d[2] + p22 * d[3];
+            P[0][1] = coeff_c;
+            P[1][0] = kl; P[1][2] = cux; P[1][3] = cuy;
+            P[2][0] = cx; P[2][2] = p11; P[2][3] = pxy;
+            P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
+        }
     }
 }
 
 // The nodal term (StressCahnHilliardElement.cpp:449-466,
-// FracStressCahnHilliardElement.cpp:528-561) of sum e: staged entry e < NK
-// of the diagonal block, residual component e - NK otherwise, as the
-// assembler leaves it before the bonds: 0 + (-K_node) and 0 + R_node.
-template <int DIM, bool FRAC>
-__device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<DIM> &nd, size_t ri, int ic, int e)
+// FracStressCahnHilliardElement.cpp:528-561, CahnHilliardElement.cpp:256-292)
+// of sum e: staged entry e < NK of the diagonal block, residual component
+// e - NK otherwise, as the assembler leaves it before the bonds:
+// 0 + (-K_node) and 0 + R_node.
+template <int DIM, int EL>
+__device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<el_nd<DIM, EL>()> &nd, size_t ri,
+                                            int ic, int e)
 {
-    constexpr int ND = DIM + 2;
+    constexpr int ND = el_nd<DIM, EL>();
     constexpr int NK = sch_nk<ND>();
     const FschConst &k = A.k;
     const double dt = A.dt;
     const double c = nd.ui[0], mu = nd.ui[1], cold = A.uold[ri];
     double fp, fpp;
-    if constexpr (FRAC) {
-        const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
-        fp = log(cc / (1.0 - cc)) + k.chi * (1.0 - 2.0 * cc);
-        fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * k.chi;
-    } else {
+    if constexpr (EL == EL_SCH) {
         double L, Lp;
         sch_reglog(c, L, Lp);
         fp = L + k.chi * (1.0 - 2.0 * c);
         fpp = Lp - 2.0 * k.chi;
+    } else {
+        const double cc = fmax(1.0e-8, fmin(1.0 - 1.0e-8, c));
+        fp = log(cc / (1.0 - cc)) + k.chi * (1.0 - 2.0 * cc);
+        fpp = 1.0 / (cc * (1.0 - cc)) - 2.0 * k.chi;
     }
-    const bool inertia = FRAC && k.rho > 0.0;
+    const bool inertia = EL == EL_FSCH && k.rho > 0.0;
     const double c0 = inertia ? k.rho / (dt * dt) : 0.0;
     if (e < NK) {
         int a, b;
         sch_entry<ND>(e, a, b);
         double kn = 0.0;
-        if (a == 1 && b == 0) kn = -fpp + k.Omega * k.Ch;
+        if (a == 1 && b == 0) kn = (EL == EL_CH) ? -fpp : -fpp + k.Omega * k.Ch;
         else if (inertia && a >= 2 && a == b) kn = 0.0 + -c0;
         return 0.0 + (-kn);
     }
     const int a = e - NK;
     if (a == 0) return 0.0 + (c - cold) / dt;
-    if (a == 1) return 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
+    if (a == 1) {
+        if constexpr (EL == EL_CH) return 0.0 + (mu - fp);
+        else return 0.0 + (mu - fp + k.Omega * k.Ch * (c - k.cref));
+    }
     if (inertia) {
         const double v = A.vel[(size_t)ic * DIM + a - 2];
         double ua = 0.0;                  // nd.ui[a] without a dynamic index
@@ -318,50 +339,55 @@ __device__ __forceinline__ double sch_nodal_only(int a, int b, double dt)
     return 0.0 + (-kn);
 }
 
-// ---- assembly: one wavefront per node, lanes over its bonds ---------------
+// ---- assembly: one node per LPN lanes of a wavefront, lanes over its bonds ----
 //
-// k_sch_rows (rows of at most MAXB node blocks): lane l computes bonds
-// l, l + 64, ... of the node and stages P's structural nonzeros and the
-// residual terms in LDS, with a map from the row's block positions to bonds.
-// One lane per diagonal-block entry / residual component then adds the
-// staged terms in family order onto the nodal term -- the reference's order,
-// so every value comes out bit for bit -- and finally the wave writes the
-// node's dim+2 CSR rows front to back, 64 consecutive values per store, the
-// K_IJ blocks as 0 + (-P) (FormResidualAndJacobian.cpp:119-128).
+// k_sch_rows (rows of at most MAXB node blocks): lane l of the node computes
+// bonds l, l + LPN, ... and stages P's structural nonzeros and the residual
+// terms in LDS, with a map from the row's block positions to bonds.  One
+// lane per diagonal-block entry / residual component then adds the staged
+// terms in family order onto the nodal term -- the reference's order, so
+// every value comes out bit for bit -- and finally the node's lanes write its
+// CSR rows front to back, LPN consecutive values per store, the K_IJ blocks
+// as 0 + (-P) (FormResidualAndJacobian.cpp:119-128).  2D families (28 bonds
+// at the usual horizon) take 32 lanes, two nodes per wavefront, and the
+// Cahn-Hilliard element's four sums 16 lanes, four nodes per wavefront.
 //
 // k_sch (any row length): the same sums, in chunks of 64 bonds, with each
 // lane writing its bond's K_IJ block straight into the rows.
 constexpr int kSchWaves = 4;   // nodes per block of k_sch
 
-template <int DIM, bool FRAC, int MAXB, int NW>
+template <int DIM, int EL, int MAXB, int NW, int LPN>
 __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
 {
-    constexpr int ND = DIM + 2;
+    constexpr int ND = el_nd<DIM, EL>();
     constexpr int NK = sch_nk<ND>();
     constexpr int NS = NK + ND;
-    __shared__ double s_v[NW][NS][MAXB + 1];
-    __shared__ double s_d[NW][ND * ND];
-    __shared__ int s_inv[NW][MAXB];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = blockIdx.x * NW + w;
+    constexpr int NPW = 64 / LPN;            // nodes per wavefront
+    constexpr int NB = NW * NPW;             // nodes per block
+    static_assert(NS + ND * ND <= LPN, "one lane per sum and per nodal-only entry");
+    __shared__ double s_v[NB][NS][MAXB + 1];
+    __shared__ double s_d[NB][ND * ND];
+    __shared__ int s_inv[NB][MAXB];
+    const int l = threadIdx.x % LPN, w = threadIdx.x / LPN;   // w: node slot in the block
+    const int i = blockIdx.x * NB + w;
     const bool active = i < A.n;
-    const int ic = active ? i : 0;   // reads of an idle wave stay in bounds
+    const int ic = active ? i : 0;   // reads of an idle slot stay in bounds
     const size_t ri = (size_t)ic * ND;
     const int r0 = A.rowptr[ri];
     const int rowlen = A.rowptr[ri + 1] - r0;
     int rowoff[ND];
 #pragma unroll
     for (int a = 0; a < ND; a++) rowoff[a] = A.rowptr[ri + a] - r0;
-    SchNode<DIM> nd;
+    SchNode<ND> nd;
 #pragma unroll
     for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
     nd.Mi = sch_mobility(nd.ui[0], A.k.D);
     nd.Vi = A.vol[ic];
     const int t0 = active ? A.off[i] : 0, deg = active ? A.off[i + 1] - t0 : 0;
-    for (int tl = lane; tl < deg; tl += 64) {
+    for (int tl = l; tl < deg; tl += LPN) {
         const int t = t0 + tl;
         double P[ND][ND], R[ND];
-        sch_bond<DIM, FRAC>(A, nd, t, P, R);
+        sch_bond<DIM, EL>(A, nd, t, P, R);
 #pragma unroll
         for (int e = 0; e < NK; e++) {
             int a, b;
@@ -372,24 +398,24 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
         for (int a = 0; a < ND; a++) s_v[w][NK + a][tl] = R[a];
         s_inv[w][(A.slot[t] - r0) / ND] = tl;
     }
-    if (active && lane == 0) {
+    if (active && l == 0) {
         s_inv[w][(A.diag[i] - r0) / ND] = -1;
         if (A.mslot[i] >= 0) s_inv[w][(A.mslot[i] - r0) / ND] = -2;
     }
     __syncthreads();
-    if (lane < NS) {
-        double acc = sch_nodal<DIM, FRAC>(A, nd, ri, ic, lane);
+    if (l < NS) {
+        double acc = sch_nodal<DIM, EL>(A, nd, ri, ic, l);
 #pragma unroll 8
-        for (int q = 0; q < deg; q++) acc += s_v[w][lane][q];   // in order; the loads pipeline
-        if (lane < NK) {
+        for (int q = 0; q < deg; q++) acc += s_v[w][l][q];   // in order; the loads pipeline
+        if (l < NK) {
             int a, b;
-            sch_entry<ND>(lane, a, b);
+            sch_entry<ND>(l, a, b);
             s_d[w][a * ND + b] = acc;
         } else if (active) {
-            A.rhs[ri + lane - NK] = acc;
+            A.rhs[ri + l - NK] = acc;
         }
-    } else if (lane < NS + ND * ND) {
-        const int q = lane - NS, a = q / ND, b = q % ND;
+    } else if (l < NS + ND * ND) {
+        const int q = l - NS, a = q / ND, b = q % ND;
         if (!sch_staged<ND>(a, b)) s_d[w][q] = sch_nodal_only<ND>(a, b, A.dt);
     }
     __syncthreads();
@@ -397,7 +423,7 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
 #pragma unroll
     for (int a = 0; a < ND; a++) {
         double *row = A.kvals + r0 + rowoff[a];
-        for (int q = lane; q < rowlen; q += 64) {
+        for (int q = l; q < rowlen; q += LPN) {
             const int kb = q / ND, b = q - kb * ND;
             const int src = s_inv[w][kb];
             double v = 0.0;                               // mirror block, structural zeros
@@ -411,10 +437,10 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
     }
 }
 
-template <int DIM, bool FRAC>
+template <int DIM, int EL>
 __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
 {
-    constexpr int ND = DIM + 2;
+    constexpr int ND = el_nd<DIM, EL>();
     constexpr int NK = sch_nk<ND>();
     constexpr int NS = NK + ND;
     __shared__ double s_v[kSchWaves][NS][65];
@@ -433,7 +459,7 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
     int rowoff[ND];
 #pragma unroll
     for (int a = 0; a < ND; a++) rowoff[a] = A.rowptr[ri + a] - rb;
-    SchNode<DIM> nd;
+    SchNode<ND> nd;
 #pragma unroll
     for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
     nd.Mi = sch_mobility(nd.ui[0], A.k.D);
@@ -443,12 +469,12 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
         // the ghost's mirror block is in the pattern but receives no term
         A.kvals[A.mslot[i] + rowoff[lane / ND] + lane % ND] = 0.0;
     }
-    double acc = (lane < NS) ? sch_nodal<DIM, FRAC>(A, nd, ri, ic, lane) : 0.0;
+    double acc = (lane < NS) ? sch_nodal<DIM, EL>(A, nd, ri, ic, lane) : 0.0;
     for (int ch = 0; ch < nch; ch++) {
         const int t = t0 + ch * 64 + lane;
         if (t < t1) {
             double P[ND][ND], R[ND];
-            sch_bond<DIM, FRAC>(A, nd, t, P, R);
+            sch_bond<DIM, EL>(A, nd, t, P, R);
             const int s0 = A.slot[t];
 #pragma unroll
             for (int a = 0; a < ND; a++)
@@ -486,21 +512,30 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
     }
 }
 
-template <int DIM, bool FRAC>
+template <int DIM, int EL>
 int launch_sch(hipStream_t st, int maxblk, const SchArgs &A)
 {
     const int n = A.n;
-    if constexpr (DIM == 2) {
-        if (maxblk <= 64) {
-            hipLaunchKernelGGL((k_sch_rows<DIM, FRAC, 64, 4>), dim3(grids(n, 4)), dim3(256), 0, st, A);
+    if constexpr (EL == EL_CH) {
+        // four sums per node: 16 lanes, four nodes per wavefront
+        if (maxblk <= 32) {
+            hipLaunchKernelGGL((k_sch_rows<DIM, EL, 32, 4, 16>), dim3(grids(n, 16)), dim3(256), 0, st, A);
+            PXG_HIP(hipGetLastError());
+            return PXG_OK;
+        }
+    } else if constexpr (DIM == 2) {
+        if (maxblk <= 32) {
+            hipLaunchKernelGGL((k_sch_rows<DIM, EL, 32, 4, 32>), dim3(grids(n, 8)), dim3(256), 0, st, A);
             PXG_HIP(hipGetLastError());
             return PXG_OK;
         }
     }
-    if (maxblk <= 128)
-        hipLaunchKernelGGL((k_sch_rows<DIM, FRAC, 128, 2>), dim3(grids(n, 2)), dim3(128), 0, st, A);
+    if (maxblk <= 64)
+        hipLaunchKernelGGL((k_sch_rows<DIM, EL, 64, 4, 64>), dim3(grids(n, 4)), dim3(256), 0, st, A);
+    else if (maxblk <= 128)
+        hipLaunchKernelGGL((k_sch_rows<DIM, EL, 128, 2, 64>), dim3(grids(n, 2)), dim3(128), 0, st, A);
     else
-        hipLaunchKernelGGL((k_sch<DIM, FRAC>), dim3(grids(n, kSchWaves)), dim3(64 * kSchWaves), 0, st, A);
+        hipLaunchKernelGGL((k_sch<DIM, EL>), dim3(grids(n, kSchWaves)), dim3(64 * kSchWaves), 0, st, A);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
@@ -699,8 +734,8 @@ extern "C" int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *
                     FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], 0.0, 1.0}, dt,
                     nullptr, nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
                     kvals_dev, rhs_dev};
-    return ctx->dim == 3 ? launch_sch<3, false>(ctx->stream, ctx->csr_maxblk, A)
-                         : launch_sch<2, false>(ctx->stream, ctx->csr_maxblk, A);
+    return ctx->dim == 3 ? launch_sch<3, EL_SCH>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, EL_SCH>(ctx->stream, ctx->csr_maxblk, A);
 }
 
 extern "C" int pxg_assemble_stressch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
@@ -832,8 +867,8 @@ extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_de
                     FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12]}, dt,
                     F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
                     kvals_dev, rhs_dev};
-    return ctx->dim == 3 ? launch_sch<3, true>(ctx->stream, ctx->csr_maxblk, A)
-                         : launch_sch<2, true>(ctx->stream, ctx->csr_maxblk, A);
+    return ctx->dim == 3 ? launch_sch<3, EL_FSCH>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, EL_FSCH>(ctx->stream, ctx->csr_maxblk, A);
 }
 
 extern "C" int pxg_assemble_fsch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
@@ -955,4 +990,19 @@ extern "C" int pxg_fsch_project(pxg_ctx *ctx, int which, const double *u, double
               "pxg_fsch_project: bad arguments (which: strain, stress, vonMises, sigmaH, gradC, damage, DamageI)");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_project: injected fault");
     return sch_project_host(ctx, true, which, u, out);
+}
+
+// The Cahn-Hilliard element's assembly on the same kernels (dofs c, mu):
+// M(c) = M0 c (1-c) clamped, the clamped logarithm, lambda-corrected
+// ghost-excluded Laplacian (CahnHilliardElement.cpp:93-292).
+int pxg::launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, double dt, const double *u_dev,
+                        const double *uold_dev, double *kvals_dev, double *rhs_dev)
+{
+    const SchArgs A{ctx->n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p,
+                    ctx->vols.p,
+                    FschConst{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, M0, kappa, chi, 0.0, 0.0, 0.0, 1.0}, dt, nullptr,
+                    nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
+                    rhs_dev};
+    return ctx->dim == 3 ? launch_sch<3, EL_CH>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, EL_CH>(ctx->stream, ctx->csr_maxblk, A);
 }
