@@ -5,11 +5,12 @@
 // (PDDODynamicFracElement.cpp:207-297 through the assembler of
 // FormResidualAndJacobian.cpp:25-132).
 //
-// The assembly runs one wavefront per node (lanes over its bonds); every sum
-// of the node's diagonal block and residual is taken in the reference's order
-// (nodal term first, then the bonds in family order), so K and the residual
-// come out bit for bit.  The commit (one thread per node) reads u_old of the
-// neighbours and writes only the node's own velocity, history and bonds.
+// The assembly runs on the block-element kernels of pxg_stressch.hip
+// (launch_dyn_rows: lanes over a node's bonds, every sum of the node's
+// diagonal block and residual taken in the reference's order, so K and the
+// residual come out bit for bit).  The commit (one thread per node) reads
+// u_old of the neighbours and writes only the node's own velocity, history
+// and bonds.
 #include <algorithm>
 #include <vector>
 #include "pxg_internal.h"
@@ -54,119 +55,6 @@ __global__ void k_dynfrac_commit(int n, const double *coords, const int *off, co
         const double s = (yr - r) / r;
         if (tension_only ? (s > sc) : (fabs(s) > sc)) health[t] = 0;
     }
-}
-
-// Assembly, one wavefront per node with lanes over its bonds: lane l
-// computes bond off[i] + 64c + l, writes its K_IJ block into the node's CSR
-// rows (the 64 lanes write into the same DIM rows, so the stores coalesce)
-// and stages its contributions to the diagonal block and the residual in
-// LDS; one lane per diagonal-block entry / residual component then adds them
-// in family order onto the nodal term, the reference's order.
-constexpr int kDynWaves = 4;   // nodes per block
-
-template <int DIM>
-__global__ void __launch_bounds__(64 * kDynWaves)
-k_dynfrac_assemble(int n, int nop, const int *off, const int *nbr, const double *ops, const int *rowptr,
-                   const int *slot, const int *diag, const int *mslot, double C11, double C12, double C66,
-                   double rho, double b0, double b1, double b2, double dt, const double *vel,
-                   const unsigned char *health, const double *u, const double *uold, double *kvals, double *rhs)
-{
-    constexpr int NK = DIM * DIM, NS = NK + DIM;
-    __shared__ double s_v[kDynWaves][NS][65];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i0 = blockIdx.x * kDynWaves;
-    const int i = i0 + w;
-    const bool active = i < n;
-    const int ic = active ? i : 0;   // reads of an idle wave stay in bounds
-    int nch = 0;
-#pragma unroll
-    for (int q = 0; q < kDynWaves; q++)
-        if (i0 + q < n) nch = max(nch, (off[i0 + q + 1] - off[i0 + q] + 63) >> 6);
-    const size_t ri = (size_t)ic * DIM;
-    const int rb = rowptr[ri];
-    int rowoff[DIM];                          // row a starts at rowptr[ri + a]
-#pragma unroll
-    for (int a = 0; a < DIM; a++) rowoff[a] = rowptr[ri + a] - rb;
-    if (active && lane < NK && mslot[i] >= 0) kvals[mslot[i] + rowoff[lane / DIM] + lane % DIM] = 0.0;
-    double ui[DIM];
-#pragma unroll
-    for (int a = 0; a < DIM; a++) ui[a] = u[ri + a];
-    // nodal: R = b - c0 (u - uold - v dt), K = -c0 I (PDDODynamicFracElement.cpp:276-297)
-    const double c0 = rho / (dt * dt);
-    double acc = 0.0;
-    if (lane < NK) {
-        const int a = lane / DIM, c = lane % DIM;
-        acc = 0.0 + (-((a == c) ? -c0 : 0.0));
-    } else if (lane < NS) {
-        const int a = lane - NK;
-        const double bf = (a == 0) ? b0 : (a == 1) ? b1 : b2;
-        acc = 0.0 + (bf - c0 * (ui[a] - uold[ri + a] - vel[ri + a] * dt));
-    }
-    const int t0 = active ? off[i] : 0, t1 = active ? off[i + 1] : 0;
-    for (int ch = 0; ch < nch; ch++) {
-        const int t = t0 + ch * 64 + lane;
-        if (t < t1) {
-            const size_t rj = (size_t)(nbr[t] - 1) * DIM;
-            const double *op = ops + (size_t)t * nop;
-            double R[DIM], P[DIM][DIM];
-#pragma unroll
-            for (int a = 0; a < DIM; a++) {
-                R[a] = 0.0;
-#pragma unroll
-                for (int c = 0; c < DIM; c++) P[a][c] = 0.0;
-            }
-            const bool intact = health[t] != 0;
-            if (intact) {
-                const double mu = 1.0;
-                if constexpr (DIM == 3) {
-                    const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
-                    const double aa = C12 + C66;
-                    const double p11 = mu * (C11 * Gxx + C66 * (Gyy + Gzz));
-                    const double p22 = mu * (C11 * Gyy + C66 * (Gxx + Gzz));
-                    const double p33 = mu * (C11 * Gzz + C66 * (Gxx + Gyy));
-                    const double p12 = mu * aa * Gxy, p13 = mu * aa * Gxz, p23 = mu * aa * Gyz;
-                    const double dux = u[rj] - ui[0], duy = u[rj + 1] - ui[1], duz = u[rj + 2] - ui[2];
-                    R[0] = p11 * dux + p12 * duy + p13 * duz;
-                    R[1] = p12 * dux + p22 * duy + p23 * duz;
-                    R[2] = p13 * dux + p23 * duy + p33 * duz;
-                    P[0][0] = p11; P[0][1] = p12; P[0][2] = p13;
-                    P[1][0] = p12; P[1][1] = p22; P[1][2] = p23;
-                    P[2][0] = p13; P[2][1] = p23; P[2][2] = p33;
-                } else {
-                    const double Gxx = op[2], Gxy = op[3], Gyy = op[4];
-                    const double p11 = mu * (C11 * Gxx + C66 * Gyy);
-                    const double p22 = mu * (C66 * Gxx + C11 * Gyy);
-                    const double pxy = mu * (C12 + C66) * Gxy;
-                    const double dux = u[rj] - ui[0], duy = u[rj + 1] - ui[1];
-                    R[0] = p11 * dux + pxy * duy;
-                    R[1] = pxy * dux + p22 * duy;
-                    P[0][0] = p11; P[0][1] = pxy; P[1][0] = pxy; P[1][1] = p22;
-                }
-            }
-            // scatter (FormResidualAndJacobian.cpp:119-128): K_II = -P, K_IJ = P;
-            // a broken bond's local blocks are zero
-            const int s0 = slot[t];
-#pragma unroll
-            for (int a = 0; a < DIM; a++) {
-#pragma unroll
-                for (int c = 0; c < DIM; c++) {
-                    kvals[s0 + rowoff[a] + c] = 0.0 + (-(intact ? P[a][c] : 0.0));
-                    s_v[w][a * DIM + c][lane] = intact ? P[a][c] : 0.0;
-                }
-                s_v[w][NK + a][lane] = R[a];
-            }
-        }
-        __syncthreads();
-        if (lane < NS) {
-            const int m = min(64, t1 - (t0 + ch * 64));
-#pragma unroll 8
-            for (int q = 0; q < m; q++) acc += s_v[w][lane][q];   // in order; the loads pipeline
-        }
-        __syncthreads();
-    }
-    if (!active) return;
-    if (lane < NK) kvals[diag[i] + rowoff[lane / DIM] + lane % DIM] = acc;
-    else if (lane < NS) rhs[ri + lane - NK] = acc;
 }
 
 __global__ void k_u8_from_seed(long long nb, const unsigned char *seed, unsigned char *h)
@@ -259,20 +147,7 @@ extern "C" int pxg_assemble_dynfrac_dev(pxg_ctx *ctx, double dt, const double *u
               "pxg_assemble_dynfrac: bad arguments (transient element, dt > 0)");
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_dynfrac: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
-    const DynFracState &D = ctx->dyn;
-    const unsigned g = gridd(ctx->n, kDynWaves);
-    if (ctx->dim == 3)
-        hipLaunchKernelGGL(k_dynfrac_assemble<3>, dim3(g), dim3(64 * kDynWaves), 0, ctx->stream, ctx->n, ctx->nop,
-                           ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p,
-                           ctx->mslot.p, D.C11, D.C12, D.C66, D.rho, D.b[0], D.b[1], D.b[2], dt, D.vel.p,
-                           D.health.p, u_dev, uold_dev, kvals_dev, rhs_dev);
-    else
-        hipLaunchKernelGGL(k_dynfrac_assemble<2>, dim3(g), dim3(64 * kDynWaves), 0, ctx->stream, ctx->n, ctx->nop,
-                           ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p,
-                           ctx->mslot.p, D.C11, D.C12, D.C66, D.rho, D.b[0], D.b[1], D.b[2], dt, D.vel.p,
-                           D.health.p, u_dev, uold_dev, kvals_dev, rhs_dev);
-    PXG_HIP(hipGetLastError());
-    return PXG_OK;
+    return launch_dyn_rows(ctx, dt, u_dev, uold_dev, kvals_dev, rhs_dev);
 }
 
 extern "C" int pxg_assemble_dynfrac(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,

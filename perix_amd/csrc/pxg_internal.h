@@ -235,6 +235,9 @@ int ch_geometry(pxg_ctx *ctx);
 // the Cahn-Hilliard assembly on the block-element kernels (pxg_stressch.hip)
 int launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, double dt, const double *u_dev,
                    const double *uold_dev, double *kvals_dev, double *rhs_dev);
+// the dynamic fracture assembly on the block-element kernels (pxg_stressch.hip)
+int launch_dyn_rows(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev, double *kvals_dev,
+                    double *rhs_dev);
 
 } // namespace pxg
 

@@ -1,5 +1,7 @@
 // pxg_stressch.hip -- the stress-coupled Cahn-Hilliard elements
-// (StressCahnHilliardElement, FracStressCahnHilliardElement) on the device: CSR assembly of the
+// (StressCahnHilliardElement, FracStressCahnHilliardElement) on the device,
+// and the block-element assembly kernels they share with the Cahn-Hilliard
+// and dynamic fracture elements: CSR assembly of the
 // (c, mu, u) system with the conservative degenerate-mobility species flux,
 // the lambda-corrected ghost-excluded gradient-energy Laplacian, the
 // chemo-mechanical coupling and the PDDO Navier operator
@@ -142,20 +144,21 @@ struct SchArgs {
     const double *vol;
     FschConst k;
     double dt;
-    const double *vel;               // fracture variant: committed velocity
-    const unsigned char *health;     // fracture variant: bond health
+    const double *vel;               // fracture / dynamic: committed velocity
+    const unsigned char *health;     // fracture / dynamic: bond health
     const int *rowptr, *slot, *diag, *mslot;
     const double *u, *uold;
     double *kvals, *rhs;
+    double b[3];                     // dynamic fracture: body force
 };
 
 // Element kinds sharing the block assembly: the stress-coupled
 // Cahn-Hilliard element, its fracture variant, and the plain Cahn-Hilliard
 // element (the species/chemical-potential part alone, dofs c and mu).
-enum { EL_SCH = 0, EL_FSCH = 1, EL_CH = 2 };
+enum { EL_SCH = 0, EL_FSCH = 1, EL_CH = 2, EL_DYN = 3 };
 
 template <int DIM, int EL>
-__device__ __forceinline__ constexpr int el_nd() { return EL == EL_CH ? 2 : DIM + 2; }
+__device__ __forceinline__ constexpr int el_nd() { return EL == EL_CH ? 2 : EL == EL_DYN ? DIM : DIM + 2; }
 
 // The structural nonzeros of a bond's dR/du_j block P, in staging order:
 // (0,1), then for rows a = 1..ND-1 the columns 0, 2, 3, ..., ND-1.  They are
@@ -179,6 +182,35 @@ __device__ __forceinline__ bool sch_staged(int a, int b) { return (a == 0 && b =
 template <int ND>
 __device__ __forceinline__ int sch_index(int a, int b) { return (a == 0) ? 0 : 1 + (a - 1) * (ND - 1) + (b == 0 ? 0 : b - 1); }
 
+// per element: the dynamic fracture element's bond blocks are full (the
+// symmetric Navier block), so every diagonal-block entry is staged, row-major
+template <int DIM, int EL>
+__device__ __forceinline__ constexpr int el_nk()
+{
+    return EL == EL_DYN ? DIM * DIM : sch_nk<el_nd<DIM, EL>()>();
+}
+
+template <int DIM, int EL>
+__device__ __forceinline__ void el_entry(int e, int &a, int &b)
+{
+    if constexpr (EL == EL_DYN) { a = e / DIM; b = e % DIM; }
+    else sch_entry<el_nd<DIM, EL>()>(e, a, b);
+}
+
+template <int DIM, int EL>
+__device__ __forceinline__ bool el_staged(int a, int b)
+{
+    if constexpr (EL == EL_DYN) return true;
+    else return sch_staged<el_nd<DIM, EL>()>(a, b);
+}
+
+template <int DIM, int EL>
+__device__ __forceinline__ int el_index(int a, int b)
+{
+    if constexpr (EL == EL_DYN) return a * DIM + b;
+    else return sch_index<el_nd<DIM, EL>()>(a, b);
+}
+
 // Node-level quantities of node i (the row owner) used by its bonds.
 template <int ND>
 struct SchNode {
@@ -200,6 +232,44 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
     const FschConst &k = A.k;
     const int j = A.nbr[t] - 1;
     const size_t rj = (size_t)j * ND;
+    if constexpr (EL == EL_DYN) {
+        // PDDODynamicFracElement.cpp:207-274: the health-gated Navier block;
+        // a broken bond's blocks are zero
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            R[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < ND; b++) P[a][b] = 0.0;
+        }
+        if (!A.health[t]) return;
+        const double *op = A.ops + (size_t)t * A.nop;
+        const double mu = 1.0;
+        if constexpr (DIM == 3) {
+            const double Gxx = op[3], Gxy = op[4], Gxz = op[5], Gyy = op[6], Gyz = op[7], Gzz = op[8];
+            const double aa = k.C12 + k.C66;
+            const double p11 = mu * (k.C11 * Gxx + k.C66 * (Gyy + Gzz));
+            const double p22 = mu * (k.C11 * Gyy + k.C66 * (Gxx + Gzz));
+            const double p33 = mu * (k.C11 * Gzz + k.C66 * (Gxx + Gyy));
+            const double p12 = mu * aa * Gxy, p13 = mu * aa * Gxz, p23 = mu * aa * Gyz;
+            const double dux = A.u[rj] - nd.ui[0], duy = A.u[rj + 1] - nd.ui[1], duz = A.u[rj + 2] - nd.ui[2];
+            R[0] = p11 * dux + p12 * duy + p13 * duz;
+            R[1] = p12 * dux + p22 * duy + p23 * duz;
+            R[2] = p13 * dux + p23 * duy + p33 * duz;
+            P[0][0] = p11; P[0][1] = p12; P[0][2] = p13;
+            P[1][0] = p12; P[1][1] = p22; P[1][2] = p23;
+            P[2][0] = p13; P[2][1] = p23; P[2][2] = p33;
+        } else {
+            const double Gxx = op[2], Gxy = op[3], Gyy = op[4];
+            const double p11 = mu * (k.C11 * Gxx + k.C66 * Gyy);
+            const double p22 = mu * (k.C66 * Gxx + k.C11 * Gyy);
+            const double pxy = mu * (k.C12 + k.C66) * Gxy;
+            const double dux = A.u[rj] - nd.ui[0], duy = A.u[rj + 1] - nd.ui[1];
+            R[0] = p11 * dux + pxy * duy;
+            R[1] = pxy * dux + p22 * duy;
+            P[0][0] = p11; P[0][1] = pxy; P[1][0] = pxy; P[1][1] = p22;
+        }
+        return;
+    } else {
     const bool ghostJ = A.isghost[j] != 0;
     double g = 1.0, coeff_c = 0.0, kl = 0.0;
     if constexpr (EL == EL_FSCH) {
@@ -277,6 +347,7 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
             P[3][0] = cy; P[3][2] = pxy; P[3][3] = p22;
         }
     }
+    }
 }
 
 // The nodal term (StressCahnHilliardElement.cpp:449-466,
@@ -289,9 +360,23 @@ __device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<el_n
                                             int ic, int e)
 {
     constexpr int ND = el_nd<DIM, EL>();
-    constexpr int NK = sch_nk<ND>();
+    constexpr int NK = el_nk<DIM, EL>();
     const FschConst &k = A.k;
     const double dt = A.dt;
+    if constexpr (EL == EL_DYN) {
+        // R = b - c0 (u - uold - v dt), K = -c0 I (PDDODynamicFracElement.cpp:276-297)
+        const double c0 = k.rho / (dt * dt);
+        if (e < NK) {
+            const int a = e / DIM, b = e % DIM;
+            return 0.0 + (-((a == b) ? -c0 : 0.0));
+        }
+        const int a = e - NK;
+        double ua = 0.0, bf = 0.0;        // nd.ui[a], b[a] without a dynamic index
+#pragma unroll
+        for (int q = 0; q < ND; q++)
+            if (q == a) { ua = nd.ui[q]; bf = A.b[q]; }
+        return 0.0 + (bf - c0 * (ua - A.uold[ri + a] - A.vel[ri + a] * dt));
+    } else {
     const double c = nd.ui[0], mu = nd.ui[1], cold = A.uold[ri];
     double fp, fpp;
     if constexpr (EL == EL_SCH) {
@@ -308,7 +393,7 @@ __device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<el_n
     const double c0 = inertia ? k.rho / (dt * dt) : 0.0;
     if (e < NK) {
         int a, b;
-        sch_entry<ND>(e, a, b);
+        el_entry<DIM, EL>(e, a, b);
         double kn = 0.0;
         if (a == 1 && b == 0) kn = (EL == EL_CH) ? -fpp : -fpp + k.Omega * k.Ch;
         else if (inertia && a >= 2 && a == b) kn = 0.0 + -c0;
@@ -329,6 +414,7 @@ __device__ __forceinline__ double sch_nodal(const SchArgs &A, const SchNode<el_n
         return 0.0 + (0.0 + -c0 * (ua - A.uold[ri + a] - v * dt));
     }
     return 0.0;
+    }
 }
 
 // the diagonal-block entries without bond terms: 0 + (-K_node)
@@ -360,7 +446,7 @@ template <int DIM, int EL, int MAXB, int NW, int LPN>
 __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
 {
     constexpr int ND = el_nd<DIM, EL>();
-    constexpr int NK = sch_nk<ND>();
+    constexpr int NK = el_nk<DIM, EL>();
     constexpr int NS = NK + ND;
     constexpr int NPW = 64 / LPN;            // nodes per wavefront
     constexpr int NB = NW * NPW;             // nodes per block
@@ -391,7 +477,7 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
 #pragma unroll
         for (int e = 0; e < NK; e++) {
             int a, b;
-            sch_entry<ND>(e, a, b);
+            el_entry<DIM, EL>(e, a, b);
             s_v[w][e][tl] = P[a][b];
         }
 #pragma unroll
@@ -409,14 +495,14 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
         for (int q = 0; q < deg; q++) acc += s_v[w][l][q];   // in order; the loads pipeline
         if (l < NK) {
             int a, b;
-            sch_entry<ND>(l, a, b);
+            el_entry<DIM, EL>(l, a, b);
             s_d[w][a * ND + b] = acc;
         } else if (active) {
             A.rhs[ri + l - NK] = acc;
         }
     } else if (l < NS + ND * ND) {
         const int q = l - NS, a = q / ND, b = q % ND;
-        if (!sch_staged<ND>(a, b)) s_d[w][q] = sch_nodal_only<ND>(a, b, A.dt);
+        if (!el_staged<DIM, EL>(a, b)) s_d[w][q] = sch_nodal_only<ND>(a, b, A.dt);
     }
     __syncthreads();
     if (!active) return;
@@ -428,7 +514,7 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
             const int src = s_inv[w][kb];
             double v = 0.0;                               // mirror block, structural zeros
             if (src >= 0) {
-                if (sch_staged<ND>(a, b)) v = 0.0 + (-s_v[w][sch_index<ND>(a, b)][src]);
+                if (el_staged<DIM, EL>(a, b)) v = 0.0 + (-s_v[w][el_index<DIM, EL>(a, b)][src]);
             } else if (src == -1) {
                 v = s_d[w][a * ND + b];
             }
@@ -441,7 +527,7 @@ template <int DIM, int EL>
 __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
 {
     constexpr int ND = el_nd<DIM, EL>();
-    constexpr int NK = sch_nk<ND>();
+    constexpr int NK = el_nk<DIM, EL>();
     constexpr int NS = NK + ND;
     __shared__ double s_v[kSchWaves][NS][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -483,7 +569,7 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
 #pragma unroll
             for (int e = 0; e < NK; e++) {
                 int a, b;
-                sch_entry<ND>(e, a, b);
+                el_entry<DIM, EL>(e, a, b);
                 s_v[w][e][lane] = P[a][b];
             }
 #pragma unroll
@@ -501,14 +587,14 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
     const int d0 = A.diag[i];
     if (lane < NK) {
         int a, b;
-        sch_entry<ND>(lane, a, b);
+        el_entry<DIM, EL>(lane, a, b);
         A.kvals[d0 + rowoff[a] + b] = acc;
     } else if (lane < NS) {
         A.rhs[ri + lane - NK] = acc;
     }
     if (lane < ND * ND) {
         const int a = lane / ND, b = lane % ND;
-        if (!sch_staged<ND>(a, b)) A.kvals[d0 + rowoff[a] + b] = sch_nodal_only<ND>(a, b, A.dt);
+        if (!el_staged<DIM, EL>(a, b)) A.kvals[d0 + rowoff[a] + b] = sch_nodal_only<ND>(a, b, A.dt);
     }
 }
 
@@ -516,8 +602,8 @@ template <int DIM, int EL>
 int launch_sch(hipStream_t st, int maxblk, const SchArgs &A)
 {
     const int n = A.n;
-    if constexpr (EL == EL_CH) {
-        // four sums per node: 16 lanes, four nodes per wavefront
+    if constexpr (EL == EL_CH || (EL == EL_DYN && DIM == 2)) {
+        // four / six sums per node: 16 lanes, four nodes per wavefront
         if (maxblk <= 32) {
             hipLaunchKernelGGL((k_sch_rows<DIM, EL, 32, 4, 16>), dim3(grids(n, 16)), dim3(256), 0, st, A);
             PXG_HIP(hipGetLastError());
@@ -733,7 +819,7 @@ extern "C" int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *
                     ctx->vols.p,
                     FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], 0.0, 1.0}, dt,
                     nullptr, nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
-                    kvals_dev, rhs_dev};
+                    kvals_dev, rhs_dev, {0.0, 0.0, 0.0}};
     return ctx->dim == 3 ? launch_sch<3, EL_SCH>(ctx->stream, ctx->csr_maxblk, A)
                          : launch_sch<2, EL_SCH>(ctx->stream, ctx->csr_maxblk, A);
 }
@@ -866,7 +952,7 @@ extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_de
                     ctx->vols.p,
                     FschConst{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12]}, dt,
                     F.vel.p, F.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
-                    kvals_dev, rhs_dev};
+                    kvals_dev, rhs_dev, {0.0, 0.0, 0.0}};
     return ctx->dim == 3 ? launch_sch<3, EL_FSCH>(ctx->stream, ctx->csr_maxblk, A)
                          : launch_sch<2, EL_FSCH>(ctx->stream, ctx->csr_maxblk, A);
 }
@@ -1002,7 +1088,20 @@ int pxg::launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, doubl
                     ctx->vols.p,
                     FschConst{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, M0, kappa, chi, 0.0, 0.0, 0.0, 1.0}, dt, nullptr,
                     nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
-                    rhs_dev};
+                    rhs_dev, {0.0, 0.0, 0.0}};
     return ctx->dim == 3 ? launch_sch<3, EL_CH>(ctx->stream, ctx->csr_maxblk, A)
                          : launch_sch<2, EL_CH>(ctx->stream, ctx->csr_maxblk, A);
+}
+
+// The dynamic fracture element's assembly on the same kernels (dofs u).
+int pxg::launch_dyn_rows(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev, double *kvals_dev,
+                         double *rhs_dev)
+{
+    const DynFracState &D = ctx->dyn;
+    const SchArgs A{ctx->n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, nullptr, nullptr, nullptr, ctx->vols.p,
+                    FschConst{D.C11, D.C12, D.C66, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, D.rho, 1.0}, dt,
+                    D.vel.p, D.health.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev,
+                    kvals_dev, rhs_dev, {D.b[0], D.b[1], D.b[2]}};
+    return ctx->dim == 3 ? launch_sch<3, EL_DYN>(ctx->stream, ctx->csr_maxblk, A)
+                         : launch_sch<2, EL_DYN>(ctx->stream, ctx->csr_maxblk, A);
 }
