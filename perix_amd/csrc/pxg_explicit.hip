@@ -357,12 +357,12 @@ int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break
     return PXG_OK;
 }
 
-// FP32C with the tiled stream layout: nodes of one family order pattern
+// FP32C with the tiled 3D stream layout: nodes of one family order pattern
 // whose operators agree to kShareTol read one representative's FP32
 // operator run (F.pofs).  On a structured lattice most interior nodes fall
 // into a few hundred patterns, and their operators differ from the
-// representative's only by the rounding of the coordinates (about 1e-14
-// relative), far below the FP32 rounding of the operators themselves.
+// representative's only through the rounding of the coordinates, far below
+// the FP32 rounding of the operators themselves.
 constexpr double kShareTol = 1e-12;
 
 static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
@@ -371,7 +371,13 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     F.pofs.release();
     F.nshared = 0;
     F.nclasses = 0;
-    if (!F.bsh.p || F.precision != PXG_FP32C) return PXG_OK;
+    // 3D only.  A shared run applies one FP32 rounding of the class's
+    // operators to every member, where the members' own roundings scatter
+    // (their FP64 operators differ by about 1e-9 relative), so sharing turns
+    // incoherent rounding errors into a coherent drift.  On the 2D plate that
+    // drift reached 5e-6 by step 1500 and moved the crack's branching; the
+    // latency-bound 2D step gains little from it.
+    if (!F.bsh.p || F.precision != PXG_FP32C || ctx->dim != 3) return PXG_OK;
     const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
     hipStream_t st = ctx->stream;
     dbuf<double> P;                     // FP64 operators, CSR order
