@@ -309,6 +309,42 @@ def block_case(problems, pxg, steps=100):
     return out
 
 
+def block_long_run(problems, pxg, steps=1000, chunk=100):
+    """FP32C against FP64 over the block's full 1,000-step run, every 100
+    steps, beside a control: FP64 against FP64 started 1e-15 m away (random
+    per-component perturbation of u0).  From about step 400 the block
+    fragments under the impactor and the trajectories become chaotic, so the
+    control bounds what any pointwise comparison can show there."""
+    runs = {}
+    for name, prec, eps in (("fp64", pxg.FP64, 0.0), ("fp32c", pxg.FP32C, 0.0), ("fp64_perturbed", pxg.FP64, 1e-15)):
+        s = problems.block_160x80x40(precision=prec, nsteps=steps)
+        u0 = s.u0.copy()
+        if eps:
+            u0 += eps * np.random.RandomState(1).standard_normal(u0.size)
+        s.ctx.frac_init(u0, s.dt)
+        runs[name] = s
+    rows = []
+    for c in range(steps // chunk):
+        st = {}
+        for name, s in runs.items():
+            s.ctx.frac_run(s.dt, s.total, nsteps_max=chunk)
+            u, _, h, _ = s.ctx.frac_state()
+            st[name] = (u, h)
+        u64, h64 = st["fp64"]
+        nrm = max(np.linalg.norm(u64), 1e-300)
+        row = {"step": (c + 1) * chunk, "broken_fp64": int((h64 == 0).sum()), "max_abs_u_fp64": float(np.abs(u64).max())}
+        for name in ("fp32c", "fp64_perturbed"):
+            u, h = st[name]
+            row[name + "_rel_l2"] = float(np.linalg.norm(u - u64) / nrm)
+            row[name + "_broken_bond_mismatch"] = int((h != h64).sum())
+        rows.append(row)
+        log("block long run", row)
+    for s in runs.values():
+        s.ctx.close()
+    return {"workload": "160x80x40 block, 1000 steps, FP32C and a 1e-15 m perturbed FP64 run against FP64",
+            "chunks": rows}
+
+
 def assembly_case(problems, pxg, reps=20):
     """Poisson and diffusion (100x100), dynamic fracture (400x200 plate),
     Cahn-Hilliard and stress-coupled Cahn-Hilliard (256x256, one Picard
@@ -706,6 +742,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and 3D block figures")
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
     ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
+    ap.add_argument("--long-run", action="store_true", help="with --only-block: the 1000-step comparison")
     ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
     ap.add_argument("--only-assembly", action="store_true", help="CSR assembly only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -728,7 +765,10 @@ def main():
 
     K, W = args.steps, args.warmup
     if args.only_block:
-        print(json.dumps({"block3d": block_case(problems, pxg, steps=K)}))
+        blk = block_case(problems, pxg, steps=K)
+        if args.long_run:
+            blk["long_run"] = block_long_run(problems, pxg)
+        print(json.dumps({"block3d": blk}))
         return 0
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_case(problems, pxg),
@@ -813,7 +853,9 @@ def main():
         out["adjoint_block3d_body_force"] = block_adjoint_case(problems, pxg)
         log("3D block (160x80x40)")
         out["block3d"] = block_case(problems, pxg)
-        log("assembly (Poisson 100x100, Cahn-Hilliard 256x256)")
+        log("3D block, 1000-step comparison")
+        out["block3d"]["long_run"] = block_long_run(problems, pxg)
+        log("assembly (Poisson, diffusion, dynamic fracture, Cahn-Hilliard, stress-coupled Cahn-Hilliard)")
         out["assembly"] = assembly_case(problems, pxg)
     print(json.dumps(out))
     return 0
