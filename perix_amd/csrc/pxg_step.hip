@@ -802,8 +802,9 @@ int build_step_gather(pxg_ctx *ctx)
     }
     if (!staged) {
         // FP32C: node order kept; the per-bond streams (and a copy of the
-        // neighbour ids) in line-aligned runs, as in the staged layout, so
-        // that the operator runs can be shared
+        // neighbour ids) in line-aligned runs as in the staged layout (on
+        // the 400x200 plate 24.0 us per step against 24.4 us in CSR order;
+        // operator runs are not shared in 2D, see build_op_classes)
         F.gcap = 0;
         F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
         std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
