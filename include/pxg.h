@@ -341,6 +341,20 @@ int pxg_stressch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, doubl
 int pxg_fsch_project(pxg_ctx *ctx, int which, const double *u, double *out);
 int pxg_fsch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev);
 
+/* ---- f3: result output ----------------------------------------------------
+ * Replaces PDProblem::savePDResults (SaveResults.cpp:53-204): the same VTU
+ * content and order (N points and VTK_VERTEX cells; point data ElmtID = the
+ * parent cell id for the first nbulk nodes and -1 for ghosts, one Float64
+ * scalar per dof named dof_names[a] from u[n*ndof], then nfields projections
+ * field_data[f] of field_comps[f] = 1, 3 or 9 values per node; the
+ * coordinates coords[3n]), written as raw little-endian binary appended data
+ * (UInt64 headers) instead of 16-digit ASCII.  Host arrays only; needs no
+ * context or GPU.  The reference names the file <deck>-PD-results-<step:05>.vtu. */
+int pxg_write_vtu(const char *path, int n, int nbulk, const double *coords, int ndof,
+                  const char *const *dof_names, const double *u, int nfields,
+                  const char *const *field_names, const int *field_comps,
+                  const double *const *field_data);
+
 /* Per-step drop-in of perix_cuda::assembleExplicitPDDOFracAccel
  * (CudaAssemble.h:251-254; CPU loop ExplicitPDDOFracElement.cpp:237-299):
  * damage with u, then health-gated Navier force, accel = (L+b)/rho.

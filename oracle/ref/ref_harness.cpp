@@ -394,6 +394,16 @@ void ref_projection_u(void *h, const char *name, const double *u, int nc, double
     }
 }
 
+// PDProblem::savePDResults for the state u (n*ndof): the reference's ASCII
+// VTU of the deck's OutputSystem fields, written next to `stem` (stem.json ->
+// stem-PD-results-<step>.vtu)
+void ref_save_vtu(void *h, const double *u, const char *stem, int step)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    std::copy(u, u + p.m_U.getSize(), p.m_U.getDataPtr());
+    p.savePDResults(std::string(stem) + ".json", step, 0.0);
+}
+
 // Elastic constants of the explicit element (C11, C12, C66, rho, sc).
 void ref_frac_consts(void *h, double *out)
 {

@@ -31,7 +31,7 @@ def lib():
         for f in ("ref_close", "ref_info", "ref_mesh", "ref_families", "ref_ops", "ref_csr",
                   "ref_initial_state", "ref_preset", "ref_assemble", "ref_assemble_parallel",
                   "ref_apply_bcs", "ref_projection", "ref_ch_geometry", "ref_assemble_step",
-                  "ref_dynfrac_state", "ref_dynfrac_set", "ref_stressch_consts", "ref_fsch_state", "ref_projection_u",
+                  "ref_dynfrac_state", "ref_dynfrac_set", "ref_stressch_consts", "ref_fsch_state", "ref_projection_u", "ref_save_vtu",
                   "ref_explicit", "ref_frac_consts", "ref_dxyz"):
             getattr(_LIB, f).argtypes = None
         _LIB.ref_horizon.restype = C.c_double
@@ -152,6 +152,11 @@ class Case:
         out = np.zeros(self.n * nc)
         lib().ref_projection_u(self.h, name.encode(), _dp(np.ascontiguousarray(u, float)), nc, _dp(out))
         return out.reshape(self.n, nc)
+
+    def save_vtu(self, u, stem, step):
+        """PDProblem::savePDResults of the state u; returns the written path."""
+        lib().ref_save_vtu(self.h, _dp(np.ascontiguousarray(u, float)), stem.encode(), step)
+        return "%s-PD-results-%05d.vtu" % (stem, step) if step > 0 else "%s-PD-results.vtu" % stem
 
     def assemble_step(self, u, uold, dt, timestep):
         """formResidualAndJacobian as time step `timestep` (per-step commits)."""

@@ -80,11 +80,38 @@ SYMBOLS = [
     "pxg_assemble_fsch", "pxg_assemble_fsch_dev", "pxg_fsch_get_state",
     "pxg_sch_components", "pxg_stressch_project", "pxg_stressch_project_dev", "pxg_fsch_project",
     "pxg_fsch_project_dev",
+    "pxg_write_vtu",
     "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
     "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
 ]
+
+
+def vtu_path(deck_path, step):
+    """The reference's result file name for a deck and output step
+    (SaveResults.cpp:17-60): <deck without .json>-PD-results[-<step:05>].vtu."""
+    stem = deck_path[:-5] if deck_path.endswith(".json") else deck_path
+    return stem + "-PD-results" + ("-%05d" % step if step > 0 else "") + ".vtu"
+
+
+def write_vtu(path, coords, nbulk, dof_names, u, fields=()):
+    """Binary VTU of PD results (pxg_write_vtu): coords (3n), the first nbulk
+    nodes bulk, u (n * len(dof_names)) and fields = [(name, n x comps array)]."""
+    coords = np.ascontiguousarray(coords, float)
+    n = coords.size // 3
+    ndof = len(dof_names)
+    u = np.ascontiguousarray(u, float)
+    if u.size != n * ndof:
+        raise ValueError("write_vtu: u must hold n * ndof values")
+    dn = (C.c_char_p * max(ndof, 1))(*[d.encode() for d in dof_names])
+    arrs = [np.ascontiguousarray(a, float).reshape(n, -1) for _, a in fields]
+    fn = (C.c_char_p * max(len(arrs), 1))(*[nm.encode() for nm, _ in fields])
+    fc = np.array([a.shape[1] for a in arrs] or [0], np.int32)
+    fd = (D * max(len(arrs), 1))(*[a.ctypes.data_as(D) for a in arrs])
+    _check(load().pxg_write_vtu(path.encode(), n, int(nbulk), _dp(coords), ndof, dn, _dp(u), len(arrs), fn,
+                                _ip(fc), fd))
+    return path
 
 
 def lib_path():
@@ -153,6 +180,9 @@ def load():
         for nm in ("pxg_stressch_project", "pxg_fsch_project"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_int, D, D]
             getattr(L, nm + "_dev").argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        CP = C.POINTER(C.c_char_p)
+        L.pxg_write_vtu.argtypes = [C.c_char_p, C.c_int, C.c_int, D, C.c_int, CP, D, C.c_int, CP, I,
+                                    C.POINTER(D)]
         L.pxg_stressch_setup.argtypes = [C.c_void_p, C.POINTER(StressChParams)]
         L.pxg_stressch_constants.argtypes = [C.c_void_p, D]
         L.pxg_assemble_stressch.argtypes = [C.c_void_p, C.c_double, D, D, D, D]

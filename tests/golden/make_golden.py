@@ -31,6 +31,8 @@ Fixtures (all small, .npz):
                     committing step (2D plane stress with inertia and a
                     residual stiffness; 3D quasi-static, tension and
                     compression failure), projections after the commit
+  vtu_stressch_2d.npz  the reference's VTU output (savePDResults) of the
+                    stressch_2d state with every projection, parsed
   bcrows_*_2d.npz   BC row elimination (BCSystem::applyBCs) of an assembled
                     Poisson (20x12) / Cahn-Hilliard (16x16) system under an
                     overlapping Dirichlet/Neumann list, with zeroed diagonals
@@ -314,6 +316,39 @@ def fsch_fixture(name, dim, nx, ny, nz, hi, crack, rho, tension_only, resid=None
     return int((h1 == 0).sum()), int((h2 == 0).sum()), len(h2)
 
 
+def vtu_fixture(name):
+    """PDProblem::savePDResults (the reference's ASCII VTU) of the plane-strain
+    stress-coupled Cahn-Hilliard state of stressch_2d.npz with every projection
+    the element lists; stored as the parsed arrays in file order."""
+    import tempfile
+    sys.path.insert(0, os.path.dirname(HERE))
+    from vtu_io import read_vtu
+    z = np.load(os.path.join(HERE, "stressch_2d.npz"))
+    params = {"E": 70e9, "nu": 0.3, "D": 1e-3, "Omega": 3.5e-6, "cref": 0.5, "chi": 2.5, "kappa": 2e-4,
+              "state": "plane_strain"}
+    d = {"Mesh": refpy.mesh_block(2, 14, 10, 1, hi=(1.4, 1.0, 1.0)),
+         "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+         "DOFs": ["c", "mu", "ux", "uy"],
+         "ElmtSystem": {"s": {"type": "stress_cahnhilliard", "params": params}},
+         "BCSystem": {"l": {"type": "neumann", "phygroup": "leftnodes_ghost", "dofs": ["c", "mu"],
+                            "value": [0.0, 0.0]}},
+         "JobSystem": {"type": "transient", "assemble": "serial"},
+         "TimeStepping": {"dt": 1e-3, "total_time": 1e-2},
+         "OutputSystem": {"interval": 1, "Fields": [nm for nm, _ in SCH_PROJECTIONS]}}
+    c = refpy.Case(d)
+    stem = os.path.join(tempfile.mkdtemp(prefix="pxvtu_"), "case")
+    path = c.save_vtu(z["u"], stem, 3)
+    v = read_vtu(path)
+    c.close()
+    names = [a[0] for a in v["point_data"]]
+    comps = [a[1] for a in v["point_data"]]
+    arrays = {"pd_%d" % k: a[2] for k, a in enumerate(v["point_data"])}
+    np.savez_compressed(os.path.join(HERE, name), names=np.array(names), comps=np.array(comps),
+                        vtu_name=os.path.basename(path), points=v["points"],
+                        connectivity=v["cells"][0][2], offsets=v["cells"][1][2], types=v["cells"][2][2],
+                        u=z["u"], **arrays)
+
+
 # BC lists of the row-elimination fixtures: (name, type, group, dofs, value,
 # velocity, direct, box) in deck order; overlapping entries on one row
 # exercise the list order
@@ -416,6 +451,7 @@ def main():
                                   rho=2.0, tension_only=True, resid=1e-3))
     print("fsch_3d", fsch_fixture("fsch_3d.npz", 3, 6, 4, 3, (0.6, 0.4, 0.3), (0.3, 0.0, 0.3, 0.15),
                                   rho=0.0, tension_only=False))
+    vtu_fixture("vtu_stressch_2d.npz")
     bcrows_fixture("bcrows_poisson_2d.npz", "poisson")
     bcrows_fixture("bcrows_ch_2d.npz", "ch")
     for f in sorted(os.listdir(HERE)):
