@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "explicit PDDO node-steps/s FP32-comp vs FP64 at stated err; adjoint/fwd cost"
 
 
@@ -343,6 +344,69 @@ def block_long_run(problems, pxg, steps=1000, chunk=100):
         s.ctx.close()
     return {"workload": "160x80x40 block, 1000 steps, FP32C and a 1e-15 m perturbed FP64 run against FP64",
             "chunks": rows}
+
+
+def gweight_model(n, nb, dim, nop):
+    """Algorithmic cost of one g-weight build (k_operators): (bytes, flops).
+
+    Bytes: per directed bond the neighbour id (4 B) and the nop FP64 weights
+    written; per node the family offset (4 B), x (dim x 8 B), V (8 B) and the
+    flag byte.  Flops (sqrt and exp counted as one each): per bond the
+    geometry twice (moment pass and weight pass: 21 + dim(dim+1)/2), the
+    moment products nop^2 x 3 and the weights nop x (2 nop + 2); per node the
+    complete-pivot LU (2/3 nop^3), the nop triangular solves (2 nop^3) and the
+    A X = B residual (2 nop^3)."""
+    geom = 21 + dim * (dim + 1) // 2
+    per_bond_f = 2 * geom + 3 * nop * nop + nop * (2 * nop + 2)
+    per_node_f = (2.0 / 3.0 + 2.0 + 2.0) * nop ** 3
+    per_bond_b = 4 + 8 * nop
+    per_node_b = 4 + 8 * dim + 8 + 1
+    return nb * per_bond_b + n * per_node_b, nb * per_bond_f + n * per_node_f, 3 * nop * nop * nb
+
+
+def gweight_case(problems, pxg, reps=3):
+    """Family search and g-weight build of the 2D plate and the 3D block.
+
+    Each build runs once to allocate, then `reps` rebuilds are timed on the
+    host clock around the synchronous call (buffers already allocated, so the
+    figure is the kernel plus a few microseconds of launch and copy-back)."""
+    from perix_amd.mesh import StructuredGrid, build_structured_cloud, horizon_radius
+    out = {}
+    for name, dim, nx, ny, nz, hi in (("plate400x200", 2, 400, 200, 1, (0.2, 0.1, 0.04)),
+                                      ("block160x80x40", 3, 160, 80, 40, (0.16, 0.08, 0.04))):
+        g = StructuredGrid(dim, nx, ny, nz, (0.0, 0.0, 0.0), hi)
+        cloud = build_structured_cloud(g)
+        delta = horizon_radius(g, 3.015)
+        ctx = pxg.Context(0)
+        ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+        ctx.build_families(delta)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.build_families(delta)
+        ctx.sync()
+        fam_ms = (time.perf_counter() - t0) * 1e3 / reps
+        dxc = float(max(cloud.dxyz))
+        ndeg = ctx.build_operators(delta, dxc)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.build_operators(delta, dxc)
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        n, nb, nop = cloud.n, ctx.nb, ctx.nop
+        b, f, fmom = gweight_model(n, nb, dim, nop)
+        gbs = b / (ms * 1e-3) / 1e9
+        tfl = f / (ms * 1e-3) / 1e12
+        out[name] = {"nodes": n, "bonds": nb, "nop": nop, "degenerate_nodes": ndeg,
+                     "families_ms": fam_ms, "gweight_ms": ms,
+                     "gweight_hbm": {"algorithmic_bytes": b, "achieved_GBs": gbs,
+                                     "frac": gbs / HBM_PEAK_GBS},
+                     "gweight_fp64": {"flops": f, "achieved_TFLOPs": tfl,
+                                      "frac": tfl / FP64_PEAK_TFLOPS,
+                                      "moment_accumulation_share": fmom / f}}
+        ctx.close()
+    return out
 
 
 def assembly_case(problems, pxg, reps=20):
@@ -745,6 +809,7 @@ def main():
     ap.add_argument("--long-run", action="store_true", help="with --only-block: the 1000-step comparison")
     ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
     ap.add_argument("--only-assembly", action="store_true", help="CSR assembly only (profiling)")
+    ap.add_argument("--only-gweight", action="store_true", help="family + g-weight build only (profiling)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--stream-ceiling", action="store_true")
     args = ap.parse_args()
@@ -776,6 +841,9 @@ def main():
         return 0
     if args.only_assembly:
         print(json.dumps({"assembly": assembly_case(problems, pxg)}))
+        return 0
+    if args.only_gweight:
+        print(json.dumps({"gweight_build": gweight_case(problems, pxg)}))
         return 0
     res = {}
     for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
@@ -857,6 +925,8 @@ def main():
         out["block3d"]["long_run"] = block_long_run(problems, pxg)
         log("assembly (Poisson, diffusion, dynamic fracture, Cahn-Hilliard, stress-coupled Cahn-Hilliard)")
         out["assembly"] = assembly_case(problems, pxg)
+        log("family search and g-weight build (plate, block)")
+        out["gweight_build"] = gweight_case(problems, pxg)
     print(json.dumps(out))
     return 0
 

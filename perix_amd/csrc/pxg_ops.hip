@@ -21,7 +21,10 @@
 //   5. consistency residual max|AX-B| <= 1e-8 * scale, then every bond's
 //      op_k = ((sum_a X[a,k] eta^{q_a}) * w vc V_j) / delta^{|q_k|}.
 // All arithmetic is compiled without FMA contraction so that it rounds like
-// the reference's x86-64 build.
+// the reference's x86-64 build.  The build is latency-bound (dependent
+// neighbour gathers, a short serial LU per node), so occupancy is what hides
+// it: empty asm memory clobbers keep the LU factors and X in LDS rather than
+// hoisted into registers, and the kernel fits 5 waves per SIMD.
 #include "pxg_internal.h"
 #include <cfloat>
 #include <cmath>
@@ -65,10 +68,9 @@ __device__ __forceinline__ void bond_geometry(int i, int j1, const double *xp, i
 }
 
 template <int NOP>
-__global__ void __launch_bounds__(64) k_operators(int n, int dim, const double *xp, const double *vols,
-                                                  const int *off, const int *nbr, double delta,
-                                                  double dx_char, int volcorr, double *ops,
-                                                  unsigned char *flags, int *ndeg)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_operators(int n, int dim, const double *xp, const double *vols, const int *off, const int *nbr,
+            double delta, double dx_char, int volcorr, double *ops, unsigned char *flags, int *ndeg)
 {
     constexpr int NN = NOP * NOP;
     __shared__ double s_bw[64];
@@ -102,11 +104,13 @@ __global__ void __launch_bounds__(64) k_operators(int n, int dim, const double *
         for (int k = 0; k < NOP; k++) s_mono[lane][k] = mono[k];
         usable += __popcll(__ballot(bw > 0.0));
         __syncthreads();
+        // An unusable bond has bw = 0 and zero monomials, so its term is
+        // +-0 and adding it leaves the sum bit-unchanged: no branch needed.
         const int cnt = min(64, b1 - s);
+#pragma unroll 8
         for (int u = 0; u < cnt; u++) {
             const double bu = s_bw[u];
-            if (!(bu > 0.0)) continue;
-            if (e0 < NN) acc0 += bu * s_mono[u][a0] * s_mono[u][c0];
+            acc0 += bu * s_mono[u][a0] * s_mono[u][c0];
             if (e1 < NN) acc1 += bu * s_mono[u][a1] * s_mono[u][c1];
         }
         __syncthreads();
@@ -195,6 +199,7 @@ __global__ void __launch_bounds__(64) k_operators(int n, int dim, const double *
             // unit-lower, panels of 4 starting at row 0
 #pragma unroll
             for (int j2 = 0; j2 < NOP; j2 += 4) {
+                asm volatile("" ::: "memory");   // re-read L per panel, not all 81 in registers
                 const int w = (NOP - j2 < 4) ? NOP - j2 : 4;
                 if (j2 > 0) {
 #pragma unroll
@@ -214,6 +219,7 @@ __global__ void __launch_bounds__(64) k_operators(int n, int dim, const double *
             const int sz = rank;
             int j2 = sz - ((sz % 4) ? (sz % 4) : 4);
             for (; j2 >= 0; j2 -= 4) {
+                asm volatile("" ::: "memory");   // same for U
                 const int w = (sz - j2 < 4) ? sz - j2 : 4;
                 const int po = j2 + w;
 #pragma unroll
@@ -298,8 +304,11 @@ __global__ void __launch_bounds__(64) k_operators(int n, int dim, const double *
         for (int k = 0; k < NOP; k++) mono[k] = 0.0;
         bond_geometry<NOP>(i, nbr[t], xp, dim, vols, delta, inv_delta, dx_char, volcorr, bw, mono);
         double *op = ops + (size_t)t * NOP;
+        // keep X in LDS (broadcast reads) instead of 81 hoisted registers
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int k = 0; k < NOP; k++) {
+            asm volatile("" ::: "memory");
             double v = 0.0;
             if (bw > 0.0) {
                 double sacc = 0.0;
