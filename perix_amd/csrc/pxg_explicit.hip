@@ -243,47 +243,23 @@ __global__ void k_class_check(int n, int nc, const int *off, const int *bsh, con
     pofs[i] = (off[r] + bsh[r]) - (off[i] + bsh[i]);
 }
 
-// Nodal output projections of the explicit element, one thread per node
-// (ExplicitPDDOFracElement::computeNodalProjection, ExplicitPDDOFracElement.cpp:
-// 324-405): which = 0 damage = 1 - (intact bonds)/|H_i|; otherwise the
-// displacement gradient sum_j (u_j - u_i) (x) G(i,j) over the whole family in
-// family order (first-derivative g-weights, health not consulted), the small
-// strain, the Navier stress (sigma_zz = C12 tr(eps) in 2D plane strain, 0 in
-// plane stress), von Mises (1), stress (2) or strain (3) as row-major 3x3.
-__global__ void k_frac_project(int n, int dim, int nop, int which, int plane_stress, double C12, double C66,
-                               const int *off, const int *nbr, const double *ops, const unsigned char *health,
-                               const int *bsh, const double *u, double *out)
+// Nodal output projections of the explicit element (ExplicitPDDOFracElement::
+// computeNodalProjection, ExplicitPDDOFracElement.cpp:324-405): which = 0
+// damage = 1 - (intact bonds)/|H_i|; otherwise the displacement gradient
+// sum_j (u_j - u_i) (x) G(i,j) over the whole family in family order
+// (first-derivative g-weights, health not consulted), the small strain, the
+// Navier stress (sigma_zz = C12 tr(eps) in 2D plane strain, 0 in plane
+// stress), von Mises (1), stress (2) or strain (3) as row-major 3x3.
+//
+// A one-wave block holds 4 nodes of 16 lanes.  Per chunk of 16 bonds each
+// lane forms its bond's 9 products (u_j - u_i)_a G_b with coalesced loads of
+// the family's contiguous ids and g-weights; lanes 0..8 then add component
+// a*3+b over the chunk in family order, so every sum is the sequential one.
+constexpr int kProjLanes = 16, kProjNodes = 64 / kProjLanes;
+
+__device__ void frac_project_out(int dim, int which, int plane_stress, double C12, double C66,
+                                 const double *gu, double *o)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int nc = (which >= 2) ? 9 : 1;
-    double *o = out + (size_t)i * nc;
-    const int b = off[i], nb = off[i + 1] - off[i];
-    if (which == 0) {
-        if (nb <= 0) { o[0] = 0.0; return; }
-        const int sh = bsh ? bsh[i] : 0;
-        double live = 0.0;
-        for (int t = b; t < b + nb; t++) live += health[t + sh] ? 1.0 : 0.0;
-        o[0] = 1.0 - live / (double)nb;
-        return;
-    }
-    const size_t rowI = (size_t)i * dim;
-    double gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int t = b; t < b + nb; t++) {
-        const size_t rowJ = (size_t)(nbr[t] - 1) * dim;
-        const double *op = ops + (size_t)t * nop;
-        const double du0 = u[rowJ] - u[rowI], du1 = u[rowJ + 1] - u[rowI + 1];
-        if (dim >= 3) {
-            const double du2 = u[rowJ + 2] - u[rowI + 2];
-            const double Gx = op[0], Gy = op[1], Gz = op[2];
-            gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[2] += du0 * Gz;
-            gu[3] += du1 * Gx; gu[4] += du1 * Gy; gu[5] += du1 * Gz;
-            gu[6] += du2 * Gx; gu[7] += du2 * Gy; gu[8] += du2 * Gz;
-        } else {
-            const double Gx = op[0], Gy = op[1];
-            gu[0] += du0 * Gx; gu[1] += du0 * Gy; gu[3] += du1 * Gx; gu[4] += du1 * Gy;
-        }
-    }
     double eps[6] = {0, 0, 0, 0, 0, 0};
     if (dim >= 3) {
         eps[0] = gu[0]; eps[1] = gu[4]; eps[2] = gu[8];
@@ -306,6 +282,70 @@ __global__ void k_frac_project(int n, int dim, int nop, int which, int plane_str
     o[0] = v[0]; o[1] = v[3]; o[2] = v[4];
     o[3] = v[3]; o[4] = v[1]; o[5] = v[5];
     o[6] = v[4]; o[7] = v[5]; o[8] = v[2];
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64) k_frac_project(int n, int nop, int which, int plane_stress, double C12,
+                                                     double C66, const int *off, const int *nbr,
+                                                     const double *ops, const unsigned char *health,
+                                                     const int *bsh, const double *u, double *out)
+{
+    __shared__ double s_p[kProjNodes][kProjLanes][9];
+    __shared__ double s_gu[kProjNodes][9];
+    const int g = threadIdx.x / kProjLanes, l = threadIdx.x % kProjLanes;
+    const int i = blockIdx.x * kProjNodes + g;
+    const bool act = i < n;
+    const int b = act ? off[i] : 0;
+    const int nb = act ? off[i + 1] - b : 0;
+    if (which == 0) {
+        // intact-bond count: integer, so the lane split does not change it
+        int live = 0;
+        if (act) {
+            const int sh = bsh ? bsh[i] : 0;
+            for (int t = b + l; t < b + nb; t += kProjLanes) live += health[t + sh] ? 1 : 0;
+        }
+        for (int m = kProjLanes / 2; m >= 1; m >>= 1) live += __shfl_xor(live, m);
+        if (act && l == 0) out[i] = nb <= 0 ? 0.0 : 1.0 - (double)live / (double)nb;
+        return;
+    }
+    double ui[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) ui[d] = act ? u[(size_t)i * DIM + d] : 0.0;
+    // block-uniform chunk count (the block is one wave)
+    int mx = nb;
+    for (int m = 32; m >= 1; m >>= 1) mx = max(mx, __shfl_xor(mx, m));
+    double acc = 0.0;   // lane l < 9 owns component l = a*3 + b
+    for (int c = 0; c < mx; c += kProjLanes) {
+        double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int t = b + c + l;
+        if (c + l < nb) {
+            const size_t rowJ = (size_t)(nbr[t] - 1) * DIM;
+            const double *op = ops + (size_t)t * nop;
+            double G[DIM], du[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) { G[d] = op[d]; du[d] = u[rowJ + d] - ui[d]; }
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int q = 0; q < DIM; q++) p[a * 3 + q] = du[a] * G[q];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) s_p[g][l][k] = p[k];
+        __syncthreads();
+        if (l < 9) {
+            const int cnt = min(kProjLanes, nb - c);
+            for (int q = 0; q < cnt; q++) acc += s_p[g][q][l];
+        }
+        __syncthreads();
+    }
+    if (l < 9) s_gu[g][l] = acc;
+    __syncthreads();
+    if (act && l == 0) {
+        double gu[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) gu[k] = s_gu[g][k];
+        frac_project_out(DIM, which, plane_stress, C12, C66, gu, out + (size_t)i * ((which >= 2) ? 9 : 1));
+    }
 }
 
 __global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
@@ -775,9 +815,9 @@ extern "C" int pxg_frac_project_dev(pxg_ctx *ctx, int which, double *out_dev)
     PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_project: injected fault");
     PXG_HIP(hipSetDevice(ctx->device));
     const FracState &F = ctx->frac;
-    hipLaunchKernelGGL(k_frac_project, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
-                       ctx->nop, which, F.plane_stress, F.C12, F.C66, ctx->off.p, ctx->nbr.p, ctx->ops.p,
-                       F.health.p, F.bsh.p, F.u[F.cur].p, out_dev);
+    hipLaunchKernelGGL(ctx->dim == 3 ? k_frac_project<3> : k_frac_project<2>, dim3(gridn(ctx->n, kProjNodes)),
+                       dim3(64), 0, ctx->stream, ctx->n, ctx->nop, which, F.plane_stress, F.C12, F.C66,
+                       ctx->off.p, ctx->nbr.p, ctx->ops.p, F.health.p, F.bsh.p, F.u[F.cur].p, out_dev);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
