@@ -90,7 +90,7 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
                            int plane_stress, long long ps, const int *bsh, long long pst, double *P64,
                            float *P32, double *Phat)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     double c11 = C11, c12 = C12, c66 = C66;
     if (nodeE) {
@@ -115,7 +115,7 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
             h11 = 1.0 / d; h12 = 1.0 * nu / d; h66 = 1.0 / (2.0 * (1.0 + nu));
         }
     }
-    for (int t = off[i]; t < off[i + 1]; t++) {
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) {
         const double *op = ops + (size_t)t * nop;
         double p[6], q[6];
         int nc;
@@ -153,14 +153,14 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
 
 // Bond state (health, break steps) lives in the step kernel's stream
 // layout: CSR bond t of node i at t + bsh[i] (bsh null: CSR order).  These
-// per-node kernels convert at the API boundary; they are not on the step path.
+// kernels convert at the API boundary; they are not on the step path.
 __global__ void k_health_init(int n, const int *off, const int *bsh, const unsigned char *seed,
                               unsigned char *health, int *bs)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     const int sh = bsh ? bsh[i] : 0;
-    for (int t = off[i]; t < off[i + 1]; t++) {
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) {
         health[t + sh] = seed[t];
         bs[t + sh] = seed[t] ? INT_MAX : -1;
     }
@@ -169,28 +169,28 @@ __global__ void k_health_init(int n, const int *off, const int *bsh, const unsig
 __global__ void k_health_from_double(int n, const int *off, const int *bsh, const double *h,
                                      unsigned char *health)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     const int sh = bsh ? bsh[i] : 0;
-    for (int t = off[i]; t < off[i + 1]; t++) health[t + sh] = (h[t] > 0.0) ? 1 : 0;
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) health[t + sh] = (h[t] > 0.0) ? 1 : 0;
 }
 
 __global__ void k_health_to_double(int n, const int *off, const int *bsh, const unsigned char *health,
                                    double *h)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     const int sh = bsh ? bsh[i] : 0;
-    for (int t = off[i]; t < off[i + 1]; t++) h[t] = health[t + sh] ? 1.0 : 0.0;
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) h[t] = health[t + sh] ? 1.0 : 0.0;
 }
 
 __global__ void k_bonds_to_csr(int n, const int *off, const int *bsh, const unsigned char *health,
                                const int *bs, unsigned char *hc, int *bc)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     const int sh = bsh[i];
-    for (int t = off[i]; t < off[i + 1]; t++) {
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) {
         if (hc) hc[t] = health[t + sh];
         if (bc) bc[t] = bs[t + sh];
     }
@@ -223,24 +223,28 @@ __global__ void k_pattern_hash(int n, int dim, const int *off, const int *nbr, c
 __global__ void k_class_check(int n, int nc, const int *off, const int *bsh, const int *rep, long long ps,
                               const double *P, double tol, int *pofs)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    pofs[i] = 0;
-    const int r = rep[i];
-    if (r < 0 || r == i) return;
-    const int len = off[i + 1] - off[i];
-    if (off[r + 1] - off[r] != len) return;
+    // 16 lanes per node; a group's lanes are consecutive in one wave, so the
+    // shuffles below stay inside the group
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
+    const int r = (i < n) ? rep[i] : -1;
+    const int len = (i < n) ? off[i + 1] - off[i] : 0;
+    const bool cand = i < n && r >= 0 && r != i && off[r + 1] - off[r] == len;
     double mx = 0.0;
-    for (int c = 0; c < nc; c++)
-        for (int k = 0; k < len; k++) {
-            const double a = fabs(P[(size_t)c * ps + off[i] + k]);
-            mx = a > mx ? a : mx;
-        }
+    if (cand)
+        for (int c = 0; c < nc; c++)
+            for (int k = l; k < len; k += kConvLanes) {
+                const double a = fabs(P[(size_t)c * ps + off[i] + k]);
+                mx = a > mx ? a : mx;
+            }
+    for (int m = kConvLanes / 2; m >= 1; m >>= 1) { const double o = __shfl_xor(mx, m); mx = o > mx ? o : mx; }
     const double lim = tol * mx;
-    for (int c = 0; c < nc; c++)
-        for (int k = 0; k < len; k++)
-            if (!(fabs(P[(size_t)c * ps + off[i] + k] - P[(size_t)c * ps + off[r] + k]) <= lim)) return;
-    pofs[i] = (off[r] + bsh[r]) - (off[i] + bsh[i]);
+    int ok = cand ? 1 : 0;
+    if (cand)
+        for (int c = 0; c < nc && ok; c++)
+            for (int k = l; k < len; k += kConvLanes)
+                if (!(fabs(P[(size_t)c * ps + off[i] + k] - P[(size_t)c * ps + off[r] + k]) <= lim)) { ok = 0; break; }
+    for (int m = kConvLanes / 2; m >= 1; m >>= 1) ok &= __shfl_xor(ok, m);
+    if (i < n && l == 0) pofs[i] = ok ? (off[r] + bsh[r]) - (off[i] + bsh[i]) : 0;
 }
 
 // Nodal output projections of the explicit element (ExplicitPDDOFracElement::
@@ -350,9 +354,9 @@ __global__ void __launch_bounds__(64) k_frac_project(int n, int nop, int which, 
 
 __global__ void k_rev(int n, const int *off, const int *nbr, int *rev)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
-    for (int t = off[i]; t < off[i + 1]; t++) {
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) {
         const int j = nbr[t] - 1;
         int r = -1;
         for (int s = off[j]; s < off[j + 1]; s++) if (nbr[s] == i + 1) { r = s; break; }
@@ -371,7 +375,7 @@ namespace pxg {
 int launch_frac_ops(pxg_ctx *ctx, const double *nodeE_dev)
 {
     FracState &F = ctx->frac;
-    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->dim,
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, ctx->stream, ctx->n, ctx->dim,
                        ctx->nop, ctx->off.p, ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress,
                        F.ps, F.bsh.p, F.pst, F.P64.p, (float *)nullptr, (double *)nullptr);
     PXG_HIP(hipGetLastError());
@@ -388,7 +392,7 @@ int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break
     }
     if (health) PXG_HIP(F.health_csr.alloc(ctx->nb));
     if (break_step) PXG_HIP(F.bs_csr.alloc(ctx->nb));
-    hipLaunchKernelGGL(k_bonds_to_csr, dim3(gridn(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->n, ctx->off.p,
+    hipLaunchKernelGGL(k_bonds_to_csr, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, ctx->stream, ctx->n, ctx->off.p,
                        F.bsh.p, F.health.p, F.break_step.p, health ? F.health_csr.p : nullptr,
                        break_step ? F.bs_csr.p : nullptr);
     PXG_HIP(hipGetLastError());
@@ -422,7 +426,7 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     hipStream_t st = ctx->stream;
     dbuf<double> P;                     // FP64 operators, CSR order
     PXG_HIP(P.alloc((size_t)F.ps * nc));
-    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(n, 128)), dim3(128), 0, st, n, ctx->dim, ctx->nop, ctx->off.p,
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, ctx->dim, ctx->nop, ctx->off.p,
                        ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress, F.ps, (const int *)nullptr,
                        F.ps, P.p, (float *)nullptr, (double *)nullptr);
     PXG_HIP(hipGetLastError());
@@ -448,7 +452,7 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     PXG_HIP(drep.alloc(n));
     PXG_HIP(F.pofs.alloc(n));
     PXG_HIP(hipMemcpyAsync(drep.p, rep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_class_check, dim3(gridn(n, 128)), dim3(128), 0, st, n, nc, ctx->off.p, F.bsh.p, drep.p,
+    hipLaunchKernelGGL(k_class_check, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p, F.bsh.p, drep.p,
                        F.ps, P.p, kShareTol, F.pofs.p);
     PXG_HIP(hipGetLastError());
     std::vector<int> pofs((size_t)n);
@@ -501,7 +505,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
         PXG_HIP(hipMemcpyAsync(F.nodeE.p, node_E, sizeof(double) * ctx->n, hipMemcpyHostToDevice, st));
         PXG_HIP(F.Phat.alloc((size_t)F.ps * nc));
         PXG_HIP(F.rev.alloc(nb));
-        hipLaunchKernelGGL(k_rev, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, ctx->nbr.p, F.rev.p);
+        hipLaunchKernelGGL(k_rev, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->off.p, ctx->nbr.p, F.rev.p);
     } else {
         F.nodeE.release(); F.Phat.release(); F.rev.release();
     }
@@ -511,7 +515,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     }
     if (precision == PXG_FP64) { PXG_HIP(F.P64.alloc((size_t)F.pst * nc)); F.P32.release(); }
     else { PXG_HIP(F.P32.alloc((size_t)F.pst * nc)); if (!node_E) F.P64.release(); else PXG_HIP(F.P64.alloc((size_t)F.pst * nc)); }
-    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->dim, ctx->nop,
+    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->dim, ctx->nop,
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
                        plane_stress, F.ps, F.bsh.p, F.pst, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
@@ -530,7 +534,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     // layout padding: dead in every mode (health 0, a break step below any step)
     PXG_HIP(hipMemsetAsync(F.break_step.p, 0x80, sizeof(int) * (size_t)F.pst, st));
     F.health_csr.release(); F.bs_csr.release();
-    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, F.bsh.p,
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->off.p, F.bsh.p,
                        ctx->seed.p, F.health.p, F.break_step.p);
     for (int k = 0; k < 2; k++) PXG_HIP(F.u[k].alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.accel.alloc((size_t)ctx->n * ctx->dim));
@@ -669,7 +673,7 @@ extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inou
     if (health_inout && !F.health_uploaded) {
         PXG_HIP(ctx->sk.alloc(ctx->nb));
         PXG_HIP(hipMemcpyAsync(ctx->sk.p, health_inout, sizeof(double) * ctx->nb, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_health_from_double, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p,
+        hipLaunchKernelGGL(k_health_from_double, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->off.p,
                            F.bsh.p, ctx->sk.p, F.health.p);
         F.health_uploaded = 1;
     }
@@ -683,7 +687,7 @@ extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inou
     PXG_HIP(hipStreamSynchronize(st));
     if (broke && health_inout) {
         PXG_HIP(ctx->sk.alloc(ctx->nb));
-        hipLaunchKernelGGL(k_health_to_double, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p,
+        hipLaunchKernelGGL(k_health_to_double, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->off.p,
                            F.bsh.p, F.health.p, ctx->sk.p);
         PXG_HIP(hipMemcpyAsync(health_inout, ctx->sk.p, sizeof(double) * ctx->nb, hipMemcpyDeviceToHost, st));
         PXG_HIP(hipStreamSynchronize(st));
@@ -703,7 +707,7 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     const size_t nd = (size_t)ctx->n * ctx->dim;
     F.cur = 0;
     // a new run starts from the seeded bond health
-    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, 128)), dim3(128), 0, st, ctx->n, ctx->off.p, F.bsh.p,
+    hipLaunchKernelGGL(k_health_init, dim3(gridn(ctx->n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->n, ctx->off.p, F.bsh.p,
                        ctx->seed.p, F.health.p, F.break_step.p);
     PXG_HIP(hipGetLastError());
     PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));

@@ -279,11 +279,11 @@ __global__ void k_seed(const double *c3, const int *mirror, int n, const int *of
                        const double *seg, int nseg, const double *quad, int nquad,
                        unsigned char *seed)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
     if (i >= n) return;
     int a = i + 1;
     if (mirror[i] >= 1 && mirror[i] <= n) a = mirror[i];
-    for (int t = off[i]; t < off[i + 1]; t++) {
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes) {
         int b = nbr[t];
         if (b >= 1 && b <= n && mirror[b - 1] >= 1 && mirror[b - 1] <= n) b = mirror[b - 1];
         unsigned char cut = 0;
@@ -483,7 +483,7 @@ extern "C" int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out)
     // crack seeds follow the families
     PXG_HIP(ctx->seed.alloc((size_t)(tot > 0 ? tot : 1)));
     if (ctx->nseg + ctx->nquad > 0) {
-        hipLaunchKernelGGL(k_seed, dim3(grid_of(n, 128)), dim3(128), 0, st, ctx->coords.p,
+        hipLaunchKernelGGL(k_seed, dim3(grid_of(n, kConvNodes)), dim3(kConvBlock), 0, st, ctx->coords.p,
                            ctx->mirror.p, n, ctx->off.p, ctx->nbr.p, ctx->crack_seg.p, ctx->nseg,
                            ctx->crack_quad.p, ctx->nquad, ctx->seed.p);
     } else {
@@ -517,7 +517,7 @@ extern "C" int pxg_set_cracks(pxg_ctx *ctx, int nseg, const double *seg, int nqu
     if (nquad) PXG_HIP(hipMemcpyAsync(ctx->crack_quad.p, quad, sizeof(double) * 12 * nquad, hipMemcpyHostToDevice, ctx->stream));
     if (ctx->nb > 0) {
         if (nseg + nquad > 0)
-            hipLaunchKernelGGL(k_seed, dim3(grid_of(ctx->n, 128)), dim3(128), 0, ctx->stream, ctx->coords.p,
+            hipLaunchKernelGGL(k_seed, dim3(grid_of(ctx->n, kConvNodes)), dim3(kConvBlock), 0, ctx->stream, ctx->coords.p,
                                ctx->mirror.p, ctx->n, ctx->off.p, ctx->nbr.p, ctx->crack_seg.p, nseg,
                                ctx->crack_quad.p, nquad, ctx->seed.p);
         else

@@ -36,6 +36,11 @@ void set_error(const std::string &msg);
         if (!(cond)) { ::pxg::set_error(msg); return (code); }                        \
     } while (0)
 
+// Per-node loops over a family whose bonds are independent (conversions,
+// set-up): 16 consecutive lanes walk one family, 8 nodes per 128-thread
+// block, so a family's loads and stores are coalesced.
+constexpr int kConvLanes = 16, kConvBlock = 128, kConvNodes = kConvBlock / kConvLanes;
+
 // device buffer with size bookkeeping.  Every allocation carries at least 64
 // zeroed elements of slack past `count` (rounded to a multiple of 64), so
 // kernels may read aligned vectors that run past the last element.
