@@ -395,8 +395,10 @@ int pxg_frac_get_state(pxg_ctx *ctx, double *u, double *uold,
  * pxg_frac_* call). */
 int pxg_frac_state_ptrs(pxg_ctx *ctx, double **u_dev, double **uold_dev);
 
-/* Per-launch timing of the step kernel inside pxg_frac_run (HIP events on
- * the context's stream): enable, run, then read the summed kernel time. */
+/* Timing of the step kernel inside pxg_frac_run (HIP events on the
+ * context's stream, one pair around each run of back-to-back launches
+ * between guard polls): enable, run, then read the summed time and the
+ * number of step launches it covers. */
 int pxg_frac_profile(pxg_ctx *ctx, int enable);
 int pxg_frac_kernel_time(pxg_ctx *ctx, double *ms_total, int *launches);
 

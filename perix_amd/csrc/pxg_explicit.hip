@@ -725,6 +725,18 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     return PXG_OK;
 }
 
+// closes the open step-timing pair after the last launch of a run
+static hipError_t close_step_events(pxg_ctx *ctx)
+{
+    FracState &F = ctx->frac;
+    if (!F.ev_open) return hipSuccess;
+    const hipError_t e = hipEventRecord(F.ev[2 * F.nev + 1], ctx->stream);
+    if (e != hipSuccess) return e;
+    F.nev++; F.nlaunch += F.ev_pending;
+    F.ev_open = 0; F.ev_pending = 0;
+    return hipSuccess;
+}
+
 extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_max, int *steps_done)
 {
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_run: call pxg_frac_setup first");
@@ -740,7 +752,9 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
         const double dtA = fmin(dt, total - F.t);
         const double tNew = F.t + dtA;
         double *ucur = F.u[F.cur].p, *uold = F.u[F.cur ^ 1].p;
-        if (F.prof) {
+        if (F.prof && !F.ev_open) {
+            // one event pair spans each run of back-to-back launches between
+            // guard polls, so the markers do not sit between the kernels
             if ((size_t)(2 * F.nev + 2) > F.ev.size()) {
                 hipEvent_t a, b;
                 PXG_HIP(hipEventCreate(&a));
@@ -748,6 +762,7 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
                 F.ev.push_back(a); F.ev.push_back(b);
             }
             PXG_HIP(hipEventRecord(F.ev[2 * F.nev], ctx->stream));
+            F.ev_open = 1; F.ev_pending = 0;
         }
         // one launch per step: damage, forces, update, BC presets at tNew and
         // the guard decision on the previous step
@@ -759,17 +774,19 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
             rc = launch_step_fused(ctx, MODE_VAR, ucur, uold, uold, rr, w, step, tNew);
         }
         if (rc) return rc;
-        if (F.prof) { PXG_HIP(hipEventRecord(F.ev[2 * F.nev + 1], ctx->stream)); F.nev++; }
+        if (F.prof) F.ev_pending++;
         F.cur ^= 1;
         F.t = tNew; F.dt_prev = dtA; F.step = step;
         done++;
         if ((done & 255) == 0) {
+            if (F.prof) PXG_HIP(close_step_events(ctx));
             int gs[3];
             PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
             PXG_HIP(hipStreamSynchronize(ctx->stream));
             if (gs[0]) break;
         }
     }
+    if (F.prof) PXG_HIP(close_step_events(ctx));
     if (done > 0) {
         rc = launch_guard_final(ctx, F.step);
         if (rc) return rc;
@@ -864,6 +881,9 @@ extern "C" int pxg_frac_profile(pxg_ctx *ctx, int enable)
     PXG_CHECK(ctx, PXG_ERR_ARG, "pxg_frac_profile: ctx is NULL");
     ctx->frac.prof = enable ? 1 : 0;
     ctx->frac.nev = 0;
+    ctx->frac.nlaunch = 0;
+    ctx->frac.ev_open = 0;
+    ctx->frac.ev_pending = 0;
     return PXG_OK;
 }
 
@@ -880,6 +900,6 @@ extern "C" int pxg_frac_kernel_time(pxg_ctx *ctx, double *ms_total, int *launche
         tot += ms;
     }
     *ms_total = tot;
-    *launches = F.nev;
+    *launches = (int)F.nlaunch;
     return PXG_OK;
 }

@@ -208,10 +208,13 @@ struct FracState {
     double t = 0.0, dt_prev = 0.0;
     int step = 0;
     int cur0 = 0;                 // buffer index at step 0
-    // optional per-launch timing of the step kernel (pxg_frac_profile)
+    // optional timing of the step kernel (pxg_frac_profile): each event pair
+    // spans a run of back-to-back step launches
     int prof = 0;
     std::vector<hipEvent_t> ev;   // start/stop pairs
-    int nev = 0;
+    int nev = 0;                  // closed pairs
+    long long nlaunch = 0;        // launches inside closed pairs
+    int ev_open = 0, ev_pending = 0;
 };
 
 // launch_step modes: u^{n+1} by the fixed-dt central difference, by the

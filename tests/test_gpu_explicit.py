@@ -168,3 +168,25 @@ def test_explicit_large_horizon_global_gathers(oracle):
     print("FP32C rel-L2 %.3e, mismatches %d" % (rel, int((h32 != h).sum())))
     assert rel <= 1e-5
     s.ctx.close(); s32.ctx.close()
+
+
+def test_step_timing_spans_every_launch():
+    """pxg_frac_kernel_time counts every step launch of the runs made while
+    profiling (event pairs around each run of launches between guard polls)
+    and its average cannot exceed the host clock's time per step."""
+    import time
+    s = problems.explicit_plate(2, 80, 40, hi=(0.02, 0.01, 1.0), nsteps=600)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    ctx.sync()
+    ctx.frac_profile(True)
+    t0 = time.perf_counter()
+    done = ctx.frac_run(s.dt, s.total)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms, launches = ctx.frac_kernel_time()
+    ctx.frac_profile(False)
+    print("launches %d, event ms %.3f, host ms %.3f" % (launches, ms, wall * 1e3))
+    assert done == 600 and launches == 600
+    assert 0.0 < ms <= wall * 1e3
+    ctx.close()
