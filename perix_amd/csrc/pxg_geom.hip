@@ -89,26 +89,48 @@ __global__ void k_pack_coords(const double *c3, int n, int dim, double *xp)
     for (int a = 0; a < dim; a++) xp[(size_t)i * dim + a] = c3[3 * (size_t)i + a];
 }
 
-__global__ void k_minmax(const double *c3, int n, double *out)
+// Bounding box of the points in two passes: 256-thread blocks reduce
+// block-strided slices to partials part[6*b + {minx,miny,minz,maxx,maxy,maxz}],
+// then one block folds the partials.  min and max are exact, so the split
+// does not change the result.
+constexpr int kMinmaxThreads = 256, kMinmaxParts = 256;
+
+__device__ void minmax_block(double (*s)[kMinmaxThreads], double *v)
 {
-    // one block of 1024: out = {minx,miny,minz,maxx,maxy,maxz}
-    __shared__ double s[6][1024];
-    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int i = threadIdx.x; i < n; i += 1024)
-        for (int a = 0; a < 3; a++) {
-            const double x = c3[3 * (size_t)i + a];
-            v[a] = fmin(v[a], x);
-            v[3 + a] = fmax(v[3 + a], x);
-        }
     for (int a = 0; a < 6; a++) s[a][threadIdx.x] = v[a];
     __syncthreads();
-    for (int d = 512; d > 0; d >>= 1) {
+    for (int d = kMinmaxThreads / 2; d > 0; d >>= 1) {
         if ((int)threadIdx.x < d)
             for (int a = 0; a < 6; a++)
                 s[a][threadIdx.x] = (a < 3) ? fmin(s[a][threadIdx.x], s[a][threadIdx.x + d])
                                             : fmax(s[a][threadIdx.x], s[a][threadIdx.x + d]);
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kMinmaxThreads) k_minmax_part(const double *c3, int n, double *part)
+{
+    __shared__ double s[6][kMinmaxThreads];
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (long long i = (long long)blockIdx.x * kMinmaxThreads + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kMinmaxThreads)
+        for (int a = 0; a < 3; a++) {
+            const double x = c3[3 * (size_t)i + a];
+            v[a] = fmin(v[a], x);
+            v[3 + a] = fmax(v[3 + a], x);
+        }
+    minmax_block(s, v);
+    if (threadIdx.x < 6) part[6 * (size_t)blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kMinmaxThreads) k_minmax_fold(const double *part, int nparts, double *out)
+{
+    __shared__ double s[6][kMinmaxThreads];
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int p = threadIdx.x; p < nparts; p += kMinmaxThreads)
+        for (int a = 0; a < 6; a++)
+            v[a] = (a < 3) ? fmin(v[a], part[6 * (size_t)p + a]) : fmax(v[a], part[6 * (size_t)p + a]);
+    minmax_block(s, v);
     if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
@@ -416,8 +438,10 @@ extern "C" int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out)
     hipStream_t st = ctx->stream;
     const int n = ctx->n, dim = ctx->dim;
     dbuf<double> mm;
-    PXG_HIP(mm.alloc(6));
-    hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, st, ctx->coords.p, n, mm.p);
+    const int nparts = (int)std::min<long long>(kMinmaxParts, ((long long)n + kMinmaxThreads - 1) / kMinmaxThreads);
+    PXG_HIP(mm.alloc(6 + 6 * (size_t)nparts));
+    hipLaunchKernelGGL(k_minmax_part, dim3(nparts), dim3(kMinmaxThreads), 0, st, ctx->coords.p, n, mm.p + 6);
+    hipLaunchKernelGGL(k_minmax_fold, dim3(1), dim3(kMinmaxThreads), 0, st, mm.p + 6, nparts, mm.p);
     double h[6];
     PXG_HIP(hipMemcpyAsync(h, mm.p, sizeof(h), hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
