@@ -68,3 +68,31 @@ def test_block_families_and_weights_full_size():
     print("polynomial reproduction over %d nodes: max rel %.2e" % (len(nodes), err))
     assert err <= 1e-12
     ctx.close()
+
+
+def test_block_rigid_and_linear_fields_full_size():
+    """Full-size step arithmetic (pxg_frac_accel, FP64): a rigid translation
+    gives exactly zero acceleration and a linear field zero to round-off
+    (the second-derivative operators reproduce linear fields), measured
+    against the acceleration of a random field of the same amplitude."""
+    from perix_amd import problems
+    s = problems.explicit_plate(3, 160, 80, 40, hi=(0.16, 0.08, 0.04), cracks=[],
+                                box=(0.04, 0.06, 1.0, -1.0, 1.0, -1.0), nsteps=10)
+    ctx, cl = s.ctx, s.cloud
+    print("block: %d nodes, %d bonds" % (cl.n, ctx.nb), flush=True)
+    X = cl.coords.reshape(-1, 3)
+    eps = 1.0e-4
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((3, 3))
+    h = np.ones(ctx.nb)
+    a, broke = ctx.frac_accel(np.tile([1e-4, -2e-4, 3e-4], cl.n), h)
+    assert broke == 0 and not np.any(a)
+    a_lin, broke = ctx.frac_accel((eps * X @ A.T).reshape(-1), h)
+    assert broke == 0
+    a_rnd, _ = ctx.frac_accel(rng.standard_normal(cl.n * 3) * eps * s.delta, h)
+    assert np.all(h == 1.0)
+    ratio = np.abs(a_lin).max() / np.abs(a_rnd).max()
+    print("linear field: max |a| %.3e, random field %.3e, ratio %.2e" %
+          (np.abs(a_lin).max(), np.abs(a_rnd).max(), ratio))
+    assert ratio <= 1e-9
+    ctx.close()
