@@ -85,84 +85,98 @@ def timed_run(ctx, s, steps, warmup, profile=False):
     return t1 - t0, done
 
 
-def plate_case(problems, pxg, precision, steps):
+def plate_case(problems, pxg, precision, steps, opts=0):
     t0 = time.perf_counter()
-    s = problems.plate_400x200(precision=precision, nsteps=steps)
+    ctx = pxg.Context(0)
+    ctx.frac_options(opts)
+    s = problems.plate_400x200(precision=precision, nsteps=steps, ctx=ctx)
     s.ctx.sync()
     return s, time.perf_counter() - t0
 
 
-def cpu_reference_baseline(problems, n_nodes, dt, budget_s=12.0):
-    """The reference's own OpenMP explicit loop (oracle/_ref, compiled from the
-    reference sources) on this host, on a bounded sample of the same plate;
-    falls back to the C oracle when that build is absent."""
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    from oracle import refpy
-    if refpy.available():
-        import perix_amd
-        g = perix_amd.StructuredGrid(2, 400, 200, hi=(0.2, 0.1, 1.0))
-        delta = perix_amd.horizon_radius(g, 3.015)
-        G0 = problems.g0_for_stretch(0.01, problems.E_STEEL, delta, 2)
+def plate_throughput(problems, pxg, K, W):
+    """The plate's timed runs: W warm-up steps, then K steps timed on the host
+    clock with the step kernel's HIP events recorded in the same run, for
+    FP32C and FP64.  The event spans cover the step launches only (no guard
+    polls), so the kernel time is at most the wall time."""
+    out = {}
+    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
+        log("plate %s: setup" % name)
+        s, setup_s = plate_case(problems, pxg, prec, K)
+        sec, done, ms, launches = timed_run(s.ctx, s, K, W, profile=True)
+        assert launches == done and ms <= sec * 1e3 * 1.0001, (ms, sec, launches, done)
+        log("plate %s: %d steps in %.3f s (step kernel %.3f ms)" % (name, done, sec, ms))
+        out[name] = {"sec": sec, "done": done, "setup_s": setup_s, "kernel_ms": ms, "launches": launches,
+                     "n": s.cloud.n, "nb": s.ctx.nb, "dt": s.dt}
+        s.ctx.close()
+    return out
 
-        def deck(nsteps):
-            d = {"Mesh": refpy.mesh_block(2, 400, 200, hi=(0.2, 0.1, 1.0)),
-                 "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
-                 "MeshModify": {"type": "pre_existing_cracks", "treatment": "force_only",
-                                "Cracks": [{"x1": 0.05, "y1": 0.0, "x2": 0.05, "y2": 0.05},
-                                           {"x1": 0.15, "y1": 0.0, "x2": 0.15, "y2": 0.05}]},
-                 "DOFs": ["ux", "uy"],
-                 "ElmtSystem": {"f": {"type": "explicit_pddo_frac", "params": {
-                     "E": problems.E_STEEL, "nu": problems.NU, "rho": problems.RHO, "G0": G0,
-                     "state": "plane_strain", "tension_only": True, "damage": True}}},
-                 "BCSystem": {}, "JobSystem": {"type": "transient", "assemble": "openmp"},
-                 "TimeStepping": {"dt": dt, "total_time": nsteps * dt, "verbose": False}}
-            for w in ("left", "right", "bottom", "top"):
-                d["BCSystem"]["n_" + w] = {"type": "neumann", "phygroup": w + "nodes_ghost",
-                                          "dofs": ["ux", "uy"], "value": [0.0, 0.0]}
-            d["BCSystem"]["impactor"] = {"type": "dirichlet", "phygroup": "bottomnodes_ghost",
-                                         "dofs": ["ux", "uy"], "value": [0.0, 0.0],
-                                         "velocity": [0.0, 16.5], "direct": True,
-                                         "box": {"xmin": 0.075, "xmax": 0.125}}
-            return d
 
-        def run(nsteps):
-            with quiet_stdout():
-                c = refpy.Case(deck(nsteps))
-                steps, setup_s, loop_s, _ = c.explicit_timed()
-                c.close()
-            return steps, setup_s, loop_s
+def plate_error(problems, pxg, nsteps=2000):
+    """FP32C against FP64 over the workload's full 2,000 steps (independent of
+    the timed step count): relative-L2 and max-abs displacement error and the
+    broken-bond mismatch count."""
+    st = {}
+    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
+        s = problems.plate_400x200(precision=prec, nsteps=nsteps)
+        s.ctx.frac_init(s.u0, s.dt)
+        done = s.ctx.frac_run(s.dt, s.total)
+        u, _, h, _ = s.ctx.frac_state()
+        st[name] = (u, h, done)
+        s.ctx.close()
+    (u32, h32, d32), (u64, h64, d64) = st["fp32c"], st["fp64"]
+    assert d32 == d64 == nsteps
+    return {"steps": nsteps, "rel_l2": float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300)),
+            "max_abs_m": float(np.abs(u32 - u64).max()), "broken_bond_mismatch": int((h32 != h64).sum()),
+            "broken_bonds_fp64": int((h64 == 0).sum())}
 
-        steps, _, loop_s = run(10)
-        per_step = loop_s / max(steps, 1)
-        sample = int(min(2000, max(20, budget_s / max(per_step, 1e-6))))
-        steps, setup_s, loop_s = run(sample)
-        return {"value": n_nodes * steps / loop_s, "unit": "node-steps/s", "cores": cores,
-                "kind": "reference",
-                "sample": "reference OpenMP explicit loop (oracle/_ref built from the reference "
-                          "sources), first %d steps of the same 400x200 plate; one-off "
-                          "operator build (%.2f s) excluded" % (steps, setup_s)}
-    from oracle import pxo
-    m = pxo.Mesh(pxo.make_grid(2, 400, 200, hi=(0.2, 0.1, 1.0)))
-    ops, _ = pxo.operators(m)
-    opc = pxo.frac_opcache(m, ops)
-    G0 = problems.g0_for_stretch(0.01, problems.E_STEEL, m.delta, 2)
-    sc = problems.critical_stretch(G0, problems.E_STEEL, m.delta, 2)
-    p = pxo.frac_params(2, problems.E_STEEL, problems.NU, problems.RHO, sc)
-    h = pxo.seed_health(m, segs=[(0.05, 0.0, 0.05, 0.05), (0.15, 0.0, 0.15, 0.05)])
-    specs = [{"type": "neumann", "nodes": m.group(w), "comps": [0, 1]} for w in range(4)]
-    specs.append({"type": "dirichlet", "nodes": m.group(2), "comps": [0, 1], "value": [0, 0],
-                  "velocity": [0, 16.5], "direct": True,
-                  "box": (0.075, 0.125, 1.0, -1.0, 1.0, -1.0)})
-    bcs = pxo.make_bcs(specs)
-    u = np.zeros(2 * m.n)
-    pxo.bc_preset(m, 2, bcs, u, 0.0)
-    uo = np.zeros_like(u)
-    nsteps = 100
-    t0 = time.perf_counter()
-    pxo.explicit_run(m, p, opc, bcs, dt, nsteps * dt, u, uo, h)
-    loop_s = time.perf_counter() - t0
-    return {"value": m.n * nsteps / loop_s, "unit": "node-steps/s", "cores": cores,
-            "kind": "port", "sample": "C oracle OpenMP loop, first %d steps" % nsteps}
+
+def physical_cores():
+    """Distinct physical cores among the CPUs this process may run on."""
+    cores = set()
+    for c in sorted(os.sched_getaffinity(0)):
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            with open(base + "core_id") as a, open(base + "physical_package_id") as b:
+                cores.add((int(b.read()), int(a.read())))
+        except OSError:
+            cores.add((0, c))
+    return len(cores)
+
+
+def cpu_run(case, threads, steps, budget_s=0.0, timeout=300):
+    """oracle/cpu_bench.py in a process of its own with the reference's
+    advised OpenMP placement set before its runtime starts."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    cmd = [sys.executable, "-m", "oracle.cpu_bench", case, "--steps", str(steps)]
+    if budget_s > 0:
+        cmd += ["--budget-s", str(budget_s)]
+    r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError("cpu_bench %s failed: %s" % (case, r.stderr[-400:]))
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baselines():
+    """The reference's OpenMP explicit loop on this host (bounded samples):
+    the plate on the physical cores this process may use (at most
+    OMP_NUM_THREADS when that is set), the plate on one thread, and the block."""
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    phys = physical_cores()
+    threads = min(phys, cap) if cap > 0 else phys
+    plate = cpu_run("plate", threads, 2000, budget_s=12.0)
+    one = cpu_run("plate", 1, 400, budget_s=6.0)
+    block = cpu_run("block", threads, 20)
+    return {"value": plate["node_steps_per_s"], "unit": "node-steps/s", "cores": threads, "kind": plate["kind"],
+            "sample": "reference OpenMP explicit loop (oracle/_ref, built from the reference sources), "
+                      "first %d steps of the same 400x200 plate; one-off element build (%.2f s) excluded"
+                      % (plate["steps"], plate["setup_s"]),
+            "cpu_model": plate["cpu_model"], "physical_cores_available": phys,
+            "omp": "OMP_NUM_THREADS=%d OMP_PLACES=cores OMP_PROC_BIND=close" % threads,
+            "one_thread_node_steps_per_s": one["node_steps_per_s"], "one_thread_steps": one["steps"],
+            "block3d_node_steps_per_s": block["node_steps_per_s"], "block3d_steps": block["steps"],
+            "block3d_setup_s": block["setup_s"], "block3d_kind": block["kind"]}
 
 
 def adjoint_case(problems, pxg, nsteps=500, nsens=64, ndirs=20, seed=2024):
@@ -253,28 +267,36 @@ def block_adjoint_case(problems, pxg, nsteps=200, nsens=64, ckpt=20, seed=2025):
 
 
 def block_case(problems, pxg, steps=100):
-    """3D block 160 x 80 x 40 (556,800 PD nodes), FP32C and FP64."""
+    """3D block 160 x 80 x 40 (556,800 PD nodes): FP32C (shared operator
+    runs), FP32C with every node on its own operators, and FP64.  Each form
+    is timed in one run: host clock around `steps` steps and the step
+    kernel's HIP events of the same run."""
     out = {}
     states = {}
-    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
+    forms = (("fp32c", pxg.FP32C, 0), ("fp32c_own_ops", pxg.FP32C, 1), ("fp64", pxg.FP64, 0))
+    for name, prec, opts in forms:
         t0 = time.perf_counter()
-        s = problems.block_160x80x40(precision=prec, nsteps=steps)
+        ctx = pxg.Context(0)
+        ctx.frac_options(opts)
+        s = problems.block_160x80x40(precision=prec, nsteps=steps, ctx=ctx)
         s.ctx.sync()
         setup = time.perf_counter() - t0
         sec, done, ms, launches = timed_run(s.ctx, s, steps, 5, profile=True)
-        sec2, done2 = timed_run(s.ctx, s, steps, 0)
+        assert launches == done and ms <= sec * 1e3 * 1.0001, (ms, sec, launches, done)
         u, _, h, _ = s.ctx.frac_state()
         states[name] = (u, h)
         nb, n = s.ctx.nb, s.cloud.n
         avg_ms = ms / max(launches, 1)
-        gbs = frac_bytes(nb, n, 3, prec) / (avg_ms * 1e-3) / 1e9
         alg = frac_bytes(nb, n, 3, prec)
-        out[name] = {"node_steps_per_s": n * done2 / sec2, "ms_per_step": sec2 * 1e3 / done2,
+        gbs = alg / (avg_ms * 1e-3) / 1e9
+        traffic = profiled_traffic("frac_step_block160x80x40", name)
+        out[name] = {"node_steps_per_s": n * done / sec, "ms_per_step": sec * 1e3 / done,
                      "step_kernel_ms": avg_ms, "setup_s": setup,
                      "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": gbs / HBM_PEAK_GBS,
-                                  "traffic": profiled_traffic("frac_step_block160x80x40", name),
-                                  "kernel": "k_frac_step<3,%s>" % name,
+                                  "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                                  "counter_frac": (traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                                   if traffic else None),
+                                  "kernel": "k_frac_step<3,%s>" % ("fp64" if prec == pxg.FP64 else "fp32c"),
                                   "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms}}
         if name == "fp64":
             # output projections of the final state (device, n or 9n doubles)
@@ -293,57 +315,88 @@ def block_case(problems, pxg, steps=100):
         pst, tiled, nclasses, nshared = s.ctx.frac_layout()
         out[name]["layout"] = {"bond_slots": pst, "tiled": bool(tiled), "shared_operator_runs": nclasses,
                                "nodes_on_shared_runs": nshared}
-        if name == "fp32c":
-            out[name]["roofline"]["note"] = (
-                "achieved prices the launch at the model's per-bond bytes (id, 6 FP32 operators, health); "
-                "the kernel reads the operators of nodes on shared runs from L2, so its measured HBM "
-                "traffic (traffic) is well below that figure")
         out["nodes"] = n
         out["bonds"] = nb
         s.ctx.close()
-    u32, h32 = states["fp32c"]
     u64, h64 = states["fp64"]
-    out["fp32c_rel_l2_err"] = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
-    out["fp32c_broken_bond_mismatch"] = int((h32 != h64).sum())
-    out["fp32c_over_fp64"] = out["fp32c"]["node_steps_per_s"] / out["fp64"]["node_steps_per_s"]
+    for name in ("fp32c", "fp32c_own_ops"):
+        u32, h32 = states[name]
+        out[name]["rel_l2_err_vs_fp64"] = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
+        out[name]["broken_bond_mismatch"] = int((h32 != h64).sum())
+        out[name]["over_fp64"] = out[name]["node_steps_per_s"] / out["fp64"]["node_steps_per_s"]
     out["steps"] = steps
     return out
 
 
+def _observables(s, pxg):
+    """Pointwise state and non-pointwise observables of a block run: broken
+    bonds, the L2 norm of the nodal damage field, kinetic energy of the bulk
+    nodes with v = (u^n - u^{n-1}) / dt."""
+    u, uo, h, _ = s.ctx.frac_state()
+    dmg = s.ctx.frac_project("damage").ravel()
+    nb = s.cloud.nbulk
+    v = (u - uo).reshape(-1, 3)[:nb] / s.dt
+    ke = 0.5 * RHO_BLOCK * float((s.cloud.vols[:nb] * (v * v).sum(axis=1)).sum())
+    return u, h, {"broken": int((h == 0).sum()), "damage_l2": float(np.linalg.norm(dmg[:nb])), "kinetic_J": ke}
+
+
+RHO_BLOCK = 8000.0
+
+
 def block_long_run(problems, pxg, steps=1000, chunk=100):
     """FP32C against FP64 over the block's full 1,000-step run, every 100
-    steps, beside a control: FP64 against FP64 started 1e-15 m away (random
-    per-component perturbation of u0).  From about step 400 the block
-    fragments under the impactor and the trajectories become chaotic, so the
-    control bounds what any pointwise comparison can show there."""
-    runs = {}
-    for name, prec, eps in (("fp64", pxg.FP64, 0.0), ("fp32c", pxg.FP32C, 0.0), ("fp64_perturbed", pxg.FP64, 1e-15)):
-        s = problems.block_160x80x40(precision=prec, nsteps=steps)
+    steps, beside controls that separate the causes of a divergence:
+      fp32c_own_ops  FP32C with every node on its own FP32 operators (no
+                     shared runs);
+      fp64_f32ops    FP64 arithmetic on operators rounded to FP32 (FP32C's
+                     operator rounding without its FP32 family sums);
+      fp64_eps15     FP64 started 1e-15 m away (random per component);
+      fp64_epsc      FP64 started away by FP32C's own error after the first
+                     chunk (max |u_fp32c - u_fp64| at step 100, random per
+                     component).
+    Each row holds the relative-L2 displacement error and the broken-bond
+    mismatch count against FP64, and the broken-bond count, damage-field L2
+    and kinetic energy of every run."""
+    def make(prec, opts, eps):
+        ctx = pxg.Context(0)
+        ctx.frac_options(opts)
+        s = problems.block_160x80x40(precision=prec, nsteps=steps, ctx=ctx)
         u0 = s.u0.copy()
         if eps:
             u0 += eps * np.random.RandomState(1).standard_normal(u0.size)
         s.ctx.frac_init(u0, s.dt)
-        runs[name] = s
+        return s
+
+    # first chunk of FP32C vs FP64 fixes the control's perturbation size
+    a64, a32 = make(pxg.FP64, 0, 0.0), make(pxg.FP32C, 0, 0.0)
+    for s in (a64, a32):
+        s.ctx.frac_run(s.dt, s.total, nsteps_max=chunk)
+    epsc = float(np.abs(a32.ctx.frac_state()[0] - a64.ctx.frac_state()[0]).max())
+    for s in (a64, a32):
+        s.ctx.close()
+    runs = {"fp64": make(pxg.FP64, 0, 0.0), "fp32c": make(pxg.FP32C, 0, 0.0),
+            "fp32c_own_ops": make(pxg.FP32C, pxg.Context.FRAC_NO_SHARED_RUNS, 0.0),
+            "fp64_f32ops": make(pxg.FP64, pxg.Context.FRAC_FP32_OPERATORS, 0.0),
+            "fp64_eps15": make(pxg.FP64, 0, 1e-15), "fp64_epsc": make(pxg.FP64, 0, epsc)}
     rows = []
     for c in range(steps // chunk):
         st = {}
         for name, s in runs.items():
             s.ctx.frac_run(s.dt, s.total, nsteps_max=chunk)
-            u, _, h, _ = s.ctx.frac_state()
-            st[name] = (u, h)
-        u64, h64 = st["fp64"]
+            st[name] = _observables(s, pxg)
+        u64, h64, ob64 = st["fp64"]
         nrm = max(np.linalg.norm(u64), 1e-300)
-        row = {"step": (c + 1) * chunk, "broken_fp64": int((h64 == 0).sum()), "max_abs_u_fp64": float(np.abs(u64).max())}
-        for name in ("fp32c", "fp64_perturbed"):
-            u, h = st[name]
-            row[name + "_rel_l2"] = float(np.linalg.norm(u - u64) / nrm)
-            row[name + "_broken_bond_mismatch"] = int((h != h64).sum())
+        row = {"step": (c + 1) * chunk, "fp64": ob64}
+        for name, (u, h, ob) in st.items():
+            if name == "fp64":
+                continue
+            row[name] = dict(ob, rel_l2=float(np.linalg.norm(u - u64) / nrm), mismatch=int((h != h64).sum()))
         rows.append(row)
-        log("block long run", row)
+        log("block long run", json.dumps(row))
     for s in runs.values():
         s.ctx.close()
-    return {"workload": "160x80x40 block, 1000 steps, FP32C and a 1e-15 m perturbed FP64 run against FP64",
-            "chunks": rows}
+    return {"workload": "160x80x40 block, 1000 steps, FP32C and controls against FP64",
+            "control_eps_m": epsc, "chunks": rows}
 
 
 def gweight_model(n, nb, dim, nop):
@@ -845,33 +898,41 @@ def main():
     if args.only_gweight:
         print(json.dumps({"gweight_build": gweight_case(problems, pxg)}))
         return 0
-    res = {}
-    for prec, name in ((pxg.FP32C, "fp32c"), (pxg.FP64, "fp64")):
-        log("plate %s: setup" % name)
-        s, setup_s = plate_case(problems, pxg, prec, K)
-        ctx = s.ctx
-        sec, done = timed_run(ctx, s, K, W)
-        log("plate %s: %d steps in %.3f s" % (name, done, sec))
-        sec_p, done_p, ms, launches = timed_run(ctx, s, K, 0, profile=True)
-        u, _, h, _ = ctx.frac_state()
-        res[name] = {"sec": sec, "done": done, "setup_s": setup_s, "kernel_ms": ms,
-                     "launches": launches, "u": u, "h": h, "n": s.cloud.n, "nb": ctx.nb}
-        ctx.close()
-    n, nb = res["fp32c"]["n"], res["fp32c"]["nb"]
-    f32, f64 = res["fp32c"], res["fp64"]
+    pt = plate_throughput(problems, pxg, K, W)
+    f32, f64 = pt["fp32c"], pt["fp64"]
+    n, nb = f32["n"], f32["nb"]
     value = n * f32["done"] / f32["sec"]
     v64 = n * f64["done"] / f64["sec"]
-    u32, u64 = f32["u"], f64["u"]
-    rel = float(np.linalg.norm(u32 - u64) / max(np.linalg.norm(u64), 1e-300))
-    mism = int((f32["h"] != f64["h"]).sum())
     avg_ms = f32["kernel_ms"] / max(f32["launches"], 1)
+    avg64 = f64["kernel_ms"] / max(f64["launches"], 1)
     bytes32 = frac_bytes(nb, n, 2, 1)
     achieved = bytes32 / (avg_ms * 1e-3) / 1e9
-    avg64 = f64["kernel_ms"] / max(f64["launches"], 1)
     achieved64 = frac_bytes(nb, n, 2, 0) / (avg64 * 1e-3) / 1e9
     traffic = profiled_traffic("frac_step_plate400x200", "fp32c")
-
-    dt = f32.get("dt")
+    if args.only_plate:
+        print(json.dumps({"value": value, "fp64": v64, "kernel_ms": avg_ms, "kernel64_ms": avg64}))
+        return 0
+    log("plate FP32C vs FP64 over 2000 steps")
+    err = plate_error(problems, pxg)
+    extra = {}
+    if not args.no_cpu:
+        log("cpu baseline (reference OpenMP loop in a process of its own)")
+        try:
+            extra["cpu_baseline"] = cpu_baselines()
+        except Exception as e:   # the baseline is reported, never required
+            extra["cpu_baseline"] = {"value": None, "unit": "node-steps/s", "cores": physical_cores(),
+                                     "kind": "reference", "sample": "failed: %s" % e}
+    if not args.no_extra:
+        log("adjoint (200x100, 500 steps)")
+        adj = adjoint_case(problems, pxg)
+        log("adjoint (160x80x40 block, body force, 200 steps)")
+        adjb = block_adjoint_case(problems, pxg)
+        log("3D block (160x80x40)")
+        blk = block_case(problems, pxg)
+        log("assembly")
+        asm = assembly_case(problems, pxg)
+        log("family search and g-weight build")
+        gw = gweight_case(problems, pxg)
     out = {
         "metric": METRIC,
         "value": value,
@@ -885,49 +946,47 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 compensated family sums + f64 damage/update",
         "data": "synthetic",
-        "config": {"workload": "explicit_pddo_frac 2D plate 0.2x0.1 m, 400x200 cells "
-                               "(81,200 PD nodes, %d directed bonds), two force-only edge "
-                               "notches, 16.5 m/s impactor, %d CFL-limited steps" % (nb, K),
+        "config": {"workload": "explicit_pddo_frac 2D plate 0.2x0.1 m, 400x200 cells (81,200 PD nodes, "
+                               "%d directed bonds), two force-only edge notches, 16.5 m/s impactor; "
+                               "throughput over %d CFL-limited steps, FP32C error over the workload's "
+                               "2000 steps" % (nb, K),
                    "nodes": n, "bonds": nb, "horizon": "3.015 dx", "order": 2},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_frac_step<2,fp32c>", "algorithmic_bytes_per_launch": bytes32,
                      "avg_launch_ms": avg_ms,
-                     "note": "the 43.5 MB working set of the 2D plate stays in the 256 MB "
-                             "Infinity Cache, so this kernel is latency/launch-bound; the HBM "
-                             "roofline claim is made on the 3D block (block3d.fp32c.roofline)"},
-        "fp64": {"node_steps_per_s": v64, "ms_per_step": f64["sec"] * 1e3 / f64["done"],
-                 "step_kernel_ms": avg64, "step_kernel_GBs": achieved64},
-        "fp32c_over_fp64": value / v64,
-        "fp32c_error_vs_fp64": {"rel_l2": rel, "max_abs": float(np.abs(u32 - u64).max()),
-                                "broken_bond_mismatch": mism,
-                                "broken_bonds_fp64": int((f64["h"] == 0).sum())},
-        "setup_s": {"fp32c": f32["setup_s"], "fp64": f64["setup_s"]},
+                     "note": "HIP events of the timed run; the 43.5 MB working set stays in the 256 MB "
+                             "Infinity Cache (latency-bound); the HBM claim is block3d's"},
     }
-    if not args.no_cpu and not args.only_plate:
-        log("cpu baseline")
-        s0 = problems.plate_400x200(precision=pxg.FP64, nsteps=K)
-        dt = s0.dt
-        s0.ctx.close()
-        try:
-            out["cpu_baseline"] = cpu_reference_baseline(problems, n, dt)
-        except Exception as e:   # the baseline is reported, never required
-            out["cpu_baseline"] = {"value": None, "unit": "node-steps/s", "cores": os.cpu_count(),
-                                   "kind": "reference", "sample": "failed: %s" % e}
-    if not args.no_extra and not args.only_plate:
-        log("adjoint (200x100, 500 steps)")
-        out["adjoint"] = adjoint_case(problems, pxg)
-        log("adjoint (160x80x40 block, body force, 200 steps)")
-        out["adjoint_block3d_body_force"] = block_adjoint_case(problems, pxg)
-        log("3D block (160x80x40)")
-        out["block3d"] = block_case(problems, pxg)
-        log("3D block, 1000-step comparison")
-        out["block3d"]["long_run"] = block_long_run(problems, pxg)
-        log("assembly (Poisson, diffusion, dynamic fracture, Cahn-Hilliard, stress-coupled Cahn-Hilliard)")
-        out["assembly"] = assembly_case(problems, pxg)
-        log("family search and g-weight build (plate, block)")
-        out["gweight_build"] = gweight_case(problems, pxg)
+    out.update(extra)
+    # the metric's components, last on the line
+    out["fp64"] = {"node_steps_per_s": v64, "ms_per_step": f64["sec"] * 1e3 / f64["done"],
+                   "step_kernel_ms": avg64, "step_kernel_GBs": achieved64}
+    out["fp32c_over_fp64"] = value / v64
+    out["fp32c_error_vs_fp64"] = err
+    if not args.no_extra:
+        out["adjoint_over_forward"] = {
+            "plate200x100_dJdE_500_steps": adj["adjoint_over_forward"],
+            "plate200x100_fd_max_rel_20dirs": adj["fd_max_rel_err_20dirs"],
+            "block3d_dJdb_200_steps": adjb["adjoint_over_forward"],
+            "block3d_fd_max_rel": adjb["fd_max_rel_err_3comps"]}
+        b32, bown, b64 = blk["fp32c"], blk["fp32c_own_ops"], blk["fp64"]
+        out["block3d"] = {
+            "fp32c_ms": b32["ms_per_step"], "fp32c_kernel_ms": b32["step_kernel_ms"],
+            "fp32c_frac_model": b32["roofline"]["frac"], "fp32c_frac_counters": b32["roofline"]["counter_frac"],
+            "fp32c_own_ops_kernel_ms": bown["step_kernel_ms"], "fp32c_own_ops_frac": bown["roofline"]["frac"],
+            "fp64_ms": b64["ms_per_step"], "fp64_kernel_ms": b64["step_kernel_ms"],
+            "fp64_frac": b64["roofline"]["frac"], "fp32c_over_fp64": b32["over_fp64"],
+            "fp32c_own_ops_over_fp64": bown["over_fp64"],
+            "fp32c_err_100_steps": [b32["rel_l2_err_vs_fp64"], b32["broken_bond_mismatch"]],
+            "node_steps_per_s_fp32c": b32["node_steps_per_s"]}
+        out["assembly_nnz_per_s"] = {k: round(v["nnz_per_s"] / 1e9, 2) for k, v in asm.items()
+                                     if isinstance(v, dict) and "nnz_per_s" in v}
+        out["assembly_nnz_per_s"]["unit"] = "1e9 nnz/s"
+        out["gweight_build_ms"] = {k: round(v["gweight_ms"], 3) for k, v in gw.items()}
+    sys.stderr.flush()
     print(json.dumps(out))
+    sys.stdout.flush()
     return 0
 
 

@@ -46,9 +46,12 @@ typedef struct pxg_ctx pxg_ctx;
 int pxg_create(pxg_ctx **out, int device);
 void pxg_destroy(pxg_ctx *ctx);
 const char *pxg_last_error(void);
-/* Host-test fault injection: the next entry point that uses the device
- * returns PXG_ERR_FAULT without launching (code 0 disables).  Mirrors the
- * `return false` seam failure of CudaAssembleKernels.cu:1112,1193. */
+/* Host-test fault injection: the next entry point that would launch work
+ * returns PXG_ERR_FAULT before any launch and consumes the fault, so the
+ * context's device state is unchanged and later calls work (code 0 clears a
+ * pending fault).  Mirrors the `return false` seam failure of
+ * CudaAssembleKernels.cu:1112,1193; the latch that keeps a failed explicit
+ * run on the CPU (ExplicitPDDOFracElement.cpp:225-234) is the caller's. */
 int pxg_set_fault(pxg_ctx *ctx, int code);
 /* Blocks until all queued work of the context has finished. */
 int pxg_sync(pxg_ctx *ctx);
@@ -187,6 +190,19 @@ typedef struct {
  * family uses C(E_i); C11/C12/C66 of params are ignored). */
 int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precision,
                    const double *node_E, double nu, int plane_stress);
+
+/* Diagnostic options of the explicit set-up (no reference counterpart; they
+ * serve the FP32C accuracy study).  The bits take effect at the next
+ * pxg_frac_setup of the context:
+ *   PXG_FRAC_NO_SHARED_RUNS  every node reads its own FP32 operators (the 3D
+ *                            FP32C step otherwise lets nodes whose FP64
+ *                            operators agree to 1e-12 share one run);
+ *   PXG_FRAC_FP32_OPERATORS  an FP64 run on operators rounded to FP32 and
+ *                            back: the operator rounding of FP32C without
+ *                            its FP32 family sums. */
+#define PXG_FRAC_NO_SHARED_RUNS 1
+#define PXG_FRAC_FP32_OPERATORS 2
+int pxg_frac_options(pxg_ctx *ctx, int flags);
 
 /* BC presets run after every central-difference update
  * (BCSystem::presetSolution, BCSystem.cpp:114-129 -> DirichletBC.cpp:107-147,

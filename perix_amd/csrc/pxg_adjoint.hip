@@ -393,19 +393,24 @@ static int frozen_segment(pxg_ctx *ctx, double *seg, size_t nd, int k0, int k1, 
 }
 
 // forward with frozen health from u0: fills checkpoints (pairs u^{k-1}, u^k at
-// k = 0, ckpt, 2ckpt, ...) when asked and returns u^K in F.u[...]
+// k = 0, ckpt, 2ckpt, ...) when asked and returns u^K in the workspace's own
+// ping-pong buffers, so the state of the last pxg_frac_run (FracState::u,
+// cur, step, t) stays as it was
 static int adj_forward(pxg_ctx *ctx, const double *u0, const double *v0, double dt, int K, int ckpt,
                        double *cks, double **uK_out, double *tk)
 {
     FracState &F = ctx->frac;
+    AdjWork &A = F.adj;
     hipStream_t st = ctx->stream;
     const size_t nd = (size_t)ctx->n * ctx->dim;
-    PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(A.fu[0].alloc(nd));
+    PXG_HIP(A.fu[1].alloc(nd));
+    PXG_HIP(hipMemcpyAsync(A.fu[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     else PXG_HIP(hipMemsetAsync(F.accel.p, 0, sizeof(double) * nd, st));
-    int rc = launch_step(ctx, MODE_INIT, F.u[0].p, F.accel.p, F.u[1].p, dt, 0.5 * dt * dt, 0, false, 0);
+    int rc = launch_step(ctx, MODE_INIT, A.fu[0].p, F.accel.p, A.fu[1].p, dt, 0.5 * dt * dt, 0, false, 0);
     if (rc) return rc;
-    double *um = F.u[1].p, *uc = F.u[0].p;
+    double *um = A.fu[1].p, *uc = A.fu[0].p;
     double t = 0.0;
     for (int k = 1; k <= K; k++) {
         if (cks && (k - 1) % ckpt == 0) {
@@ -435,7 +440,7 @@ extern "C" int pxg_frac_loss(pxg_ctx *ctx, const double *u0, const double *v0, d
               "pxg_frac_loss: needs pxg_frac_setup with a per-node E");
     PXG_CHECK(u0 && dt > 0.0 && nsteps >= 1 && nsens >= 0 && (nsens == 0 || sens) && comp >= 0 &&
               comp < ctx->dim && J_out, PXG_ERR_ARG, "pxg_frac_loss: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_loss: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_loss");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     AdjWork &A = F.adj;
@@ -481,7 +486,7 @@ extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double 
               "pxg_frac_adjoint: needs pxg_frac_setup with a per-node E (and a recorded forward run)");
     PXG_CHECK(u0 && dt > 0.0 && nsteps >= 1 && ckpt >= 1 && nsens >= 0 && (nsens == 0 || sens) &&
               comp >= 0 && comp < ctx->dim, PXG_ERR_ARG, "pxg_frac_adjoint: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_adjoint: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_adjoint");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     AdjWork &A = F.adj;

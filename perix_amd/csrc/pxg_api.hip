@@ -4,6 +4,26 @@
 namespace pxg {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+void invalidate(pxg_ctx *ctx, int what)
+{
+    if (what & INV_OPS) {
+        // bond data sized and computed for the old cloud / families / operators
+        ctx->ops_gen++;
+        ctx->lap_kind = LAP_NONE;
+        ctx->ch_ready = 0;
+        ctx->frac.ready = 0;
+        ctx->dyn.ready = 0;
+        ctx->sch.ready = 0;
+        ctx->fsch.ready = 0;
+    }
+    if (what & INV_CRACKS) {
+        // element health started from the old crack seed
+        ctx->frac.ready = 0;
+        ctx->dyn.ready = 0;
+        ctx->fsch.ready = 0;
+    }
+}
 } // namespace pxg
 
 using namespace pxg;

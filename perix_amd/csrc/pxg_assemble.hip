@@ -140,14 +140,20 @@ using namespace pxg;
 
 static unsigned gridb(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ctx->lap is shared: the plain pdLaplacian of the Poisson and diffusion
+// assemblies, or the lambda-corrected one of the Cahn-Hilliard geometry.  It
+// is rebuilt unless it holds the plain Laplacian of the current operators.
 static int ensure_lap_poisson(pxg_ctx *ctx)
 {
-    if (ctx->lap.n == (size_t)ctx->nb && ctx->lap.p && !ctx->ch_ready) return PXG_OK;
+    if (ctx->lap_kind == LAP_POISSON && ctx->lap_gen == ctx->ops_gen && ctx->lap.n == (size_t)ctx->nb && ctx->lap.p)
+        return PXG_OK;
     PXG_HIP(ctx->lap.alloc(ctx->nb));
     hipLaunchKernelGGL(k_lap, dim3(gridb(ctx->nb, 256)), dim3(256), 0, ctx->stream, ctx->nb, ctx->dim, ctx->nop,
                        ctx->ops.p, ctx->lap.p);
     PXG_HIP(hipGetLastError());
-    ctx->ch_ready = 0;
+    ctx->lap_kind = LAP_POISSON;
+    ctx->lap_gen = ctx->ops_gen;
+    ctx->ch_ready = 0;          // the Cahn-Hilliard geometry was overwritten
     return PXG_OK;
 }
 
@@ -157,7 +163,7 @@ extern "C" int pxg_assemble_poisson_dev(pxg_ctx *ctx, double sigma, const double
     PXG_CHECK(ctx && ctx->nop > 0 && ctx->nnz > 0 && ctx->csr_ndof == 1, PXG_ERR_STATE,
               "pxg_assemble_poisson: needs operators and a 1-dof CSR pattern");
     PXG_CHECK(u_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_poisson: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_poisson: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_poisson");
     PXG_HIP(hipSetDevice(ctx->device));
     int rc = ensure_lap_poisson(ctx);
     if (rc) return rc;
@@ -199,7 +205,7 @@ extern "C" int pxg_assemble_diffusion_dev(pxg_ctx *ctx, double D, const double *
               "pxg_assemble_diffusion: needs operators and a 1-dof CSR pattern");
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev && dt > 0.0, PXG_ERR_ARG,
               "pxg_assemble_diffusion: bad arguments (the element is transient-only, dt > 0)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_diffusion: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_diffusion");
     PXG_HIP(hipSetDevice(ctx->device));
     int rc = ensure_lap_poisson(ctx);
     if (rc) return rc;
@@ -238,7 +244,7 @@ extern "C" int pxg_assemble_diffusion(pxg_ctx *ctx, double D, const double *f_no
 
 int pxg::ch_geometry(pxg_ctx *ctx)
 {
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_ch_setup: injected fault");
+    PXG_FAULT(ctx, "pxg_ch_setup");
     PXG_HIP(hipSetDevice(ctx->device));
     PXG_HIP(ctx->lap.alloc(ctx->nb));
     PXG_HIP(ctx->rev.alloc(ctx->nb));
@@ -248,6 +254,8 @@ int pxg::ch_geometry(pxg_ctx *ctx)
                        ctx->rev.p, ctx->isghost.p);
     PXG_HIP(hipGetLastError());
     PXG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->lap_kind = LAP_CH;
+    ctx->lap_gen = ctx->ops_gen;
     ctx->ch_ready = 1;
     return PXG_OK;
 }
@@ -279,7 +287,7 @@ extern "C" int pxg_assemble_ch_dev(pxg_ctx *ctx, double kappa, double chi, doubl
     PXG_CHECK(ctx->csr_ndof == 2, PXG_ERR_STATE, "pxg_assemble_ch: needs the 2-dof CSR pattern");
     PXG_CHECK(dt > 0.0, PXG_ERR_ARG, "pxg_assemble_ch: Cahn-Hilliard is transient-only (dt > 0)");
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_ch: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_ch: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_ch");
     PXG_HIP(hipSetDevice(ctx->device));
     return launch_ch_rows(ctx, kappa, chi, M0, dt, u_dev, uold_dev, kvals_dev, rhs_dev);
 }

@@ -96,7 +96,7 @@ extern "C" int pxg_csr_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
     PXG_CHECK(ctx && ctx->nnz > 0 && ctx->csr_ndof > 0, PXG_ERR_STATE,
               "pxg_csr_set_bcs: call pxg_build_csr first");
     PXG_CHECK(nbc >= 0 && (nbc == 0 || bcs), PXG_ERR_ARG, "pxg_csr_set_bcs: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_csr_set_bcs: injected fault");
+    PXG_FAULT(ctx, "pxg_csr_set_bcs");
     PXG_HIP(hipSetDevice(ctx->device));
     CsrBcProgram &P = ctx->csrbc;
     P.set = 0;
@@ -202,7 +202,7 @@ extern "C" int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev
 {
     PXG_CHECK(ctx && ctx->csrbc.set, PXG_ERR_STATE, "pxg_csr_apply_bcs: call pxg_csr_set_bcs first");
     PXG_CHECK(u_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_csr_apply_bcs: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_csr_apply_bcs: injected fault");
+    PXG_FAULT(ctx, "pxg_csr_apply_bcs");
     PXG_HIP(hipSetDevice(ctx->device));
     const CsrBcProgram &P = ctx->csrbc;
     if (P.count == 0) return PXG_OK;

@@ -82,7 +82,7 @@ extern "C" int pxg_dynfrac_setup(pxg_ctx *ctx, const pxg_frac_params *p, const d
     PXG_CHECK(ctx->nnz > 0 && ctx->csr_ndof == ctx->dim && p->dim == ctx->dim, PXG_ERR_STATE,
               "pxg_dynfrac_setup: needs a CSR pattern with dim dofs per node");
     PXG_CHECK(p->rho > 0.0, PXG_ERR_ARG, "pxg_dynfrac_setup: rho must be > 0");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_dynfrac_setup: injected fault");
+    PXG_FAULT(ctx, "pxg_dynfrac_setup");
     PXG_HIP(hipSetDevice(ctx->device));
     DynFracState &D = ctx->dyn;
     hipStream_t st = ctx->stream;
@@ -108,7 +108,7 @@ extern "C" int pxg_dynfrac_commit_dev(pxg_ctx *ctx, const double *uold_dev, doub
 {
     PXG_CHECK(ctx && ctx->dyn.ready, PXG_ERR_STATE, "pxg_dynfrac_commit: call pxg_dynfrac_setup first");
     PXG_CHECK(uold_dev && dt_prev > 0.0, PXG_ERR_ARG, "pxg_dynfrac_commit: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_dynfrac_commit: injected fault");
+    PXG_FAULT(ctx, "pxg_dynfrac_commit");
     PXG_HIP(hipSetDevice(ctx->device));
     DynFracState &D = ctx->dyn;
     const unsigned g = gridd(ctx->n, 128);
@@ -145,7 +145,7 @@ extern "C" int pxg_assemble_dynfrac_dev(pxg_ctx *ctx, double dt, const double *u
     PXG_CHECK(ctx->csr_ndof == ctx->dim, PXG_ERR_STATE, "pxg_assemble_dynfrac: CSR pattern changed");
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev && dt > 0.0, PXG_ERR_ARG,
               "pxg_assemble_dynfrac: bad arguments (transient element, dt > 0)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_dynfrac: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_dynfrac");
     PXG_HIP(hipSetDevice(ctx->device));
     return launch_dyn_rows(ctx, dt, u_dev, uold_dev, kvals_dev, rhs_dev);
 }

@@ -151,6 +151,13 @@ __global__ void k_frac_ops(int n, int dim, int nop, const int *off, const double
     }
 }
 
+// PXG_FRAC_FP32_OPERATORS: the FP64 operators rounded to FP32 and back
+__global__ void k_round_f32(size_t count, double *P)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < count) P[q] = (double)(float)P[q];
+}
+
 // Bond state (health, break steps) lives in the step kernel's stream
 // layout: CSR bond t of node i at t + bsh[i] (bsh null: CSR order).  These
 // kernels convert at the API boundary; they are not on the step path.
@@ -421,7 +428,7 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     // incoherent rounding errors into a coherent drift.  On the 2D plate that
     // drift reached 5e-6 by step 1500 and moved the crack's branching; the
     // latency-bound 2D step gains little from it.
-    if (!F.bsh.p || F.precision != PXG_FP32C || ctx->dim != 3) return PXG_OK;
+    if (!F.bsh.p || F.precision != PXG_FP32C || ctx->dim != 3 || (F.opts & PXG_FRAC_NO_SHARED_RUNS)) return PXG_OK;
     const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
     hipStream_t st = ctx->stream;
     dbuf<double> P;                     // FP64 operators, CSR order
@@ -488,7 +495,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_CHECK(p->dim == ctx->dim, PXG_ERR_ARG, "pxg_frac_setup: dim mismatch");
     PXG_CHECK(precision == PXG_FP64 || precision == PXG_FP32C, PXG_ERR_ARG, "pxg_frac_setup: bad precision");
     PXG_CHECK(p->rho > 0.0, PXG_ERR_ARG, "pxg_frac_setup: rho must be > 0");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_setup: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_setup");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     hipStream_t st = ctx->stream;
@@ -519,6 +526,11 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
                        ctx->off.p, ctx->ops.p, node_E ? F.nodeE.p : nullptr, p->C11, p->C12, p->C66, nu,
                        plane_stress, F.ps, F.bsh.p, F.pst, F.P64.p, F.P32.p, node_E ? F.Phat.p : nullptr);
     PXG_HIP(hipGetLastError());
+    if (precision == PXG_FP64 && (F.opts & PXG_FRAC_FP32_OPERATORS)) {
+        const size_t cnt = (size_t)F.pst * nc;
+        hipLaunchKernelGGL(k_round_f32, dim3(gridn((long long)cnt, 256)), dim3(256), 0, st, cnt, F.P64.p);
+        PXG_HIP(hipGetLastError());
+    }
     {
         const int rc = build_op_classes(ctx, node_E ? F.nodeE.p : nullptr);
         if (rc) return rc;
@@ -561,6 +573,14 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     F.cur = 0; F.t = 0.0; F.step = 0; F.dt_prev = 0.0;
     PXG_HIP(hipStreamSynchronize(st));
     F.ready = 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_frac_options(pxg_ctx *ctx, int flags)
+{
+    PXG_CHECK(ctx && (flags & ~(PXG_FRAC_NO_SHARED_RUNS | PXG_FRAC_FP32_OPERATORS)) == 0, PXG_ERR_ARG,
+              "pxg_frac_options: unknown flags");
+    ctx->frac.opts = flags;
     return PXG_OK;
 }
 
@@ -665,7 +685,7 @@ extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inou
 {
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_accel: call pxg_frac_setup first");
     PXG_CHECK(u && accel_out, PXG_ERR_ARG, "pxg_frac_accel: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_accel: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_accel");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     hipStream_t st = ctx->stream;
@@ -700,7 +720,7 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
 {
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_init: call pxg_frac_setup first");
     PXG_CHECK(u0 && dt > 0.0, PXG_ERR_ARG, "pxg_frac_init: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_init: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_init");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     hipStream_t st = ctx->stream;
@@ -741,7 +761,7 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
 {
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_run: call pxg_frac_setup first");
     PXG_CHECK(dt > 0.0 && total > 0.0, PXG_ERR_ARG, "pxg_frac_run: dt and total_time must be positive");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_run: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_run");
     PXG_HIP(hipSetDevice(ctx->device));
     FracState &F = ctx->frac;
     const double tEps = 1.0e-12 * total;
@@ -833,7 +853,7 @@ extern "C" int pxg_frac_project_dev(pxg_ctx *ctx, int which, double *out_dev)
 {
     PXG_CHECK(ctx && ctx->frac.ready, PXG_ERR_STATE, "pxg_frac_project: no explicit state");
     PXG_CHECK(which >= 0 && which <= 3 && out_dev, PXG_ERR_ARG, "pxg_frac_project: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_frac_project: injected fault");
+    PXG_FAULT(ctx, "pxg_frac_project");
     PXG_HIP(hipSetDevice(ctx->device));
     const FracState &F = ctx->frac;
     hipLaunchKernelGGL(ctx->dim == 3 ? k_frac_project<3> : k_frac_project<2>, dim3(gridn(ctx->n, kProjNodes)),

@@ -407,7 +407,7 @@ extern "C" int pxg_set_points(pxg_ctx *ctx, int n, int dim, const double *coords
 {
     PXG_CHECK(ctx && n > 0 && (dim == 2 || dim == 3) && coords && vols && mirror && dxyz,
               PXG_ERR_ARG, "pxg_set_points: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_set_points: injected fault");
+    PXG_FAULT(ctx, "pxg_set_points");
     PXG_HIP(hipSetDevice(ctx->device));
     ctx->n = n; ctx->dim = dim;
     for (int a = 0; a < 3; a++) ctx->dxyz[a] = dxyz[a];
@@ -424,7 +424,8 @@ extern "C" int pxg_set_points(pxg_ctx *ctx, int n, int dim, const double *coords
                        ctx->coords.p, n, dim, ctx->xp.p);
     PXG_HIP(hipGetLastError());
     // invalidate everything derived from the old cloud
-    ctx->nb = 0; ctx->nop = 0; ctx->nnz = 0; ctx->ch_ready = 0; ctx->frac.ready = 0;
+    ctx->nb = 0; ctx->nop = 0; ctx->nnz = 0;
+    invalidate(ctx, INV_OPS | INV_CRACKS);
     PXG_HIP(hipStreamSynchronize(ctx->stream));
     return PXG_OK;
 }
@@ -433,7 +434,7 @@ extern "C" int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out)
 {
     PXG_CHECK(ctx && ctx->n > 0, PXG_ERR_STATE, "pxg_build_families: no points");
     PXG_CHECK(r > 0.0, PXG_ERR_ARG, "pxg_build_families: horizon radius must be positive");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_build_families: injected fault");
+    PXG_FAULT(ctx, "pxg_build_families");
     PXG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int n = ctx->n, dim = ctx->dim;
@@ -503,7 +504,8 @@ extern "C" int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out)
     PXG_HIP(hipStreamSynchronize(st));
     cell.release(); count.release(); start.release(); cursor.release(); members.release(); famc.release();
     ctx->delta = r;
-    ctx->nop = 0; ctx->nnz = 0; ctx->ch_ready = 0; ctx->frac.ready = 0;
+    ctx->nop = 0; ctx->nnz = 0;
+    invalidate(ctx, INV_OPS | INV_CRACKS);
     // crack seeds follow the families
     PXG_HIP(ctx->seed.alloc((size_t)(tot > 0 ? tot : 1)));
     if (ctx->nseg + ctx->nquad > 0) {
@@ -548,7 +550,7 @@ extern "C" int pxg_set_cracks(pxg_ctx *ctx, int nseg, const double *seg, int nqu
             PXG_HIP(hipMemsetAsync(ctx->seed.p, 1, (size_t)ctx->nb, ctx->stream));
         PXG_HIP(hipGetLastError());
     }
-    ctx->frac.ready = 0;
+    invalidate(ctx, INV_CRACKS);
     PXG_HIP(hipStreamSynchronize(ctx->stream));
     return PXG_OK;
 }
@@ -566,7 +568,7 @@ extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
 {
     PXG_CHECK(ctx && ctx->nb > 0, PXG_ERR_STATE, "pxg_build_csr: no families");
     PXG_CHECK(ndof >= 1 && ndof <= 8, PXG_ERR_ARG, "pxg_build_csr: ndof must be 1..8");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_build_csr: injected fault");
+    PXG_FAULT(ctx, "pxg_build_csr");
     PXG_HIP(hipSetDevice(ctx->device));
     ctx->csrbc.set = 0;                 // a BC row program refers to the old pattern
     hipStream_t st = ctx->stream;

@@ -36,6 +36,18 @@ void set_error(const std::string &msg);
         if (!(cond)) { ::pxg::set_error(msg); return (code); }                        \
     } while (0)
 
+// Host-test fault injection (pxg_set_fault): the first entry point that
+// reaches this check after the fault was set returns PXG_ERR_FAULT before
+// any launch and consumes the fault, so the context works again afterwards.
+#define PXG_FAULT(ctx, name)                                                          \
+    do {                                                                              \
+        if ((ctx)->fault) {                                                           \
+            (ctx)->fault = 0;                                                         \
+            ::pxg::set_error(std::string(name) + ": injected fault");                 \
+            return PXG_ERR_FAULT;                                                     \
+        }                                                                             \
+    } while (0)
+
 // Per-node loops over a family whose bonds are independent (conversions,
 // set-up): 16 consecutive lanes walk one family, 8 nodes per 128-thread
 // block, so a family's loads and stores are coalesced.
@@ -141,6 +153,7 @@ struct AdjWork {
     dbuf<double> spg;        // family sums of intact Phat (sweeps without the E-gradient)
     dbuf<int> cur, brk_off, brk_t;
     dbuf<int> ds;            // sensor node ids
+    dbuf<double> fu[2];      // frozen forward's own u ping-pong (the run state in FracState::u stays)
 };
 
 struct FracState {
@@ -149,6 +162,7 @@ struct FracState {
     int dim = 2;
     double C11 = 0, C12 = 0, C66 = 0, rho = 1, b[3] = {0, 0, 0}, sc = 0;
     int tension_only = 1, damage_on = 1;
+    int opts = 0;            // PXG_FRAC_* diagnostic options (pxg_frac_options)
     // per-bond force operators P (CSR bond order): 3 (2D) or 6 (3D) components
     // component-major: component c of bond t at [c*ps + t]
     long long ps = 0;        // CSR-order component stride: nb rounded up to a multiple of 64
@@ -247,6 +261,17 @@ int launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, double dt,
 int launch_dyn_rows(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev, double *kvals_dev,
                     double *rhs_dev);
 
+// What a rebuild invalidates.  INV_OPS: the point cloud, the families or the
+// operators changed, so every element's bond data (explicit, dynamic
+// fracture, both stress-coupled Cahn-Hilliard elements) and the cached
+// Laplacian are stale and each element must be set up again.  INV_CRACKS: the
+// crack seed changed (elements that start from it must be set up again).
+enum { INV_OPS = 1, INV_CRACKS = 2 };
+void invalidate(pxg_ctx *ctx, int what);
+
+// which Laplacian ctx->lap holds
+enum { LAP_NONE = 0, LAP_POISSON = 1, LAP_CH = 2 };
+
 } // namespace pxg
 
 struct pxg_ctx {
@@ -286,6 +311,9 @@ struct pxg_ctx {
     pxg::dbuf<int> diag;            // N: CSR index of (i*ndof, i*ndof)
     pxg::dbuf<int> mslot;           // N: CSR index of mirror block (-1 if in family/none)
     pxg::dbuf<double> lap;          // nb: pdLaplacian per bond (Poisson) / lambda-corrected (CH)
+    int lap_kind = 0;               // LAP_NONE / LAP_POISSON / LAP_CH
+    long long lap_gen = -1;         // ops_gen the Laplacian was built from
+    long long ops_gen = 0;          // bumped by every cloud, family or operator rebuild
     pxg::dbuf<int> rev;             // nb
     pxg::dbuf<unsigned char> isghost;
     int ch_ready = 0;

@@ -331,7 +331,7 @@ extern "C" int pxg_build_operators(pxg_ctx *ctx, int order, double delta, double
     PXG_CHECK(order == 2, PXG_ERR_ARG, "pxg_build_operators: Order 2 is the supported PDDO order "
                                        "(5 slots in 2D, 9 in 3D)");
     PXG_CHECK(delta > 0.0, PXG_ERR_ARG, "pxg_build_operators: delta must be > 0");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_build_operators: injected fault");
+    PXG_FAULT(ctx, "pxg_build_operators");
     PXG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int dim = ctx->dim;
@@ -354,6 +354,9 @@ extern "C" int pxg_build_operators(pxg_ctx *ctx, int order, double delta, double
     PXG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_qq), qq, sizeof(qq), 0, hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_qr), qr, sizeof(qr), 0, hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_bdiag), bd, sizeof(bd), 0, hipMemcpyHostToDevice, st));
+    // the operators are rewritten: every element built on the old ones is stale
+    invalidate(ctx, INV_OPS);
+    ctx->nop = 0;
     PXG_HIP(ctx->ops.alloc((size_t)ctx->nb * nop));
     PXG_HIP(ctx->flags.alloc(ctx->n));
     dbuf<int> nd;
@@ -372,7 +375,6 @@ extern "C" int pxg_build_operators(pxg_ctx *ctx, int order, double delta, double
     PXG_HIP(hipMemcpyAsync(&ndeg, nd.p, sizeof(int), hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
     ctx->order = order; ctx->nop = nop; ctx->delta = delta; ctx->ndegenerate = ndeg;
-    ctx->ch_ready = 0; ctx->frac.ready = 0;
     if (ndeg_out) *ndeg_out = ndeg;
     return PXG_OK;
 }

@@ -812,7 +812,7 @@ extern "C" int pxg_assemble_stressch_dev(pxg_ctx *ctx, double dt, const double *
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_stressch: bad arguments");
     PXG_CHECK(dt > 0.0, PXG_ERR_ARG,
               "pxg_assemble_stressch: the element is transient-only (StressCahnHilliardElement.cpp:443-447)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_stressch: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_stressch");
     PXG_HIP(hipSetDevice(ctx->device));
     const double *q = ctx->sch.k;
     const SchArgs A{ctx->n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->rev.p, ctx->lap.p, ctx->isghost.p,
@@ -903,10 +903,11 @@ extern "C" int pxg_fsch_constants(pxg_ctx *ctx, double *out)
 
 extern "C" int pxg_fsch_commit_dev(pxg_ctx *ctx, const double *uold_dev, double dt_prev)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_commit: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_fsch_commit: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_fsch_commit: CSR pattern changed");
     PXG_CHECK(uold_dev && dt_prev > 0.0, PXG_ERR_ARG, "pxg_fsch_commit: bad arguments");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_commit: injected fault");
+    PXG_FAULT(ctx, "pxg_fsch_commit");
     PXG_HIP(hipSetDevice(ctx->device));
     FracSchState &F = ctx->fsch;
     const unsigned g = grids(ctx->n, 128);
@@ -924,7 +925,8 @@ extern "C" int pxg_fsch_commit_dev(pxg_ctx *ctx, const double *uold_dev, double 
 
 extern "C" int pxg_fsch_commit(pxg_ctx *ctx, const double *uold, double dt_prev)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_commit: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_fsch_commit: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(uold, PXG_ERR_ARG, "pxg_fsch_commit: bad arguments");
     PXG_HIP(hipSetDevice(ctx->device));
     const size_t nu = (size_t)ctx->n * (ctx->dim + 2);
@@ -939,12 +941,13 @@ extern "C" int pxg_fsch_commit(pxg_ctx *ctx, const double *uold, double dt_prev)
 extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_dev, const double *uold_dev,
                                      double *kvals_dev, double *rhs_dev)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_assemble_fsch: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_assemble_fsch: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_assemble_fsch: CSR pattern changed");
     PXG_CHECK(u_dev && uold_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_fsch: bad arguments");
     PXG_CHECK(dt > 0.0, PXG_ERR_ARG,
               "pxg_assemble_fsch: the element is transient-only (FracStressCahnHilliardElement.cpp:536-540)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_assemble_fsch: injected fault");
+    PXG_FAULT(ctx, "pxg_assemble_fsch");
     PXG_HIP(hipSetDevice(ctx->device));
     const FracSchState &F = ctx->fsch;
     const double *q = F.k;
@@ -960,7 +963,8 @@ extern "C" int pxg_assemble_fsch_dev(pxg_ctx *ctx, double dt, const double *u_de
 extern "C" int pxg_assemble_fsch(pxg_ctx *ctx, double dt, const double *u, const double *uold, double *kvals,
                                  double *rhs)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_assemble_fsch: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_assemble_fsch: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(u && uold && kvals && rhs, PXG_ERR_ARG, "pxg_assemble_fsch: bad arguments");
     PXG_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -1047,7 +1051,7 @@ extern "C" int pxg_stressch_project_dev(pxg_ctx *ctx, int which, const double *u
     PXG_CHECK(ctx && ctx->sch.ready, PXG_ERR_STATE, "pxg_stressch_project: call pxg_stressch_setup first");
     PXG_CHECK(stressch_has(which) && u_dev && out_dev, PXG_ERR_ARG,
               "pxg_stressch_project: bad arguments (which: strain .. gradC)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_stressch_project: injected fault");
+    PXG_FAULT(ctx, "pxg_stressch_project");
     return sch_project(ctx, false, which, u_dev, out_dev);
 }
 
@@ -1056,25 +1060,27 @@ extern "C" int pxg_stressch_project(pxg_ctx *ctx, int which, const double *u, do
     PXG_CHECK(ctx && ctx->sch.ready, PXG_ERR_STATE, "pxg_stressch_project: call pxg_stressch_setup first");
     PXG_CHECK(stressch_has(which) && u && out, PXG_ERR_ARG,
               "pxg_stressch_project: bad arguments (which: strain .. gradC)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_stressch_project: injected fault");
+    PXG_FAULT(ctx, "pxg_stressch_project");
     return sch_project_host(ctx, false, which, u, out);
 }
 
 extern "C" int pxg_fsch_project_dev(pxg_ctx *ctx, int which, const double *u_dev, double *out_dev)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_project: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_fsch_project: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(fsch_has(which) && u_dev && out_dev, PXG_ERR_ARG,
               "pxg_fsch_project: bad arguments (which: strain, stress, vonMises, sigmaH, gradC, damage, DamageI)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_project: injected fault");
+    PXG_FAULT(ctx, "pxg_fsch_project");
     return sch_project(ctx, true, which, u_dev, out_dev);
 }
 
 extern "C" int pxg_fsch_project(pxg_ctx *ctx, int which, const double *u, double *out)
 {
-    PXG_CHECK(ctx && ctx->fsch.ready, PXG_ERR_STATE, "pxg_fsch_project: call pxg_fsch_setup first");
+    PXG_CHECK(ctx && ctx->fsch.ready && ctx->ch_ready, PXG_ERR_STATE,
+              "pxg_fsch_project: call pxg_fsch_setup first (again after a CSR rebuild or a Poisson or diffusion assembly)");
     PXG_CHECK(fsch_has(which) && u && out, PXG_ERR_ARG,
               "pxg_fsch_project: bad arguments (which: strain, stress, vonMises, sigmaH, gradC, damage, DamageI)");
-    PXG_CHECK(!ctx->fault, PXG_ERR_FAULT, "pxg_fsch_project: injected fault");
+    PXG_FAULT(ctx, "pxg_fsch_project");
     return sch_project_host(ctx, true, which, u, out);
 }
 

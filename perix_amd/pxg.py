@@ -81,7 +81,7 @@ SYMBOLS = [
     "pxg_sch_components", "pxg_stressch_project", "pxg_stressch_project_dev", "pxg_fsch_project",
     "pxg_fsch_project_dev",
     "pxg_write_vtu",
-    "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
+    "pxg_frac_options", "pxg_frac_accel", "pxg_frac_init", "pxg_frac_run",
     "pxg_frac_project", "pxg_frac_project_dev", "pxg_frac_layout", "pxg_frac_get_state",
     "pxg_frac_state_ptrs", "pxg_frac_profile", "pxg_frac_kernel_time",
     "pxg_frac_adjoint", "pxg_frac_adjoint_ex", "pxg_frac_set_body", "pxg_frac_loss",
@@ -281,6 +281,10 @@ class Context:
         self.nop = 0
         self.nnz = 0
         self.ndof = 0
+        # bumped by every call that changes the explicit state (set-up, init,
+        # run, per-step accel): lets a caller tell whether the state it
+        # recorded is still the context's
+        self.frac_serial = 0
 
     def close(self):
         if self.h:
@@ -307,6 +311,7 @@ class Context:
         dxyz = np.ascontiguousarray(dxyz, float)
         self.n = len(vols)
         self.dim = dim
+        self.nb = self.nop = self.nnz = self.ndof = 0
         _check(load().pxg_set_points(self.h, self.n, dim, _dp(coords), _dp(vols), _ip(mirror),
                                      _dp(dxyz)))
 
@@ -532,8 +537,16 @@ class Context:
     # -- a15-a20 ---------------------------------------------------------------
     def frac_setup(self, params, precision=FP64, node_E=None, nu=0.25, plane_stress=False):
         ne = None if node_E is None else np.ascontiguousarray(node_E, float)
+        self.frac_serial += 1
         _check(load().pxg_frac_setup(self.h, C.byref(params), precision, _dp(ne), nu,
                                      int(plane_stress)))
+
+    FRAC_NO_SHARED_RUNS = 1
+    FRAC_FP32_OPERATORS = 2
+
+    def frac_options(self, flags):
+        """PXG_FRAC_* diagnostic options, applied at the next frac_setup."""
+        _check(load().pxg_frac_options(self.h, int(flags)))
 
     def frac_set_bcs(self, specs):
         arr, n, keep = make_bcs(specs)
@@ -542,6 +555,7 @@ class Context:
     def frac_accel(self, u, health):
         a = np.zeros(self.n * self.dim)
         ab = C.c_int()
+        self.frac_serial += 1
         _check(load().pxg_frac_accel(self.h, _dp(np.ascontiguousarray(u, float)), _dp(health),
                                      _dp(a), C.byref(ab)))
         return a, ab.value
@@ -567,10 +581,12 @@ class Context:
 
     def frac_init(self, u0, dt, v0=None):
         v = None if v0 is None else np.ascontiguousarray(v0, float)
+        self.frac_serial += 1
         _check(load().pxg_frac_init(self.h, _dp(np.ascontiguousarray(u0, float)), _dp(v), dt))
 
     def frac_run(self, dt, total, nsteps_max=0):
         done = C.c_int()
+        self.frac_serial += 1
         rc = load().pxg_frac_run(self.h, dt, total, nsteps_max, C.byref(done))
         if rc != 0:
             err = PxgError(rc, load().pxg_last_error().decode(errors="replace"))

@@ -1,0 +1,204 @@
+"""Parity at the full size of every benchmark workload.
+
+- 400 x 200 plate (81,200 nodes), 2,000 CFL-limited steps: the FP64 device
+  loop is bit-exact against the oracle run on the device's g-weights (u^n,
+  u^{n-1}, health, break step of every bond); with each side's own g-weights
+  the broken-bond set is the same and u agrees to 1e-7 relative; the FP32C
+  loop stays within 1e-5 relative-L2 of FP64 with no broken-bond mismatch
+  (ExplicitPDDOFracElement.cpp:237-299, TimeStepping.cpp:417-599).
+- 256 x 256 Cahn-Hilliard (66,560 nodes, 7,646,928 nonzeros): the CSR K is
+  bit-exact and the residual within 1e-12 against the oracle at the RandomIC
+  state and at the state after one Picard-frozen backward-Euler iteration,
+  with sum_i V_i R_c,i = 0 (CahnHilliardElement.cpp:66-292).
+- 200 x 100 plate adjoint (20,600 nodes), 500 steps, checkpoints every 50,
+  64 sensors, per-node E = 191 GPa (1 + 0.05 N(0,1)): dJ/dE within 1e-10 of
+  the oracle's FP64 adjoint and within 1e-4 of central differences along 20
+  seeded directions.
+- 160 x 80 x 40 block (556,800 nodes, 64,718,816 bonds) with the notch and
+  the impactor: 50 FP64 steps bit-exact against the oracle on the device's
+  g-weights, and FP32C within 1e-5 of FP64 with no mismatch over them.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import perix_amd
+from perix_amd import pxg, problems
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _oracle_explicit(oracle, s, ops, nsteps):
+    g = s.cloud.grid
+    m = oracle.Mesh(oracle.make_grid(g.dim, g.nx, g.ny, g.nz, hi=g.hi))
+    if ops is None:
+        ops, _ = oracle.operators(m)
+    opc = oracle.frac_opcache(m, ops)
+    del ops
+    p = oracle.frac_params(g.dim, problems.E_STEEL, problems.NU, problems.RHO, s.params.sc)
+    h = oracle.seed_health(m, segs=s.cracks)
+    bcs = oracle.make_bcs(s.bcs)
+    u = s.u0.copy()
+    uo = np.zeros_like(u)
+    steps, bs = oracle.explicit_run(m, p, opc, bcs, s.dt, nsteps * s.dt, u, uo, h)
+    return m, steps, u, uo, h, bs
+
+
+def _same_bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+def test_plate_400x200_2000_steps(oracle):
+    s = problems.plate_400x200(precision=pxg.FP64, nsteps=2000)
+    assert s.cloud.n == 81200 and s.ctx.nb == 2251924
+    s.ctx.frac_init(s.u0, s.dt)
+    assert s.ctx.frac_run(s.dt, s.total) == 2000
+    u, uo, h, bs = s.ctx.frac_state()
+    ops, _ = s.ctx.operators()
+    print("plate: device FP64 done, %d broken bonds" % int((h == 0).sum()), flush=True)
+    m, steps, ru, ruo, rh, rbs = _oracle_explicit(oracle, s, ops, 2000)
+    del ops
+    off, nbr = s.ctx.families()
+    assert np.array_equal(off, m.off) and np.array_equal(nbr, m.nbr)
+    assert steps == 2000
+    assert (rh == 0).sum() > 100000, "the crack must run"
+    assert np.array_equal(h.astype(float), rh)
+    assert np.array_equal(bs, rbs)
+    assert _same_bits(u, ru) and _same_bits(uo, ruo)
+    # each side with its own g-weights (device exp vs libm exp, ~1 ulp)
+    _, _, ru2, _, rh2, _ = _oracle_explicit(oracle, s, None, 2000)
+    rel = np.linalg.norm(u - ru2) / np.linalg.norm(ru2)
+    print("plate end to end: rel-L2 %.3e, mismatches %d" % (rel, int((h.astype(float) != rh2).sum())))
+    assert np.array_equal(h.astype(float), rh2)
+    assert rel <= 1e-7
+    s.ctx.close()
+    # FP32C against the FP64 trajectory over the same 2,000 steps
+    s32 = problems.plate_400x200(precision=pxg.FP32C, nsteps=2000)
+    s32.ctx.frac_init(s32.u0, s32.dt)
+    assert s32.ctx.frac_run(s32.dt, s32.total) == 2000
+    u32, _, h32, _ = s32.ctx.frac_state()
+    rel32 = np.linalg.norm(u32 - u) / np.linalg.norm(u)
+    mism = int((h32 != h).sum())
+    print("plate FP32C: rel-L2 %.3e, max abs %.3e, mismatches %d" % (rel32, np.abs(u32 - u).max(), mism))
+    assert rel32 <= 1e-5
+    assert mism == 0
+    s32.ctx.close()
+
+
+def _ch_state_check(oracle, ctx, cloud, m, rlap, rrev, risg, u, uold, kappa):
+    k, r = ctx.assemble_ch(kappa, 3.0, 1.0, 1e-3, u, uold)
+    rp, cols = oracle.csr_pattern(m, 2)
+    mn = oracle.ch_mobility(m, u, 1.0)
+    rk, rr = oracle.assemble_ch(m, rlap, rrev, risg, mn, kappa, 3.0, 1e-3, rp, cols, u, uold)
+    assert _same_bits(k, rk)
+    rel = np.linalg.norm(r - rr) / np.linalg.norm(rr)
+    print("CH 256: K bit-exact, rhs rel %.2e (bit-equal fraction %.6f)" % (rel, np.mean(r == rr)))
+    assert rel <= 1e-12
+    assert np.abs(r - rr).max() <= 1e-12 * np.abs(rr).max()
+    # conservation of the bond flux: sum_i V_i R_c,i = 0 with uold = u
+    _, r2 = ctx.assemble_ch(kappa, 3.0, 1.0, 1e-3, u, u)
+    bulk = np.arange(cloud.n) < cloud.nbulk
+    rc = r2[0::2][bulk] * cloud.vols[bulk]
+    assert abs(rc.sum()) <= 1e-10 * np.abs(rc).sum()
+    return k, r
+
+
+def test_cahnhilliard_256(oracle):
+    nx = 256
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=nx, ny=nx)
+    assert cloud.n == 66560 and ctx.nnz == 7646928
+    m = oracle.Mesh(oracle.make_grid(2, nx, nx))
+    ops, _ = ctx.operators()
+    lap, rev, isg = ctx.ch_geometry()
+    rlap, rrev, risg = oracle.ch_geometry(m, ops)
+    assert _same_bits(lap, rlap)
+    assert np.array_equal(rev, rrev) and np.array_equal(isg, risg)
+    kappa = 1e-2 * (1.0 / nx) ** 2
+    _ch_state_check(oracle, ctx, cloud, m, rlap, rrev, risg, u0, u0, kappa)
+    # the iterate after one Picard-frozen backward-Euler solve from the
+    # RandomIC state (fixture written by tests/golden/make_ch_picard.py)
+    z = np.load(os.path.join(GOLDEN, "ch_picard_256.npz"))
+    assert hashlib.sha256(u0.tobytes()).digest() == z["u0_sha256"].tobytes()
+    u1 = z["u1"]
+    assert np.abs(u1 - u0).max() > 1e-4
+    _ch_state_check(oracle, ctx, cloud, m, rlap, rrev, risg, u1, u0, kappa)
+    ctx.close()
+
+
+def test_adjoint_200x100_500_steps(oracle):
+    nsteps, ckpt, nsens = 500, 50, 64
+    rs = np.random.RandomState(2024)
+    probe = problems.explicit_plate(2, 200, 100, nsteps=nsteps)
+    n, nbulk = probe.cloud.n, probe.cloud.nbulk
+    probe.ctx.close()
+    assert n == 20600
+    E = problems.E_STEEL * (1.0 + 0.05 * rs.standard_normal(n))
+    sens = rs.choice(nbulk, nsens, replace=False).astype(np.int32)
+    s = problems.explicit_plate(2, 200, 100, nsteps=nsteps, node_E=E)
+    ctx = s.ctx
+    ctx.frac_init(s.u0, s.dt)
+    assert ctx.frac_run(s.dt, s.total) == nsteps        # records the break steps
+    _, _, h, bs = ctx.frac_state()
+    assert ((bs >= 0) & (bs <= nsteps)).sum() > 0, "bonds must break during the run"
+    J, g = ctx.frac_adjoint(s.u0, s.dt, nsteps, sens, 1, ckpt=ckpt)
+    # FP64 oracle adjoint on the same operators
+    m = oracle.Mesh(oracle.make_grid(2, 200, 100, hi=s.cloud.grid.hi))
+    ops, _ = ctx.operators()
+    opc = oracle.frac_opcache(m, ops)
+    p = oracle.frac_params(2, problems.E_STEEL, problems.NU, problems.RHO, s.params.sc)
+    Jr, gr = oracle.frac_adjoint(m, p, opc, oracle.make_bcs(s.bcs), s.dt, nsteps, s.u0, bs, E, sens, 1)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print("adjoint 200x100: J %.9e (oracle %.9e), gradient rel-L2 %.3e" % (J, Jr, rel))
+    assert abs(J - Jr) <= 1e-12 * abs(Jr)
+    assert rel <= 1e-10
+    # central differences of the frozen-health map along 20 seeded directions
+    hh = 1e-6
+    worst = 0.0
+    for _ in range(20):
+        v = rs.standard_normal(n) * E
+        jp = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1, node_E=E + hh * v)
+        jm = ctx.frac_loss(s.u0, s.dt, nsteps, sens, 1, node_E=E - hh * v)
+        fd = (jp - jm) / (2 * hh)
+        ad = float(g @ v)
+        worst = max(worst, abs(fd - ad) / max(abs(fd), 1e-300))
+    print("adjoint 200x100 vs central differences, 20 directions: worst rel %.3e" % worst)
+    assert worst <= 1e-4
+    ctx.close()
+
+
+def test_block_160x80x40_fp64_steps(oracle):
+    nsteps = 50
+    s = problems.block_160x80x40(precision=pxg.FP64, nsteps=nsteps)
+    assert s.cloud.n == 556800 and s.ctx.nb == 64718816
+    s.ctx.frac_init(s.u0, s.dt)
+    assert s.ctx.frac_run(s.dt, s.total) == nsteps
+    u, uo, h, bs = s.ctx.frac_state()
+    ops, _ = s.ctx.operators()
+    off, nbr = s.ctx.families()
+    seed = s.ctx.seed()
+    s.ctx.close()
+    print("block: device FP64 done, %d bonds broke during the run" % int(((bs >= 1) & (bs <= nsteps)).sum()),
+          flush=True)
+    m, steps, ru, ruo, rh, rbs = _oracle_explicit(oracle, s, ops, nsteps)
+    del ops
+    assert np.array_equal(off, m.off) and np.array_equal(nbr, m.nbr)
+    assert steps == nsteps
+    broke = int(((rbs >= 1) & (rbs <= nsteps)).sum())
+    print("block oracle: %d bonds broke during the run, %d seeded" % (broke, int((seed == 0).sum())))
+    assert broke > 0, "the impactor must break bonds within the run"
+    assert np.array_equal(h.astype(float), rh)
+    assert np.array_equal(bs, rbs)
+    assert _same_bits(u, ru) and _same_bits(uo, ruo)
+    # FP32C over the same steps
+    s32 = problems.block_160x80x40(precision=pxg.FP32C, nsteps=nsteps)
+    s32.ctx.frac_init(s32.u0, s32.dt)
+    assert s32.ctx.frac_run(s32.dt, s32.total) == nsteps
+    u32, _, h32, _ = s32.ctx.frac_state()
+    rel = np.linalg.norm(u32 - u) / np.linalg.norm(u)
+    print("block FP32C after %d steps: rel-L2 %.3e, mismatches %d" % (nsteps, rel, int((h32 != h).sum())))
+    assert rel <= 1e-5 and int((h32 != h).sum()) == 0
+    s32.ctx.close()
