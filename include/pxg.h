@@ -139,6 +139,19 @@ int pxg_assemble_poisson_dev(pxg_ctx *ctx, double sigma, const double *f_node_de
                              double f_const, const double *u_dev,
                              double *kvals_dev, double *rhs_dev);
 
+/* Allocation-free variant of pxg_assemble_poisson_dev.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same bits as pxg_assemble_poisson_dev;
+ * it calls no hipMalloc, hipFree or allocating routine and keeps its one
+ * scratch buffer (the per-bond Laplacian) in a 16 MiB device array of the
+ * library, so the context must have at most 2,097,152 directed bonds
+ * (PXG_ERR_ARG otherwise).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams or two host threads) race. */
+int pxg_assemble_poisson_noalloc(pxg_ctx *ctx, double sigma, const double *f_node_dev,
+                                 double f_const, const double *u_dev, double *kvals_dev,
+                                 double *rhs_dev);
+
 /* ---- remaining elements: diffusion ---------------------------------------
  * DiffusionElement (DiffusionElement.cpp:28-62) through the same assembler:
  * backward Euler, K = -dF/dU with bond coeff = -D lap_ij and nodal

@@ -36,6 +36,10 @@ extern "C" int pxg_create(pxg_ctx **out, int device)
     PXG_HIP(hipGetDeviceCount(&ndev));
     PXG_CHECK(device >= 0 && device < ndev, PXG_ERR_ARG, "pxg_create: no such device");
     PXG_HIP(hipSetDevice(device));
+    {
+        const int rc = scratch_init(device);
+        if (rc) return rc;
+    }
     pxg_ctx *c = new pxg_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -19,6 +19,7 @@
 // block first, then each bond).
 #include "pxg_internal.h"
 #include <cmath>
+#include <mutex>
 
 namespace pxg {
 
@@ -194,6 +195,62 @@ extern "C" int pxg_assemble_poisson(pxg_ctx *ctx, double sigma, const double *f_
     PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+// ---- allocation-free Poisson assembly -------------------------------------
+// pxg_assemble_poisson_noalloc keeps its one scratch buffer, the per-bond
+// Laplacian, in this file-scope device array instead of the context's lap
+// buffer (which the first Poisson call allocates).  Every GPU the library
+// runs on holds its own copy of the array for the life of the process; host
+// code reaches it only through the per-device address taken below, never
+// through the symbol's name (the host-side name is a stand-in, not a device
+// address).
+constexpr size_t kScratchDoubles = (size_t)2 << 20;     // 16 MiB: 2,097,152 bonds
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static double *g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Called by pxg_create with `device` current: takes this device's address
+// of the scratch array once per device.
+int pxg::scratch_init(int device)
+{
+    PXG_CHECK(device >= 0 && device < kScratchDevices, PXG_ERR_ARG, "scratch_init: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PXG_OK;
+    void *p = nullptr;
+    PXG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PXG_CHECK(p, PXG_ERR_HIP, "scratch_init: no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_poisson_noalloc(pxg_ctx *ctx, double sigma, const double *f_node_dev,
+                                            double f_const, const double *u_dev, double *kvals_dev,
+                                            double *rhs_dev)
+{
+    PXG_CHECK(ctx && ctx->nop > 0 && ctx->nnz > 0 && ctx->csr_ndof == 1, PXG_ERR_STATE,
+              "pxg_assemble_poisson_noalloc: needs operators and a 1-dof CSR pattern");
+    PXG_CHECK(u_dev && kvals_dev && rhs_dev, PXG_ERR_ARG, "pxg_assemble_poisson_noalloc: bad arguments");
+    PXG_CHECK((size_t)ctx->nb <= kScratchDoubles, PXG_ERR_ARG,
+              "pxg_assemble_poisson_noalloc: " + std::to_string(ctx->nb) +
+                  " directed bonds exceed the scratch limit of 2097152 (use pxg_assemble_poisson_dev)");
+    PXG_FAULT(ctx, "pxg_assemble_poisson_noalloc");
+    PXG_HIP(hipSetDevice(ctx->device));
+    double *lap = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        lap = (ctx->device >= 0 && ctx->device < kScratchDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+    }
+    PXG_CHECK(lap, PXG_ERR_STATE, "pxg_assemble_poisson_noalloc: scratch not set up on this device");
+    // k_lap writes every one of the nb entries k_poisson reads: nothing to clear
+    hipLaunchKernelGGL(k_lap, dim3(gridb(ctx->nb, 256)), dim3(256), 0, ctx->stream, ctx->nb, ctx->dim, ctx->nop,
+                       ctx->ops.p, lap);
+    hipLaunchKernelGGL(k_poisson, dim3(gridb(ctx->n, AT)), dim3(256), 0, ctx->stream, ctx->n, ctx->off.p,
+                       ctx->nbr.p, lap, ctx->slot.p, ctx->diag.p, ctx->mslot.p, sigma, f_node_dev, f_const, 0.0,
+                       (const double *)nullptr, u_dev, kvals_dev, rhs_dev);
+    PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
 

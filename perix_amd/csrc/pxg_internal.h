@@ -269,6 +269,10 @@ int launch_dyn_rows(pxg_ctx *ctx, double dt, const double *u_dev, const double *
 enum { INV_OPS = 1, INV_CRACKS = 2 };
 void invalidate(pxg_ctx *ctx, int what);
 
+// takes this device's address of the allocation-free entry point's scratch
+// array (pxg_assemble.hip); called by pxg_create with the device current
+int scratch_init(int device);
+
 // which Laplacian ctx->lap holds
 enum { LAP_NONE = 0, LAP_POISSON = 1, LAP_CH = 2 };
 

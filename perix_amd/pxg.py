@@ -69,7 +69,7 @@ SYMBOLS = [
     "pxg_structured_sizes", "pxg_structured_cloud", "pxg_horizon_radius", "pxg_random_ic",
     "pxg_schedule", "pxg_set_points", "pxg_build_families", "pxg_get_families",
     "pxg_set_cracks", "pxg_get_seed", "pxg_build_csr", "pxg_get_csr", "pxg_build_operators",
-    "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev",
+    "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev", "pxg_assemble_poisson_noalloc",
     "pxg_assemble_diffusion", "pxg_assemble_diffusion_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
@@ -146,6 +146,8 @@ def load():
         L.pxg_assemble_poisson.argtypes = [C.c_void_p, C.c_double, D, C.c_double, D, D, D]
         L.pxg_assemble_poisson_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
                                                C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pxg_assemble_poisson_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p]
         L.pxg_assemble_diffusion.argtypes = [C.c_void_p, C.c_double, D, C.c_double, C.c_double,
                                              D, D, D, D]
         L.pxg_assemble_diffusion_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
@@ -376,6 +378,10 @@ class Context:
 
     def assemble_poisson_dev(self, sigma, u_ptr, k_ptr, r_ptr, f_ptr=None, f_const=0.0):
         _check(load().pxg_assemble_poisson_dev(self.h, sigma, f_ptr, f_const, u_ptr, k_ptr, r_ptr))
+
+    def assemble_poisson_noalloc(self, sigma, u_ptr, k_ptr, r_ptr, f_ptr=None, f_const=0.0):
+        """pxg_assemble_poisson_noalloc: device pointers, no allocation."""
+        _check(load().pxg_assemble_poisson_noalloc(self.h, sigma, f_ptr, f_const, u_ptr, k_ptr, r_ptr))
 
     def assemble_diffusion(self, D, dt, u, uold, f_node=None, f_const=0.0):
         k = np.zeros(self.nnz)
