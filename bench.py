@@ -343,7 +343,33 @@ def _observables(s, pxg):
 RHO_BLOCK = 8000.0
 
 
-def block_long_run(problems, pxg, steps=1000, chunk=100):
+def stability_probe(problems, pxg, dim, iters=60, seed=5):
+    """Largest eigenvalue of the intact discrete operator u -> -a(u) (a =
+    L u / rho, the step's acceleration) by power iteration through the
+    per-step accel entry point, and the central-difference limit
+    dt_crit = 2 / sqrt(lambda_max) beside the step the workload uses
+    (0.9 x the reference's estimate dx / sqrt(C11 / rho),
+    ExplicitPDDOFracElement.cpp:54-77)."""
+    s = problems.block_160x80x40(precision=pxg.FP64, nsteps=1) if dim == 3 else \
+        problems.plate_400x200(precision=pxg.FP64, nsteps=1)
+    rs = np.random.RandomState(seed)
+    nd = s.cloud.n * dim
+    h = np.ones(s.ctx.nb)
+    v = rs.standard_normal(nd)
+    v /= np.linalg.norm(v)
+    lam = 0.0
+    for _ in range(iters):
+        a, _ = s.ctx.frac_accel(1e-12 * v, h)
+        w = -a / 1e-12
+        lam = float(v @ w)
+        v = w / np.linalg.norm(w)
+    dt_crit = 2.0 / math.sqrt(abs(lam))
+    s.ctx.close()
+    return {"lambda_max": lam, "dt_crit": dt_crit, "dt": s.dt, "dt_over_dt_crit": s.dt / dt_crit,
+            "iterations": iters}
+
+
+def block_long_run(problems, pxg, steps=1000, chunk=100, cfl=0.9):
     """FP32C against FP64 over the block's full 1,000-step run, every 100
     steps, beside controls that separate the causes of a divergence:
       fp32c_own_ops  FP32C with every node on its own FP32 operators (no
@@ -360,7 +386,7 @@ def block_long_run(problems, pxg, steps=1000, chunk=100):
     def make(prec, opts, eps):
         ctx = pxg.Context(0)
         ctx.frac_options(opts)
-        s = problems.block_160x80x40(precision=prec, nsteps=steps, ctx=ctx)
+        s = problems.block_160x80x40(precision=prec, nsteps=steps, ctx=ctx, cfl=cfl)
         u0 = s.u0.copy()
         if eps:
             u0 += eps * np.random.RandomState(1).standard_normal(u0.size)
@@ -396,7 +422,7 @@ def block_long_run(problems, pxg, steps=1000, chunk=100):
     for s in runs.values():
         s.ctx.close()
     return {"workload": "160x80x40 block, 1000 steps, FP32C and controls against FP64",
-            "control_eps_m": epsc, "chunks": rows}
+            "cfl": cfl, "control_eps_m": epsc, "chunks": rows}
 
 
 def gweight_model(n, nb, dim, nop):
@@ -860,6 +886,10 @@ def main():
     ap.add_argument("--only-plate", action="store_true", help="plate FP32C/FP64 only (profiling)")
     ap.add_argument("--only-block", action="store_true", help="3D block only (profiling)")
     ap.add_argument("--long-run", action="store_true", help="with --only-block: the 1000-step comparison")
+    ap.add_argument("--cfl", type=float, default=0.9, help="with --long-run: step as a fraction of the CFL estimate")
+    ap.add_argument("--long-steps", type=int, default=1000, help="with --long-run: steps of the comparison")
+    ap.add_argument("--stability", action="store_true",
+                    help="largest eigenvalue of the plate's and the block's step operator (power iteration)")
     ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
     ap.add_argument("--only-assembly", action="store_true", help="CSR assembly only (profiling)")
     ap.add_argument("--only-gweight", action="store_true", help="family + g-weight build only (profiling)")
@@ -882,10 +912,13 @@ def main():
     from perix_amd import pxg, problems
 
     K, W = args.steps, args.warmup
+    if args.stability:
+        print(json.dumps({"plate": stability_probe(problems, pxg, 2), "block": stability_probe(problems, pxg, 3)}))
+        return 0
     if args.only_block:
-        blk = block_case(problems, pxg, steps=K)
+        blk = {} if args.long_run else block_case(problems, pxg, steps=K)
         if args.long_run:
-            blk["long_run"] = block_long_run(problems, pxg)
+            blk["long_run"] = block_long_run(problems, pxg, steps=args.long_steps, cfl=args.cfl)
         print(json.dumps({"block3d": blk}))
         return 0
     if args.only_adjoint:
