@@ -97,6 +97,13 @@ int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out);
  * off[n+1], nbr[nb] (1-based). */
 int pxg_get_families(pxg_ctx *ctx, int *off, int *nbr);
 
+/* The host's own families instead of pxg_build_families: off[n+1] (off[0] =
+ * 0) and nbr[off[n]] with 1-based ids, in the host's order (the flat layout
+ * FormResidualAndJacobianCUDA.cpp:187-205 and ExplicitPDDOFracElement.cpp:
+ * 115-153 marshal into AssembleMesh::neigh_off / neigh_id, CudaAssemble.h:
+ * 66-67).  Everything built on the old families is dropped. */
+int pxg_set_families(pxg_ctx *ctx, const int *off, const int *nbr);
+
 /* ---- a3: force-only crack seeds ------------------------------------------
  * Replaces the force-only health seeding of ExplicitPDDOFracElement.cpp:
  * 134-155 through initialCrackCutsBond (PDMesh.cpp:104-159): segments
@@ -112,6 +119,15 @@ int pxg_get_seed(pxg_ctx *ctx, unsigned char *seed);
  * elements use 1, dim, 2 and dim+2). */
 int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out);
 int pxg_get_csr(pxg_ctx *ctx, int *rowptr, int *cols);
+/* The host's own CSR pattern (SparseMatrix::getCSRRowsIndexPtr /
+ * getCSRColsIndexPtr, passed as AssembleMesh::csr_off / csr_col,
+ * CudaAssemble.h:64-65) instead of pxg_build_csr: rowptr[n*ndof+1],
+ * cols[rowptr[n*ndof]], 0-based, sorted whole ndof x ndof blocks, the rows of
+ * one node sharing one pattern that holds the node's diagonal block, every
+ * family member's block and at most one other block (the ghost's mirror,
+ * SparseMatrix.cpp:90-115).  PXG_ERR_ARG names the first violation.
+ * Needs the families. */
+int pxg_set_csr(pxg_ctx *ctx, int ndof, const int *rowptr, const int *cols);
 
 /* ---- a5-a8: PDDO moment matrix and g-weights (FP64) ---------------------
  * Replaces PDOperators::calcAMatrixCompute/calcOperatorsCompute
@@ -174,6 +190,18 @@ int pxg_ch_setup(pxg_ctx *ctx);
 /* lambda-corrected g2 per bond, reverse-bond index, ghost flags (optional) */
 int pxg_ch_get_geometry(pxg_ctx *ctx, double *lapbond, int *rev,
                         unsigned char *isghost);
+/* The host element's own geometry instead of pxg_ch_setup: the arrays of
+ * perix_cuda::CahnHilliardMesh (lapbond, rev, isghost; CudaAssemble.h:
+ * 116-125), built by CahnHilliardElement::preprocessIteration
+ * (CahnHilliardElement.cpp:93-153) in the family order of the context.
+ * isghost may be NULL (no ghosts). */
+int pxg_ch_set_geometry(pxg_ctx *ctx, const double *lapbond, const int *rev,
+                        const unsigned char *isghost);
+/* The host's Picard-frozen mobility M_i (CahnHilliardMesh::mnode,
+ * CudaAssemble.h:124; CahnHilliardElement.cpp:155-175): the following
+ * Cahn-Hilliard assemblies use it in place of M0 c (1 - c) from u (their M0
+ * argument is then unused).  NULL returns to M0 c (1 - c). */
+int pxg_ch_set_mobility(pxg_ctx *ctx, const double *mnode);
 /* Picard-frozen mobility M_i = M0 c_i(1-c_i) (CahnHilliardElement.cpp:
  * 155-175) then assembly (:178-292).  u, uold: 2n (c, mu); kvals[nnz];
  * rhs[2n]. */
@@ -277,6 +305,20 @@ int pxg_assemble_dynfrac_dev(pxg_ctx *ctx, double dt, const double *u_dev,
                              const double *uold_dev, double *kvals_dev, double *rhs_dev);
 /* bond health (nb bytes, CSR order) and committed velocity (n*dim); either may be NULL */
 int pxg_dynfrac_get_state(pxg_ctx *ctx, unsigned char *health, double *vel);
+/* The same assembly on the host element's own state, as
+ * perix_cuda::assembleCached passes it (CudaAssemble.h:196-219, called from
+ * FormResidualAndJacobianCUDA.cpp:237-266): the constants of CachedParams,
+ * the committed velocity (n*dim; the caller unpacks the cache's 3 slots per
+ * node) and the committed bond health (nb doubles, > 0 intact; NULL: all
+ * intact).  Needs order-2 operators and a CSR pattern with dim dofs per
+ * node; no pxg_dynfrac_setup.  Host arrays. */
+typedef struct pxg_dynfrac_state {
+    double C11, C12, C66, rho, bx, by, bz;
+    const double *vel;
+    const double *health;
+} pxg_dynfrac_state;
+int pxg_assemble_dynfrac_state(pxg_ctx *ctx, const pxg_dynfrac_state *s, double dt, const double *u,
+                               const double *uold, double *kvals, double *rhs);
 
 /* ---- remaining elements: stress-coupled Cahn-Hilliard ---------------------
  * StressCahnHilliardElement (stress_cahnhilliard): species c, chemical
@@ -328,6 +370,29 @@ typedef struct pxg_fsch_params {
  * displacement dofs of uold0 (n*(dim+2) host; only read when rho > 0)
  * (FracStressCahnHilliardElement.cpp:160-301). */
 int pxg_fsch_setup(pxg_ctx *ctx, const pxg_fsch_params *p, const double *uold0);
+/* The fracture stress-coupled Cahn-Hilliard assembly on the host element's
+ * own state, as perix_cuda::assembleStressCahnHilliard passes it
+ * (CudaAssemble.h:150-191, called from FormResidualAndJacobianCUDA.cpp:
+ * 268-297): the element constants of StressCahnHilliardParams, the reverse
+ * bonds, the species Laplacian that drives both the flux and the kappa term,
+ * the ghost flags (NULL: none), the Picard-frozen mobility, the committed
+ * bond health (nb doubles, > 0 intact; NULL: all intact; a broken bond's
+ * mechanical terms are scaled by kres) and, with rho > 0, the committed
+ * velocity (n*dim) of the backward-Euler inertia.  Needs order-2 operators
+ * and a CSR pattern with dim + 2 dofs per node.  Host arrays; the state is
+ * uploaded on every call. */
+typedef struct pxg_sch_state {
+    double C11, C12, C66, A, Kh, Ch, Omega, cref, chi, kappa, rho, kres;
+    const int *rev;
+    const double *lap;
+    const unsigned char *isghost;
+    const double *mnode;
+    const double *health;
+    const double *vel;
+} pxg_sch_state;
+int pxg_assemble_sch_state(pxg_ctx *ctx, const pxg_sch_state *s, double dt, const double *u,
+                           const double *uold, double *kvals, double *rhs);
+
 /* out[14] = C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref, rho,
  * residual stiffness, critical stretch */
 int pxg_fsch_constants(pxg_ctx *ctx, double *out);

@@ -417,6 +417,35 @@ void ref_frac_consts(void *h, double *out)
 } // extern "C"
 
 extern "C" {
+// Runs a deck's whole job through the reference's own driver
+// (PDProblem::run, RunPDProblem.cpp:21-: init, initial conditions, BC
+// presets, then the static Newton solve or the transient loop, with the
+// output the deck asks for) and copies the final U (at most cap values) and,
+// for an explicit fracture deck, the element's bond health (at most hcap).
+// Returns the dof count; *nhealth gets the bond count (0 for other
+// elements) and *seconds the wall time of run().
+long long ref_run(const char *deck, double *u_out, long long cap, double *health_out, long long hcap,
+                  long long *nhealth, double *seconds)
+{
+    PDProblem prob;
+    std::string a0 = "perix", a1 = "-i", a2 = deck;
+    char *argv[3] = {a0.data(), a1.data(), a2.data()};
+    const auto t0 = std::chrono::steady_clock::now();
+    prob.run(3, argv);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const long long n = prob.m_U.getSize();
+    if (u_out) std::copy(prob.m_U.getDataPtr(), prob.m_U.getDataPtr() + std::min(n, cap), u_out);
+    long long nh = 0;
+    if (prob.m_ElmtSystem.getElementsNum() == 1) {
+        if (auto *e = dynamic_cast<const ExplicitPDDOFracElement *>(&prob.m_ElmtSystem.getElementByIndex(0))) {
+            nh = (long long)e->m_HealthFlat.size();
+            if (health_out) std::copy(e->m_HealthFlat.begin(), e->m_HealthFlat.begin() + std::min(nh, hcap), health_out);
+        }
+    }
+    if (nhealth) *nhealth = nh;
+    return n;
+}
+
 // Timed explicit run: the element's one-off build (families flattened, op
 // cache, crack seeds) is triggered by a first rate evaluation and timed as
 // setup; the central-difference loop of the deck's TimeStepping block is

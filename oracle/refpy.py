@@ -42,6 +42,113 @@ def lib():
     return _LIB
 
 
+_PXG_PATH = os.path.join(_HERE, "_ref", "libperix_ref_pxg.so")
+_PXG_LIB = None
+
+
+def pxg_available():
+    return os.path.exists(_PXG_PATH)
+
+
+def pxg_lib():
+    """The reference built with PERIX_CUDA_ASSEMBLE over the repository's
+    perix_cuda implementation (integration/perix_cuda_pxg.cpp, libpxg).  The
+    HIP library is loaded through perix_amd first, so that torch and libpxg
+    share one HIP runtime."""
+    global _PXG_LIB
+    if _PXG_LIB is None:
+        from perix_amd import pxg
+        pxg.load()
+        _PXG_LIB = C.CDLL(_PXG_PATH)
+        _PXG_LIB.ref_run.restype = C.c_longlong
+        _PXG_LIB.ref_run.argtypes = [C.c_char_p, D, C.c_longlong, D, C.c_longlong,
+                                     C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        _PXG_LIB.perix_cuda_pxg_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        _PXG_LIB.perix_cuda_pxg_reset_stats.argtypes = [C.c_longlong]
+    return _PXG_LIB
+
+
+def deck_dofs(deck):
+    """PD nodes x dofs of a structured deck: cells plus one ghost per boundary
+    face (CreatePDMesh.cpp:679-725), times the DOFs block."""
+    m = deck["Mesh"]
+    if m["dim"] == 2:
+        n = m["nx"] * m["ny"] + 2 * (m["nx"] + m["ny"])
+    else:
+        n = m["nx"] * m["ny"] * m["nz"] + 2 * (m["nx"] * m["ny"] + m["nx"] * m["nz"] + m["ny"] * m["nz"])
+    return n * len(deck["DOFs"])
+
+
+def run_job(deck, fault_at_call=0, timeout=900):
+    """Runs a deck's whole job with the reference's own driver (PDProblem::run)
+    in the PERIX_CUDA_ASSEMBLE build (JobSystem.assemble picks "openmp" or
+    "cuda", the latter through the seam over libpxg), in a child process: the
+    reference ends its process on a deck or solver error.  Returns U and a
+    record: the explicit element's bond health (empty for other elements),
+    seconds of run(), seam calls served by the device (seam_ok) and
+    refused (seam_failed), and the assembly and linear-solve seconds and
+    Newton iterations of the reference's own Newton log.  The deck and its
+    output files live in a temporary directory."""
+    import subprocess
+    import sys
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "deck.json")
+        with open(path, "w") as fh:
+            json.dump(deck, fh)
+        out = os.path.join(td, "u.npy")
+        r = subprocess.run([sys.executable, "-m", "oracle.refpy", path, out, str(int(fault_at_call))],
+                           cwd=os.path.dirname(_HERE), capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            tail = "\n".join(line for line in (r.stdout + r.stderr).splitlines() if "Error" in line)[-2000:]
+            raise RuntimeError("reference job failed (rc %d): %s" % (r.returncode, tail))
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        u = np.load(out)
+        rec["health"] = np.load(out + ".health.npy")
+    # the reference's Newton log: "[assemble=<s>s,solve=<s>s]" per iteration
+    import re
+    pairs = re.findall(r"assemble=\s*([0-9.eE+-]+)s,\s*solve=\s*([0-9.eE+-]+)s", r.stderr)
+    rec["assemble_s"] = sum(float(a) for a, _ in pairs)
+    rec["solve_s"] = sum(float(b) for _, b in pairs)
+    rec["newton_iterations"] = len(pairs)
+    return u, rec
+
+
+def _job_main(deck_path, out_path, fault_at_call):
+    """Child side of run_job."""
+    import sys
+    L = pxg_lib()
+    L.perix_cuda_pxg_reset_stats(fault_at_call)
+    with open(deck_path) as fh:
+        cap = deck_dofs(json.load(fh))
+    u = np.zeros(cap)
+    hcap = 200_000_000
+    health = np.zeros(0)
+    sec = C.c_double()
+    nh = C.c_longlong()
+    with open(deck_path) as fh:
+        explicit = "explicit_pddo_frac" in fh.read()
+    if explicit:
+        health = np.zeros(hcap // 8)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)         # the reference's progress goes to stderr
+    try:
+        n = L.ref_run(deck_path.encode(), u.ctypes.data_as(D), cap,
+                      health.ctypes.data_as(D) if explicit else None, health.size, C.byref(nh), C.byref(sec))
+    finally:
+        C.CDLL(None).fflush(None)      # the reference's buffered C stdio
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+    if n != cap:
+        raise SystemExit("ref_run: %d dofs, expected %d" % (n, cap))
+    ok, failed = C.c_longlong(), C.c_longlong()
+    L.perix_cuda_pxg_stats(C.byref(ok), C.byref(failed))
+    np.save(out_path, u)
+    np.save(out_path + ".health.npy", health[:nh.value])
+    print(json.dumps({"seconds": sec.value, "seam_ok": ok.value, "seam_failed": failed.value}))
+
+
 def _dp(a):
     return a.ctypes.data_as(D) if a is not None else None
 
@@ -210,3 +317,8 @@ class Case:
         sc = C.c_double()
         steps = lib().ref_explicit(self.h, _dp(u), _dp(uo), _dp(h), _dp(opc), C.byref(sc))
         return steps, u, uo, h, opc.reshape(self.nb, nops), sc.value
+
+
+if __name__ == "__main__":
+    import sys
+    _job_main(sys.argv[1], sys.argv[2], int(sys.argv[3]))

@@ -16,6 +16,7 @@ void invalidate(pxg_ctx *ctx, int what)
         ctx->dyn.ready = 0;
         ctx->sch.ready = 0;
         ctx->fsch.ready = 0;
+        ctx->mob_set = 0;
     }
     if (what & INV_CRACKS) {
         // element health started from the old crack seed
@@ -63,6 +64,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->crack_seg); rel(c->crack_quad); rel(c->seed); rel(c->ops); rel(c->flags);
     rel(c->rowptr); rel(c->cols); rel(c->slot); rel(c->diag); rel(c->mslot); rel(c->lap);
     rel(c->rev); rel(c->isghost); rel(c->su); rel(c->suo); rel(c->sk);
+    rel(c->mob); rel(c->xrev); rel(c->xlap); rel(c->xmob); rel(c->xvel); rel(c->xghost); rel(c->xhealth);
     rel(c->sr); rel(c->sf);
     rel(c->csrbc.row); rel(c->csrbc.dslot); rel(c->csrbc.pslot); rel(c->csrbc.pdof);
     rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);

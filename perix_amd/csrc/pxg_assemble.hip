@@ -324,6 +324,43 @@ extern "C" int pxg_ch_setup(pxg_ctx *ctx)
     return ch_geometry(ctx);
 }
 
+extern "C" int pxg_ch_set_geometry(pxg_ctx *ctx, const double *lapbond, const int *rev,
+                                   const unsigned char *isghost)
+{
+    PXG_CHECK(ctx && ctx->nb > 0, PXG_ERR_STATE, "pxg_ch_set_geometry: no families");
+    PXG_CHECK(lapbond && rev, PXG_ERR_ARG, "pxg_ch_set_geometry: bad arguments");
+    for (long long t = 0; t < ctx->nb; t++)
+        PXG_CHECK(rev[t] >= -1 && rev[t] < ctx->nb, PXG_ERR_ARG, "pxg_ch_set_geometry: reverse bond out of range");
+    PXG_FAULT(ctx, "pxg_ch_set_geometry");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PXG_HIP(ctx->lap.alloc(ctx->nb));
+    PXG_HIP(ctx->rev.alloc(ctx->nb));
+    PXG_HIP(ctx->isghost.alloc(ctx->n));
+    PXG_HIP(hipMemcpyAsync(ctx->lap.p, lapbond, sizeof(double) * ctx->nb, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->rev.p, rev, sizeof(int) * ctx->nb, hipMemcpyHostToDevice, st));
+    if (isghost) PXG_HIP(hipMemcpyAsync(ctx->isghost.p, isghost, ctx->n, hipMemcpyHostToDevice, st));
+    else PXG_HIP(hipMemsetAsync(ctx->isghost.p, 0, ctx->n, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    ctx->lap_kind = LAP_CH;
+    ctx->lap_gen = ctx->ops_gen;
+    ctx->ch_ready = 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_ch_set_mobility(pxg_ctx *ctx, const double *mnode)
+{
+    PXG_CHECK(ctx && ctx->n > 0, PXG_ERR_STATE, "pxg_ch_set_mobility: no points");
+    if (!mnode) { ctx->mob_set = 0; return PXG_OK; }
+    PXG_FAULT(ctx, "pxg_ch_set_mobility");
+    PXG_HIP(hipSetDevice(ctx->device));
+    PXG_HIP(ctx->mob.alloc(ctx->n));
+    PXG_HIP(hipMemcpyAsync(ctx->mob.p, mnode, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->mob_set = 1;
+    return PXG_OK;
+}
+
 extern "C" int pxg_ch_get_geometry(pxg_ctx *ctx, double *lapbond, int *rev, unsigned char *isghost)
 {
     PXG_CHECK(ctx && ctx->ch_ready, PXG_ERR_STATE, "pxg_ch_get_geometry: call pxg_ch_setup first");

@@ -14,6 +14,7 @@
 // of each cell (ascending id) that pass the distance test.  No atomics touch
 // the lists, and the order is the reference's by construction.
 #include "pxg_internal.h"
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -518,6 +519,139 @@ extern "C" int pxg_build_families(pxg_ctx *ctx, double r, long long *nb_out)
     PXG_HIP(hipGetLastError());
     PXG_HIP(hipStreamSynchronize(st));
     if (nb_out) *nb_out = tot;
+    return PXG_OK;
+}
+
+// seed of a new family set: the crack cut when cracks are set, else all intact
+static int reseed(pxg_ctx *ctx)
+{
+    const long long nb = ctx->nb;
+    PXG_HIP(ctx->seed.alloc((size_t)(nb > 0 ? nb : 1)));
+    if (ctx->nseg + ctx->nquad > 0 && nb > 0)
+        hipLaunchKernelGGL(k_seed, dim3(grid_of(ctx->n, kConvNodes)), dim3(kConvBlock), 0, ctx->stream, ctx->coords.p,
+                           ctx->mirror.p, ctx->n, ctx->off.p, ctx->nbr.p, ctx->crack_seg.p, ctx->nseg,
+                           ctx->crack_quad.p, ctx->nquad, ctx->seed.p);
+    else
+        PXG_HIP(hipMemsetAsync(ctx->seed.p, 1, (size_t)(nb > 0 ? nb : 1), ctx->stream));
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_families(pxg_ctx *ctx, const int *off, const int *nbr)
+{
+    PXG_CHECK(ctx && ctx->n > 0, PXG_ERR_STATE, "pxg_set_families: no points");
+    PXG_CHECK(off, PXG_ERR_ARG, "pxg_set_families: bad arguments");
+    const int n = ctx->n;
+    PXG_CHECK(off[0] == 0, PXG_ERR_ARG, "pxg_set_families: off[0] must be 0");
+    for (int i = 0; i < n; i++)
+        PXG_CHECK(off[i + 1] >= off[i], PXG_ERR_ARG, "pxg_set_families: offsets must not decrease");
+    const long long nb = off[n];
+    PXG_CHECK(nb == 0 || nbr, PXG_ERR_ARG, "pxg_set_families: bad arguments");
+    for (long long t = 0; t < nb; t++)
+        PXG_CHECK(nbr[t] >= 1 && nbr[t] <= n, PXG_ERR_ARG,
+                  "pxg_set_families: neighbour id outside 1..n (ids are 1-based)");
+    PXG_FAULT(ctx, "pxg_set_families");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PXG_HIP(ctx->off.alloc((size_t)n + 1));
+    PXG_HIP(ctx->nbr.alloc((size_t)(nb > 0 ? nb : 1)));
+    PXG_HIP(hipMemcpyAsync(ctx->off.p, off, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+    if (nb > 0) PXG_HIP(hipMemcpyAsync(ctx->nbr.p, nbr, sizeof(int) * (size_t)nb, hipMemcpyHostToDevice, st));
+    ctx->nb = nb;
+    ctx->nop = 0; ctx->nnz = 0;
+    invalidate(ctx, INV_OPS | INV_CRACKS);
+    const int rc = reseed(ctx);
+    if (rc) return rc;
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_csr(pxg_ctx *ctx, int ndof, const int *rowptr, const int *cols)
+{
+    PXG_CHECK(ctx && ctx->nb > 0, PXG_ERR_STATE, "pxg_set_csr: no families");
+    PXG_CHECK(ndof >= 1 && ndof <= 8 && rowptr && cols, PXG_ERR_ARG, "pxg_set_csr: bad arguments");
+    const int n = ctx->n;
+    const long long rows = (long long)n * ndof;
+    PXG_CHECK(rowptr[0] == 0, PXG_ERR_ARG, "pxg_set_csr: rowptr[0] must be 0");
+    for (long long r = 0; r < rows; r++)
+        PXG_CHECK(rowptr[r + 1] >= rowptr[r], PXG_ERR_ARG, "pxg_set_csr: row offsets must not decrease");
+    const long long nnz = rowptr[rows];
+    PXG_CHECK(nnz > 0, PXG_ERR_ARG, "pxg_set_csr: empty pattern");
+    std::vector<int> off((size_t)n + 1), nbr((size_t)ctx->nb);
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(nbr.data(), ctx->nbr.p, sizeof(int) * (size_t)ctx->nb, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    // Every node owns ndof rows with one column pattern: whole ndof x ndof
+    // blocks, sorted, holding the diagonal block and the family's blocks
+    // (SparseMatrix.cpp:90-115).  At most one more block (the ghost's mirror)
+    // may be present; it receives no term and is zeroed by the assembly.
+    std::vector<int> slot((size_t)ctx->nb), diag((size_t)n), mslot((size_t)n, -1);
+    int maxblk = 0;
+    std::vector<int> blk;
+    for (int i = 0; i < n; i++) {
+        const long long r0 = rowptr[(size_t)i * ndof], len = rowptr[(size_t)i * ndof + 1] - r0;
+        PXG_CHECK(len % ndof == 0, PXG_ERR_ARG, "pxg_set_csr: a row does not hold whole dof blocks");
+        const int nbk = (int)(len / ndof);
+        blk.resize((size_t)nbk);
+        for (int k = 0; k < nbk; k++) {
+            const int c = cols[r0 + (long long)k * ndof];
+            PXG_CHECK(c >= 0 && c < rows && c % ndof == 0 && (k == 0 || c / ndof > blk[(size_t)k - 1]), PXG_ERR_ARG,
+                      "pxg_set_csr: columns must be sorted whole dof blocks");
+            blk[(size_t)k] = c / ndof;
+            for (int b = 1; b < ndof; b++)
+                PXG_CHECK(cols[r0 + (long long)k * ndof + b] == c + b, PXG_ERR_ARG,
+                          "pxg_set_csr: columns must be sorted whole dof blocks");
+        }
+        for (int a = 1; a < ndof; a++) {
+            const long long ra = rowptr[(size_t)i * ndof + a];
+            PXG_CHECK(rowptr[(size_t)i * ndof + a + 1] - ra == len, PXG_ERR_ARG,
+                      "pxg_set_csr: the rows of one node must share their pattern");
+            for (long long q = 0; q < len; q++)
+                PXG_CHECK(cols[ra + q] == cols[r0 + q], PXG_ERR_ARG,
+                          "pxg_set_csr: the rows of one node must share their pattern");
+        }
+        auto pos = [&](int node0) {
+            const auto it = std::lower_bound(blk.begin(), blk.end(), node0);
+            return (it != blk.end() && *it == node0) ? (int)(it - blk.begin()) : -1;
+        };
+        const int pd = pos(i);
+        PXG_CHECK(pd >= 0, PXG_ERR_ARG, "pxg_set_csr: a row lacks its diagonal block");
+        diag[(size_t)i] = (int)(r0 + (long long)pd * ndof);
+        std::vector<char> used((size_t)nbk, 0);
+        used[(size_t)pd] = 1;
+        for (int t = off[(size_t)i]; t < off[(size_t)i + 1]; t++) {
+            const int pj = pos(nbr[(size_t)t] - 1);
+            PXG_CHECK(pj >= 0, PXG_ERR_ARG, "pxg_set_csr: a family member is missing from its row");
+            slot[(size_t)t] = (int)(r0 + (long long)pj * ndof);
+            used[(size_t)pj] = 1;
+        }
+        for (int k = 0; k < nbk; k++) {
+            if (used[(size_t)k]) continue;
+            PXG_CHECK(mslot[(size_t)i] < 0, PXG_ERR_ARG,
+                      "pxg_set_csr: a row holds more than one block outside the node's family");
+            mslot[(size_t)i] = (int)(r0 + (long long)k * ndof);
+        }
+        maxblk = std::max(maxblk, nbk);
+    }
+    PXG_FAULT(ctx, "pxg_set_csr");
+    ctx->csrbc.set = 0;
+    PXG_HIP(ctx->rowptr.alloc((size_t)rows + 1));
+    PXG_HIP(ctx->cols.alloc((size_t)nnz));
+    PXG_HIP(ctx->slot.alloc((size_t)ctx->nb));
+    PXG_HIP(ctx->diag.alloc((size_t)n));
+    PXG_HIP(ctx->mslot.alloc((size_t)n));
+    PXG_HIP(hipMemcpyAsync(ctx->rowptr.p, rowptr, sizeof(int) * ((size_t)rows + 1), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->cols.p, cols, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->slot.p, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->diag.p, diag.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->mslot.p, mslot.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    ctx->nnz = nnz;
+    ctx->csr_ndof = ndof;
+    ctx->csr_maxblk = maxblk;
+    ctx->ch_ready = 0;
     return PXG_OK;
 }
 

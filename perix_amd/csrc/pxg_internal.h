@@ -321,6 +321,12 @@ struct pxg_ctx {
     pxg::dbuf<int> rev;             // nb
     pxg::dbuf<unsigned char> isghost;
     int ch_ready = 0;
+    pxg::dbuf<double> mob;          // n: host-frozen Cahn-Hilliard mobility (pxg_ch_set_mobility)
+    int mob_set = 0;
+    // element state handed in by the host on every call (pxg_assemble_*_state)
+    pxg::dbuf<int> xrev;
+    pxg::dbuf<double> xlap, xmob, xvel;
+    pxg::dbuf<unsigned char> xghost, xhealth;
     pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
     pxg::DynFracState dyn;          // implicit dynamic fracture element
     pxg::StressChState sch;         // stress-coupled Cahn-Hilliard element

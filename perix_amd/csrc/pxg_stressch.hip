@@ -150,6 +150,7 @@ struct SchArgs {
     const double *u, *uold;
     double *kvals, *rhs;
     double b[3];                     // dynamic fracture: body force
+    const double *mnode;             // Picard-frozen nodal mobility from the host (null: from u)
 };
 
 // Element kinds sharing the block assembly: the stress-coupled
@@ -278,7 +279,7 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
         if (!ghostJ) {
             g2c = A.lap[t];
             const int rv = A.rev[t];
-            if (rv >= 0) wf = 0.5 * (nd.Mi * g2c + (A.vol[j] / nd.Vi) * sch_mobility(A.u[rj], k.D) * A.lap[rv]);
+            if (rv >= 0) wf = 0.5 * (nd.Mi * g2c + (A.vol[j] / nd.Vi) * (A.mnode ? A.mnode[j] : sch_mobility(A.u[rj], k.D)) * A.lap[rv]);
             else wf = nd.Mi * g2c;
         }
         coeff_c = -wf;
@@ -289,7 +290,7 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
             const int rv = A.rev[t];
             double wij;
             if (rv >= 0 && nd.Vi > 0.0)
-                wij = 0.5 * (nd.Mi * g2ij + (A.vol[j] / nd.Vi) * sch_mobility(A.u[rj], k.D) * A.lap[rv]);
+                wij = 0.5 * (nd.Mi * g2ij + (A.vol[j] / nd.Vi) * (A.mnode ? A.mnode[j] : sch_mobility(A.u[rj], k.D)) * A.lap[rv]);
             else wij = nd.Mi * g2ij;
             coeff_c = -wij;
             kl = k.kappa * g2ij;
@@ -467,7 +468,7 @@ __global__ void __launch_bounds__(64 * NW) k_sch_rows(SchArgs A)
     SchNode<ND> nd;
 #pragma unroll
     for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
-    nd.Mi = sch_mobility(nd.ui[0], A.k.D);
+    nd.Mi = A.mnode ? A.mnode[ic] : sch_mobility(nd.ui[0], A.k.D);
     nd.Vi = A.vol[ic];
     const int t0 = active ? A.off[i] : 0, deg = active ? A.off[i + 1] - t0 : 0;
     for (int tl = l; tl < deg; tl += LPN) {
@@ -548,7 +549,7 @@ __global__ void __launch_bounds__(64 * kSchWaves) k_sch(SchArgs A)
     SchNode<ND> nd;
 #pragma unroll
     for (int a = 0; a < ND; a++) nd.ui[a] = A.u[ri + a];
-    nd.Mi = sch_mobility(nd.ui[0], A.k.D);
+    nd.Mi = A.mnode ? A.mnode[ic] : sch_mobility(nd.ui[0], A.k.D);
     nd.Vi = A.vol[ic];
     const int t0 = active ? A.off[i] : 0, t1 = active ? A.off[i + 1] : 0;
     if (active && lane < ND * ND && A.mslot[i] >= 0) {
@@ -1094,7 +1095,7 @@ int pxg::launch_ch_rows(pxg_ctx *ctx, double kappa, double chi, double M0, doubl
                     ctx->vols.p,
                     FschConst{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, M0, kappa, chi, 0.0, 0.0, 0.0, 1.0}, dt, nullptr,
                     nullptr, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, u_dev, uold_dev, kvals_dev,
-                    rhs_dev, {0.0, 0.0, 0.0}};
+                    rhs_dev, {0.0, 0.0, 0.0}, ctx->mob_set ? ctx->mob.p : nullptr};
     return ctx->dim == 3 ? launch_sch<3, EL_CH>(ctx->stream, ctx->csr_maxblk, A)
                          : launch_sch<2, EL_CH>(ctx->stream, ctx->csr_maxblk, A);
 }
@@ -1110,4 +1111,117 @@ int pxg::launch_dyn_rows(pxg_ctx *ctx, double dt, const double *u_dev, const dou
                     kvals_dev, rhs_dev, {D.b[0], D.b[1], D.b[2]}};
     return ctx->dim == 3 ? launch_sch<3, EL_DYN>(ctx->stream, ctx->csr_maxblk, A)
                          : launch_sch<2, EL_DYN>(ctx->stream, ctx->csr_maxblk, A);
+}
+
+// ---- element state handed in by the host ------------------------------------
+// The reference's seam passes the element's own host-side state on every
+// call (perix_cuda::assembleStressCahnHilliard, CudaAssemble.h:185-191, and
+// perix_cuda::assembleCached, CudaAssemble.h:216-219, called from
+// FormResidualAndJacobianCUDA.cpp:229-302): the committed bond health, the
+// committed velocity and the Picard-frozen mobility.  These entry points
+// upload that state and run the same block-element kernels as the
+// device-resident elements.
+
+static int upload_health(pxg_ctx *ctx, const double *health)
+{
+    // health > 0 intact (the reference keeps 0.0 / 1.0 per bond)
+    const long long nb = ctx->nb;
+    std::vector<unsigned char> h((size_t)(nb > 0 ? nb : 1), 1);
+    if (health)
+        for (long long t = 0; t < nb; t++) h[(size_t)t] = health[t] > 0.0 ? 1 : 0;
+    PXG_HIP(ctx->xhealth.alloc(h.size()));
+    PXG_HIP(hipMemcpyAsync(ctx->xhealth.p, h.data(), h.size(), hipMemcpyHostToDevice, ctx->stream));
+    // the host vector must outlive the copy
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_sch_state(pxg_ctx *ctx, const pxg_sch_state *s, double dt, const double *u,
+                                      const double *uold, double *kvals, double *rhs)
+{
+    PXG_CHECK(ctx && ctx->nop >= (ctx->dim == 3 ? 9 : 5) && ctx->nnz > 0, PXG_ERR_STATE,
+              "pxg_assemble_sch_state: needs order-2 operators and a CSR pattern");
+    PXG_CHECK(ctx->csr_ndof == ctx->dim + 2, PXG_ERR_STATE, "pxg_assemble_sch_state: needs dim+2 dofs per node");
+    PXG_CHECK(s && s->rev && s->lap && s->mnode && u && uold && kvals && rhs && dt > 0.0, PXG_ERR_ARG,
+              "pxg_assemble_sch_state: bad arguments (the element is transient-only, dt > 0)");
+    PXG_CHECK(!(s->rho > 0.0) || s->vel, PXG_ERR_ARG, "pxg_assemble_sch_state: rho > 0 needs the velocity");
+    PXG_FAULT(ctx, "pxg_assemble_sch_state");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n, dim = ctx->dim;
+    const long long nb = ctx->nb;
+    const size_t nd = (size_t)n * (dim + 2);
+    PXG_HIP(ctx->xrev.alloc((size_t)nb));
+    PXG_HIP(ctx->xlap.alloc((size_t)nb));
+    PXG_HIP(ctx->xmob.alloc((size_t)n));
+    PXG_HIP(ctx->xghost.alloc((size_t)n));
+    PXG_HIP(hipMemcpyAsync(ctx->xrev.p, s->rev, sizeof(int) * (size_t)nb, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->xlap.p, s->lap, sizeof(double) * (size_t)nb, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->xmob.p, s->mnode, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, st));
+    if (s->isghost) PXG_HIP(hipMemcpyAsync(ctx->xghost.p, s->isghost, (size_t)n, hipMemcpyHostToDevice, st));
+    else PXG_HIP(hipMemsetAsync(ctx->xghost.p, 0, (size_t)n, st));
+    const bool inertia = s->rho > 0.0;
+    if (inertia) {
+        PXG_HIP(ctx->xvel.alloc((size_t)n * dim));
+        PXG_HIP(hipMemcpyAsync(ctx->xvel.p, s->vel, sizeof(double) * (size_t)n * dim, hipMemcpyHostToDevice, st));
+    }
+    {
+        const int rc = upload_health(ctx, s->health);
+        if (rc) return rc;
+    }
+    PXG_HIP(ctx->su.alloc(nd));
+    PXG_HIP(ctx->suo.alloc(nd));
+    PXG_HIP(ctx->sk.alloc((size_t)ctx->nnz));
+    PXG_HIP(ctx->sr.alloc(nd));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    const SchArgs A{n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, ctx->xrev.p, ctx->xlap.p, ctx->xghost.p,
+                    ctx->vols.p,
+                    FschConst{s->C11, s->C12, s->C66, s->A, s->Kh, s->Ch, 0.0, s->kappa, s->chi, s->Omega, s->cref,
+                              inertia ? s->rho : 0.0, s->kres},
+                    dt, inertia ? ctx->xvel.p : nullptr, ctx->xhealth.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p,
+                    ctx->mslot.p, ctx->su.p, ctx->suo.p, ctx->sk.p, ctx->sr.p, {0.0, 0.0, 0.0}, ctx->xmob.p};
+    const int rc = dim == 3 ? launch_sch<3, EL_FSCH>(st, ctx->csr_maxblk, A) : launch_sch<2, EL_FSCH>(st, ctx->csr_maxblk, A);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * (size_t)ctx->nnz, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_assemble_dynfrac_state(pxg_ctx *ctx, const pxg_dynfrac_state *s, double dt, const double *u,
+                                          const double *uold, double *kvals, double *rhs)
+{
+    PXG_CHECK(ctx && ctx->nop >= (ctx->dim == 3 ? 9 : 5) && ctx->nnz > 0, PXG_ERR_STATE,
+              "pxg_assemble_dynfrac_state: needs order-2 operators and a CSR pattern");
+    PXG_CHECK(ctx->csr_ndof == ctx->dim, PXG_ERR_STATE, "pxg_assemble_dynfrac_state: needs dim dofs per node");
+    PXG_CHECK(s && s->vel && u && uold && kvals && rhs && dt > 0.0 && s->rho > 0.0, PXG_ERR_ARG,
+              "pxg_assemble_dynfrac_state: bad arguments (dt > 0 and rho > 0)");
+    PXG_FAULT(ctx, "pxg_assemble_dynfrac_state");
+    PXG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n, dim = ctx->dim;
+    const size_t nd = (size_t)n * dim;
+    PXG_HIP(ctx->xvel.alloc(nd));
+    PXG_HIP(hipMemcpyAsync(ctx->xvel.p, s->vel, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    {
+        const int rc = upload_health(ctx, s->health);
+        if (rc) return rc;
+    }
+    PXG_HIP(ctx->su.alloc(nd));
+    PXG_HIP(ctx->suo.alloc(nd));
+    PXG_HIP(ctx->sk.alloc((size_t)ctx->nnz));
+    PXG_HIP(ctx->sr.alloc(nd));
+    PXG_HIP(hipMemcpyAsync(ctx->su.p, u, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(ctx->suo.p, uold, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    const SchArgs A{n, ctx->nop, ctx->off.p, ctx->nbr.p, ctx->ops.p, nullptr, nullptr, nullptr, ctx->vols.p,
+                    FschConst{s->C11, s->C12, s->C66, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, s->rho, 1.0}, dt,
+                    ctx->xvel.p, ctx->xhealth.p, ctx->rowptr.p, ctx->slot.p, ctx->diag.p, ctx->mslot.p, ctx->su.p,
+                    ctx->suo.p, ctx->sk.p, ctx->sr.p, {s->bx, s->by, s->bz}, nullptr};
+    const int rc = dim == 3 ? launch_sch<3, EL_DYN>(st, ctx->csr_maxblk, A) : launch_sch<2, EL_DYN>(st, ctx->csr_maxblk, A);
+    if (rc) return rc;
+    PXG_HIP(hipMemcpyAsync(kvals, ctx->sk.p, sizeof(double) * (size_t)ctx->nnz, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipMemcpyAsync(rhs, ctx->sr.p, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
 }

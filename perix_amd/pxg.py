@@ -56,6 +56,21 @@ class FschParams(C.Structure):
                 ("damage_on", C.c_int)]
 
 
+class SchState(C.Structure):
+    """pxg_sch_state: a fracture stress-CH element's host-side state."""
+    _fields_ = [(k, C.c_double) for k in ("C11", "C12", "C66", "A", "Kh", "Ch", "Omega", "cref", "chi",
+                                          "kappa", "rho", "kres")] + \
+               [("rev", C.POINTER(C.c_int)), ("lap", C.POINTER(C.c_double)), ("isghost", C.POINTER(C.c_ubyte)),
+                ("mnode", C.POINTER(C.c_double)), ("health", C.POINTER(C.c_double)),
+                ("vel", C.POINTER(C.c_double))]
+
+
+class DynFracState(C.Structure):
+    """pxg_dynfrac_state: a dynamic fracture element's host-side state."""
+    _fields_ = [(k, C.c_double) for k in ("C11", "C12", "C66", "rho", "bx", "by", "bz")] + \
+               [("vel", C.POINTER(C.c_double)), ("health", C.POINTER(C.c_double))]
+
+
 class BC(C.Structure):
     _fields_ = [("type", C.c_int), ("direct", C.c_int), ("ncomp", C.c_int),
                 ("comp", C.c_int * 3), ("value", C.c_double * 3),
@@ -67,7 +82,9 @@ class BC(C.Structure):
 SYMBOLS = [
     "pxg_create", "pxg_destroy", "pxg_last_error", "pxg_set_fault", "pxg_sync",
     "pxg_structured_sizes", "pxg_structured_cloud", "pxg_horizon_radius", "pxg_random_ic",
-    "pxg_schedule", "pxg_set_points", "pxg_build_families", "pxg_get_families",
+    "pxg_schedule", "pxg_set_points", "pxg_build_families", "pxg_get_families", "pxg_set_families",
+    "pxg_set_csr", "pxg_ch_set_geometry", "pxg_ch_set_mobility", "pxg_assemble_sch_state",
+    "pxg_assemble_dynfrac_state",
     "pxg_set_cracks", "pxg_get_seed", "pxg_build_csr", "pxg_get_csr", "pxg_build_operators",
     "pxg_get_operators", "pxg_assemble_poisson", "pxg_assemble_poisson_dev", "pxg_assemble_poisson_noalloc",
     "pxg_assemble_diffusion", "pxg_assemble_diffusion_dev", "pxg_ch_setup",
@@ -146,6 +163,8 @@ def load():
         L.pxg_assemble_poisson.argtypes = [C.c_void_p, C.c_double, D, C.c_double, D, D, D]
         L.pxg_assemble_poisson_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
                                                C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pxg_assemble_sch_state.argtypes = [C.c_void_p, C.c_void_p, C.c_double, D, D, D, D]
+        L.pxg_assemble_dynfrac_state.argtypes = [C.c_void_p, C.c_void_p, C.c_double, D, D, D, D]
         L.pxg_assemble_poisson_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_double,
                                                    C.c_void_p, C.c_void_p, C.c_void_p]
         L.pxg_assemble_diffusion.argtypes = [C.c_void_p, C.c_double, D, C.c_double, C.c_double,
@@ -322,6 +341,58 @@ class Context:
         _check(load().pxg_build_families(self.h, C.c_double(r), C.byref(nb)))
         self.nb = nb.value
         return self.nb
+
+    def set_families(self, off, nbr):
+        """pxg_set_families: the caller's own flat families (1-based ids)."""
+        off = np.ascontiguousarray(off, np.int32)
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        _check(load().pxg_set_families(self.h, _ip(off), _ip(nbr)))
+        self.nb = int(off[-1])
+        self.nop = self.nnz = 0
+
+    def set_csr(self, ndof, rowptr, cols):
+        """pxg_set_csr: the caller's own CSR pattern."""
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        _check(load().pxg_set_csr(self.h, int(ndof), _ip(rowptr), _ip(cols)))
+        self.nnz = int(rowptr[-1])
+        self.ndof = ndof
+
+    def ch_set_geometry(self, lap, rev, isghost=None):
+        lap = np.ascontiguousarray(lap, float)
+        rev = np.ascontiguousarray(rev, np.int32)
+        g = None if isghost is None else np.ascontiguousarray(isghost, np.uint8)
+        _check(load().pxg_ch_set_geometry(self.h, _dp(lap), _ip(rev), _up(g)))
+
+    def ch_set_mobility(self, mnode):
+        m = None if mnode is None else np.ascontiguousarray(mnode, float)
+        _check(load().pxg_ch_set_mobility(self.h, _dp(m)))
+
+    def assemble_sch_state(self, consts, dt, u, uold, rev, lap, mnode, isghost=None, health=None, vel=None):
+        """pxg_assemble_sch_state; consts: dict of the 12 pxg_sch_state constants."""
+        keep = [np.ascontiguousarray(rev, np.int32), np.ascontiguousarray(lap, float),
+                None if isghost is None else np.ascontiguousarray(isghost, np.uint8),
+                np.ascontiguousarray(mnode, float),
+                None if health is None else np.ascontiguousarray(health, float),
+                None if vel is None else np.ascontiguousarray(vel, float)]
+        st = SchState(**consts)
+        st.rev, st.lap, st.isghost = _ip(keep[0]), _dp(keep[1]), _up(keep[2])
+        st.mnode, st.health, st.vel = _dp(keep[3]), _dp(keep[4]), _dp(keep[5])
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n * (self.dim + 2))
+        _check(load().pxg_assemble_sch_state(self.h, C.byref(st), dt, _dp(np.ascontiguousarray(u, float)),
+                                             _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
+
+    def assemble_dynfrac_state(self, consts, dt, u, uold, vel, health=None):
+        keep = [np.ascontiguousarray(vel, float), None if health is None else np.ascontiguousarray(health, float)]
+        st = DynFracState(**consts)
+        st.vel, st.health = _dp(keep[0]), _dp(keep[1])
+        k = np.zeros(self.nnz)
+        r = np.zeros(self.n * self.dim)
+        _check(load().pxg_assemble_dynfrac_state(self.h, C.byref(st), dt, _dp(np.ascontiguousarray(u, float)),
+                                                 _dp(np.ascontiguousarray(uold, float)), _dp(k), _dp(r)))
+        return k, r
 
     def families(self):
         off = np.zeros(self.n + 1, np.int32)
