@@ -37,11 +37,16 @@
 // Family sum.  FP64: every lane parks its bond forces in LDS (broken bonds as
 // -0.0, the additive identity), then one lane per component adds the node's
 // bonds in family order, which is the reference's exact summation order.
-// FP32C: forces are formed in FP32 from FP32 operators and FP32 displacement
-// differences; each lane adds a group's G forces pairwise and accumulates the
-// group sums with TwoSum (an FP32 sum/compensation pair); the S lanes of a
-// node combine their pairs with a lane butterfly (DPP within 16 lanes), and
-// the FP64 update consumes sum + compensation.
+// FP32C: forces are formed in FP32 pair arithmetic from FP32 operators: the
+// FP64 displacement difference splits exactly into an FP32 pair (hi + lo),
+// each operator product is error-free (the fma residual of p*hi plus p*lo),
+// the products of a force component are added with TwoSum, and every bond's
+// (value, error) pair enters the lane's TwoSum-compensated (sum,
+// compensation) pair.  The error of a bond force is then that of the FP32
+// operators alone, a fixed perturbation of the operator, with no rounding
+// noise added per step.  The S lanes of a node combine their pairs with a
+// lane butterfly (DPP within 16 lanes), and the FP64 update consumes
+// sum + compensation.
 #include "pxg_internal.h"
 #include <algorithm>
 #include <cfloat>
@@ -184,26 +189,39 @@ __device__ __forceinline__ float partner(float v)
     else return __shfl_xor(v, M);
 }
 
-// a group's G bond forces are added pairwise in FP32 and the group sum
-// enters the lane's TwoSum-compensated (sum, compensation) pair
-template <int G, int DIM>
-__device__ __forceinline__ void accumulate_group(const float (&f)[G][DIM], float (&s)[DIM], float (&c)[DIM])
+// TwoProd: a * b = p + e exactly (fma residual)
+__device__ __forceinline__ void two_prod(float a, float b, float &p, float &e)
 {
+    p = a * b;
+    e = fmaf(a, b, -p);
+}
+
+// one component of a bond force, sum_q a[q] (dh[q] + dl[q]), as an FP32 pair
+// (s, c): error-free products of the hi parts, their lo-part products and
+// residuals in c, the hi products added with TwoSum
+template <int NT>
+__device__ __forceinline__ void pair_dot(const float (&a)[NT], const float (&dh)[NT], const float (&dl)[NT],
+                                         float &s, float &c)
+{
+    float e;
+    two_prod(a[0], dh[0], s, e);
+    c = fmaf(a[0], dl[0], e);
 #pragma unroll
-    for (int k = 0; k < DIM; k++) {
-        float g;
-        if constexpr (G == 4) g = (f[0][k] + f[1][k]) + (f[2][k] + f[3][k]);
-        else if constexpr (G == 2) g = f[0][k] + f[1][k];
-        else {
-            g = f[0][k];
-#pragma unroll
-            for (int q = 1; q < G; q++) g += f[q][k];
-        }
-        float sn, e;
-        two_sum(s[k], g, sn, e);
-        s[k] = sn;
-        c[k] += e;
+    for (int q = 1; q < NT; q++) {
+        float pq, eq, es;
+        two_prod(a[q], dh[q], pq, eq);
+        two_sum(s, pq, s, es);
+        c = c + (fmaf(a[q], dl[q], eq) + es);
     }
+}
+
+// a bond force pair enters the lane's TwoSum-compensated (sum, compensation)
+__device__ __forceinline__ void accumulate_pair(float fs, float fc, float &s, float &c)
+{
+    float sn, e;
+    two_sum(s, fs, sn, e);
+    s = sn;
+    c = c + (e + fc);
 }
 
 template <int M, int S, int DIM>
@@ -422,7 +440,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 
     // damage and forces of one loaded group (dead bonds give f = 0 / -0.0)
     using F64G = double[F32 ? 1 : G][DIM];
-    using F32G = float[F32 ? G : 1][DIM];
+    using F32G = float[F32 ? 2 : 1][DIM];       // FP32C: the lane's (sum, compensation) pair
     auto compute_group = [&](unsigned t0, const GD &dd, F64G &f64, F32G &f32) {
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -500,18 +518,26 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 }
             }
             if constexpr (F32) {
-                float dv[DIM];
+                // exact FP32 pair of the FP64 displacement difference (a dead
+                // bond contributes the pair (0, 0), which leaves the sums as they are)
+                float dh[DIM], dl[DIM];
 #pragma unroll
-                for (int c = 0; c < DIM; c++) dv[c] = live ? (float)du[c] : 0.0f;
-                const auto &p = dd.p32;
-                if (DIM == 2) {
-                    f32[k][0] = fmaf(p[0][k], dv[0], p[2][k] * dv[1]);
-                    f32[k][1] = fmaf(p[2][k], dv[0], p[1][k] * dv[1]);
-                } else {
-                    f32[k][0] = fmaf(p[0][k], dv[0], fmaf(p[3][k], dv[1], p[4][k] * dv[DIM - 1]));
-                    f32[k][1] = fmaf(p[3][k], dv[0], fmaf(p[1][k], dv[1], p[5][k] * dv[DIM - 1]));
-                    f32[k][DIM - 1] = fmaf(p[4][k], dv[0], fmaf(p[5][k], dv[1], p[2][k] * dv[DIM - 1]));
+                for (int c = 0; c < DIM; c++) {
+                    dh[c] = live ? (float)du[c] : 0.0f;
+                    dl[c] = live ? (float)(du[c] - (double)dh[c]) : 0.0f;
                 }
+                const auto &p = dd.p32;
+                float fs[DIM], fc[DIM];
+                if constexpr (DIM == 2) {
+                    pair_dot<2>({p[0][k], p[2][k]}, dh, dl, fs[0], fc[0]);
+                    pair_dot<2>({p[2][k], p[1][k]}, dh, dl, fs[1], fc[1]);
+                } else {
+                    pair_dot<3>({p[0][k], p[3][k], p[4][k]}, dh, dl, fs[0], fc[0]);
+                    pair_dot<3>({p[3][k], p[1][k], p[5][k]}, dh, dl, fs[1], fc[1]);
+                    pair_dot<3>({p[4][k], p[5][k], p[2][k]}, dh, dl, fs[DIM - 1], fc[DIM - 1]);
+                }
+#pragma unroll
+                for (int c = 0; c < DIM; c++) accumulate_pair(fs[c], fc[c], f32[0][c], f32[1][c]);
             } else {
                 // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295);
                 // a dead bond contributes -0.0, the additive identity
@@ -535,27 +561,26 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 
     double Lown = 0.0;     // family sum of component sl (lanes sl < DIM)
     if constexpr (F32) {
-        float Ls[DIM], Lc[DIM];
+        F32G acc;              // the lane's (sum, compensation) pair per component
 #pragma unroll
-        for (int c = 0; c < DIM; c++) { Ls[c] = 0.0f; Lc[c] = 0.0f; }
+        for (int c = 0; c < DIM; c++) { acc[0][c] = 0.0f; acc[1][c] = 0.0f; }
         for (int t0 = tfirst; t0 < b1; t0 += W) {
             if constexpr (PF2) {
                 // the next group's loads fly while this one is computed
                 GD dn;
                 if (t0 + W < b1) load_group((unsigned)(t0 + W), dn);
                 F64G f64;
-                F32G f32;
-                compute_group((unsigned)t0, d, f64, f32);
-                accumulate_group<G, DIM>(f32, Ls, Lc);
+                compute_group((unsigned)t0, d, f64, acc);
                 d = dn;
             } else {
                 if (STAGE || t0 != tfirst) load_group((unsigned)t0, d);
                 F64G f64;
-                F32G f32;
-                compute_group((unsigned)t0, d, f64, f32);
-                accumulate_group<G, DIM>(f32, Ls, Lc);
+                compute_group((unsigned)t0, d, f64, acc);
             }
         }
+        float Ls[DIM], Lc[DIM];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) { Ls[c] = acc[0][c]; Lc[c] = acc[1][c]; }
         butterfly<1, S, DIM>(Ls, Lc);
         double L[DIM];
 #pragma unroll
