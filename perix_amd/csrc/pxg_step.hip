@@ -37,16 +37,15 @@
 // Family sum.  FP64: every lane parks its bond forces in LDS (broken bonds as
 // -0.0, the additive identity), then one lane per component adds the node's
 // bonds in family order, which is the reference's exact summation order.
-// FP32C: forces are formed in FP32 pair arithmetic from FP32 operators: the
-// FP64 displacement difference splits exactly into an FP32 pair (hi + lo),
-// each operator product is error-free (the fma residual of p*hi plus p*lo),
-// the products of a force component are added with TwoSum, and every bond's
-// (value, error) pair enters the lane's TwoSum-compensated (sum,
-// compensation) pair.  The error of a bond force is then that of the FP32
-// operators alone, a fixed perturbation of the operator, with no rounding
-// noise added per step.  The S lanes of a node combine their pairs with a
-// lane butterfly (DPP within 16 lanes), and the FP64 update consumes
-// sum + compensation.
+// FP32C: the operators are stored in FP32 (the bytes the step streams) and
+// every bond force is formed and summed in FP64: an FP64 fma on an FP32
+// operator costs the same issue slot as an FP32 fma on CDNA4 (packed FP32 is
+// not used), and it makes the bond force exact up to the operators' own
+// FP32 rounding -- the result an FP32-pair (error-free product + TwoSum)
+// evaluation reaches at ten times the instructions.  The error against FP64
+// is then a fixed perturbation of the operator, with no rounding noise added
+// per step.  The S lanes of a node combine their FP64 sums with a lane
+// butterfly (DPP within 16 lanes).
 #include "pxg_internal.h"
 #include <algorithm>
 #include <cfloat>
@@ -165,14 +164,6 @@ __device__ __forceinline__ void vload(const T *p, T (&o)[G])
     }
 }
 
-// TwoSum (Knuth): a + b = s + e exactly, FP32
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
-{
-    s = a + b;
-    const float bp = s - a;
-    e = (a - (s - bp)) + (b - bp);
-}
-
 // value held by the partner lane at butterfly level M.  Levels run M = 1, 2,
 // 4, ...; before level M every lane of an aligned M-lane group holds the same
 // value, so any pairing of the two halves of a 2M-lane group is a valid
@@ -189,55 +180,26 @@ __device__ __forceinline__ float partner(float v)
     else return __shfl_xor(v, M);
 }
 
-// TwoProd: a * b = p + e exactly (fma residual)
-__device__ __forceinline__ void two_prod(float a, float b, float &p, float &e)
+// FP64 value held by the partner lane at butterfly level M (two 32-bit moves)
+template <int M>
+__device__ __forceinline__ double partner64(double v)
 {
-    p = a * b;
-    e = fmaf(a, b, -p);
+    const long long b = __builtin_bit_cast(long long, v);
+    const float lo = __builtin_bit_cast(float, (int)(b & 0xffffffffLL));
+    const float hi = __builtin_bit_cast(float, (int)(b >> 32));
+    const long long rl = (long long)(unsigned)__builtin_bit_cast(int, partner<M>(lo));
+    const long long rh = (long long)__builtin_bit_cast(int, partner<M>(hi));
+    return __builtin_bit_cast(double, (rh << 32) | rl);
 }
 
-// one component of a bond force, sum_q a[q] (dh[q] + dl[q]), as an FP32 pair
-// (s, c): error-free products of the hi parts, their lo-part products and
-// residuals in c, the hi products added with TwoSum
-template <int NT>
-__device__ __forceinline__ void pair_dot(const float (&a)[NT], const float (&dh)[NT], const float (&dl)[NT],
-                                         float &s, float &c)
-{
-    float e;
-    two_prod(a[0], dh[0], s, e);
-    c = fmaf(a[0], dl[0], e);
-#pragma unroll
-    for (int q = 1; q < NT; q++) {
-        float pq, eq, es;
-        two_prod(a[q], dh[q], pq, eq);
-        two_sum(s, pq, s, es);
-        c = c + (fmaf(a[q], dl[q], eq) + es);
-    }
-}
-
-// a bond force pair enters the lane's TwoSum-compensated (sum, compensation)
-__device__ __forceinline__ void accumulate_pair(float fs, float fc, float &s, float &c)
-{
-    float sn, e;
-    two_sum(s, fs, sn, e);
-    s = sn;
-    c = c + (e + fc);
-}
-
+// sums the FP64 family sums of the S lanes of a node (every lane ends with the total)
 template <int M, int S, int DIM>
-__device__ __forceinline__ void butterfly(float (&s)[DIM], float (&c)[DIM])
+__device__ __forceinline__ void butterfly64(double (&L)[DIM])
 {
     if constexpr (M < S) {
 #pragma unroll
-        for (int k = 0; k < DIM; k++) {
-            const float os = partner<M>(s[k]);
-            const float oc = partner<M>(c[k]);
-            float sn, e;
-            two_sum(s[k], os, sn, e);
-            s[k] = sn;
-            c[k] = c[k] + oc + e;
-        }
-        butterfly<2 * M, S, DIM>(s, c);
+        for (int k = 0; k < DIM; k++) L[k] = L[k] + partner64<M>(L[k]);
+        butterfly64<2 * M, S, DIM>(L);
     }
 }
 
@@ -440,7 +402,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 
     // damage and forces of one loaded group (dead bonds give f = 0 / -0.0)
     using F64G = double[F32 ? 1 : G][DIM];
-    using F32G = float[F32 ? 2 : 1][DIM];       // FP32C: the lane's (sum, compensation) pair
+    using F32G = double[DIM];                  // FP32C: the lane's FP64 family sum
     auto compute_group = [&](unsigned t0, const GD &dd, F64G &f64, F32G &f32) {
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -518,26 +480,21 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 }
             }
             if constexpr (F32) {
-                // exact FP32 pair of the FP64 displacement difference (a dead
-                // bond contributes the pair (0, 0), which leaves the sums as they are)
-                float dh[DIM], dl[DIM];
-#pragma unroll
-                for (int c = 0; c < DIM; c++) {
-                    dh[c] = live ? (float)du[c] : 0.0f;
-                    dl[c] = live ? (float)(du[c] - (double)dh[c]) : 0.0f;
+                // FP32 operators, FP64 products and sums (a dead bond adds nothing)
+                if (live) {
+                    const auto &p = dd.p32;
+                    if constexpr (DIM == 2) {
+                        const double q0 = p[0][k], q1 = p[1][k], q2 = p[2][k];
+                        f32[0] = fma(q2, du[1], fma(q0, du[0], f32[0]));
+                        f32[1] = fma(q1, du[1], fma(q2, du[0], f32[1]));
+                    } else {
+                        const double q0 = p[0][k], q1 = p[1][k], q2 = p[2][k];
+                        const double q3 = p[3][k], q4 = p[4][k], q5 = p[5][k];
+                        f32[0] = fma(q4, du[DIM - 1], fma(q3, du[1], fma(q0, du[0], f32[0])));
+                        f32[1] = fma(q5, du[DIM - 1], fma(q1, du[1], fma(q3, du[0], f32[1])));
+                        f32[DIM - 1] = fma(q2, du[DIM - 1], fma(q5, du[1], fma(q4, du[0], f32[DIM - 1])));
+                    }
                 }
-                const auto &p = dd.p32;
-                float fs[DIM], fc[DIM];
-                if constexpr (DIM == 2) {
-                    pair_dot<2>({p[0][k], p[2][k]}, dh, dl, fs[0], fc[0]);
-                    pair_dot<2>({p[2][k], p[1][k]}, dh, dl, fs[1], fc[1]);
-                } else {
-                    pair_dot<3>({p[0][k], p[3][k], p[4][k]}, dh, dl, fs[0], fc[0]);
-                    pair_dot<3>({p[3][k], p[1][k], p[5][k]}, dh, dl, fs[1], fc[1]);
-                    pair_dot<3>({p[4][k], p[5][k], p[2][k]}, dh, dl, fs[DIM - 1], fc[DIM - 1]);
-                }
-#pragma unroll
-                for (int c = 0; c < DIM; c++) accumulate_pair(fs[c], fc[c], f32[0][c], f32[1][c]);
             } else {
                 // health-gated Navier force (ExplicitPDDOFracElement.cpp:268-295);
                 // a dead bond contributes -0.0, the additive identity
@@ -561,9 +518,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 
     double Lown = 0.0;     // family sum of component sl (lanes sl < DIM)
     if constexpr (F32) {
-        F32G acc;              // the lane's (sum, compensation) pair per component
+        F32G acc;              // the lane's FP64 family sum per component
 #pragma unroll
-        for (int c = 0; c < DIM; c++) { acc[0][c] = 0.0f; acc[1][c] = 0.0f; }
+        for (int c = 0; c < DIM; c++) acc[c] = 0.0;
         for (int t0 = tfirst; t0 < b1; t0 += W) {
             if constexpr (PF2) {
                 // the next group's loads fly while this one is computed
@@ -578,14 +535,8 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 compute_group((unsigned)t0, d, f64, acc);
             }
         }
-        float Ls[DIM], Lc[DIM];
-#pragma unroll
-        for (int c = 0; c < DIM; c++) { Ls[c] = acc[0][c]; Lc[c] = acc[1][c]; }
-        butterfly<1, S, DIM>(Ls, Lc);
-        double L[DIM];
-#pragma unroll
-        for (int c = 0; c < DIM; c++) L[c] = (double)Ls[c] + (double)Lc[c];
-        Lown = pick(L, sl);
+        butterfly64<1, S, DIM>(acc);
+        Lown = pick(acc, sl);
     } else {
         int nr = nvalid ? (b1 - A + W - 1) / W : 0;
 #pragma unroll

@@ -84,6 +84,38 @@ def test_explicit_fp32c_error(oracle):
     s64.ctx.close(); s32.ctx.close()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32c_is_fp64_on_fp32_operators(dim):
+    """FP32C stores the operators in FP32 and forms and sums the bond forces in
+    FP64, so its trajectory is that of the FP64 step run on the FP32-rounded
+    operators (PXG_FRAC_FP32_OPERATORS): the two differ only by the order of
+    the family sum (lane partial sums vs family order), far below the last
+    bit of u, and break the same bonds at the same steps."""
+    if dim == 2:
+        kw = dict(hi=(0.02, 0.01, 1.0), sc=0.003, cracks=[(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+                  box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=300)
+        args = (2, 80, 40)
+    else:
+        kw = dict(hi=(0.024, 0.012, 0.006), sc=0.003, cracks=[(0.012, 0.0, 0.012, 0.0012)],
+                  box=(0.006, 0.009, 1.0, -1.0, 1.0, -1.0), nsteps=200)
+        args = (3, 24, 12, 6)
+    runs = []
+    for prec, opts in ((pxg.FP32C, pxg.Context.FRAC_NO_SHARED_RUNS), (pxg.FP64, pxg.Context.FRAC_FP32_OPERATORS)):
+        ctx = pxg.Context(0)
+        ctx.frac_options(opts)
+        s = problems.explicit_plate(*args, precision=prec, ctx=ctx, **kw)
+        s.ctx.frac_init(s.u0, s.dt)
+        s.ctx.frac_run(s.dt, s.total)
+        runs.append(s.ctx.frac_state())
+        s.ctx.close()
+    (u32, _, h32, b32), (u64, _, h64, b64) = runs
+    rel = np.linalg.norm(u32 - u64) / np.linalg.norm(u64)
+    print("FP32C vs FP64 on FP32 operators: rel-L2 %.2e, bit-equal fraction %.4f, broken %d"
+          % (rel, np.mean(u32 == u64), int((h64 == 0).sum())))
+    assert np.array_equal(h32, h64) and np.array_equal(b32, b64)
+    assert rel <= 1e-13
+
+
 def test_drop_in_accel(oracle):
     """pxg_frac_accel = one preprocessIteration (damage + force + accel)."""
     s = problems.explicit_plate(2, 40, 20, hi=(0.02, 0.01, 1.0), sc=0.003,
