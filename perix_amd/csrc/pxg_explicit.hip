@@ -552,7 +552,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_HIP(F.accel.alloc((size_t)ctx->n * ctx->dim));
     PXG_HIP(F.preset_mask.alloc(ctx->n));
     PXG_HIP(hipMemsetAsync(F.preset_mask.p, 0, ctx->n, st));
-    PXG_HIP(F.gmax.alloc(8));
+    PXG_HIP(F.gmax.alloc(kGuardWords));
     PXG_HIP(F.ghist.alloc(50));
     PXG_HIP(F.ghist64.alloc(64));
     // empty BC program until pxg_frac_set_bcs
@@ -563,7 +563,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_HIP(F.bc.inv_q.alloc(1));
     PXG_HIP(F.gstate.alloc(4));
     PXG_HIP(F.anybroke.alloc(1));
-    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 8, st));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * kGuardWords, st));
     PXG_HIP(hipMemsetAsync(F.ghist.p, 0, sizeof(double) * 50, st));
     PXG_HIP(hipMemsetAsync(F.ghist64.p, 0, sizeof(double) * 64, st));
     PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
@@ -733,7 +733,7 @@ extern "C" int pxg_frac_init(pxg_ctx *ctx, const double *u0, const double *v0, d
     PXG_HIP(hipMemcpyAsync(F.u[0].p, u0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     if (v0) PXG_HIP(hipMemcpyAsync(F.accel.p, v0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
     else PXG_HIP(hipMemsetAsync(F.accel.p, 0, sizeof(double) * nd, st));
-    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * 8, st));
+    PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * kGuardWords, st));
     PXG_HIP(hipMemsetAsync(F.gstate.p, 0, sizeof(int) * 4, st));
     PXG_HIP(hipMemsetAsync(F.ghist64.p, 0, sizeof(double) * 64, st));
     // t = 0 rate evaluation (damage included), then u^{-1} = u0 - dt v0 + dt^2/2 a0

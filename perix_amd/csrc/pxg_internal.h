@@ -51,6 +51,11 @@ void set_error(const std::string &msg);
 // Per-node loops over a family whose bonds are independent (conversions,
 // set-up): 16 consecutive lanes walk one family, 8 nodes per 128-thread
 // block, so a family's loads and stores are coalesced.
+// fused-path guard slots: kGuardSlots 256-byte-spaced words per slot set,
+// three sets after kGuardBase words of the separate-preset path
+constexpr int kGuardBase = 8, kGuardSlots = 64, kGuardStride = 32;
+constexpr int kGuardWords = kGuardBase + 3 * kGuardSlots * kGuardStride;
+static_assert(kGuardSlots <= 64 && (kGuardSlots & (kGuardSlots - 1)) == 0, "one wave reads every slot");
 constexpr int kConvLanes = 16, kConvBlock = 128, kConvNodes = kConvBlock / kConvLanes;
 
 // device buffer with size bookkeeping.  Every allocation carries at least 64
@@ -201,7 +206,8 @@ struct FracState {
     BcNodeProgram bc;
     // blow-up guard state
     // separate-preset path (k_bc_preset): [0] CD part, [1] preset part,
-    // [2] block counter; fused path: max|u^{s}| of step s in [4 + s % 3]
+    // [2] block counter; fused path: max|u^{s}| of step s spread over the
+    // kGuardSlots slots of set s % 3 (kGuardWords words in all)
     dbuf<unsigned long long> gmax;
     dbuf<double> ghist;                // 50-entry ring of max|u| (separate-preset path)
     dbuf<double> ghist64;              // max|u^{s}| at [s % 64] (fused path)

@@ -7,18 +7,17 @@
 //
 // Work layout.  A 256-thread block owns 4*64/S nodes; each wavefront owns
 // 64/S of them and gives each node S adjacent lanes.
-//  * 3D: neighbour rows are staged in LDS first.  Blocks own compact tiles
-//    of nodes (2 x 4 x 4 cells on a structured grid, via the tile order
-//    perm), whose horizon union is about 14 rows per node; the block copies
-//    those rows {x_j, u_j} into LDS with coalesced loads, and a bond reads its
-//    neighbour from the LDS slot recorded for it at setup.  Gathering straight
-//    from global memory costs one cache-line lookup per lane and load, which
-//    saturates the texture units long before HBM.  2D (latency-bound, in the
-//    Infinity Cache) gathers straight from global memory over consecutive
-//    node blocks.
+//  * Neighbour rows are staged in LDS first.  Blocks own compact tiles of
+//    nodes (2 x 4 x 4 cells on a structured grid, 2 x 4 in 2D, via the tile
+//    order perm), whose horizon union is about 14 rows per node in 3D and 4
+//    in 2D; the block copies those rows {x_j, u_j} into LDS with coalesced
+//    loads, and a bond reads its neighbour from the LDS slot recorded for it
+//    at setup.  Gathering straight from global memory costs one cache-line
+//    lookup per lane and load, which saturates the texture units long
+//    before HBM (the kernel keeps that path for blocks whose rows exceed
+//    the LDS budget).
 //  * The per-bond streams use the kernel's own bond layout (bsh: line-aligned
-//    node runs; tile order in 3D, node order for 2D FP32C, CSR order for 2D
-//    FP64), where a node's bonds are contiguous; the lanes walk them in
+//    node runs in tile order), where a node's bonds are contiguous; the lanes walk them in
 //    groups of G consecutive bonds aligned to G, so one G-wide vector load
 //    fetches a group's LDS slots (or ids), its health bytes and each operator
 //    component (all bond arrays are allocated with zeroed slack and the
@@ -102,10 +101,30 @@ struct StepArgs {
 
 // blow-up guard of TimeStepping.cpp:550-573 for step s (fused path): trips
 // on a non-finite max|u^s| or on growth beyond 1e3 * max(1, max|u| of step
-// s-50); max|u^s| was accumulated in gmax[4 + s % 3] by the launch of step s.
-__device__ __forceinline__ unsigned long long guard_max(int s, unsigned long long *gmax)
+// s-50).  The launch of step s folds max|u^s| into kGuardSlots slots (a
+// wave of logical block b into slot b % kGuardSlots, 256 bytes apart), so
+// that no single address takes an atomic from every wave; a later launch
+// reads the slots with one load per lane and reduces them.  Three slot sets
+// rotate with s % 3.
+__device__ __forceinline__ int guard_slot(int s, int q) { return kGuardBase + ((s % 3) * kGuardSlots + q) * kGuardStride; }
+
+// max|u^s| in two parts: each lane loads one slot (written by an earlier
+// launch, so a plain load sees the final value), and the wave reduces the
+// loaded values where the decision is taken, so that the load's latency
+// overlaps the work issued between the two (all 64 lanes call the reduction)
+__device__ __forceinline__ unsigned long long guard_slot_load(int s, const unsigned long long *gmax)
 {
-    return __hip_atomic_load(&gmax[4 + s % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int l = threadIdx.x & 63;
+    return l < kGuardSlots ? gmax[guard_slot(s, l)] : 0ULL;
+}
+__device__ __forceinline__ unsigned long long guard_reduce(unsigned long long m)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) {
+        const unsigned long long o = __shfl_xor(m, k);
+        m = o > m ? o : m;
+    }
+    return m;
 }
 __device__ __forceinline__ double guard_past(int s, const double *h64)
 {
@@ -129,14 +148,16 @@ __device__ __forceinline__ void guard_record(int s, bool trip, double umax, doub
 
 template <int DIM, bool F32> struct StepCfg;
 // S lanes per node (S divides 64), G bonds per lane group, STAGE: neighbour
-// rows staged in LDS.  The 2D plate is latency-bound (its working set lives
-// in the Infinity Cache) and runs faster gathering straight from global
-// memory; the 3D block saturates the texture units without staging.  2D
-// FP32C gives a node 4 lanes (16 nodes per wave: the 81,200-node plate fits
-// the GPU in one round of resident waves); 2D FP64 keeps 8 (its LDS family
-// sums and registers favour fewer nodes per wave).
-template <> struct StepCfg<2, true> { static constexpr int S = 4, G = 4; static constexpr bool STAGE = false, PF2 = true; };
-template <> struct StepCfg<2, false> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = false, PF2 = false; };
+// rows staged in LDS (both precisions of a dimension share the node tiling,
+// so S is common to them).  2D: 4 lanes per node, and with G = 8 one round
+// of the FP32C loop covers the 28-bond family of the 400x200 plate (81,200
+// nodes, 16 per wave: one round of resident waves on the GPU); 15.2 us per
+// step against 16.4 us at G = 4, 21.7 us at S = 2 and 20.9 us gathering
+// from global memory at S = 4.  2D FP64 keeps G = 4 (its LDS force rows
+// grow with G: 18.4 us, 26.2 us at G = 8).  3D: 8 lanes per node; with 4 the
+// 64-node tiles overflow the LDS row budget.
+template <> struct StepCfg<2, true> { static constexpr int S = 4, G = 8; static constexpr bool STAGE = true, PF2 = false; };
+template <> struct StepCfg<2, false> { static constexpr int S = 4, G = 4; static constexpr bool STAGE = true, PF2 = false; };
 template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = true, PF2 = false; };
 template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; static constexpr bool STAGE = true, PF2 = false; };
 
@@ -255,6 +276,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     using GD = GroupData<DIM, F32, G, STAGE>;
     // dynamic LDS: [FP64 force rows][gcap neighbour records {x, u}]
     extern __shared__ double s_dyn[];
+    __shared__ unsigned long long s_gmax;     // guard maximum of the block (staged)
     double *s_f = s_dyn;
     double *s_rec = s_dyn + FROWS;
 
@@ -264,15 +286,16 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     const int blk = xcd_block(blockIdx.x, a.nblk);
     if (blk < 0) return;
     // Everything that does not read LDS is issued before the staging
-    // barrier: the fused guard's inputs, the node's own data and the first
-    // bond group's stream loads; the guard is decided before the first write.
+    // barrier: the fused guard's inputs (read by wave 0 alone in a staged
+    // block), the node's own data and, unstaged, the first bond group's
+    // stream loads; the guard is decided before the first write.
     int g_tripped = 0;
     unsigned long long g_max = 0;
     double g_past = -1.0;
     if constexpr (FUSED) {
         if (a.guard_on) {
             g_tripped = a.gstate[0];
-            if (a.step >= 2) { g_max = guard_max(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
+            if (a.step >= 2) { if (!STAGE || (threadIdx.x >> 6) == 0) g_max = guard_slot_load(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -382,6 +405,14 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 for (int c = 0; c < 2 * DIM; c++) s_rec[qb * (2 * DIM) + c] = rb[c];
             }
         }
+        if constexpr (FUSED) {
+            // wave 0 reduces the guard slots for the block, overlapped with
+            // the staging loads (which the barrier waits for anyway)
+            if (a.guard_on && a.step >= 2 && wv == 0) {
+                const unsigned long long m = guard_reduce(g_max);
+                if (lane == 0) s_gmax = m;
+            }
+        }
         __syncthreads();
     }
     if constexpr (FUSED) {
@@ -391,11 +422,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             // the same data; block 0 records it and clears the maximum slot
             // of step+1 (last used by step-2, already decided)
             double umax;
-            const bool trip = guard_trips(g_max, g_past, umax);
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                guard_record(a.step - 1, trip, umax, a.ghist64, a.gstate);
-                a.gmax[4 + (a.step + 1) % 3] = 0;
-            }
+            const bool trip = guard_trips(STAGE ? s_gmax : guard_reduce(g_max), g_past, umax);
+            if (blockIdx.x == 0 && threadIdx.x == 0) guard_record(a.step - 1, trip, umax, a.ghist64, a.gstate);
+            if (blockIdx.x == 0 && threadIdx.x < kGuardSlots) a.gmax[guard_slot(a.step + 1, threadIdx.x)] = 0;
             if (trip) return;
         }
     }
@@ -631,9 +660,12 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             const unsigned long long o = __shfl_xor(mymax, m);
             mymax = o > mymax ? o : mymax;
         }
-        unsigned long long *slot = FUSED ? &a.gmax[4 + a.step % 3] : &a.gmax[0];
+        unsigned long long *slot = FUSED ? &a.gmax[guard_slot(a.step, blk & (kGuardSlots - 1))] : &a.gmax[0];
         if (lane == 0 && mymax && (!FUSED || a.guard_on)) {
-            const unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the workgroup-scope load is a cached hint that can only lag the
+            // slot's value (it only grows during the launch), so it skips
+            // atomics that cannot raise it and never skips one that can
+            const unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (mymax > cur) atomicMax(slot, mymax);
         }
     }
@@ -642,9 +674,11 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 // guard decision on the last step of a fused run
 __global__ void k_guard_final(int s, unsigned long long *gmax, double *h64, int *gstate)
 {
-    if (threadIdx.x != 0 || gstate[0] || s < 1) return;
+    if (gstate[0] || s < 1) return;            // uniform over the one wave
+    const unsigned long long m = guard_reduce(guard_slot_load(s, gmax));
+    if (threadIdx.x != 0) return;
     double umax;
-    const bool trip = guard_trips(guard_max(s, gmax), guard_past(s, h64), umax);
+    const bool trip = guard_trips(m, guard_past(s, h64), umax);
     guard_record(s, trip, umax, h64, gstate);
 }
 
