@@ -161,25 +161,35 @@ template <> struct StepCfg<2, false> { static constexpr int S = 4, G = 4; static
 template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = true, PF2 = false; };
 template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; static constexpr bool STAGE = true, PF2 = false; };
 
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <int B> struct VecT;
 template <> struct VecT<2> { using T = unsigned short; };
 template <> struct VecT<4> { using T = unsigned; };
-template <> struct VecT<8> { using T = uint2; };
-template <> struct VecT<16> { using T = uint4; };
+template <> struct VecT<8> { using T = u32x2; };
+template <> struct VecT<16> { using T = u32x4; };
+
+template <bool NT, typename V>
+__device__ __forceinline__ V load_hint(const V *q)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(q);
+    else return *q;
+}
 
 // G consecutive elements starting at an address aligned to their total size
-template <typename T, int G>
+// (NT: a non-temporal load, for a stream read once per step)
+template <bool NT = false, typename T, int G>
 __device__ __forceinline__ void vload(const T *p, T (&o)[G])
 {
     constexpr int B = (int)sizeof(T) * G;
     if constexpr (B <= 16) {
-        const auto v = *reinterpret_cast<const typename VecT<B>::T *>(p);
+        const auto v = load_hint<NT>(reinterpret_cast<const typename VecT<B>::T *>(p));
         __builtin_memcpy(o, &v, B);
     } else {
         static_assert(B % 16 == 0, "vector width");
 #pragma unroll
         for (int h = 0; h < B / 16; h++) {
-            const uint4 v = reinterpret_cast<const uint4 *>(p)[h];
+            const u32x4 v = load_hint<NT>(reinterpret_cast<const u32x4 *>(p) + h);
             __builtin_memcpy(reinterpret_cast<char *>(o) + 16 * h, &v, 16);
         }
     }
@@ -260,7 +270,10 @@ struct GroupData {
     double p64[F32 ? 1 : NC][G];
 };
 
-template <int DIM, bool F32, int MODE, bool FUSED>
+// NTO: the operator streams are read with non-temporal loads (each node
+// reads its own operators once per step, so keeping them in the caches only
+// evicts the staged rows and the other streams)
+template <int DIM, bool F32, int MODE, bool FUSED, bool NTO = false>
 __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 {
     constexpr int S = StepCfg<DIM, F32>::S;
@@ -365,8 +378,8 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            if constexpr (F32) vload(P32c[c] + (unsigned)((int)t0 + po), d.p32[c]);
-            else vload(P64c[c] + t0, d.p64[c]);
+            if constexpr (F32) vload<NTO>(P32c[c] + (unsigned)((int)t0 + po), d.p32[c]);
+            else vload<NTO>(P64c[c] + t0, d.p64[c]);
         }
     };
     GD d;
@@ -739,19 +752,32 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     hipStream_t st = ctx->stream;
     a.nblk = (int)gridn(ctx->n, launch_npb(ctx->dim, f32));
     const dim3 grid(8 * gridn(a.nblk, 8));
-#define PXG_LAUNCH(D, FF, M, FU)                                                                    \
-    hipLaunchKernelGGL((k_frac_step<D, FF, M, FU>), grid, dim3(256),                                  \
+    // Non-temporal operator loads for the stepping loop when every node reads
+    // its own operators (no shared FP32C runs, which are re-read from L2).
+    // FP64 operators that fit the 256 MB Infinity Cache stay cached from step
+    // to step instead (400x200 plate: 18.4 us cached against 19.0 us
+    // non-temporal; 160x80x40 block FP64 706 -> 656 us, FP32C with own
+    // operators 425 -> 393 us, 2D FP32C 15.2 -> 14.5 us).
+    const size_t opbytes = (size_t)F.pst * (ctx->dim == 3 ? 6 : 3) * (f32 ? sizeof(float) : sizeof(double));
+    const bool nto = fused && !(f32 && a.pofs) && (f32 || opbytes >= ((size_t)128 << 20));
+#define PXG_LAUNCH(D, FF, M, FU, NT)                                                                \
+    hipLaunchKernelGGL((k_frac_step<D, FF, M, FU, NT>), grid, dim3(256),                              \
                        sizeof(double) * (step_force_rows<D, FF>() + 2 * (size_t)F.gcap * D), st, a)
+#define PXG_FUSED(D, FF, M)                                                                         \
+    if (!fused) PXG_LAUNCH(D, FF, M, false, false);                                                 \
+    else if (nto) PXG_LAUNCH(D, FF, M, true, true);                                                 \
+    else PXG_LAUNCH(D, FF, M, true, false);
 #define PXG_MODES(D, FF)                                                                            \
     switch (mode) {                                                                                 \
-    case MODE_CD: if (fused) PXG_LAUNCH(D, FF, MODE_CD, true); else PXG_LAUNCH(D, FF, MODE_CD, false); break; \
-    case MODE_VAR: if (fused) PXG_LAUNCH(D, FF, MODE_VAR, true); else PXG_LAUNCH(D, FF, MODE_VAR, false); break; \
-    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL, false); break;                                   \
-    default: PXG_LAUNCH(D, FF, MODE_INIT, false); break;                                            \
+    case MODE_CD: PXG_FUSED(D, FF, MODE_CD) break;                                                  \
+    case MODE_VAR: PXG_FUSED(D, FF, MODE_VAR) break;                                                \
+    case MODE_ACCEL: PXG_LAUNCH(D, FF, MODE_ACCEL, false, false); break;                            \
+    default: PXG_LAUNCH(D, FF, MODE_INIT, false, false); break;                                     \
     }
     if (ctx->dim == 2) { if (f32) { PXG_MODES(2, true) } else { PXG_MODES(2, false) } }
     else { if (f32) { PXG_MODES(3, true) } else { PXG_MODES(3, false) } }
 #undef PXG_MODES
+#undef PXG_FUSED
 #undef PXG_LAUNCH
     PXG_HIP(hipGetLastError());
     return PXG_OK;
