@@ -10,7 +10,9 @@ stretch 0.01, CFL-limited dt (0.9 x dx/c), 2000 central-difference steps.
 A "step" is one explicit time step (damage, Navier force, central-difference
 update, BC presets, blow-up guard) over all nodes.  Inputs are resident in
 HBM; the one-off family / g-weight build is timed separately.  `value` is the
-FP32-compensated form's node-steps/s; the FP64 form, the FP32C error against
+FP32C form's node-steps/s (operators stored in FP32, bond products and family
+sums formed in FP64, which is at least as accurate as FP32-pair compensated
+sums of the same FP32 operators); the FP64 form, the FP32C error against
 FP64, the adjoint/forward cost (200 x 100 plate, 500 steps) and the 3D block
 (160 x 80 x 40, 556,800 nodes) are reported beside it.
 
@@ -176,7 +178,25 @@ def cpu_baselines():
             "omp": "OMP_NUM_THREADS=%d OMP_PLACES=cores OMP_PROC_BIND=close" % threads,
             "one_thread_node_steps_per_s": one["node_steps_per_s"], "one_thread_steps": one["steps"],
             "block3d_node_steps_per_s": block["node_steps_per_s"], "block3d_steps": block["steps"],
-            "block3d_setup_s": block["setup_s"], "block3d_kind": block["kind"]}
+            "block3d_setup_s": block["setup_s"], "block3d_kind": block["kind"],
+            "implicit_poisson_100x100": implicit_solve_split()}
+
+
+def implicit_solve_split(nx=100):
+    """Config 1 run whole by the reference's own driver (oracle/_ref, Newton
+    with its default LDU solver), once on its OpenMP assembler and once with
+    "assemble": "cuda" through the seam over libpxg: the reference's own
+    timers split each run into assembly and linear solve, the part that
+    stays on the CPU.  None when the seam build is absent."""
+    from oracle import refpy
+    if not refpy.pxg_available():
+        return None
+    out = {}
+    for assemble in ("openmp", "cuda"):
+        _, rec = refpy.run_job(refpy.poisson_deck(nx, assemble), timeout=300)
+        out[assemble] = {k: rec[k] for k in ("seconds", "assemble_s", "solve_s", "newton_iterations", "seam_ok")}
+    out["deck"] = "Poisson %dx%d, static Newton, default LDU (reference driver)" % (nx, nx)
+    return out
 
 
 def adjoint_case(problems, pxg, nsteps=500, nsens=64, ndirs=20, seed=2024):
@@ -977,7 +997,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 compensated family sums + f64 damage/update",
+        "dtype": "f32 operators; f64 bond products, family sums, damage and update",
         "data": "synthetic",
         "config": {"workload": "explicit_pddo_frac 2D plate 0.2x0.1 m, 400x200 cells (81,200 PD nodes, "
                                "%d directed bonds), two force-only edge notches, 16.5 m/s impactor; "

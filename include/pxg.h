@@ -35,7 +35,7 @@ extern "C" {
 #define PXG_ERR_BLOWUP (-6)   /* explicit blow-up guard tripped */
 
 #define PXG_FP64 0            /* plain FP64 (bit-exact with the reference) */
-#define PXG_FP32C 1           /* FP32 operators + compensated FP32 family sums */
+#define PXG_FP32C 1           /* FP32 operators; FP64 bond products and family sums */
 
 typedef struct pxg_ctx pxg_ctx;
 

@@ -157,6 +157,20 @@ def _ip(a):
     return a.ctypes.data_as(I) if a is not None else None
 
 
+def poisson_deck(nx, assemble):
+    """Config 1: Poisson on the nx x nx unit square, sigma = 1, scalar source
+    2 pi^2 (PoissonElement.h:74), Dirichlet u = 0 on the four ghost walls,
+    static Newton with the reference's default LDU solver; "assemble" is
+    "openmp" (the reference's CPU assembler) or "cuda" (the seam)."""
+    import math
+    walls = ["left", "right", "bottom", "top"]
+    return {"Mesh": mesh_block(2, nx, nx), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+            "DOFs": ["u"], "ElmtSystem": {"p": {"type": "poisson", "params": {"sigma": 1.0, "f": 2.0 * math.pi ** 2}}},
+            "BCSystem": {w: {"type": "dirichlet", "phygroup": w + "nodes_ghost", "dofs": ["u"], "value": 0.0}
+                         for w in walls},
+            "JobSystem": {"type": "static", "assemble": assemble}}
+
+
 def mesh_block(dim, nx, ny, nz=1, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
     m = {"type": "perix", "dim": dim, "nx": nx, "ny": ny, "xmin": lo[0], "xmax": hi[0],
          "ymin": lo[1], "ymax": hi[1], "meshtype": "quad4" if dim == 2 else "hex8",

@@ -40,14 +40,9 @@ def _walls(dim):
 
 
 def poisson_deck(refpy, nx, assemble):
-    """Config 1: Poisson on the 100 x 100 unit square, sigma = 1, scalar source
-    (PoissonElement.h:74), Dirichlet u = 0 on the four ghost walls, static
-    Newton with the reference's default LDU solver."""
-    return {"Mesh": refpy.mesh_block(2, nx, nx), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
-            "DOFs": ["u"], "ElmtSystem": {"p": {"type": "poisson", "params": {"sigma": 1.0, "f": 2.0 * math.pi ** 2}}},
-            "BCSystem": {w: {"type": "dirichlet", "phygroup": w + "nodes_ghost", "dofs": ["u"], "value": 0.0}
-                         for w in _walls(2)},
-            "JobSystem": {"type": "static", "assemble": assemble}}
+    """Config 1 (oracle/refpy.poisson_deck): Poisson on the 100 x 100 unit
+    square, Dirichlet ghost walls, static Newton with the default LDU."""
+    return refpy.poisson_deck(nx, assemble)
 
 
 def ch_deck(refpy, nx, steps, assemble):
