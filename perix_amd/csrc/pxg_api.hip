@@ -76,6 +76,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(F.preset_mask); rel(F.bc.node); rel(F.bc.src); rel(F.bc.mode); rel(F.bc.val0);
     rel(F.bc.vel); rel(F.gmax); rel(F.ghist); rel(F.gstate); rel(F.nodeE); rel(F.Phat);
     rel(F.rev); rel(F.accel); rel(F.anybroke); rel(F.bc.inv_off); rel(F.bc.inv_q);
+    if (F.gstate_host) (void)hipHostFree(F.gstate_host);
     for (hipEvent_t e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     if (c->stream) (void)hipStreamDestroy(c->stream);

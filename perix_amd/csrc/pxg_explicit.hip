@@ -562,6 +562,7 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_HIP(hipMemsetAsync(F.bc.inv_off.p, 0, sizeof(int) * ((size_t)ctx->n + 1), st));
     PXG_HIP(F.bc.inv_q.alloc(1));
     PXG_HIP(F.gstate.alloc(4));
+    if (!F.gstate_host) PXG_HIP(hipHostMalloc((void **)&F.gstate_host, 4 * sizeof(int), hipHostMallocDefault));
     PXG_HIP(F.anybroke.alloc(1));
     PXG_HIP(hipMemsetAsync(F.gmax.p, 0, sizeof(unsigned long long) * kGuardWords, st));
     PXG_HIP(hipMemsetAsync(F.ghist.p, 0, sizeof(double) * 50, st));
@@ -807,13 +808,17 @@ extern "C" int pxg_frac_run(pxg_ctx *ctx, double dt, double total, int nsteps_ma
         }
     }
     if (F.prof) PXG_HIP(close_step_events(ctx));
+    int gs[3];
     if (done > 0) {
+        // the last step's guard decision, delivered to the pinned mirror
         rc = launch_guard_final(ctx, F.step);
         if (rc) return rc;
+        PXG_HIP(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 3; c++) gs[c] = __atomic_load_n(&F.gstate_host[c], __ATOMIC_ACQUIRE);
+    } else {
+        PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
+        PXG_HIP(hipStreamSynchronize(ctx->stream));
     }
-    int gs[3];
-    PXG_HIP(hipMemcpyAsync(gs, F.gstate.p, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
-    PXG_HIP(hipStreamSynchronize(ctx->stream));
     if (gs[0]) {
         // state at the aborted step: the buffer written by that step
         const int abort_step = gs[1];

@@ -212,6 +212,7 @@ struct FracState {
     dbuf<double> ghist;                // 50-entry ring of max|u| (separate-preset path)
     dbuf<double> ghist64;              // max|u^{s}| at [s % 64] (fused path)
     dbuf<int> gstate;                  // [0] aborted flag, [1] abort step, [2] steps done
+    int *gstate_host = nullptr;        // pinned host mirror of gstate, written by k_guard_final
     // node-E (optional): per-node Young's modulus for the adjoint configuration
     dbuf<double> nodeE;
     double nu = 0.25;

@@ -685,14 +685,20 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 }
 
 // guard decision on the last step of a fused run
-__global__ void k_guard_final(int s, unsigned long long *gmax, double *h64, int *gstate)
+__global__ void k_guard_final(int s, unsigned long long *gmax, double *h64, int *gstate, int *mirror)
 {
-    if (gstate[0] || s < 1) return;            // uniform over the one wave
-    const unsigned long long m = guard_reduce(guard_slot_load(s, gmax));
-    if (threadIdx.x != 0) return;
-    double umax;
-    const bool trip = guard_trips(m, guard_past(s, h64), umax);
-    guard_record(s, trip, umax, h64, gstate);
+    if (!gstate[0] && s >= 1) {                // uniform over the one wave
+        const unsigned long long m = guard_reduce(guard_slot_load(s, gmax));
+        if (threadIdx.x == 0) {
+            double umax;
+            const bool trip = guard_trips(m, guard_past(s, h64), umax);
+            guard_record(s, trip, umax, h64, gstate);
+        }
+    }
+    // the run's outcome, straight into pinned host memory: the host reads it
+    // after the stream synchronisation without a copy
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; c++) __hip_atomic_store(&mirror[c], gstate[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 static unsigned gridn(long long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -799,7 +805,8 @@ int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uol
 int launch_guard_final(pxg_ctx *ctx, int step)
 {
     FracState &F = ctx->frac;
-    hipLaunchKernelGGL(k_guard_final, dim3(1), dim3(64), 0, ctx->stream, step, F.gmax.p, F.ghist64.p, F.gstate.p);
+    hipLaunchKernelGGL(k_guard_final, dim3(1), dim3(64), 0, ctx->stream, step, F.gmax.p, F.ghist64.p, F.gstate.p,
+                       F.gstate_host);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
