@@ -72,7 +72,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->fsch.health); rel(c->fsch.lapc); rel(c->fsch.vel); rel(c->fsch.uprev);
     FracState &F = c->frac;
     rel(F.P64); rel(F.P32); rel(F.health); rel(F.break_step); rel(F.u[0]); rel(F.u[1]);
-    rel(F.perm); rel(F.roff); rel(F.ulist); rel(F.gslot); rel(F.bovf); rel(F.ghist64); rel(F.bc.prog_of); rel(F.bsh); rel(F.pofs); rel(F.nbr_t); rel(F.health_csr); rel(F.bs_csr);
+    rel(F.perm); rel(F.rcnt); rel(F.ulist); rel(F.nrec); rel(F.brec); rel(F.gslot); rel(F.bovf); rel(F.ghist64); rel(F.bc.prog_of); rel(F.bsh); rel(F.pofs); rel(F.health_csr); rel(F.bs_csr);
     rel(F.preset_mask); rel(F.bc.node); rel(F.bc.src); rel(F.bc.mode); rel(F.bc.val0);
     rel(F.bc.vel); rel(F.gmax); rel(F.ghist); rel(F.gstate); rel(F.nodeE); rel(F.Phat);
     rel(F.rev); rel(F.accel); rel(F.anybroke); rel(F.bc.inv_off); rel(F.bc.inv_q);

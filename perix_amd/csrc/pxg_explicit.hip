@@ -254,6 +254,48 @@ __global__ void k_class_check(int n, int nc, const int *off, const int *bsh, con
     if (i < n && l == 0) pofs[i] = ok ? (off[r] + bsh[r]) - (off[i] + bsh[i]) : 0;
 }
 
+// FNV-1a hash of node i's FP32 operator run in the step layout: its length
+// and the bit patterns of every component, bond by bond (one thread per
+// node; a set-up pass)
+__global__ void k_run_hash32(int n, int nc, const int *off, const int *bsh, long long pst, const float *P32,
+                             unsigned long long *hash)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int len = off[i + 1] - off[i];
+    const long long p0 = off[i] + bsh[i];
+    unsigned long long h = 1469598103934665603ull ^ (unsigned long long)len;
+    for (int c = 0; c < nc; c++)
+        for (int k = 0; k < len; k++)
+            h = (h ^ (unsigned long long)__float_as_uint(P32[(size_t)c * pst + p0 + k])) * 1099511628211ull;
+    hash[i] = h;
+}
+
+// Exact form of k_class_check on the FP32 operators themselves: node i
+// shares candidate r's run when the runs have the same length and every
+// FP32 operator is the same bit pattern, bond by bond.  The step then reads
+// the very values of i's own run, so sharing cannot change a result.
+__global__ void k_class_check32(int n, int nc, const int *off, const int *bsh, const int *rep, long long pst,
+                                const float *P32, int *pofs)
+{
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
+    const int r = (i < n) ? rep[i] : -1;
+    const int len = (i < n) ? off[i + 1] - off[i] : 0;
+    const bool cand = i < n && r >= 0 && r != i && off[r + 1] - off[r] == len;
+    int ok = cand ? 1 : 0;
+    if (cand) {
+        const long long pi = off[i] + bsh[i], pr = off[r] + bsh[r];
+        for (int c = 0; c < nc && ok; c++)
+            for (int k = l; k < len; k += kConvLanes)
+                if (__float_as_uint(P32[(size_t)c * pst + pi + k]) != __float_as_uint(P32[(size_t)c * pst + pr + k])) {
+                    ok = 0;
+                    break;
+                }
+    }
+    for (int m = kConvLanes / 2; m >= 1; m >>= 1) ok &= __shfl_xor(ok, m);
+    if (i < n && l == 0) pofs[i] = ok ? (off[r] + bsh[r]) - (off[i] + bsh[i]) : 0;
+}
+
 // Nodal output projections of the explicit element (ExplicitPDDOFracElement::
 // computeNodalProjection, ExplicitPDDOFracElement.cpp:324-405): which = 0
 // damage = 1 - (intact bonds)/|H_i|; otherwise the displacement gradient
@@ -408,12 +450,22 @@ int bond_state_csr(pxg_ctx *ctx, const unsigned char **health, const int **break
     return PXG_OK;
 }
 
-// FP32C with the tiled 3D stream layout: nodes of one family order pattern
-// whose operators agree to kShareTol read one representative's FP32
-// operator run (F.pofs).  On a structured lattice most interior nodes fall
-// into a few hundred patterns, and their operators differ from the
-// representative's only through the rounding of the coordinates, far below
-// the FP32 rounding of the operators themselves.
+// FP32C shared operator runs (F.pofs): a node reads a representative's FP32
+// operator run instead of its own.
+//  * 2D: only runs that are bit for bit the node's own FP32 operators, so
+//    the step's results are exactly those of every node on its own run.  On
+//    the 400x200 plate the interior nodes' FP64 operators differ by about
+//    1e-14 relative, far inside one FP32 rounding, and the 81,200 runs
+//    collapse to about 3,900 distinct ones, which stay in L2.
+//  * 3D: nodes of one family order pattern whose FP64 operators agree to
+//    kShareTol.  The 122-bond 3D runs almost never round to identical FP32
+//    values (about 41,000 distinct runs for 75,200 nodes of an 80x40x20
+//    block), so exact sharing would find little; a tolerance class applies
+//    one FP32 rounding of the representative to every member.  On the
+//    160x80x40 block its trajectory is the own-operator trajectory to 1e-12.
+//    On the 2D plate such classes turned the members' scattered roundings
+//    into a coherent drift that moved the crack's branching, hence the exact
+//    rule there.
 constexpr double kShareTol = 1e-12;
 
 static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
@@ -422,27 +474,27 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     F.pofs.release();
     F.nshared = 0;
     F.nclasses = 0;
-    // 3D only.  A shared run applies one FP32 rounding of the class's
-    // operators to every member, where the members' own roundings scatter
-    // (their FP64 operators differ by about 1e-9 relative), so sharing turns
-    // incoherent rounding errors into a coherent drift.  On the 2D plate that
-    // drift reached 5e-6 by step 1500 and moved the crack's branching; the
-    // latency-bound 2D step gains little from it.
-    if (!F.bsh.p || F.precision != PXG_FP32C || ctx->dim != 3 || (F.opts & PXG_FRAC_NO_SHARED_RUNS)) return PXG_OK;
+    if (!F.bsh.p || F.precision != PXG_FP32C || (F.opts & PXG_FRAC_NO_SHARED_RUNS)) return PXG_OK;
     const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
+    const bool exact = ctx->dim != 3;
     hipStream_t st = ctx->stream;
-    dbuf<double> P;                     // FP64 operators, CSR order
-    PXG_HIP(P.alloc((size_t)F.ps * nc));
-    hipLaunchKernelGGL(k_frac_ops, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, ctx->dim, ctx->nop, ctx->off.p,
-                       ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress, F.ps, (const int *)nullptr,
-                       F.ps, P.p, (float *)nullptr, (double *)nullptr);
-    PXG_HIP(hipGetLastError());
+    dbuf<double> P;                     // FP64 operators, CSR order (tolerance classes)
     dbuf<unsigned long long> hash;
     PXG_HIP(hash.alloc(n));
-    const double h = std::max(ctx->dxyz[0], std::max(ctx->dxyz[1], ctx->dxyz[2]));
-    PXG_CHECK(h > 0.0, PXG_ERR_STATE, "build_op_classes: zero grid spacing");
-    hipLaunchKernelGGL(k_pattern_hash, dim3(gridn(n, 128)), dim3(128), 0, st, n, ctx->dim, ctx->off.p,
-                       ctx->nbr.p, ctx->xp.p, 1.0 / h, hash.p);
+    if (exact) {
+        hipLaunchKernelGGL(k_run_hash32, dim3(gridn(n, 128)), dim3(128), 0, st, n, nc, ctx->off.p, F.bsh.p, F.pst,
+                           F.P32.p, hash.p);
+    } else {
+        PXG_HIP(P.alloc((size_t)F.ps * nc));
+        hipLaunchKernelGGL(k_frac_ops, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, ctx->dim, ctx->nop,
+                           ctx->off.p, ctx->ops.p, nodeE_dev, F.C11, F.C12, F.C66, F.nu, F.plane_stress, F.ps,
+                           (const int *)nullptr, F.ps, P.p, (float *)nullptr, (double *)nullptr);
+        PXG_HIP(hipGetLastError());
+        const double h = std::max(ctx->dxyz[0], std::max(ctx->dxyz[1], ctx->dxyz[2]));
+        PXG_CHECK(h > 0.0, PXG_ERR_STATE, "build_op_classes: zero grid spacing");
+        hipLaunchKernelGGL(k_pattern_hash, dim3(gridn(n, 128)), dim3(128), 0, st, n, ctx->dim, ctx->off.p,
+                           ctx->nbr.p, ctx->xp.p, 1.0 / h, hash.p);
+    }
     PXG_HIP(hipGetLastError());
     std::vector<unsigned long long> hh((size_t)n);
     PXG_HIP(hipMemcpyAsync(hh.data(), hash.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, st));
@@ -459,8 +511,12 @@ static int build_op_classes(pxg_ctx *ctx, const double *nodeE_dev)
     PXG_HIP(drep.alloc(n));
     PXG_HIP(F.pofs.alloc(n));
     PXG_HIP(hipMemcpyAsync(drep.p, rep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_class_check, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p, F.bsh.p, drep.p,
-                       F.ps, P.p, kShareTol, F.pofs.p);
+    if (exact)
+        hipLaunchKernelGGL(k_class_check32, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p,
+                           F.bsh.p, drep.p, F.pst, F.P32.p, F.pofs.p);
+    else
+        hipLaunchKernelGGL(k_class_check, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p,
+                           F.bsh.p, drep.p, F.ps, P.p, kShareTol, F.pofs.p);
     PXG_HIP(hipGetLastError());
     std::vector<int> pofs((size_t)n);
     PXG_HIP(hipMemcpyAsync(pofs.data(), F.pofs.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
@@ -572,6 +628,10 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     F.bc.count = 0;
     F.health_uploaded = 0;
     F.cur = 0; F.t = 0.0; F.step = 0; F.dt_prev = 0.0;
+    {
+        const int rc = build_step_records(ctx);
+        if (rc) return rc;
+    }
     PXG_HIP(hipStreamSynchronize(st));
     F.ready = 1;
     return PXG_OK;
@@ -667,7 +727,10 @@ extern "C" int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
     // inverse of the source map (adjoint of the preset), entries in program order
     std::vector<int> ioff(n + 1, 0), iq;
     for (int s2 = 0; s2 < cnt; s2++) if (tsrc[s2] >= 0) ioff[tsrc[s2] + 1]++;
-    for (int m = 0; m < n; m++) ioff[m + 1] += ioff[m];
+    for (int m = 0; m < n; m++) {
+        PXG_CHECK(ioff[m + 1] <= 0xffff, PXG_ERR_ARG, "pxg_frac_set_bcs: a node mirrors more than 65535 targets");
+        ioff[m + 1] += ioff[m];
+    }
     iq.assign(ioff[n] > 0 ? ioff[n] : 1, 0);
     {
         std::vector<int> cur(ioff.begin(), ioff.end() - 1);
@@ -678,7 +741,7 @@ extern "C" int pxg_frac_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
     PXG_HIP(hipMemcpyAsync(F.bc.inv_off.p, ioff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.bc.inv_q.p, iq.data(), sizeof(int) * iq.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipStreamSynchronize(st));
-    return PXG_OK;
+    return build_step_records(ctx);
 }
 
 extern "C" int pxg_frac_accel(pxg_ctx *ctx, const double *u, double *health_inout, double *accel_out,

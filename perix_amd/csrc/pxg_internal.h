@@ -171,29 +171,36 @@ struct FracState {
     // per-bond force operators P (CSR bond order): 3 (2D) or 6 (3D) components
     // component-major: component c of bond t at [c*ps + t]
     long long ps = 0;        // CSR-order component stride: nb rounded up to a multiple of 64
-    // P64 / P32 (and gslot) use the tiled bond layout of the step blocks when
-    // the step kernel stages rows: CSR bond t of node i at t + bsh[i], with
-    // component stride pst; without staging bsh is empty and pst = ps
+    // P64 / P32 (and gslot, health, break_step) use the tiled bond layout of
+    // the step blocks: CSR bond t of node i at t + bsh[i], with component
+    // stride pst
     long long pst = 0;
     dbuf<int> bsh;
     // FP32C shared operator runs: node i reads its FP32 operators at bond
-    // position + pofs[i] (the run of a representative node whose FP64
-    // operators agree to 1e-12 relative); empty when nothing is shared
+    // position + pofs[i], the run of a representative node (2D: the same
+    // FP32 bits; 3D: FP64 operators that agree to 1e-12 relative); empty
+    // when nothing is shared
     dbuf<int> pofs;
-    dbuf<int> nbr_t;              // unstaged (2D): neighbour ids in the bond layout
     long long nshared = 0, nclasses = 0;
     dbuf<double> P64;
     dbuf<float> P32;
     // LDS neighbour staging of the step kernel: step block b owns the npb
-    // nodes perm[b*npb ..] (a compact tile) and copies the neighbour rows
-    // ulist[roff[b] .. roff[b+1]) (node ids, slot order) into LDS; bond t
-    // reads its neighbour from LDS slot gslot[t].  A block whose
-    // neighbourhood exceeds the slot budget has bovf set and gathers its
-    // neighbours from global memory via nbr.
+    // nodes perm[b*npb ..] (a compact tile) and copies its rcnt[b] neighbour
+    // rows ulist[b*gcap ..] (node ids in slot order, padded with -1 to gcap)
+    // into LDS; bond t reads its neighbour from LDS slot gslot[t].  A block
+    // whose neighbourhood exceeds the slot budget has bovf set and gathers
+    // its neighbours from global memory via nbr.
     int npb = 0, gcap = 0;        // nodes per step block, LDS slots per block
     dbuf<int> perm;
-    dbuf<int> roff;
+    dbuf<int> rcnt;
     dbuf<int> ulist;
+    // per step-block node slot q (perm order), so that one load gives what a
+    // lane needs to start: nrec[q] = {i, first and end bond position of i in
+    // the stream layout, FP32C shared-run offset}; brec[q] = the node's BC
+    // program (fused step) {own entry or -1, first inverse entry, count |
+    // own modes << 16 | first inverse entry's modes << 24 (2 bits per
+    // component), program entry of the first inverse entry or -1}
+    dbuf<int4> nrec, brec;
     dbuf<unsigned short> gslot;
     dbuf<unsigned char> bovf;
     dbuf<unsigned char> health;   // pst, stream layout (see bsh)
@@ -252,6 +259,9 @@ int launch_step_fused(pxg_ctx *ctx, int mode, const double *u, const double *uol
                       int frozen_k = INT_MIN);
 int launch_guard_final(pxg_ctx *ctx, int step);
 int build_step_gather(pxg_ctx *ctx);
+// node and BC records of the step kernel, after any change of the tiling,
+// the shared runs or the BC program
+int build_step_records(pxg_ctx *ctx);
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 // device pointers to the bond health / break steps in CSR order (a gathered
 // copy when the stream layout is tiled); either output may be NULL

@@ -16,6 +16,14 @@
 //    lookup per lane and load, which saturates the texture units long
 //    before HBM (the kernel keeps that path for blocks whose rows exceed
 //    the LDS budget).
+//  * Dependent loads come in two rounds before the family loop.  Round one
+//    reads what needs no other load: the node record of the lane's slot
+//    (node id, bond range, shared-run offset), its BC record, the block's
+//    row ids and the guard slots.  Round two reads the node's own u, x,
+//    u^{n-1} and BC values and the staged rows.  When every load a step
+//    needs hung off index loads (perm, then off/bsh, then the bond groups;
+//    the BC program after the family sum), the latency of that chain set
+//    the time of the latency-bound 2D step.
 //  * The per-bond streams use the kernel's own bond layout (bsh: line-aligned
 //    node runs in tile order), where a node's bonds are contiguous; the lanes walk them in
 //    groups of G consecutive bonds aligned to G, so one G-wide vector load
@@ -58,16 +66,14 @@ struct StepArgs {
     int n;
     int nblk;                  // logical blocks (grid is rounded up to 8*)
     long long pst;             // component stride of the operator arrays
-    const int *off;
     const int *bsh;            // tiled bond layout: the per-bond streams (operators,
                                // slots, health, break steps) of CSR bond t of node i
-                               // sit at t + bsh[i] (null: CSR order)
-    const int *nbr_t;          // unstaged path: neighbour ids (1-based) in the bond layout
-    const int *pofs;           // FP32C: node i's FP32 operators at stream position
-                               // + pofs[i] (a shared run; null: none shared)
+                               // sit at t + bsh[i]
     const int *nbr;
-    const int *perm;           // LDS staging: nodes of step block b at perm[b*npb ..]
-    const int *roff;           // rows of block b: ulist[roff[b] .. roff[b+1])
+    const int4 *nrec;          // node slot q of the tiling: {node, first and end bond
+                               // position, FP32C shared-run offset}
+    const int4 *brec;          // its BC program record (fused path)
+    const int *rcnt;           // rows of block b: ulist[b*gcap .. + rcnt[b]), -1 beyond
     const int *ulist;
     const unsigned char *bovf; // block gathers from global memory (no LDS rows)
     const unsigned short *gslot;
@@ -92,7 +98,7 @@ struct StepArgs {
     int frozen;                // >0: health frozen from break_step (no damage)
     int frozen_k;              // bond t is intact iff break_step[t] > frozen_k
     // fused BC presets (merged program, see BcNodeProgram) and guard history
-    const int *prog_of, *bnode, *bmode, *inv_off, *inv_q;
+    const int *bnode, *bmode, *inv_q;
     const double *bval0, *bvel;
     double tpre;               // time of the presets (t^{n+1})
     double *ghist64;
@@ -147,19 +153,19 @@ __device__ __forceinline__ void guard_record(int s, bool trip, double umax, doub
 }
 
 template <int DIM, bool F32> struct StepCfg;
-// S lanes per node (S divides 64), G bonds per lane group, STAGE: neighbour
-// rows staged in LDS (both precisions of a dimension share the node tiling,
-// so S is common to them).  2D: 4 lanes per node, and with G = 8 one round
-// of the FP32C loop covers the 28-bond family of the 400x200 plate (81,200
-// nodes, 16 per wave: one round of resident waves on the GPU); 15.2 us per
-// step against 16.4 us at G = 4, 21.7 us at S = 2 and 20.9 us gathering
-// from global memory at S = 4.  2D FP64 keeps G = 4 (its LDS force rows
-// grow with G: 18.4 us, 26.2 us at G = 8).  3D: 8 lanes per node; with 4 the
-// 64-node tiles overflow the LDS row budget.
-template <> struct StepCfg<2, true> { static constexpr int S = 4, G = 8; static constexpr bool STAGE = true, PF2 = false; };
-template <> struct StepCfg<2, false> { static constexpr int S = 4, G = 4; static constexpr bool STAGE = true, PF2 = false; };
-template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; static constexpr bool STAGE = true, PF2 = false; };
-template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; static constexpr bool STAGE = true, PF2 = false; };
+// S lanes per node (S divides 64) and G bonds per lane group; both
+// precisions of a dimension share the node tiling, so S is common to them.
+// 2D: 4 lanes per node, and with G = 8 one round of the FP32C loop covers
+// the 28-bond family of the 400x200 plate (81,200 nodes, 16 per wave: one
+// round of resident waves on the GPU); 15.2 us per step against 16.4 us at
+// G = 4, 21.7 us at S = 2 and 20.9 us gathering from global memory at S = 4.
+// 2D FP64 keeps G = 4 (its LDS force rows grow with G: 18.4 us, 26.2 us at
+// G = 8).  3D: 8 lanes per node; with 4 the 64-node tiles overflow the LDS
+// row budget.
+template <> struct StepCfg<2, true> { static constexpr int S = 4, G = 8; };
+template <> struct StepCfg<2, false> { static constexpr int S = 4, G = 4; };
+template <> struct StepCfg<3, true> { static constexpr int S = 8, G = 4; };
+template <> struct StepCfg<3, false> { static constexpr int S = 8, G = 2; };
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -260,11 +266,10 @@ __device__ __forceinline__ double pick(const double (&v)[DIM], int c)
 }
 
 // stream data of one group of G consecutive bonds (one vector load per array)
-template <int DIM, bool F32, int G, bool STAGE>
+template <int DIM, bool F32, int G>
 struct GroupData {
     static constexpr int NC = (DIM == 3) ? 6 : 3;
-    unsigned short sv[STAGE ? G : 1];      // LDS slots of the neighbours
-    int jv[STAGE ? 1 : G];                 // neighbour ids (1-based)
+    unsigned short sv[G];                  // LDS slots of the neighbours
     bool ok[G];                            // bond intact
     float p32[F32 ? NC : 1][G];
     double p64[F32 ? 1 : NC][G];
@@ -278,18 +283,15 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 {
     constexpr int S = StepCfg<DIM, F32>::S;
     constexpr int G = StepCfg<DIM, F32>::G;
-    constexpr bool STAGE = StepCfg<DIM, F32>::STAGE;
-    constexpr bool PF2 = StepCfg<DIM, F32>::PF2;
-    static_assert(!(STAGE && PF2), "double-buffered groups rely on the first group loaded up front");
     constexpr int NPW = 64 / S;               // nodes per wavefront
     constexpr int W = G * S;                  // bonds per node and round
     constexpr int NC = (DIM == 3) ? 6 : 3;
     constexpr int ROW = W * DIM + 1;          // LDS row per node (odd: no bank aliasing)
     constexpr int FROWS = F32 ? 0 : 4 * NPW * ROW;
-    using GD = GroupData<DIM, F32, G, STAGE>;
+    using GD = GroupData<DIM, F32, G>;
     // dynamic LDS: [FP64 force rows][gcap neighbour records {x, u}]
     extern __shared__ double s_dyn[];
-    __shared__ unsigned long long s_gmax;     // guard maximum of the block (staged)
+    __shared__ unsigned long long s_gmax;     // guard maximum of the block
     double *s_f = s_dyn;
     double *s_rec = s_dyn + FROWS;
 
@@ -298,59 +300,78 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     }
     const int blk = xcd_block(blockIdx.x, a.nblk);
     if (blk < 0) return;
-    // Everything that does not read LDS is issued before the staging
-    // barrier: the fused guard's inputs (read by wave 0 alone in a staged
-    // block), the node's own data and, unstaged, the first bond group's
-    // stream loads; the guard is decided before the first write.
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const bool live_dmg = a.damage && !a.frozen;
+    const int ln = lane / S, sl = lane % S;
+    const int gi = (blk * 4 + wv) * NPW + ln;
+    const bool nvalid = gi < a.n;
+    const bool upd = nvalid && sl < DIM;      // lane sl < DIM updates component sl
+
+    // round one: loads that depend on no other load -- the fused guard's
+    // inputs (wave 0 reduces them for the block), the node and BC records of
+    // the lane's slot, and the block's first 512 row ids
     int g_tripped = 0;
     unsigned long long g_max = 0;
     double g_past = -1.0;
     if constexpr (FUSED) {
         if (a.guard_on) {
             g_tripped = a.gstate[0];
-            if (a.step >= 2) { if (!STAGE || (threadIdx.x >> 6) == 0) g_max = guard_slot_load(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
+            if (a.step >= 2) { if (wv == 0) g_max = guard_slot_load(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
         }
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const bool live_dmg = a.damage && !a.frozen;
-    const int ln = lane / S, sl = lane % S;
-    const int gi = (blk * 4 + wv) * NPW + ln;
-    const bool nvalid = gi < a.n;
-    const int i = (STAGE && nvalid) ? a.perm[gi] : gi;
-    const bool govf = STAGE && a.bovf[blk];
-    // [b0, b1): the node's bonds in the stream layout (tiled when staged;
-    // nbr stays in CSR order, at t - sh)
-    int b0 = 0, b1 = 0, sh = 0;
-    if (nvalid) { b0 = a.off[i]; b1 = a.off[i + 1]; }
-    if (nvalid && (STAGE || a.bsh)) { sh = a.bsh[i]; b0 += sh; b1 += sh; }
-    int po = 0;                // operator run offset (shared runs, FP32C)
-    if constexpr (F32) {
-        if (nvalid && a.pofs) po = a.pofs[i];
+    int4 nr = make_int4(0, 0, 0, 0);
+    int4 br = make_int4(-1, 0, 0, -1);
+    if (nvalid) {
+        nr = a.nrec[gi];
+        if constexpr (FUSED) br = a.brec[gi];
     }
-    // BC program of this node (fused path): own entry, entries it mirrors
-    int bq = -1, bp0 = 0, bp1 = 0;
-    if constexpr (FUSED) {
-        if (nvalid && sl < DIM) { bq = a.prog_of[i]; bp0 = a.inv_off[i]; bp1 = a.inv_off[i + 1]; }
-    }
+    const int *rows = a.ulist + (size_t)blk * a.gcap;
+    const int ida0 = tid < a.gcap ? rows[tid] : -1, idb0 = tid + 256 < a.gcap ? rows[tid + 256] : -1;
+    const int nrows = a.rcnt[blk];
+    const bool govf = a.bovf[blk];
+
+    // [b0, b1): the node's bonds in the stream layout; po: FP32C shared run
+    const int i = nr.x, b0 = nr.y, b1 = nr.z;
+    const int po = F32 ? nr.w : 0;
+    // BC program of the node (fused path): own entry bq and its mode m_own;
+    // the bcnt entries bp0.. whose mirror source it is, the first of them
+    // (entry qq0, mode m_inv0, target node tn0) resolved here
+    const int bq = br.x, bp0 = br.y, bcnt = br.z & 0xffff, qq0 = br.w;
+    const int bmodes = (int)((unsigned)br.z >> 16);
+    const int m_own = (FUSED && upd && bq >= 0) ? (bmodes >> (2 * sl)) & 3 : 0;
+    const int m_inv0 = (FUSED && upd && bcnt > 0) ? (bmodes >> (8 + 2 * sl)) & 3 : 0;
+
+    // round two: the node's own state and BC values, then the staged rows
     double ui[DIM], xi[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
         ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
         xi[c] = (nvalid && live_dmg) ? a.xp[(size_t)i * DIM + c] : 0.0;
     }
-    // unstaged (latency-bound) path: the update's own inputs are loaded up
-    // front so that their latency hides behind the family loop (when out
-    // aliases uold, the lane reads its own entry before writing it; preset
-    // targets written by other lanes do not use theirs).  The staged path
-    // loads them at the update: holding them costs it occupancy.
-    double uo_pre = 0.0;
-    int m_pre = 0;
-    if (!STAGE && nvalid && sl < DIM) {
-        if constexpr (MODE != MODE_ACCEL) uo_pre = a.uold[(size_t)i * DIM + sl];
-        if constexpr (FUSED) {
-            if (bq >= 0) m_pre = a.bmode[bq * 3 + sl];
+    // The update's own inputs: u^{n-1} of the lane's component (when out
+    // aliases uold the lane reads its own entry before writing it; preset
+    // targets written by other lanes do not use theirs), and g(0), dg/dt of
+    // the one program entry whose value the lane needs first (its own direct
+    // value, else the first reflection it writes).  2D loads them here, so
+    // that their latency hides behind the staging: its grids are one round
+    // of resident waves, bound by the latency of each wave.  3D loads them
+    // at the update: its grids run many rounds, and holding them would cost
+    // a resident wave per SIMD (100 VGPRs against 92).
+    constexpr bool EARLY = DIM == 2;
+    double uo_pre = 0.0, pv0 = 0.0, pvel = 0.0;
+    int tn0 = -1;
+    auto load_update = [&]() {
+        if constexpr (MODE != MODE_ACCEL) {
+            if (upd) uo_pre = a.uold[(size_t)i * DIM + sl];
         }
-    }
+        if constexpr (FUSED) {
+            if (m_inv0 == 1 || m_inv0 == 2) tn0 = a.bnode[qq0];
+            const int qv = (m_own == 3) ? bq : (m_inv0 == 2 ? qq0 : -1);
+            if (qv >= 0) { pv0 = a.bval0[qv * 3 + sl]; pvel = a.bvel[qv * 3 + sl]; }
+        }
+    };
+    if constexpr (EARLY) load_update();
     const int A = b0 & ~(G - 1);
     int broke = 0;
 
@@ -363,8 +384,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         else P64c[c] = a.P64 + c * a.pst;
     }
     auto load_group = [&](unsigned t0, GD &d) {
-        if constexpr (STAGE) vload(a.gslot + t0, d.sv);
-        else vload((a.bsh ? a.nbr_t : a.nbr) + t0, d.jv);
+        vload(a.gslot + t0, d.sv);
         if (a.frozen) {
             int bs[G];
             vload(a.break_step + t0, bs);
@@ -384,50 +404,45 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     };
     GD d;
     const int tfirst = A + G * sl;
-    // the first group's stream loads overlap the staging of the neighbour
-    // rows -- except where rows are staged: there the extra bytes in flight
-    // push the shared rows out of L2 before the next block reuses them
-    if (!STAGE && tfirst < b1) load_group((unsigned)tfirst, d);
+    // nbr keeps CSR order: a block without LDS rows reads bond t at t - sh
+    const int sh = (govf && nvalid) ? a.bsh[i] : 0;
 
-    if constexpr (STAGE) {
-        // stage the block's neighbour rows as LDS records {x_j, u_j}: a row
-        // per thread, two rows per thread and round, and every load of a
-        // round is issued before its LDS writes (the row ids are sorted node
-        // ids, so neighbouring lanes read neighbouring nodes)
-        const int r0 = a.roff[blk];
-        const int nrows = a.roff[blk + 1] - r0;
-        const int *rows = a.ulist + r0;
-        for (int q0 = 0; q0 < nrows; q0 += 512) {
-            const int qa = q0 + (int)threadIdx.x, qb = qa + 256;
-            const bool ha = qa < nrows, hb = qb < nrows;
-            const size_t ja = ha ? (size_t)rows[qa] * DIM : 0, jb = hb ? (size_t)rows[qb] * DIM : 0;
-            double ra[2 * DIM], rb[2 * DIM];
+    // stage the block's neighbour rows as LDS records {x_j, u_j}: two rows
+    // per thread and round, every load of a round issued before its LDS
+    // writes (the row ids are sorted node ids, so neighbouring lanes read
+    // neighbouring nodes); rounds beyond the first read their ids first
+    for (int q0 = 0; q0 == 0 || q0 < nrows; q0 += 512) {
+        const int qa = q0 + tid, qb = qa + 256;
+        const int ida = q0 == 0 ? ida0 : (qa < a.gcap ? rows[qa] : -1);
+        const int idb = q0 == 0 ? idb0 : (qb < a.gcap ? rows[qb] : -1);
+        const bool ha = ida >= 0, hb = idb >= 0;
+        const size_t ja = ha ? (size_t)ida * DIM : 0, jb = hb ? (size_t)idb * DIM : 0;
+        double ra[2 * DIM], rb[2 * DIM];
 #pragma unroll
-            for (int c = 0; c < DIM; c++) {
-                ra[c] = (ha && live_dmg) ? a.xp[ja + c] : 0.0;
-                ra[DIM + c] = ha ? a.u[ja + c] : 0.0;
-                rb[c] = (hb && live_dmg) ? a.xp[jb + c] : 0.0;
-                rb[DIM + c] = hb ? a.u[jb + c] : 0.0;
-            }
-            if (ha) {
-#pragma unroll
-                for (int c = 0; c < 2 * DIM; c++) s_rec[qa * (2 * DIM) + c] = ra[c];
-            }
-            if (hb) {
-#pragma unroll
-                for (int c = 0; c < 2 * DIM; c++) s_rec[qb * (2 * DIM) + c] = rb[c];
-            }
+        for (int c = 0; c < DIM; c++) {
+            ra[c] = (ha && live_dmg) ? a.xp[ja + c] : 0.0;
+            ra[DIM + c] = ha ? a.u[ja + c] : 0.0;
+            rb[c] = (hb && live_dmg) ? a.xp[jb + c] : 0.0;
+            rb[DIM + c] = hb ? a.u[jb + c] : 0.0;
         }
-        if constexpr (FUSED) {
-            // wave 0 reduces the guard slots for the block, overlapped with
-            // the staging loads (which the barrier waits for anyway)
-            if (a.guard_on && a.step >= 2 && wv == 0) {
-                const unsigned long long m = guard_reduce(g_max);
-                if (lane == 0) s_gmax = m;
-            }
+        if (ha) {
+#pragma unroll
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qa * (2 * DIM) + c] = ra[c];
         }
-        __syncthreads();
+        if (hb) {
+#pragma unroll
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qb * (2 * DIM) + c] = rb[c];
+        }
     }
+    if constexpr (FUSED) {
+        // wave 0 reduces the guard slots for the block, overlapped with the
+        // staging loads (which the barrier waits for anyway)
+        if (a.guard_on && a.step >= 2 && wv == 0) {
+            const unsigned long long m = guard_reduce(g_max);
+            if (lane == 0) s_gmax = m;
+        }
+    }
+    __syncthreads();
     if constexpr (FUSED) {
         if (g_tripped) return;                // the guard tripped earlier
         if (a.guard_on && a.step >= 2) {
@@ -435,7 +450,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             // the same data; block 0 records it and clears the maximum slot
             // of step+1 (last used by step-2, already decided)
             double umax;
-            const bool trip = guard_trips(STAGE ? s_gmax : guard_reduce(g_max), g_past, umax);
+            const bool trip = guard_trips(s_gmax, g_past, umax);
             if (blockIdx.x == 0 && threadIdx.x == 0) guard_record(a.step - 1, trip, umax, a.ghist64, a.gstate);
             if (blockIdx.x == 0 && threadIdx.x < kGuardSlots) a.gmax[guard_slot(a.step + 1, threadIdx.x)] = 0;
             if (trip) return;
@@ -457,31 +472,19 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 // neighbour row: the block's LDS copy, or global memory in a
                 // block whose neighbourhood did not fit the LDS slots
                 double uj[DIM], xj[DIM];
-                if constexpr (STAGE) {
-                    if (!govf) {
-                        const double *rec = s_rec + dd.sv[k] * (2 * DIM);
-#pragma unroll
-                        for (int c = 0; c < DIM; c++) {
-                            xj[c] = rec[c];
-                            uj[c] = rec[DIM + c];
-                        }
-                    } else {
-                        const size_t q = (size_t)(a.nbr[t - sh] - 1) * DIM;
-#pragma unroll
-                        for (int c = 0; c < DIM; c++) {
-                            uj[c] = a.u[q + c];
-                            xj[c] = live_dmg ? a.xp[q + c] : 0.0;
-                        }
-                    }
-                } else {
-                    // 32-bit byte offset from the array base
-                    const unsigned jo = (unsigned)(dd.jv[k] - 1) * (unsigned)(DIM * sizeof(double));
-                    const double *gu = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.u) + jo);
-                    const double *gx = reinterpret_cast<const double *>(reinterpret_cast<const char *>(a.xp) + jo);
+                if (!govf) {
+                    const double *rec = s_rec + dd.sv[k] * (2 * DIM);
 #pragma unroll
                     for (int c = 0; c < DIM; c++) {
-                        uj[c] = gu[c];
-                        xj[c] = live_dmg ? gx[c] : 0.0;
+                        xj[c] = rec[c];
+                        uj[c] = rec[DIM + c];
+                    }
+                } else {
+                    const size_t q = (size_t)(a.nbr[t - sh] - 1) * DIM;
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) {
+                        uj[c] = a.u[q + c];
+                        xj[c] = live_dmg ? a.xp[q + c] : 0.0;
                     }
                 }
 #pragma unroll
@@ -542,7 +545,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 // a dead bond contributes -0.0, the additive identity
                 const auto &p = dd.p64;
                 if (live) {
-                    if (DIM == 2) {
+                    if constexpr (DIM == 2) {
                         f64[k][0] = p[0][k] * du[0] + p[2][k] * du[1];
                         f64[k][1] = p[2][k] * du[0] + p[1][k] * du[1];
                     } else {
@@ -557,25 +560,15 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             }
         }
     };
-
     double Lown = 0.0;     // family sum of component sl (lanes sl < DIM)
     if constexpr (F32) {
         F32G acc;              // the lane's FP64 family sum per component
 #pragma unroll
         for (int c = 0; c < DIM; c++) acc[c] = 0.0;
         for (int t0 = tfirst; t0 < b1; t0 += W) {
-            if constexpr (PF2) {
-                // the next group's loads fly while this one is computed
-                GD dn;
-                if (t0 + W < b1) load_group((unsigned)(t0 + W), dn);
-                F64G f64;
-                compute_group((unsigned)t0, d, f64, acc);
-                d = dn;
-            } else {
-                if (STAGE || t0 != tfirst) load_group((unsigned)t0, d);
-                F64G f64;
-                compute_group((unsigned)t0, d, f64, acc);
-            }
+            load_group((unsigned)t0, d);
+            F64G f64;
+            compute_group((unsigned)t0, d, f64, acc);
         }
         butterfly64<1, S, DIM>(acc);
         Lown = pick(acc, sl);
@@ -589,7 +582,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             const int t0 = base + G * sl;
             F64G f64;
             if (t0 < b1) {
-                if (STAGE || r > 0) load_group((unsigned)t0, d);
+                load_group((unsigned)t0, d);
                 F32G f32;
                 compute_group((unsigned)t0, d, f64, f32);
             } else {
@@ -619,21 +612,14 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(w));
         mymax = bits > mymax ? bits : mymax;
     };
-    if (nvalid && sl < DIM) {
+    if constexpr (!EARLY) load_update();
+    if (upd) {
         const int c = sl;
         const size_t idx = (size_t)i * DIM + c;
         // preset mode of this component (fused path): 0 none, 1 copy and
         // 2 reflection (written by the mirror source's lane), 3 direct
-        const int q = bq;
-        int m = m_pre;
-        if constexpr (STAGE && FUSED) {
-            if (q >= 0) m = a.bmode[q * 3 + c];
-        }
-        if constexpr (STAGE) {
-            if (MODE != MODE_ACCEL) uo_pre = a.uold[idx];
-        }
         double v = 0.0;
-        if (m == 0) {
+        if (m_own == 0) {
             const double bb = (c == 0) ? a.b0 : (c == 1 ? a.b1 : a.b2);
             const double acc = (Lown + bb) / a.rho;
             const double uu = pick(ui, c);
@@ -643,25 +629,30 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             else v = uu - a.c1 * uo_pre + a.c2 * acc;
             a.out[idx] = v;
             if (FUSED || ((MODE == MODE_CD || MODE == MODE_VAR) && !a.mask[i])) fold(v);
-        } else if (m == 3) {
+        } else if (m_own == 3) {
             // direct Dirichlet value (DirichletBC.cpp:107-147)
-            v = a.bval0[q * 3 + c] + a.bvel[q * 3 + c] * a.tpre;
+            v = pv0 + pvel * a.tpre;
             a.out[idx] = v;
             fold(v);
         }
         if constexpr (FUSED) {
             // this node as the mirror source of preset targets (never itself
             // a target: the program has no chains), BCSystem.cpp:114-129
-            for (int p = bp0; p < bp1; p++) {
-                const int qq = a.inv_q[p];
-                const int mm = a.bmode[qq * 3 + c];
+            for (int k = 0; k < bcnt; k++) {
+                int qq = qq0, mm = m_inv0, tn = tn0;
+                if (k > 0) {
+                    qq = a.inv_q[bp0 + k];
+                    mm = a.bmode[qq * 3 + c];
+                    tn = (mm == 1 || mm == 2) ? a.bnode[qq] : -1;
+                }
                 if (mm != 1 && mm != 2) continue;
                 double w = v;
                 if (mm == 2) {
-                    const double g = a.bval0[qq * 3 + c] + a.bvel[qq * 3 + c] * a.tpre;
+                    const double g = (k == 0 && m_own != 3) ? pv0 + pvel * a.tpre
+                                                            : a.bval0[qq * 3 + c] + a.bvel[qq * 3 + c] * a.tpre;
                     w = 2.0 * g - v;
                 }
-                a.out[(size_t)a.bnode[qq] * DIM + c] = w;
+                a.out[(size_t)tn * DIM + c] = w;
                 fold(w);
             }
         }
@@ -710,14 +701,12 @@ constexpr size_t step_force_rows()
     return FF ? 0 : (size_t)4 * (64 / StepCfg<D, FF>::S) * (StepCfg<D, FF>::G * StepCfg<D, FF>::S * D + 1);
 }
 
-// nodes per step block of the staged (3D) path; both precisions tile nodes
-// alike there, so one LDS neighbour table serves either
+// nodes per step block; both precisions tile nodes alike, so one LDS
+// neighbour table serves either
 template <int D>
 constexpr int step_npb()
 {
-    // staged blocks are node tiles that both precisions share
-    static_assert(!StepCfg<D, true>::STAGE || StepCfg<D, true>::S == StepCfg<D, false>::S,
-                  "precisions must share the node tiling");
+    static_assert(StepCfg<D, true>::S == StepCfg<D, false>::S, "precisions must share the node tiling");
     return 4 * (64 / StepCfg<D, true>::S);
 }
 
@@ -735,8 +724,8 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
 {
     FracState &F = ctx->frac;
     StepArgs a;
-    a.n = ctx->n; a.pst = F.pst; a.off = ctx->off.p; a.bsh = F.bsh.p; a.pofs = F.pofs.p; a.nbr_t = F.nbr_t.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
-    a.perm = F.perm.p; a.roff = F.roff.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
+    a.n = ctx->n; a.pst = F.pst; a.bsh = F.bsh.p; a.nbr = ctx->nbr.p; a.xp = ctx->xp.p;
+    a.nrec = F.nrec.p; a.brec = F.brec.p; a.rcnt = F.rcnt.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p;
     a.gslot = F.gslot.p; a.gcap = F.gcap;
     a.P64 = F.P64.p; a.P32 = F.P32.p; a.health = F.health.p; a.break_step = F.break_step.p;
     a.anybroke = F.anybroke.p; a.step = step; a.u = u; a.uold = uold; a.out = out;
@@ -746,15 +735,15 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     a.b0 = F.b[0]; a.b1 = F.b[1]; a.b2 = F.b[2]; a.rho = F.rho; a.c1 = c1; a.c2 = c2;
     a.mask = F.preset_mask.p; a.gmax = F.gmax.p; a.gstate = guard ? F.gstate.p : nullptr;
     a.frozen = (frozen_k > INT_MIN) ? 1 : 0; a.frozen_k = frozen_k;
-    a.prog_of = F.bc.prog_of.p; a.bnode = F.bc.node.p; a.bmode = F.bc.mode.p;
-    a.inv_off = F.bc.inv_off.p; a.inv_q = F.bc.inv_q.p; a.bval0 = F.bc.val0.p; a.bvel = F.bc.vel.p;
+    a.bnode = F.bc.node.p; a.bmode = F.bc.mode.p;
+    a.inv_q = F.bc.inv_q.p; a.bval0 = F.bc.val0.p; a.bvel = F.bc.vel.p;
     a.tpre = tpre; a.ghist64 = F.ghist64.p; a.guard_on = guard ? 1 : 0;
     const bool f32 = (F.precision == PXG_FP32C) && !a.frozen;
-    const bool staged = (ctx->dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
-    PXG_CHECK(F.npb == (ctx->dim == 3 ? step_npb<3>() : step_npb<2>()) && (!staged || F.roff.p), PXG_ERR_STATE,
+    PXG_CHECK(F.npb == (ctx->dim == 3 ? step_npb<3>() : step_npb<2>()) && F.rcnt.p && F.ulist.p && F.nrec.p &&
+              F.brec.p && F.nrec.n == (size_t)ctx->n, PXG_ERR_STATE,
               "launch_step: neighbour staging tables missing (pxg_frac_setup)");
-    PXG_CHECK(!fused || ((mode == MODE_CD || mode == MODE_VAR) && a.prog_of && a.inv_off),
-              PXG_ERR_STATE, "launch_step_fused: CD/VAR steps with a BC program only (pxg_frac_set_bcs)");
+    PXG_CHECK(!fused || mode == MODE_CD || mode == MODE_VAR, PXG_ERR_STATE,
+              "launch_step_fused: CD/VAR steps only");
     hipStream_t st = ctx->stream;
     a.nblk = (int)gridn(ctx->n, launch_npb(ctx->dim, f32));
     const dim3 grid(8 * gridn(a.nblk, 8));
@@ -765,7 +754,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     // non-temporal; 160x80x40 block FP64 706 -> 656 us, FP32C with own
     // operators 425 -> 393 us, 2D FP32C 15.2 -> 14.5 us).
     const size_t opbytes = (size_t)F.pst * (ctx->dim == 3 ? 6 : 3) * (f32 ? sizeof(float) : sizeof(double));
-    const bool nto = fused && !(f32 && a.pofs) && (f32 || opbytes >= ((size_t)128 << 20));
+    const bool nto = fused && !(f32 && F.pofs.p) && (f32 || opbytes >= ((size_t)128 << 20));
 #define PXG_LAUNCH(D, FF, M, FU, NT)                                                                \
     hipLaunchKernelGGL((k_frac_step<D, FF, M, FU, NT>), grid, dim3(256),                              \
                        sizeof(double) * (step_force_rows<D, FF>() + 2 * (size_t)F.gcap * D), st, a)
@@ -817,11 +806,11 @@ int launch_guard_final(pxg_ctx *ctx, int step)
 // block b owns the npb nodes perm[b*npb ...]: on a structured grid every
 // block is one compact 2 x 4 x 4 tile, whose horizon union holds about 14
 // rows per node where a 32-node row would need 31.  Per block the sorted
-// distinct neighbours form the row list ulist[roff[b] ..], and every bond
-// records the LDS slot of its neighbour.  Blocks whose neighbourhood exceeds
-// the slot budget gather from global memory instead (bovf).  The per-bond
-// operator and slot streams are stored in the same tile order (bsh), so each
-// block reads one contiguous range of them.
+// distinct neighbours form the row list ulist[b*gcap ..] (rcnt[b] ids, -1
+// beyond), and every bond records the LDS slot of its neighbour.  Blocks
+// whose neighbourhood exceeds the slot budget gather from global memory
+// instead (bovf).  The per-bond operator and slot streams are stored in the
+// same tile order (bsh), so each block reads one contiguous range of them.
 // node runs of the per-bond streams start on a 128-byte line of the FP32
 // operator arrays (32 bonds)
 constexpr int kBondAlign = 32;
@@ -832,49 +821,8 @@ int build_step_gather(pxg_ctx *ctx)
     const int n = ctx->n, dim = ctx->dim;
     const long long nb = ctx->nb;
     const int npb = (dim == 3) ? step_npb<3>() : step_npb<2>();
-    const bool staged = (dim == 3) ? StepCfg<3, true>::STAGE : StepCfg<2, true>::STAGE;
     F.npb = npb;
-    if (!staged && F.precision != PXG_FP32C) {
-        // FP64 reads every bond's own operators: CSR order (line-aligned runs
-        // only add padding reads to its latency-bound 2D path)
-        F.gcap = 0;
-        F.pst = F.ps;
-        F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
-        F.bsh.release(); F.nbr_t.release();
-        return PXG_OK;
-    }
-    if (!staged) {
-        // FP32C: node order kept; the per-bond streams (and a copy of the
-        // neighbour ids) in line-aligned runs as in the staged layout (on
-        // the 400x200 plate 24.0 us per step against 24.4 us in CSR order;
-        // operator runs are not shared in 2D, see build_op_classes)
-        F.gcap = 0;
-        F.perm.release(); F.roff.release(); F.ulist.release(); F.gslot.release(); F.bovf.release();
-        std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
-        PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream));
-        if (nb > 0)
-            PXG_HIP(hipMemcpyAsync(nbr.data(), ctx->nbr.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, ctx->stream));
-        PXG_HIP(hipStreamSynchronize(ctx->stream));
-        std::vector<int> bsh((size_t)n);
-        long long pos = 0;
-        for (int i = 0; i < n; i++) {
-            pos = (pos + kBondAlign - 1) / kBondAlign * kBondAlign;
-            bsh[(size_t)i] = (int)(pos - off[(size_t)i]);
-            pos += off[(size_t)i + 1] - off[(size_t)i];
-        }
-        PXG_CHECK(pos < INT_MAX - 64, PXG_ERR_ARG, "build_step_gather: bond layout exceeds 2^31 entries");
-        F.pst = (pos + 63) / 64 * 64;
-        std::vector<int> nt((size_t)std::max(F.pst, 1LL), 0);
-        for (int i = 0; i < n; i++)
-            for (int t = off[(size_t)i]; t < off[(size_t)i + 1]; t++) nt[(size_t)(t + bsh[(size_t)i])] = nbr[(size_t)t];
-        PXG_HIP(F.bsh.alloc(bsh.size()));
-        PXG_HIP(F.nbr_t.alloc(nt.size()));
-        PXG_HIP(hipMemcpyAsync(F.bsh.p, bsh.data(), sizeof(int) * bsh.size(), hipMemcpyHostToDevice, ctx->stream));
-        PXG_HIP(hipMemcpyAsync(F.nbr_t.p, nt.data(), sizeof(int) * nt.size(), hipMemcpyHostToDevice, ctx->stream));
-        PXG_HIP(hipStreamSynchronize(ctx->stream));
-        return PXG_OK;
-    }
-    F.nbr_t.release();
+    F.nrec.release(); F.brec.release();   // rebuilt by build_step_records
     constexpr int TX = 2, TY = 4, TZ = 4;
     const int capmax = 640;   // rows per block: 640 x 48 B = 30 KB of LDS
     const int nblk = (int)gridn(n, npb);
@@ -952,17 +900,18 @@ int build_step_gather(pxg_ctx *ctx)
     std::vector<std::thread> pool;
     for (int k = 0; k < nth; k++) pool.emplace_back(work, k, nth);
     for (auto &th : pool) th.join();
-    std::vector<int> roff((size_t)nblk + 1, 0);
-    int gcap = 0;
+    int gcap = 1;
+    std::vector<int> rcnt((size_t)nblk, 0);
     for (int b = 0; b < nblk; b++) {
-        roff[(size_t)b + 1] = roff[(size_t)b] + (int)brows[(size_t)b].size();
-        gcap = std::max(gcap, (int)brows[(size_t)b].size());
+        rcnt[(size_t)b] = (int)brows[(size_t)b].size();
+        gcap = std::max(gcap, rcnt[(size_t)b]);
     }
-    std::vector<int> ulist((size_t)std::max(1, roff[(size_t)nblk]));
+    PXG_CHECK((long long)nblk * gcap < INT_MAX, PXG_ERR_ARG, "build_step_gather: row table exceeds 2^31 entries");
+    std::vector<int> ulist((size_t)nblk * gcap, -1);
     for (int b = 0; b < nblk; b++)
-        std::copy(brows[(size_t)b].begin(), brows[(size_t)b].end(), ulist.begin() + roff[(size_t)b]);
+        std::copy(brows[(size_t)b].begin(), brows[(size_t)b].end(), ulist.begin() + (size_t)b * gcap);
     PXG_HIP(F.perm.alloc(perm.size()));
-    PXG_HIP(F.roff.alloc(roff.size()));
+    PXG_HIP(F.rcnt.alloc(rcnt.size()));
     PXG_HIP(F.ulist.alloc(ulist.size()));
     PXG_HIP(F.gslot.alloc(slot.size()));
     PXG_HIP(F.bovf.alloc(ovf.size()));
@@ -970,12 +919,53 @@ int build_step_gather(pxg_ctx *ctx)
     hipStream_t st = ctx->stream;
     PXG_HIP(hipMemcpyAsync(F.bsh.p, bsh.data(), sizeof(int) * bsh.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.perm.p, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice, st));
-    PXG_HIP(hipMemcpyAsync(F.roff.p, roff.data(), sizeof(int) * roff.size(), hipMemcpyHostToDevice, st));
+    PXG_HIP(hipMemcpyAsync(F.rcnt.p, rcnt.data(), sizeof(int) * rcnt.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.ulist.p, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.gslot.p, slot.data(), sizeof(unsigned short) * slot.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(F.bovf.p, ovf.data(), ovf.size(), hipMemcpyHostToDevice, st));
     PXG_HIP(hipStreamSynchronize(st));
-    F.gcap = std::max(gcap, 1);
+    F.gcap = gcap;
+    return PXG_OK;
+}
+
+// the node and BC records of the step kernel (StepArgs::nrec, brec), one
+// per node slot q of the tiling: set-up data read with one load per lane
+__global__ void k_step_records(int n, const int *perm, const int *off, const int *bsh, const int *pofs,
+                               const int *prog_of, const int *inv_off, const int *inv_q, const int *bmode,
+                               int4 *nrec, int4 *brec)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = perm[q];
+    nrec[q] = make_int4(i, off[i] + bsh[i], off[i + 1] + bsh[i], pofs ? pofs[i] : 0);
+    const int bq = prog_of[i], p0 = inv_off[i], cnt = inv_off[i + 1] - p0;
+    int modes = 0, qq0 = -1;
+    if (bq >= 0)
+        for (int c = 0; c < 3; c++) modes |= (bmode[bq * 3 + c] & 3) << (2 * c);
+    if (cnt > 0) {
+        qq0 = inv_q[p0];
+        for (int c = 0; c < 3; c++) modes |= (bmode[qq0 * 3 + c] & 3) << (8 + 2 * c);
+    }
+    brec[q] = make_int4(bq, p0, cnt | (modes << 16), qq0);
+}
+
+// (re)builds the step records from the tiling, the shared runs and the BC
+// program; called whenever one of them changes (pxg_frac_setup,
+// pxg_frac_set_bcs).  A node mirrors at most a few ghosts, far below the
+// 16-bit count of the record.
+int build_step_records(pxg_ctx *ctx)
+{
+    FracState &F = ctx->frac;
+    const int n = ctx->n;
+    PXG_CHECK(F.perm.p && F.bsh.p && F.bc.prog_of.p && F.bc.inv_off.p, PXG_ERR_STATE,
+              "build_step_records: tiling or BC program missing");
+    PXG_HIP(F.nrec.alloc(n));
+    PXG_HIP(F.brec.alloc(n));
+    hipLaunchKernelGGL(k_step_records, dim3(gridn(n, 256)), dim3(256), 0, ctx->stream, n, F.perm.p, ctx->off.p,
+                       F.bsh.p, F.pofs.p, F.bc.prog_of.p, F.bc.inv_off.p, F.bc.inv_q.p, F.bc.mode.p, F.nrec.p,
+                       F.brec.p);
+    PXG_HIP(hipGetLastError());
+    PXG_HIP(hipStreamSynchronize(ctx->stream));
     return PXG_OK;
 }
 

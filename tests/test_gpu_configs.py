@@ -86,7 +86,20 @@ def test_plate_400x200_2000_steps(oracle):
     print("plate FP32C: rel-L2 %.3e, max abs %.3e, mismatches %d" % (rel32, np.abs(u32 - u).max(), mism))
     assert rel32 <= 1e-5
     assert mism == 0
+    # the exact shared runs change no bit of the FP32C trajectory
+    _, _, nclasses, nshared = s32.ctx.frac_layout()
     s32.ctx.close()
+    ctx = pxg.Context(0)
+    ctx.frac_options(pxg.Context.FRAC_NO_SHARED_RUNS)
+    own = problems.plate_400x200(precision=pxg.FP32C, nsteps=2000, ctx=ctx)
+    own.ctx.frac_init(own.u0, own.dt)
+    assert own.ctx.frac_run(own.dt, own.total) == 2000
+    uown, _, hown, _ = own.ctx.frac_state()
+    print("plate FP32C: %d of %d nodes on %d shared runs; own-operator run bit-equal: %s"
+          % (nshared, own.cloud.n, nclasses, _same_bits(u32, uown)))
+    assert nshared > 0.9 * own.cloud.n
+    assert _same_bits(u32, uown) and np.array_equal(h32, hown)
+    own.ctx.close()
 
 
 def _ch_state_check(oracle, ctx, cloud, m, rlap, rrev, risg, u, uold, kappa):

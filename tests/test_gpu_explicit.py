@@ -176,6 +176,35 @@ def test_explicit_fp32c_3d_shared_runs():
     s64.ctx.close(); s32.ctx.close()
 
 
+def test_explicit_fp32c_2d_exact_shared_runs():
+    """2D FP32C shares only operator runs whose FP32 bits equal the node's
+    own, so the run is bit for bit the run with every node on its own
+    operators (PXG_FRAC_NO_SHARED_RUNS): u^n, u^{n-1}, health and break
+    steps, with most nodes on a shared run."""
+    kw = dict(hi=(0.02, 0.01, 1.0), sc=0.003, cracks=[(0.005, 0.0, 0.005, 0.005), (0.015, 0.0, 0.015, 0.005)],
+              box=(0.0075, 0.0125, 1.0, -1.0, 1.0, -1.0), nsteps=300)
+    runs = []
+    for opts in (0, pxg.Context.FRAC_NO_SHARED_RUNS):
+        ctx = pxg.Context(0)
+        ctx.frac_options(opts)
+        s = problems.explicit_plate(2, 80, 40, precision=pxg.FP32C, ctx=ctx, **kw)
+        pst, tiled, nclasses, nshared = s.ctx.frac_layout()
+        print("opts %d: %d shared runs, %d of %d nodes on them" % (opts, nclasses, nshared, s.cloud.n))
+        if opts == 0:
+            assert nshared > 0.4 * s.cloud.n and nclasses < 0.25 * s.cloud.n
+        else:
+            assert nshared == 0
+        s.ctx.frac_init(s.u0, s.dt)
+        assert s.ctx.frac_run(s.dt, s.total) == 300
+        runs.append(s.ctx.frac_state())
+        s.ctx.close()
+    (u, uo, h, b), (u2, uo2, h2, b2) = runs
+    assert (h == 0).sum() > 0
+    assert np.array_equal(u.view(np.uint64), u2.view(np.uint64))
+    assert np.array_equal(uo.view(np.uint64), uo2.view(np.uint64))
+    assert np.array_equal(h, h2) and np.array_equal(b, b2)
+
+
 def test_explicit_large_horizon_global_gathers(oracle):
     """A 4.5 dx horizon (about 380 bonds per family): every interior step
     block's neighbourhood exceeds the LDS row budget, so those blocks gather
