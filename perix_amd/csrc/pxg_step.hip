@@ -152,6 +152,13 @@ __device__ __forceinline__ void guard_record(int s, bool trip, double umax, doub
     gstate[2] = s;
 }
 
+// Doubles per staged neighbour record {x_j, u_j} in LDS: one more than the
+// record's 2*DIM values.  Lanes read the records of random slots; at 32
+// bytes (2D) apart they fall on 4 distinct bank groups of the 32 four-byte
+// LDS banks, at an odd number of doubles on 16 (2D plate step: 526 bank
+// conflict cycles per wave against 62 cycles of LDS instruction issue).
+__host__ __device__ constexpr int rec_stride(int dim) { return 2 * dim + 1; }
+
 template <int DIM, bool F32> struct StepCfg;
 // S lanes per node (S divides 64) and G bonds per lane group; both
 // precisions of a dimension share the node tiling, so S is common to them.
@@ -288,6 +295,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     constexpr int NC = (DIM == 3) ? 6 : 3;
     constexpr int ROW = W * DIM + 1;          // LDS row per node (odd: no bank aliasing)
     constexpr int FROWS = F32 ? 0 : 4 * NPW * ROW;
+    constexpr int RS = rec_stride(DIM);      // doubles per staged row record
     using GD = GroupData<DIM, F32, G>;
     // dynamic LDS: [FP64 force rows][gcap neighbour records {x, u}]
     extern __shared__ double s_dyn[];
@@ -427,11 +435,11 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         }
         if (ha) {
 #pragma unroll
-            for (int c = 0; c < 2 * DIM; c++) s_rec[qa * (2 * DIM) + c] = ra[c];
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qa * RS + c] = ra[c];
         }
         if (hb) {
 #pragma unroll
-            for (int c = 0; c < 2 * DIM; c++) s_rec[qb * (2 * DIM) + c] = rb[c];
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qb * RS + c] = rb[c];
         }
     }
     if constexpr (FUSED) {
@@ -473,7 +481,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
                 // block whose neighbourhood did not fit the LDS slots
                 double uj[DIM], xj[DIM];
                 if (!govf) {
-                    const double *rec = s_rec + dd.sv[k] * (2 * DIM);
+                    const double *rec = s_rec + dd.sv[k] * RS;
 #pragma unroll
                     for (int c = 0; c < DIM; c++) {
                         xj[c] = rec[c];
@@ -757,7 +765,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     const bool nto = fused && !(f32 && F.pofs.p) && (f32 || opbytes >= ((size_t)128 << 20));
 #define PXG_LAUNCH(D, FF, M, FU, NT)                                                                \
     hipLaunchKernelGGL((k_frac_step<D, FF, M, FU, NT>), grid, dim3(256),                              \
-                       sizeof(double) * (step_force_rows<D, FF>() + 2 * (size_t)F.gcap * D), st, a)
+                       sizeof(double) * (step_force_rows<D, FF>() + (size_t)F.gcap * rec_stride(D)), st, a)
 #define PXG_FUSED(D, FF, M)                                                                         \
     if (!fused) PXG_LAUNCH(D, FF, M, false, false);                                                 \
     else if (nto) PXG_LAUNCH(D, FF, M, true, true);                                                 \
@@ -824,7 +832,7 @@ int build_step_gather(pxg_ctx *ctx)
     F.npb = npb;
     F.nrec.release(); F.brec.release();   // rebuilt by build_step_records
     constexpr int TX = 2, TY = 4, TZ = 4;
-    const int capmax = 640;   // rows per block: 640 x 48 B = 30 KB of LDS
+    const int capmax = 640;   // rows per block: 640 x 56 B = 35 KB of LDS in 3D
     const int nblk = (int)gridn(n, npb);
     std::vector<int> off((size_t)n + 1), nbr((size_t)(nb > 0 ? nb : 1));
     PXG_HIP(hipMemcpyAsync(off.data(), ctx->off.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream));
