@@ -22,7 +22,11 @@
 // Phat[rev(t)]) once; every node then writes only its own entries (no
 // atomics) and every bond array is streamed, not gathered.  Displacements
 // are checkpointed every `ckpt` steps and each segment is recomputed forward
-// (one fused, frozen-health launch per step) before its reverse sweep.
+// (one fused, frozen-health launch per step) before its reverse sweep.  With
+// the E-gradient a reverse step is one launch of k_adj_step (pxg_step.hip,
+// on the forward step's tiling: S^T lam_k is formed while the neighbour rows
+// {u_j, E_j wbar_j} are staged in LDS); the body-force gradient alone needs
+// no states and runs k_preset_T and k_adj_rev below.
 #include "pxg_internal.h"
 #include <algorithm>
 #include <climits>
@@ -91,13 +95,14 @@ __global__ void __launch_bounds__(256) k_body_last(int n, int dim, const double 
 }
 
 // dJ/db = c sum_k S^T lam_k + (c/2) ubar^{-1}, block partials in block order
-__global__ void k_body_final(int nblk, int dim, const double *part, const double *part_last, double c,
-                             double *gb)
+__global__ void k_body_final(int nblk, int nblk_last, int dim, const double *part, const double *part_last,
+                             double c, double *gb)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int a = 0; a < dim; a++) {
         double s = 0.0, l = 0.0;
-        for (int b = 0; b < nblk; b++) { s += part[b * 3 + a]; l += part_last[b * 3 + a]; }
+        for (int b = 0; b < nblk; b++) s += part[b * 3 + a];
+        for (int b = 0; b < nblk_last; b++) l += part_last[b * 3 + a];
         gb[a] = c * s + 0.5 * c * l;
     }
 }
@@ -504,10 +509,15 @@ extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double 
     PXG_HIP(A.wbar.alloc(nd)); PXG_HIP(A.ew.alloc(nd));
     PXG_HIP(A.Ebar.alloc(n)); PXG_HIP(A.dJ.alloc(1));
     PXG_HIP(A.ds.alloc(nsens > 0 ? nsens : 1));
+    // body-force partials: per block of the reverse step (node blocks of 256
+    // without the E-gradient, the step tiling's blocks with it), then per
+    // node block of 256 for the u^{-1} term
     const int nblk = (int)gridn2(n, 256);
-    PXG_HIP(A.bpart.alloc((size_t)6 * nblk));     // step partials, then the u^{-1} partials
+    const int nblk_rev = grad ? adj_step_blocks(ctx) : nblk;
+    PXG_HIP(A.bpart.alloc((size_t)3 * (nblk_rev + nblk)));
     PXG_HIP(A.gb.alloc(3));
-    PXG_HIP(hipMemsetAsync(A.bpart.p, 0, sizeof(double) * 6 * nblk, st));
+    PXG_HIP(hipMemsetAsync(A.bpart.p, 0, sizeof(double) * 3 * (nblk_rev + nblk), st));
+    double *bpart_last = A.bpart.p + (size_t)3 * nblk_rev;
     if (nsens) PXG_HIP(hipMemcpyAsync(A.ds.p, sens, sizeof(int) * nsens, hipMemcpyHostToDevice, st));
     std::vector<double> tk(ncks, 0.0);
     double *uK = nullptr;
@@ -558,8 +568,8 @@ extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double 
             if (rc) return rc;
         }
         if (grad_b) {
-            hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, A.bpart.p + 3 * nblk);
-            hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk, dim, A.bpart.p, A.bpart.p + 3 * nblk,
+            hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, bpart_last);
+            hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk_rev, nblk, dim, A.bpart.p, bpart_last,
                                c, A.gb.p);
             PXG_HIP(hipGetLastError());
             PXG_HIP(hipMemcpyAsync(grad_b, A.gb.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));
@@ -568,6 +578,12 @@ extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double 
         PXG_HIP(hipStreamSynchronize(st));
         return PXG_OK;
     }
+    // the reverse sweep: one fused launch per step (S^T lam_k staged with the
+    // neighbour rows), lam ping-ponging between two buffers
+    rc = adj_step_setup(ctx);
+    if (rc) return rc;
+    PXG_HIP(A.lam2.alloc(nd));
+    double *lam_k = A.lam.p, *lam_km1 = A.lam2.p;
     for (int cidx = ncks - 1; cidx >= 0; cidx--) {
         const int k0 = cidx * ckpt;                 // segment covers steps k0+1 .. k1
         const int k1 = (k0 + ckpt < K) ? k0 + ckpt : K;
@@ -577,22 +593,18 @@ extern "C" int pxg_frac_adjoint_ex(pxg_ctx *ctx, const double *u0, const double 
         if (rc) return rc;
         for (int k = k1; k >= k0 + 1; k--) {
             const double *ukm1 = A.seg.p + (size_t)(k - k0) * nd;     // u^{k-1}
-            hipLaunchKernelGGL(k_preset_T, dim3(gridn2(n, 256)), dim3(256), 0, st, n, dim, A.lam.p,
-                               F.bc.prog_of.p, F.bc.node.p, F.bc.mode.p, F.bc.inv_off.p, F.bc.inv_q.p,
-                               F.nodeE.p, A.wbar.p, A.ew.p, grad_b ? A.bpart.p : nullptr);
-            PXG_HIP(hipGetLastError());
-            rc = (dim == 2) ? launch_adj_rev<2, false>(ctx, k, ukm1, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, A.Ebar.p, c)
-                            : launch_adj_rev<3, false>(ctx, k, ukm1, A.wbar.p, A.ew.p, A.lam.p, A.lamp.p, A.Ebar.p, c);
+            rc = launch_adj_step(ctx, k, ukm1, lam_k, lam_km1, A.lamp.p, A.Ebar.p, grad_b ? A.bpart.p : nullptr, c);
             if (rc) return rc;
+            double *tmp = lam_k; lam_k = lam_km1; lam_km1 = tmp;
         }
     }
     // u^{-1} term with h^0 at u^0 (checkpoint 0 holds u^{-1}, u^0)
-    rc = (dim == 2) ? launch_adj_rev<2, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, A.lam.p, A.lamp.p, A.Ebar.p, 0.5 * c)
-                    : launch_adj_rev<3, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, A.lam.p, A.lamp.p, A.Ebar.p, 0.5 * c);
+    rc = (dim == 2) ? launch_adj_rev<2, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, lam_k, A.lamp.p, A.Ebar.p, 0.5 * c)
+                    : launch_adj_rev<3, true>(ctx, 0, A.cks.p + nd, A.lamp.p, nullptr, lam_k, A.lamp.p, A.Ebar.p, 0.5 * c);
     if (rc) return rc;
     if (grad_b) {
-        hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, A.bpart.p + 3 * nblk);
-        hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk, dim, A.bpart.p, A.bpart.p + 3 * nblk, c,
+        hipLaunchKernelGGL(k_body_last, dim3(nblk), dim3(256), 0, st, n, dim, A.lamp.p, bpart_last);
+        hipLaunchKernelGGL(k_body_final, dim3(1), dim3(64), 0, st, nblk_rev, nblk, dim, A.bpart.p, bpart_last, c,
                            A.gb.p);
         PXG_HIP(hipGetLastError());
         PXG_HIP(hipMemcpyAsync(grad_b, A.gb.p, sizeof(double) * dim, hipMemcpyDeviceToHost, st));

@@ -159,6 +159,9 @@ struct AdjWork {
     dbuf<int> cur, brk_off, brk_t;
     dbuf<int> ds;            // sensor node ids
     dbuf<double> fu[2];      // frozen forward's own u ping-pong (the run state in FracState::u stays)
+    dbuf<double> lam2;       // lam ping-pong of the fused reverse step
+    dbuf<double> phs, phts;  // Phat, PhatT in the step kernel's stream layout
+    dbuf<int4> trec;         // per node: its S^T preset program (k_adj_step)
 };
 
 struct FracState {
@@ -262,6 +265,13 @@ int build_step_gather(pxg_ctx *ctx);
 // node and BC records of the step kernel, after any change of the tiling,
 // the shared runs or the BC program
 int build_step_records(pxg_ctx *ctx);
+// the fused adjoint reverse step on the step tiling (pxg_step.hip): its
+// tables, and one launch reading lam_k from lam_in and writing lam_{k-1}
+// (bpart: per-block sums of S^T lam for dJ/db, adj_step_blocks() blocks, or null)
+int adj_step_setup(pxg_ctx *ctx);
+int adj_step_blocks(pxg_ctx *ctx);
+int launch_adj_step(pxg_ctx *ctx, int k, const double *u, const double *lam_in, double *lam_out, double *lamp,
+                    double *Ebar, double *bpart, double cf);
 int launch_preset(pxg_ctx *ctx, double *u, double t, int step);
 // device pointers to the bond health / break steps in CSR order (a gathered
 // copy when the stream layout is tiled); either output may be NULL

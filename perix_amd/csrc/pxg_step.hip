@@ -266,6 +266,21 @@ __device__ __forceinline__ int xcd_block(int b, int nblk)
     return l < nblk ? l : -1;
 }
 
+// o = P v for a symmetric per-bond operator given by its 3 (2D) or 6 (3D)
+// components
+template <int DIM>
+__device__ __forceinline__ void pmul_sym(const double (&p)[DIM == 3 ? 6 : 3], const double (&v)[DIM], double (&o)[DIM])
+{
+    if constexpr (DIM == 2) {
+        o[0] = p[0] * v[0] + p[2] * v[1];
+        o[1] = p[2] * v[0] + p[1] * v[1];
+    } else {
+        o[0] = p[0] * v[0] + p[3] * v[1] + p[4] * v[2];
+        o[1] = p[3] * v[0] + p[1] * v[1] + p[5] * v[2];
+        o[2] = p[4] * v[0] + p[5] * v[1] + p[2] * v[2];
+    }
+}
+
 template <int DIM>
 __device__ __forceinline__ double pick(const double (&v)[DIM], int c)
 {
@@ -804,6 +819,306 @@ int launch_guard_final(pxg_ctx *ctx, int step)
     FracState &F = ctx->frac;
     hipLaunchKernelGGL(k_guard_final, dim3(1), dim3(64), 0, ctx->stream, step, F.gmax.p, F.ghist64.p, F.gstate.p,
                        F.gstate_host);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One reverse step of the adjoint with the E-gradient (the derivative of the
+// frozen-health step above; the recursion is stated in pxg_adjoint.hip):
+//
+//   wbar = S^T lam_k,  ew = E .* wbar
+//   Ebar_i += c (Khat_k u^{k-1})_i . wbar_i
+//   lam_{k-1,i} = lamp_i + 2 wbar_i + c ((Khat_k^T ew)_i),   lamp_i <- -wbar_i
+//
+// on the step kernel's tiling: the block stages its neighbour rows
+// {u^{k-1}_j, ew_j} in LDS, forming ew_j = E_j (S^T lam_k)_j while it stages
+// (so the transposed BC preset needs no launch of its own), and walks each
+// family in aligned bond groups of the tiled streams (slots, break steps,
+// Phat and its pre-gathered transpose PhatT).  lam_k is read from lam_in and
+// lam_{k-1} written to lam_out: a block may stage a row after another block
+// has finished its node.
+struct AdjStepArgs {
+    int n, nblk, k;            // bond t is intact iff bs[t] > k
+    long long pst;
+    const int4 *nrec;          // node records of the step tiling
+    const int *rcnt, *ulist;
+    const unsigned char *bovf;
+    const unsigned short *gslot;
+    int gcap;
+    const int *nbr, *bsh;      // global gathers of blocks without LDS rows
+    const int *bs;             // break steps, stream layout
+    const double *Phs, *PhTs;  // Phat and PhatT, stream layout
+    const double *u;           // u^{k-1}
+    const double *E;
+    const int4 *trec;          // per node: its S^T program (adj_st_lam)
+    const int *inv_q, *bmode, *bnode;
+    const double *lam_in;
+    double *lam_out, *lamp, *Ebar;
+    double *bpart;             // dJ/db: per-block sums of wbar added to bpart[blk*3 + c], or null
+    double cf;
+};
+
+// (S^T lam)_m, component c, for the merged BC program (k_preset_T's rule):
+// a preset component of m takes none of its own lam, and m as the mirror
+// source of a copied or reflected target g adds +lam_g or -lam_g, in program
+// order.  tr = {own modes | first inverse entry's modes << 8 | entries << 16,
+// target node of the first entry, first entry}.
+template <int DIM>
+__device__ __forceinline__ double adj_st_lam(int m, int c, int4 tr, const double *lam, const int *inv_q,
+                                             const int *bmode, const int *bnode)
+{
+    double v = ((tr.x >> (2 * c)) & 3) ? 0.0 : lam[(size_t)m * DIM + c];
+    const int cnt = (int)((unsigned)tr.x >> 16);
+    for (int p = 0; p < cnt; p++) {
+        int md, g;
+        if (p == 0) {
+            md = (tr.x >> (8 + 2 * c)) & 3;
+            g = tr.y;
+        } else {
+            const int qq = inv_q[tr.z + p];
+            md = bmode[qq * 3 + c];
+            g = bnode[qq];
+        }
+        if (md == 1) v += lam[(size_t)g * DIM + c];
+        else if (md == 2) v += -lam[(size_t)g * DIM + c];
+    }
+    return v;
+}
+
+template <int DIM> struct AdjStepCfg { static constexpr int S = StepCfg<DIM, false>::S, G = 2; };
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_adj_step(AdjStepArgs a)
+{
+    constexpr int S = AdjStepCfg<DIM>::S, G = AdjStepCfg<DIM>::G;
+    constexpr int NPW = 64 / S, W = G * S;
+    constexpr int NC = (DIM == 3) ? 6 : 3;
+    constexpr int RS = rec_stride(DIM);
+    extern __shared__ double s_dyn[];
+    double *s_rec = s_dyn;
+    const int blk = xcd_block(blockIdx.x, a.nblk);
+    if (blk < 0) return;
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int ln = lane / S, sl = lane % S;
+    const int gi = (blk * 4 + wv) * NPW + ln;
+    const bool nvalid = gi < a.n;
+    const bool upd = nvalid && sl < DIM;
+
+    // round one: the node record and the block's row ids
+    int4 nr = make_int4(0, 0, 0, 0);
+    if (nvalid) nr = a.nrec[gi];
+    const int *rows = a.ulist + (size_t)blk * a.gcap;
+    const int ida0 = tid < a.gcap ? rows[tid] : -1, idb0 = tid + 256 < a.gcap ? rows[tid + 256] : -1;
+    const int nrows = a.rcnt[blk];
+    const bool govf = a.bovf[blk];
+    const int i = nr.x, b0 = nr.y, b1 = nr.z;
+
+    // round two: the node's own u, E, S^T program and lam, then the rows
+    double ui[DIM];
+    double wb = 0.0, lp = 0.0;      // the lane's component of wbar_i, lamp_i
+    double wi[DIM], ewi[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
+    {
+        const int4 tr = nvalid ? a.trec[i] : make_int4(0, 0, 0, 0);
+        const double Ei = nvalid ? a.E[i] : 0.0;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            wi[c] = nvalid ? adj_st_lam<DIM>(i, c, tr, a.lam_in, a.inv_q, a.bmode, a.bnode) : 0.0;
+            ewi[c] = Ei * wi[c];
+        }
+    }
+    if (upd) lp = a.lamp[(size_t)i * DIM + sl];
+    wb = pick(wi, sl);
+    const int sh = (govf && nvalid) ? a.bsh[i] : 0;
+    // stage {u_j, ew_j}: rounds beyond the first read their ids first
+    auto row_of = [&](int j, double (&r)[2 * DIM]) {
+        const int4 tr = a.trec[j];
+        const double Ej = a.E[j];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            r[c] = a.u[(size_t)j * DIM + c];
+            r[DIM + c] = Ej * adj_st_lam<DIM>(j, c, tr, a.lam_in, a.inv_q, a.bmode, a.bnode);
+        }
+    };
+    for (int q0 = 0; q0 == 0 || q0 < nrows; q0 += 512) {
+        const int qa = q0 + tid, qb = qa + 256;
+        const int ida = q0 == 0 ? ida0 : (qa < a.gcap ? rows[qa] : -1);
+        const int idb = q0 == 0 ? idb0 : (qb < a.gcap ? rows[qb] : -1);
+        double ra[2 * DIM], rb[2 * DIM];
+        if (ida >= 0) row_of(ida, ra);
+        if (idb >= 0) row_of(idb, rb);
+        if (ida >= 0) {
+#pragma unroll
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qa * RS + c] = ra[c];
+        }
+        if (idb >= 0) {
+#pragma unroll
+            for (int c = 0; c < 2 * DIM; c++) s_rec[qb * RS + c] = rb[c];
+        }
+    }
+    __syncthreads();
+
+    // family sums over the intact bonds: ku = sum Phat (u_j - u_i),
+    // zo = sum PhatT ew_j, sp = sum Phat (per-lane partial sums, then a lane
+    // butterfly: a fixed order)
+    double ku[DIM], zo[DIM], sp[NC];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) { ku[c] = 0.0; zo[c] = 0.0; }
+#pragma unroll
+    for (int c = 0; c < NC; c++) sp[c] = 0.0;
+    for (int t0 = (b0 & ~(G - 1)) + G * sl; t0 < b1; t0 += W) {
+        unsigned short sv[G];
+        int bv[G];
+        double p[NC][G], pt[NC][G];
+        vload(a.gslot + t0, sv);
+        vload(a.bs + t0, bv);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            vload(a.Phs + c * a.pst + t0, p[c]);
+            vload(a.PhTs + c * a.pst + t0, pt[c]);
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int t = t0 + g;
+            if (t < b0 || t >= b1 || bv[g] <= a.k) continue;
+            double uj[DIM], ej[DIM];
+            if (!govf) {
+                const double *rec = s_rec + sv[g] * RS;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) { uj[c] = rec[c]; ej[c] = rec[DIM + c]; }
+            } else {
+                double r[2 * DIM];
+                row_of(a.nbr[t - sh] - 1, r);
+#pragma unroll
+                for (int c = 0; c < DIM; c++) { uj[c] = r[c]; ej[c] = r[DIM + c]; }
+            }
+            double pc[NC], ptc[NC], du[DIM], o[DIM];
+#pragma unroll
+            for (int c = 0; c < NC; c++) { pc[c] = p[c][g]; ptc[c] = pt[c][g]; sp[c] += pc[c]; }
+#pragma unroll
+            for (int c = 0; c < DIM; c++) du[c] = uj[c] - ui[c];
+            pmul_sym<DIM>(pc, du, o);
+#pragma unroll
+            for (int c = 0; c < DIM; c++) ku[c] += o[c];
+            pmul_sym<DIM>(ptc, ej, o);
+#pragma unroll
+            for (int c = 0; c < DIM; c++) zo[c] += o[c];
+        }
+    }
+    butterfly64<1, S, DIM>(ku);
+    butterfly64<1, S, DIM>(zo);
+    butterfly64<1, S, NC>(sp);
+    if (a.bpart) {
+        // the body force enters every central-difference value: dJ/db +=
+        // c S^T lam, summed per block in a fixed order (reproducible)
+        __shared__ double red[4][3];
+        double v[3] = {0.0, 0.0, 0.0};
+        if (nvalid && sl == 0)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) v[c] = wi[c];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] += __shfl_xor(v[c], o, 64);
+        if (lane == 0)
+            for (int c = 0; c < 3; c++) red[wv][c] = v[c];
+        __syncthreads();
+        if (tid == 0)
+            for (int c = 0; c < DIM; c++) a.bpart[blk * 3 + c] += ((red[0][c] + red[1][c]) + (red[2][c] + red[3][c]));
+    }
+    if (!upd) return;
+    if (sl == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) s += ku[c] * wi[c];
+        a.Ebar[i] += a.cf * s;
+    }
+    double d[DIM];
+    pmul_sym<DIM>(sp, ewi, d);
+    const size_t q = (size_t)i * DIM + sl;
+    const double z = pick(zo, sl) - pick(d, sl);
+    a.lam_out[q] = lp + 2.0 * wb + a.cf * z;
+    a.lamp[q] = -wb;
+}
+
+// the step tiling's staging tables with the adjoint's streams: S^T program
+// records per node (trec) and the operators in the stream layout
+__global__ void k_adj_trec(int n, const int *prog_of, const int *bmode, const int *inv_off, const int *inv_q,
+                           const int *bnode, int4 *trec)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int q = prog_of[m], p0 = inv_off[m], cnt = inv_off[m + 1] - p0;
+    int x = 0, g0 = -1;
+    if (q >= 0)
+        for (int c = 0; c < 3; c++) x |= (bmode[q * 3 + c] & 3) << (2 * c);
+    if (cnt > 0) {
+        const int qq = inv_q[p0];
+        for (int c = 0; c < 3; c++) x |= (bmode[qq * 3 + c] & 3) << (8 + 2 * c);
+        g0 = bnode[qq];
+    }
+    trec[m] = make_int4(x | (cnt << 16), g0, p0, 0);
+}
+
+__global__ void k_to_stream(int n, int nc, const int *off, const int *bsh, long long ps, long long pst,
+                            const double *src, double *dst)
+{
+    const int i = blockIdx.x * kConvNodes + threadIdx.x / kConvLanes, l = threadIdx.x % kConvLanes;
+    if (i >= n) return;
+    for (int t = off[i] + l; t < off[i + 1]; t += kConvLanes)
+        for (int c = 0; c < nc; c++) dst[(size_t)c * pst + t + bsh[i]] = src[(size_t)c * ps + t];
+}
+
+int adj_step_setup(pxg_ctx *ctx)
+{
+    FracState &F = ctx->frac;
+    AdjWork &A = F.adj;
+    hipStream_t st = ctx->stream;
+    const int n = ctx->n, nc = (ctx->dim == 3) ? 6 : 3;
+    PXG_CHECK(F.nrec.p && F.bsh.p && F.Phat.p && F.PhatT.p, PXG_ERR_STATE, "adj_step_setup: step tiling missing");
+    PXG_HIP(A.trec.alloc(n));
+    hipLaunchKernelGGL(k_adj_trec, dim3(gridn(n, 256)), dim3(256), 0, st, n, F.bc.prog_of.p, F.bc.mode.p,
+                       F.bc.inv_off.p, F.bc.inv_q.p, F.bc.node.p, A.trec.p);
+    PXG_HIP(A.phs.alloc((size_t)F.pst * nc));
+    PXG_HIP(A.phts.alloc((size_t)F.pst * nc));
+    PXG_HIP(hipMemsetAsync(A.phs.p, 0, sizeof(double) * (size_t)F.pst * nc, st));
+    PXG_HIP(hipMemsetAsync(A.phts.p, 0, sizeof(double) * (size_t)F.pst * nc, st));
+    hipLaunchKernelGGL(k_to_stream, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p, F.bsh.p,
+                       F.ps, F.pst, F.Phat.p, A.phs.p);
+    hipLaunchKernelGGL(k_to_stream, dim3(gridn(n, kConvNodes)), dim3(kConvBlock), 0, st, n, nc, ctx->off.p, F.bsh.p,
+                       F.ps, F.pst, F.PhatT.p, A.phts.p);
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int adj_step_blocks(pxg_ctx *ctx)
+{
+    return (int)gridn(ctx->n, 4 * (64 / (ctx->dim == 3 ? AdjStepCfg<3>::S : AdjStepCfg<2>::S)));
+}
+
+int launch_adj_step(pxg_ctx *ctx, int k, const double *u, const double *lam_in, double *lam_out, double *lamp,
+                    double *Ebar, double *bpart, double cf)
+{
+    FracState &F = ctx->frac;
+    AdjWork &A = F.adj;
+    AdjStepArgs a;
+    a.n = ctx->n; a.k = k; a.pst = F.pst;
+    a.nrec = F.nrec.p; a.rcnt = F.rcnt.p; a.ulist = F.ulist.p; a.bovf = F.bovf.p; a.gslot = F.gslot.p;
+    a.gcap = F.gcap; a.nbr = ctx->nbr.p; a.bsh = F.bsh.p; a.bs = F.break_step.p;
+    a.Phs = A.phs.p; a.PhTs = A.phts.p; a.u = u; a.E = F.nodeE.p; a.trec = A.trec.p;
+    a.inv_q = F.bc.inv_q.p; a.bmode = F.bc.mode.p; a.bnode = F.bc.node.p;
+    a.lam_in = lam_in; a.lam_out = lam_out; a.lamp = lamp; a.Ebar = Ebar; a.bpart = bpart; a.cf = cf;
+    PXG_CHECK(a.nrec && a.Phs && a.trec && a.E, PXG_ERR_STATE, "launch_adj_step: adjoint step tables missing");
+    const int npb = 4 * (64 / (ctx->dim == 3 ? AdjStepCfg<3>::S : AdjStepCfg<2>::S));
+    PXG_CHECK(npb == F.npb, PXG_ERR_STATE, "launch_adj_step: tiling mismatch");
+    a.nblk = (int)gridn(ctx->n, npb);
+    const dim3 grid(8 * gridn(a.nblk, 8));
+    const size_t lds = sizeof(double) * (size_t)F.gcap * rec_stride(ctx->dim);
+    if (ctx->dim == 3) hipLaunchKernelGGL(k_adj_step<3>, grid, dim3(256), lds, ctx->stream, a);
+    else hipLaunchKernelGGL(k_adj_step<2>, grid, dim3(256), lds, ctx->stream, a);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
 }
