@@ -152,14 +152,17 @@ template <int DIM> struct AdjCfg;
 template <> struct AdjCfg<2> { static constexpr int S = 8, G = 2; };
 template <> struct AdjCfg<3> { static constexpr int S = 16, G = 2; };
 
-// One reverse step (LAST: the u^{-1} term only).  S lanes per node walk the
-// family in aligned groups of G bonds with vector loads of ids, break steps
-// and both operators; the per-lane FP64 partial sums are combined by a lane
-// butterfly (a fixed order, so results are reproducible).
-template <int DIM, bool LAST, bool EGRAD = true>
+// Two forms: LAST (= EGRAD) adds the u^{-1} term of the E-gradient; the
+// other is one reverse step of the body-force gradient alone (no states, the
+// intact family sums kept in spg).  The E-gradient's reverse steps run
+// k_adj_step (pxg_step.hip).  S lanes per node walk the family in aligned
+// groups of G bonds with vector loads of ids, break steps and operators; the
+// per-lane FP64 partial sums are combined by a lane butterfly (a fixed
+// order, so results are reproducible).
+template <int DIM, bool LAST, bool EGRAD = LAST>
 __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
 {
-    static_assert(EGRAD || !LAST, "the u^{-1} term is part of the E-gradient");
+    static_assert(EGRAD == LAST, "the u^{-1} term of the E-gradient, or a body-force step");
     constexpr int S = AdjCfg<DIM>::S, G = AdjCfg<DIM>::G;
     constexpr int NPW = 64 / S, W = S * G;
     constexpr int NC = (DIM == 3) ? 6 : 3;
@@ -216,10 +219,6 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
                 for (int c = 0; c < DIM; c++) ku[c] += o[c];
             }
             if constexpr (!LAST) {
-                if constexpr (EGRAD) {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) sp[c] += pc[c];
-                }
                 double ej[DIM], ptc[NC];
 #pragma unroll
                 for (int c = 0; c < DIM; c++) ej[c] = a.ew[jo + c];
@@ -237,10 +236,6 @@ __global__ void __launch_bounds__(256) k_adj_rev(AdjArgs a)
         for (int c = 0; c < DIM; c++) {
             if constexpr (EGRAD) ku[c] += __shfl_xor(ku[c], m);
             if constexpr (!LAST) zo[c] += __shfl_xor(zo[c], m);
-        }
-        if constexpr (!LAST && EGRAD) {
-#pragma unroll
-            for (int c = 0; c < NC; c++) sp[c] += __shfl_xor(sp[c], m);
         }
     }
     if (!nvalid || sl >= DIM) return;
@@ -364,7 +359,7 @@ int launch_transpose_ops(pxg_ctx *ctx)
     return PXG_OK;
 }
 
-template <int DIM, bool LAST, bool EGRAD = true>
+template <int DIM, bool LAST, bool EGRAD = LAST>
 static int launch_adj_rev(pxg_ctx *ctx, int k, const double *u, const double *wbar, const double *ew,
                           double *lam, double *lamp, double *Ebar, double cf)
 {
