@@ -271,26 +271,37 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
         }
         return;
     } else {
+    // Every load of the bond is issued before the first use: the reverse
+    // bond, the bond's Laplacian and health hang off t alone, the
+    // neighbour's flag, volume, mobility, state and the reverse bond's
+    // Laplacian off j or rv.  Loaded only where a branch needs them, they
+    // formed a chain of five dependent rounds (off, nbr, isghost, rev, lap).
+    const int rv = A.rev[t];
+    const double lapt = A.lap[t];
+    const double lapr = (rv >= 0) ? A.lap[rv] : 0.0;
     const bool ghostJ = A.isghost[j] != 0;
+    const double volj = A.vol[j];
+    double uj[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) uj[a] = A.u[rj + a];
+    const double mj = A.mnode ? A.mnode[j] : sch_mobility(uj[0], k.D);
     double g = 1.0, coeff_c = 0.0, kl = 0.0;
     if constexpr (EL == EL_FSCH) {
         g = A.health[t] ? 1.0 : k.resid;
         double wf = 0.0, g2c = 0.0;
         if (!ghostJ) {
-            g2c = A.lap[t];
-            const int rv = A.rev[t];
-            if (rv >= 0) wf = 0.5 * (nd.Mi * g2c + (A.vol[j] / nd.Vi) * (A.mnode ? A.mnode[j] : sch_mobility(A.u[rj], k.D)) * A.lap[rv]);
+            g2c = lapt;
+            if (rv >= 0) wf = 0.5 * (nd.Mi * g2c + (volj / nd.Vi) * mj * lapr);
             else wf = nd.Mi * g2c;
         }
         coeff_c = -wf;
         kl = k.kappa * g2c;
     } else {
-        const double g2ij = A.lap[t];
+        const double g2ij = lapt;
         if (!ghostJ) {
-            const int rv = A.rev[t];
             double wij;
             if (rv >= 0 && nd.Vi > 0.0)
-                wij = 0.5 * (nd.Mi * g2ij + (A.vol[j] / nd.Vi) * (A.mnode ? A.mnode[j] : sch_mobility(A.u[rj], k.D)) * A.lap[rv]);
+                wij = 0.5 * (nd.Mi * g2ij + (volj / nd.Vi) * mj * lapr);
             else wij = nd.Mi * g2ij;
             coeff_c = -wij;
             kl = k.kappa * g2ij;
@@ -298,7 +309,7 @@ __device__ __forceinline__ void sch_bond(const SchArgs &A, const SchNode<el_nd<D
     }
     double d[ND];
 #pragma unroll
-    for (int a = 0; a < ND; a++) d[a] = A.u[rj + a] - nd.ui[a];
+    for (int a = 0; a < ND; a++) d[a] = uj[a] - nd.ui[a];
 #pragma unroll
     for (int a = 0; a < ND; a++)
 #pragma unroll
