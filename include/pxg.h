@@ -33,6 +33,7 @@ extern "C" {
 #define PXG_ERR_FAULT (-4)    /* injected fault (pxg_set_fault) */
 #define PXG_ERR_DEGENERATE (-5) /* degenerate moment matrix */
 #define PXG_ERR_BLOWUP (-6)   /* explicit blow-up guard tripped */
+#define PXG_ERR_SINGULAR (-7) /* zero pivot in the direct solve */
 
 #define PXG_FP64 0            /* plain FP64 (bit-exact with the reference) */
 #define PXG_FP32C 1           /* FP32 operators; FP64 bond products and family sums */
@@ -279,6 +280,48 @@ int pxg_csr_bc_rows(pxg_ctx *ctx, int *count);
 int pxg_csr_apply_bcs(pxg_ctx *ctx, double t, const double *u, double *kvals, double *rhs);
 int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *kvals_dev,
                           double *rhs_dev);
+
+/* ---- direct solve of the implicit system on the device ---------------------
+ * Replaces LinearSolver::solve (LinearSolver.cpp:97-121) with its default
+ * back-end, DefaultSolver::solveLinearSystem (DefaultSolver.cpp:464-485):
+ * the same reverse Cuthill-McKee ordering (:52-150) and skyline envelope
+ * (:164-194), factored A = L U without pivoting as 64 x 64 dense blocks of
+ * that envelope, then substitution (:372-424).  The symbolic phase runs on
+ * the host once per CSR pattern (again after pxg_build_csr / pxg_set_csr).
+ * pxg_solve_options mirrors the solver's "use_rcm" and "max_profile_gb"
+ * parameters (defaults 1 and 16; a profile above the limit is PXG_ERR_ARG, as
+ * the reference refuses it, :196-238; <= 0 disables the guard).
+ * pxg_solve_factor_dev factors the device CSR values kvals (the context's
+ * pattern, BC rows already applied); an exactly zero pivot is
+ * PXG_ERR_SINGULAR naming the equation (:362-367).  pxg_solve_apply_dev
+ * solves with the current factors; pxg_solve_dev does both; pxg_solve takes
+ * host arrays.  Every _dev call is asynchronous on the context stream except
+ * the factorisation's pivot check (one host wait).
+ * pxg_solve_info: out[8] = {equations, blocks, block edge, reference profile
+ * size, off-diagonal blocks per triangle, largest active set, bytes of block
+ * storage, block products per factorisation}.  pxg_solve_ordering copies the
+ * ordering (perm[n]: new -> old) and the reference skyline pointer
+ * (jp[n+1]); either may be NULL. */
+int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib);
+int pxg_solve_factor_dev(pxg_ctx *ctx, const double *kvals_dev);
+int pxg_solve_apply_dev(pxg_ctx *ctx, const double *rhs_dev, double *x_dev);
+int pxg_solve_dev(pxg_ctx *ctx, const double *kvals_dev, const double *rhs_dev, double *x_dev);
+int pxg_solve(pxg_ctx *ctx, const double *kvals, const double *rhs, double *x);
+int pxg_solve_info(pxg_ctx *ctx, long long *out);
+int pxg_solve_ordering(pxg_ctx *ctx, int *perm, long long *jp);
+
+/* Newton iterate on the device (NonlinearSolver.cpp:90-125 and
+ * TimeStepping.cpp:167).  pxg_csr_preset_dev applies the BC presets of the
+ * resolved pxg_csr_set_bcs list to u (n*ndof) at time t
+ * (BCSystem::presetSolution, BCSystem.cpp:114-129).  pxg_newton_update_dev
+ * computes norms[0] = |rhs| and norms[1] = |du| and, when |du| is finite,
+ * applies u += du and the presets at t (the reference refuses a non-finite
+ * update and leaves u unchanged, NonlinearSolver.cpp:100-110); it waits for
+ * the stream to read the two norms.  Both refuse (PXG_ERR_STATE) a BC list in
+ * which a preset reads a dof that another entry presets. */
+int pxg_csr_preset_dev(pxg_ctx *ctx, double t, double *u_dev);
+int pxg_newton_update_dev(pxg_ctx *ctx, double t, double *u_dev, const double *du_dev,
+                          const double *rhs_dev, double *norms);
 
 /* ---- remaining elements: implicit dynamic fracture ------------------------
  * PDDODynamicFracElement (pddo_dynamic_frac): backward-Euler elastodynamics

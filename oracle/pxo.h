@@ -230,6 +230,15 @@ double pxo_frac_loss_frozen(int n, const pxo_frac *p, const double *coords,
                             const double *node_E, int nsens, const int *sens,
                             int comp, int nthreads);
 
+/* ---- f2: the reference's default linear solver (pxo_solve.cpp) ---------- */
+/* DefaultSolver.cpp:52-194: RCM ordering (perm: new -> old; identity when
+ * use_rcm is 0) and the skyline pointer jp[n+1]; returns the profile size. */
+long long pxo_ldu_profile(int n, const int *rowptr, const int *cols, int use_rcm, int *perm, long long *jp);
+/* DefaultSolver.cpp:248-424: LDU without pivoting and substitution of
+ * K x = b.  Returns 0, or 1 + the (new-order) equation of a zero pivot. */
+int pxo_ldu_solve(int n, const int *rowptr, const int *cols, const double *vals, const double *b,
+                  int use_rcm, double *x, int *perm, long long *jp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,7 @@ def lib():
         L.pxo_stable_dt.restype = C.c_double
         L.pxo_frac_adjoint.restype = C.c_double
         L.pxo_frac_loss_frozen.restype = C.c_double
+        L.pxo_ldu_profile.restype = C.c_longlong
         L.pxo_random_ic.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_int, I,
                                     C.c_int, C.c_int, I, D]
     return _LIB
@@ -452,3 +453,24 @@ def random_ic(seed, lo, hi, nodes, ndof, dofs, values):
     lib().pxo_random_ic(C.c_uint64(seed), C.c_double(lo), C.c_double(hi), len(nodes),
                         ip(nodes), ndof, len(dofs), ip(dofs), dp(values))
     return values
+
+
+def ldu_profile(rowptr, cols, use_rcm=True):
+    """The default linear solver's ordering and skyline pointer: (perm, jp)."""
+    n = len(rowptr) - 1
+    perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
+    lib().pxo_ldu_profile(n, ip(np.ascontiguousarray(rowptr, np.int32)), ip(np.ascontiguousarray(cols, np.int32)),
+                          int(use_rcm), ip(perm), jp.ctypes.data_as(C.POINTER(C.c_longlong)))
+    return perm, jp
+
+
+def ldu_solve(rowptr, cols, vals, b, use_rcm=True):
+    """The default linear solver on K x = b: (rc, x, perm, jp); rc 0, or
+    1 + the equation (new order) of a zero pivot."""
+    n = len(rowptr) - 1
+    x = np.zeros(n); perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
+    rc = lib().pxo_ldu_solve(n, ip(np.ascontiguousarray(rowptr, np.int32)),
+                             ip(np.ascontiguousarray(cols, np.int32)), dp(np.ascontiguousarray(vals, float)),
+                             dp(np.ascontiguousarray(b, float)), int(use_rcm), dp(x), ip(perm),
+                             jp.ctypes.data_as(C.POINTER(C.c_longlong)))
+    return rc, x, perm, jp

@@ -43,6 +43,7 @@
 #include "ElmtSystem/PDDODynamicFracElement.h"
 #include "ElmtSystem/StressCahnHilliardElement.h"
 #include "ElmtSystem/FracStressCahnHilliardElement.h"
+#include "LinearSolver/DefaultSolver.h"
 #undef private
 #undef protected
 
@@ -476,3 +477,32 @@ int ref_explicit_timed(void *h, double *setup_s, double *loop_s, double *u_out)
     return ok ? p.m_TimeStepping.m_LastSteps : -1;
 }
 }
+
+extern "C" {
+// The reference's default linear solver (DefaultSolver::initSolver, then
+// solveLinearSystem, DefaultSolver.cpp:428-485: RCM ordering, unified
+// skyline envelope, LDU without pivoting, substitution) on the case's CSR
+// pattern with caller values kvals and right-hand side rhs.  Writes x; perm
+// (new -> old, n ints) and jp (skyline pointer, n+1) when not NULL.  use_rcm
+// mirrors the solver's "use_rcm" parameter.  Returns 0, or 1 when the
+// factorisation meets a zero pivot.
+int ref_default_solve(void *h, const double *kvals, const double *rhs, int use_rcm, double *x, int *perm,
+                      long long *jp)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int n = p.m_U.getSize();
+    const int nnz = p.m_Matrix.getNNZNum();
+    std::copy(kvals, kvals + nnz, p.m_Matrix.getCSRValuesPtr());
+    VectorXd b(n), xs(n);
+    std::copy(rhs, rhs + n, b.getDataPtr());
+    DefaultSolver solver;
+    nlohmann::ordered_json prm;
+    prm["use_rcm"] = use_rcm != 0;
+    solver.initSolver(p.m_Matrix, prm);
+    const bool ok = solver.solveLinearSystem(p.m_Matrix, b, xs);
+    std::copy(xs.getDataPtr(), xs.getDataPtr() + n, x);
+    if (perm) std::copy(solver.m_Perm.begin(), solver.m_Perm.end(), perm);
+    if (jp) std::copy(solver.m_JP.begin(), solver.m_JP.end(), jp);
+    return ok ? 0 : 1;
+}
+} // extern "C"

@@ -39,6 +39,8 @@ def lib():
         _LIB.ref_explicit_timed.argtypes = [C.c_void_p, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), D]
         _LIB.ref_group.restype = C.c_int
+        _LIB.ref_default_solve.restype = C.c_int
+        _LIB.ref_default_solve.argtypes = [C.c_void_p, D, D, C.c_int, D, I, C.POINTER(C.c_longlong)]
     return _LIB
 
 
@@ -256,6 +258,17 @@ class Case:
         k = np.ascontiguousarray(k, float).copy(); r = np.ascontiguousarray(r, float).copy()
         lib().ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
         return k, r
+
+    def default_solve(self, k, r, use_rcm=True):
+        """The reference's DefaultSolver on the case's CSR pattern with values k
+        and right-hand side r: returns (rc, x, perm, jp) -- rc 1 on a zero
+        pivot, perm new -> old, jp the skyline pointer."""
+        n = self.n * self.ndof
+        x = np.zeros(n); perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
+        rc = lib().ref_default_solve(self.h, _dp(np.ascontiguousarray(k, float)),
+                                     _dp(np.ascontiguousarray(r, float)), int(use_rcm), _dp(x), _ip(perm),
+                                     jp.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return rc, x, perm, jp
 
     def last_bc_seconds(self):
         """Wall time of the applyBCs call alone in the last apply_bcs."""

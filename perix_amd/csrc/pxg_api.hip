@@ -68,6 +68,11 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->sr); rel(c->sf);
     rel(c->csrbc.row); rel(c->csrbc.dslot); rel(c->csrbc.pslot); rel(c->csrbc.pdof);
     rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);
+    {
+        pxg::SolveState &S = c->solve;
+        rel(S.F); rel(S.Li); rel(S.Ui); rel(S.y); rel(S.z); rel(S.w); rel(S.inv); rel(S.fb); rel(S.al);
+        rel(S.zpiv); rel(S.loc); rel(S.lp); rel(S.part); rel(S.norms);
+    }
     rel(c->dyn.health); rel(c->dyn.vel); rel(c->dyn.uprev);
     rel(c->fsch.health); rel(c->fsch.lapc); rel(c->fsch.vel); rel(c->fsch.uprev);
     FracState &F = c->frac;

@@ -172,6 +172,9 @@ extern "C" int pxg_csr_set_bcs(pxg_ctx *ctx, int nbc, const pxg_bc *bcs)
         kind[q] = ent[q].kind; val0[q] = ent[q].val0; vel[q] = ent[q].vel;
     }
     P.count = (int)m;
+    P.chained = 0;
+    for (size_t q = 0; q < m; q++)
+        if (pdof[q] >= 0 && at[(size_t)pdof[q]] >= 0) P.chained = 1;
     if (m > 0) {
         PXG_HIP(P.row.alloc(m)); PXG_HIP(P.dslot.alloc(m)); PXG_HIP(P.pslot.alloc(m));
         PXG_HIP(P.pdof.alloc(m)); PXG_HIP(P.kind.alloc(m)); PXG_HIP(P.val0.alloc(m));

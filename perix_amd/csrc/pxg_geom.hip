@@ -637,6 +637,7 @@ extern "C" int pxg_set_csr(pxg_ctx *ctx, int ndof, const int *rowptr, const int 
     }
     PXG_FAULT(ctx, "pxg_set_csr");
     ctx->csrbc.set = 0;
+    ctx->csr_gen++;
     PXG_HIP(ctx->rowptr.alloc((size_t)rows + 1));
     PXG_HIP(ctx->cols.alloc((size_t)nnz));
     PXG_HIP(ctx->slot.alloc((size_t)ctx->nb));
@@ -705,6 +706,7 @@ extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
     PXG_FAULT(ctx, "pxg_build_csr");
     PXG_HIP(hipSetDevice(ctx->device));
     ctx->csrbc.set = 0;                 // a BC row program refers to the old pattern
+    ctx->csr_gen++;
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const long long rows = (long long)n * ndof;

@@ -115,6 +115,7 @@ struct CsrBcProgram {
     dbuf<double> val0, vel;  // Dirichlet g(t) = val0 + vel t
     int count = 0;
     int set = 0;             // program built for the current CSR pattern
+    int chained = 0;         // some entry's partner dof is itself a constrained row
 };
 enum { BCROW_DIRECT = 0, BCROW_REFLECT = 1, BCROW_MIRROR = 2 };
 
@@ -300,6 +301,27 @@ void invalidate(pxg_ctx *ctx, int what);
 // array (pxg_assemble.hip); called by pxg_create with the device current
 int scratch_init(int device);
 
+// direct solve of the implicit system (pxg_solve.hip): block edge of the
+// block-skyline LU
+constexpr int kSolveBlock = 64;
+struct SolveState {
+    int ready = 0;                  // symbolic phase done for pattern `gen`
+    int factored = 0;               // numeric factors valid
+    long long gen = -1;
+    int use_rcm = 1;                // DefaultSolver "use_rcm"
+    double max_gib = 16.0;          // DefaultSolver "max_profile_gb" (<= 0: no guard)
+    int n = 0, nb = 0, amax = 0;    // equations, blocks, longest active set
+    long long ns = 0, nnz = 0;      // strip blocks per triangle, CSR entries
+    std::vector<int> hperm;         // new -> old (reference RCM)
+    std::vector<long long> hjp;     // reference skyline pointer (n + 1)
+    std::vector<int> hap, hfb;      // active-set offsets (nb + 1), first envelope block
+    dbuf<double> F, Li, Ui;         // block storage, inverted diagonal factors
+    dbuf<double> y, z, w;           // substitution vectors (nb * kSolveBlock)
+    dbuf<int> inv, fb, al, zpiv;
+    dbuf<long long> loc, lp;
+    dbuf<double> part, norms;       // Newton norms
+};
+
 // which Laplacian ctx->lap holds
 enum { LAP_NONE = 0, LAP_POISSON = 1, LAP_CH = 2 };
 
@@ -337,6 +359,7 @@ struct pxg_ctx {
     int csr_ndof = 0;
     int csr_maxblk = 0;             // most node blocks in one block row
     long long nnz = 0;
+    long long csr_gen = 0;          // bumped by every CSR pattern (re)build
     pxg::dbuf<int> rowptr, cols;
     pxg::dbuf<int> slot;            // nb: CSR index of (row i*ndof, col j*ndof)
     pxg::dbuf<int> diag;            // N: CSR index of (i*ndof, i*ndof)
@@ -355,6 +378,7 @@ struct pxg_ctx {
     pxg::dbuf<double> xlap, xmob, xvel;
     pxg::dbuf<unsigned char> xghost, xhealth;
     pxg::CsrBcProgram csrbc;        // BC rows of the implicit system
+    pxg::SolveState solve;          // direct solve of the implicit system
     pxg::DynFracState dyn;          // implicit dynamic fracture element
     pxg::StressChState sch;         // stress-coupled Cahn-Hilliard element
     pxg::FracSchState fsch;         // fracture stress-coupled Cahn-Hilliard element

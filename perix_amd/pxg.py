@@ -10,6 +10,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 FP64 = 0
 FP32C = 1
+ERR_SINGULAR = -7      # zero pivot in the direct solve
 _LIB = None
 
 D = C.POINTER(C.c_double)
@@ -90,7 +91,9 @@ SYMBOLS = [
     "pxg_assemble_diffusion", "pxg_assemble_diffusion_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
-    "pxg_csr_apply_bcs_dev", "pxg_dynfrac_setup", "pxg_dynfrac_commit", "pxg_dynfrac_commit_dev",
+    "pxg_csr_apply_bcs_dev", "pxg_solve_options", "pxg_solve_factor_dev", "pxg_solve_apply_dev",
+    "pxg_solve_dev", "pxg_solve", "pxg_solve_info", "pxg_solve_ordering", "pxg_csr_preset_dev",
+    "pxg_newton_update_dev", "pxg_dynfrac_setup", "pxg_dynfrac_commit", "pxg_dynfrac_commit_dev",
     "pxg_assemble_dynfrac", "pxg_assemble_dynfrac_dev", "pxg_dynfrac_get_state",
     "pxg_stressch_setup", "pxg_stressch_constants", "pxg_assemble_stressch", "pxg_assemble_stressch_dev",
     "pxg_fsch_setup", "pxg_fsch_constants", "pxg_fsch_commit", "pxg_fsch_commit_dev",
@@ -221,6 +224,15 @@ def load():
         L.pxg_frac_project_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.pxg_frac_state_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]
+        L.pxg_solve_options.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.pxg_solve_factor_dev.argtypes = [C.c_void_p, C.c_void_p]
+        L.pxg_solve_apply_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pxg_solve_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pxg_solve.argtypes = [C.c_void_p, D, D, D]
+        L.pxg_solve_info.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+        L.pxg_solve_ordering.argtypes = [C.c_void_p, I, C.POINTER(C.c_longlong)]
+        L.pxg_csr_preset_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.pxg_newton_update_dev.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, D]
         _LIB = L
     return _LIB
 
@@ -296,6 +308,7 @@ class Context:
         h = C.c_void_p()
         _check(L.pxg_create(C.byref(h), device))
         self.h = h
+        self.device = device
         self.n = 0
         self.dim = 0
         self.nb = 0
@@ -510,6 +523,47 @@ class Context:
 
     def csr_apply_bcs_dev(self, t, u_ptr, k_ptr, r_ptr):
         _check(load().pxg_csr_apply_bcs_dev(self.h, t, u_ptr, k_ptr, r_ptr))
+
+    # -- direct solve of the implicit system (DefaultSolver on the device) -------
+    def solve_options(self, use_rcm=True, max_profile_gib=16.0):
+        _check(load().pxg_solve_options(self.h, int(use_rcm), max_profile_gib))
+
+    def solve(self, k, r):
+        """K x = r for host values k (nnz, BC rows applied) and r (rows)."""
+        x = np.zeros(len(r))
+        _check(load().pxg_solve(self.h, _dp(np.ascontiguousarray(k, float)),
+                                _dp(np.ascontiguousarray(r, float)), _dp(x)))
+        return x
+
+    def solve_dev(self, k_ptr, r_ptr, x_ptr):
+        _check(load().pxg_solve_dev(self.h, k_ptr, r_ptr, x_ptr))
+
+    def solve_factor_dev(self, k_ptr):
+        _check(load().pxg_solve_factor_dev(self.h, k_ptr))
+
+    def solve_apply_dev(self, r_ptr, x_ptr):
+        _check(load().pxg_solve_apply_dev(self.h, r_ptr, x_ptr))
+
+    def solve_info(self):
+        out = np.zeros(8, np.int64)
+        _check(load().pxg_solve_info(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        keys = ("equations", "blocks", "block", "profile", "strip_blocks", "max_active", "bytes", "block_products")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def solve_ordering(self):
+        n = self.n * self.ndof
+        perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
+        _check(load().pxg_solve_ordering(self.h, _ip(perm), jp.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return perm, jp
+
+    def csr_preset_dev(self, t, u_ptr):
+        _check(load().pxg_csr_preset_dev(self.h, t, u_ptr))
+
+    def newton_update_dev(self, t, u_ptr, du_ptr, r_ptr):
+        """u += du and the presets at t when |du| is finite; returns (|r|, |du|)."""
+        nrm = np.zeros(2)
+        _check(load().pxg_newton_update_dev(self.h, t, u_ptr, du_ptr, r_ptr, _dp(nrm)))
+        return float(nrm[0]), float(nrm[1])
 
     # -- implicit dynamic fracture element ---------------------------------------
     def dynfrac_setup(self, params, uold0, v0=None):

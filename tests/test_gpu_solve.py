@@ -1,0 +1,192 @@
+"""GPU parity of the direct solve of the implicit system (pxg_solve*, the
+reference's DefaultSolver as a block-skyline LU on the device) and of the
+device-resident Newton drivers (perix_amd.implicit).
+
+The ordering and the skyline pointer must equal the oracle's (which equals
+the reference build bit for bit, tests/test_oracle_solve.py).  The factors are
+the same mathematically but summed in another order, so the solution is held
+to the oracle's within 1e-9 relative (the Poisson matrices have condition
+numbers near 1e5) and its residual to 1e-12 of |rhs|.  Whole jobs are held to
+the reference's own driver (PDProblem::run with its default LDU solver, run
+through oracle.refpy in a child process): same Newton iteration counts, final
+state within 1e-9 relative."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from perix_amd import implicit, problems, pxg
+
+pytestmark = pytest.mark.gpu
+
+
+def _poisson(nx):
+    ctx, cloud, delta, f = problems.poisson_square(nx=nx, ny=nx)
+    u = np.zeros(cloud.n)
+    k, r = ctx.assemble_poisson(1.0, u, f_node=f)
+    k, r = ctx.csr_apply_bcs(0.0, u, k, r)
+    return ctx, k, r
+
+
+def _ch(nx):
+    ctx, cloud, delta, u = problems.cahnhilliard_square(nx=nx, ny=nx)
+    k, r = ctx.assemble_ch(1e-2, 3.0, 1.0, 1e-3, u, u)
+    k, r = ctx.csr_apply_bcs(1e-3, u, k, r)
+    return ctx, k, r
+
+
+@pytest.mark.parametrize("kind,nx", [("poisson", 30), ("poisson", 100), ("ch", 24), ("ch", 64)])
+def test_solve_matches_oracle(oracle, kind, nx):
+    print("solve parity %s %d" % (kind, nx), flush=True)
+    ctx, k, r = _poisson(nx) if kind == "poisson" else _ch(nx)
+    rp, cols = ctx.csr()
+    perm, jp = ctx.solve_ordering()
+    orc, ox, operm, ojp = oracle.ldu_solve(rp, cols, k, r)
+    assert orc == 0
+    assert np.array_equal(perm, operm) and np.array_equal(jp, ojp)
+    x = ctx.solve(k, r)
+    info = ctx.solve_info()
+    A = sp.csr_matrix((k, cols, rp), shape=(len(r), len(r)))
+    err = np.abs(x - ox).max() / np.abs(ox).max()
+    res = np.linalg.norm(A @ x - r) / np.linalg.norm(r)
+    print("  %s: vs oracle %.2e, residual %.2e" % (info, err, res))
+    assert err <= 1e-9 and res <= 1e-12
+    # a second solve on new values reuses the symbolic phase
+    k2 = k * 1.5
+    x2 = ctx.solve(k2, r)
+    assert np.abs(x2 - x / 1.5).max() <= 1e-9 * np.abs(x).max()
+    ctx.close()
+
+
+def test_solve_device_pointers_and_refactor():
+    """pxg_solve_factor_dev once, pxg_solve_apply_dev on two right-hand sides;
+    then a refactorisation after the values change."""
+    ctx, k, r = _poisson(40)
+    dev = torch.device("cuda", 0)
+    kd = torch.tensor(k, device=dev)
+    rd = torch.tensor(np.stack([r, np.roll(r, 7)]), device=dev)
+    xd = torch.zeros_like(rd)
+    torch.cuda.synchronize()
+    ctx.solve_factor_dev(kd.data_ptr())
+    ctx.solve_apply_dev(rd[0].data_ptr(), xd[0].data_ptr())
+    ctx.solve_apply_dev(rd[1].data_ptr(), xd[1].data_ptr())
+    ctx.sync()
+    x = xd.cpu().numpy()
+    assert np.array_equal(x[0], ctx.solve(k, r))
+    assert np.array_equal(x[1], ctx.solve(k, np.roll(r, 7)))
+    ctx.close()
+
+
+def test_solve_options_and_errors(oracle):
+    ctx, k, r = _poisson(24)
+    rp, cols = ctx.csr()
+    # identity ordering ("use_rcm": false)
+    ctx.solve_options(use_rcm=False)
+    perm, jp = ctx.solve_ordering()
+    assert np.array_equal(perm, np.arange(len(r)))
+    orc, ox, operm, ojp = oracle.ldu_solve(rp, cols, k, r, False)
+    assert np.array_equal(jp, ojp)
+    x = ctx.solve(k, r)
+    assert np.abs(x - ox).max() <= 1e-9 * np.abs(ox).max()
+    # the storage guard (DefaultSolver.cpp:196-238)
+    ctx.solve_options(use_rcm=True, max_profile_gib=1e-6)
+    with pytest.raises(pxg.PxgError) as e:
+        ctx.solve(k, r)
+    assert e.value.code == -1 and "max_profile_gb" in str(e.value)
+    ctx.solve_options()
+    # an empty row and column: an exact zero pivot, at the oracle's equation
+    e_old = 300
+    kz = k.copy()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    kz[(rows == e_old) | (cols == e_old)] = 0.0
+    orc, _, operm, _ = oracle.ldu_solve(rp, cols, kz, r)
+    with pytest.raises(pxg.PxgError) as e:
+        ctx.solve(kz, r)
+    assert e.value.code == pxg.ERR_SINGULAR
+    assert ("equation %d " % orc) in str(e.value)
+    # the context still solves afterwards
+    assert np.abs(ctx.solve(k, r) - x).max() <= 1e-9 * np.abs(x).max()
+    ctx.close()
+
+
+def test_newton_update_and_presets(oracle):
+    """pxg_newton_update_dev: norms, u += du, the presets; a non-finite du
+    leaves u untouched."""
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=20, ny=20)
+    rs = np.random.RandomState(3)
+    n = len(u0)
+    u = rs.standard_normal(n)
+    du = rs.standard_normal(n)
+    r = rs.standard_normal(n)
+    dev = torch.device("cuda", 0)
+    ud, dud, rd = (torch.tensor(a, device=dev) for a in (u, du, r))
+    torch.cuda.synchronize()
+    rn, dn = ctx.newton_update_dev(0.5, ud.data_ptr(), dud.data_ptr(), rd.data_ptr())
+    ctx.sync()
+    want = u + du
+    m = oracle.Mesh(oracle.make_grid(2, 20, 20))
+    bcs = oracle.make_bcs([{"type": "neumann", "nodes": m.group(oracle.WALLS[w]), "comps": [0, 1]}
+                           for w in ("left", "right", "bottom", "top")])
+    oracle.bc_preset(m, 2, bcs, want, 0.5)
+    assert np.array_equal(ud.cpu().numpy(), want)
+    assert math.isclose(rn, np.linalg.norm(r), rel_tol=1e-13)
+    assert math.isclose(dn, np.linalg.norm(du), rel_tol=1e-13)
+    du[5] = np.nan
+    dud = torch.tensor(du, device=dev)
+    torch.cuda.synchronize()
+    rn, dn = ctx.newton_update_dev(0.5, ud.data_ptr(), dud.data_ptr(), rd.data_ptr())
+    ctx.sync()
+    assert not math.isfinite(dn)
+    assert np.array_equal(ud.cpu().numpy(), want)
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def refpy():
+    from oracle import refpy as r
+    if not r.available():
+        pytest.fail("oracle/_ref/libperix_ref.so is not built (make -C oracle/ref)")
+    return r
+
+
+def test_static_poisson_matches_reference_driver(refpy):
+    """Config 1 (Poisson 100 x 100, Dirichlet ghost walls, f = 2 pi^2) as the
+    reference's static job with its default LDU solver, and on the device."""
+    print("static poisson 100", flush=True)
+    nx = 100
+    U_ref, rec = refpy.run_job(refpy.poisson_deck(nx, "openmp"), timeout=300)
+    ctx, cloud, delta, f = problems.poisson_square(nx=nx, ny=nx)
+    dev = torch.device("cuda", 0)
+    u = torch.zeros(cloud.n, dtype=torch.float64, device=dev)
+    f0 = 2.0 * math.pi ** 2
+    res = implicit.static_solve(ctx, lambda up, uop, kp, rp: ctx.assemble_poisson_dev(1.0, up, kp, rp, f_const=f0), u)
+    U = u.cpu().numpy()
+    err = np.abs(U - U_ref).max() / np.abs(U_ref).max()
+    print("  device newton %s; reference iterations %d; rel %.2e" % (res, rec["newton_iterations"], err))
+    assert res.converged and res.iters == rec["newton_iterations"]
+    assert err <= 1e-9
+    ctx.close()
+
+
+def test_transient_ch_matches_reference_driver(refpy):
+    """Config 3's loop on a 48 x 48 square for 5 backward-Euler steps (Newton
+    max 20, abs 1e-9, rel 1e-8, Neumann ghosts, RandomIC seed 12345)."""
+    import test_gpu_seam as seam
+    print("transient ch 48", flush=True)
+    nx, steps = 48, 5
+    U_ref, rec = refpy.run_job(seam.ch_deck(refpy, nx, steps, "openmp"), timeout=600)
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=nx, ny=nx)
+    dev = torch.device("cuda", 0)
+    u = torch.tensor(u0, device=dev)
+    torch.cuda.synchronize()
+    iters = implicit.transient(
+        ctx, lambda dt, up, uop, kp, rp: ctx.assemble_ch_dev(1e-2, 3.0, 1.0, dt, up, uop, kp, rp),
+        u, 1e-3, steps * 1e-3, max_iters=20, abs_tol=1e-9, rel_tol=1e-8)
+    U = u.cpu().numpy()
+    err = np.abs(U - U_ref).max() / np.abs(U_ref).max()
+    print("  device iterations %s (total %d), reference %d; rel %.2e" % (iters, sum(iters), rec["newton_iterations"], err))
+    assert sum(iters) == rec["newton_iterations"]
+    assert err <= 1e-9
+    ctx.close()
