@@ -290,7 +290,9 @@ int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *k
  * the host once per CSR pattern (again after pxg_build_csr / pxg_set_csr).
  * pxg_solve_options mirrors the solver's "use_rcm" and "max_profile_gb"
  * parameters (defaults 1 and 16; a profile above the limit is PXG_ERR_ARG, as
- * the reference refuses it, :196-238; <= 0 disables the guard).
+ * the reference refuses it, :196-238; <= 0 disables the guard); flags
+ * PXG_SOLVE_NO_GRAPHS launches the per-block kernel chains one by one instead
+ * of replaying them as HIP graphs captured once per pattern.
  * pxg_solve_factor_dev factors the device CSR values kvals (the context's
  * pattern, BC rows already applied); an exactly zero pivot is
  * PXG_ERR_SINGULAR naming the equation (:362-367).  pxg_solve_apply_dev
@@ -302,7 +304,8 @@ int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *k
  * storage, block products per factorisation}.  pxg_solve_ordering copies the
  * ordering (perm[n]: new -> old) and the reference skyline pointer
  * (jp[n+1]); either may be NULL. */
-int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib);
+#define PXG_SOLVE_NO_GRAPHS 1
+int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib, int flags);
 int pxg_solve_factor_dev(pxg_ctx *ctx, const double *kvals_dev);
 int pxg_solve_apply_dev(pxg_ctx *ctx, const double *rhs_dev, double *x_dev);
 int pxg_solve_dev(pxg_ctx *ctx, const double *kvals_dev, const double *rhs_dev, double *x_dev);

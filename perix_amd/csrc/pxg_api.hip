@@ -70,6 +70,7 @@ extern "C" void pxg_destroy(pxg_ctx *c)
     rel(c->csrbc.kind); rel(c->csrbc.val0); rel(c->csrbc.vel);
     {
         pxg::SolveState &S = c->solve;
+        S.drop_graphs();
         rel(S.F); rel(S.Li); rel(S.Ui); rel(S.y); rel(S.z); rel(S.w); rel(S.inv); rel(S.fb); rel(S.al);
         rel(S.zpiv); rel(S.loc); rel(S.lp); rel(S.part); rel(S.norms);
     }

@@ -320,6 +320,15 @@ struct SolveState {
     dbuf<int> inv, fb, al, zpiv;
     dbuf<long long> loc, lp;
     dbuf<double> part, norms;       // Newton norms
+    int use_graphs = 1;             // replay the per-step launch chains as HIP graphs
+    hipGraphExec_t gfac = nullptr;  // factor steps of the current pattern
+    hipGraphExec_t gsub = nullptr;  // forward and backward sweeps
+    void drop_graphs()
+    {
+        if (gfac) (void)hipGraphExecDestroy(gfac);
+        if (gsub) (void)hipGraphExecDestroy(gsub);
+        gfac = gsub = nullptr;
+    }
 };
 
 // which Laplacian ctx->lap holds

@@ -42,7 +42,6 @@ namespace {
 constexpr int SB = kSolveBlock;
 constexpr int SB2 = SB * SB;
 constexpr int kThreads = 256;
-constexpr int LS = SB + 1;            // LDS row stride of the diagonal factor (bank spread)
 constexpr int TS = SB + 2;            // LDS row stride of GEMM tiles (16-byte aligned rows)
 
 __global__ __launch_bounds__(256) void k_fill(double *p, size_t count, double v)
@@ -69,117 +68,210 @@ __global__ __launch_bounds__(256) void k_scatter(long long nnz, const double *__
     if (e < npad) D[(e / SB) * SB2 + (e % SB) * (SB + 1)] = 1.0;
 }
 
-// (1) D_K = L U in place (unit L below the diagonal, U on and above), then
-// L^-1 (unit lower) and U^-1 (upper).  A zero pivot records the least failing
-// equation (new order) in *zpiv and continues with 1 so that the arithmetic
-// stays finite; the host reports the failure after the factorisation.
+// (1) D_K = L U in place (unit L below the diagonal, U on and above), with
+// L^-1 (unit lower) and U^-1 (upper) formed in the same pivot walk.  Thread t
+// keeps a 4 x 4 tile (rows 4 (t >> 4).., columns 4 (t & 15)..) of the block
+// and of both inverses in registers.  Pivot k: the owners of row k and of
+// column k publish them to LDS (double-buffered, so one barrier per pivot),
+// then every thread applies
+//   A: l_i = a_ik / a_kk (stored as L), a_ij -= l_i a_kj  (i, j > k);
+//   L^-1 by row operations: X[i][:] -= l_i X[k][:]  (i > k; row k is final);
+//   U^-1 by column operations: Y[:][k] /= a_kk, Y[:][j] -= Y[:][k] a_kj (j > k).
+// A zero pivot records the least failing equation (new order) in *zpiv and
+// continues with 1, so the arithmetic stays finite; the host reports it.
 __global__ __launch_bounds__(256) void k_diag(int K, double *__restrict__ Dall, double *__restrict__ Liall,
                                               double *__restrict__ Uiall, int *zpiv)
 {
-    __shared__ double a[SB * LS];
-    __shared__ double x[SB * LS];     // L^-1
-    __shared__ double z[SB * LS];     // U'^-1 of the unit upper U' = diag(U)^-1 U
-    const int t = (int)threadIdx.x;
+    __shared__ __align__(16) double rowA[2][SB], colA[2][SB], rowX[2][SB], colY[2][SB];
+    const int t = (int)threadIdx.x, tr = t >> 4, tc = t & 15, r0 = 4 * tr, c0 = 4 * tc;
     double *D = Dall + (size_t)K * SB2;
-    for (int e = t; e < SB2; e += kThreads) {
-        const int r = e / SB, c = e % SB;
-        a[r * LS + c] = D[e];
-        x[r * LS + c] = (r == c) ? 1.0 : 0.0;
-        z[r * LS + c] = (r == c) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    for (int k = 0; k < SB; k++) {
-        double piv = a[k * LS + k];
-        if (piv == 0.0) {
-            if (t == 0) atomicMin(zpiv, K * SB + k);
-            piv = 1.0;
+    double a[4][4], x[4][4], y[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double2 v01 = *reinterpret_cast<const double2 *>(D + (r0 + i) * SB + c0);
+        const double2 v23 = *reinterpret_cast<const double2 *>(D + (r0 + i) * SB + c0 + 2);
+        a[i][0] = v01.x; a[i][1] = v01.y; a[i][2] = v23.x; a[i][3] = v23.y;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double e = (r0 + i == c0 + j) ? 1.0 : 0.0;
+            x[i][j] = e;
+            y[i][j] = e;
         }
-        if (t > k && t < SB) a[t * LS + k] /= piv;
-        __syncthreads();
-        // rank-1 update of the trailing (SB-1-k)^2 entries
-        const int m = SB - 1 - k;
-        for (int e = t; e < m * m; e += kThreads) {
-            const int r = k + 1 + e / m, c = k + 1 + e % m;
-            a[r * LS + c] -= a[r * LS + k] * a[k * LS + c];
-        }
-        __syncthreads();
     }
-    // a zero pivot continues as 1 (see above): the inverses use the same value
-    // L^-1: row operations X[i][:] -= L[i][k] X[k][:] for i > k (X[k] is
-    // final after step k-1); U'^-1: Z[i][:] -= (U[i][k]/U[i][i]) Z[k][:] for
-    // i < k, k from the bottom.  The two walks share the barriers.
-    for (int s = 0; s < SB; s++) {
-        const int kl = s, ku = SB - 1 - s;
-        for (int e = t; e < SB * SB; e += kThreads) {
-            const int i = e / SB, c = e % SB;
-            if (i > kl && c <= kl) x[i * LS + c] -= a[i * LS + kl] * x[kl * LS + c];
-            if (i < ku && c >= ku) {
-                double d = a[i * LS + i];
-                if (d == 0.0) d = 1.0;
-                z[i * LS + c] -= (a[i * LS + ku] / d) * z[ku * LS + c];
+    for (int kk = 0; kk < SB; kk += 4) {
+        const bool own_row = tr == (kk >> 2), own_col = tc == (kk >> 2);
+#pragma unroll
+        for (int kr = 0; kr < 4; kr++) {
+            const int k = kk + kr, b = k & 1;
+            if (own_row) {
+                *reinterpret_cast<double2 *>(&rowA[b][c0]) = make_double2(a[kr][0], a[kr][1]);
+                *reinterpret_cast<double2 *>(&rowA[b][c0 + 2]) = make_double2(a[kr][2], a[kr][3]);
+                *reinterpret_cast<double2 *>(&rowX[b][c0]) = make_double2(x[kr][0], x[kr][1]);
+                *reinterpret_cast<double2 *>(&rowX[b][c0 + 2]) = make_double2(x[kr][2], x[kr][3]);
+            }
+            if (own_col) {
+                *reinterpret_cast<double2 *>(&colA[b][r0]) = make_double2(a[0][kr], a[1][kr]);
+                *reinterpret_cast<double2 *>(&colA[b][r0 + 2]) = make_double2(a[2][kr], a[3][kr]);
+                *reinterpret_cast<double2 *>(&colY[b][r0]) = make_double2(y[0][kr], y[1][kr]);
+                *reinterpret_cast<double2 *>(&colY[b][r0 + 2]) = make_double2(y[2][kr], y[3][kr]);
+            }
+            __syncthreads();
+            double p = rowA[b][k];
+            if (p == 0.0) {
+                if (t == 0) atomicMin(zpiv, K * SB + k);
+                p = 1.0;
+            }
+            // 1/p: hardware estimate and two Newton steps (full double accuracy)
+            double rinv = __builtin_amdgcn_rcp(p);
+            rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
+            rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
+            const double2 ca01 = *reinterpret_cast<const double2 *>(&colA[b][r0]);
+            const double2 ca23 = *reinterpret_cast<const double2 *>(&colA[b][r0 + 2]);
+            const double2 cy01 = *reinterpret_cast<const double2 *>(&colY[b][r0]);
+            const double2 cy23 = *reinterpret_cast<const double2 *>(&colY[b][r0 + 2]);
+            const double2 ra01 = *reinterpret_cast<const double2 *>(&rowA[b][c0]);
+            const double2 ra23 = *reinterpret_cast<const double2 *>(&rowA[b][c0 + 2]);
+            const double2 rx01 = *reinterpret_cast<const double2 *>(&rowX[b][c0]);
+            const double2 rx23 = *reinterpret_cast<const double2 *>(&rowX[b][c0 + 2]);
+            // multipliers masked to zero outside the update ranges, so every
+            // entry takes the same (exact no-op where masked) fused update
+            const double cl[4] = {ca01.x, ca01.y, ca23.x, ca23.y};
+            const double cyv[4] = {cy01.x, cy01.y, cy23.x, cy23.y};
+            const double rav[4] = {ra01.x, ra01.y, ra23.x, ra23.y};
+            const double rx[4] = {rx01.x, rx01.y, rx23.x, rx23.y};
+            double l[4], lm[4], yk[4], ra[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                l[i] = cl[i] * rinv;
+                lm[i] = (r0 + i > k) ? l[i] : 0.0;
+                yk[i] = cyv[i] * rinv;
+                ra[i] = (c0 + i > k) ? rav[i] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    a[i][j] = fma(-lm[i], ra[j], a[i][j]);
+                    x[i][j] = fma(-lm[i], rx[j], x[i][j]);
+                    y[i][j] = fma(-yk[i], ra[j], y[i][j]);
+                }
+            }
+            if (own_col) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    if (r0 + i > k) a[i][kr] = l[i];
+                    y[i][kr] = yk[i];
+                }
             }
         }
-        __syncthreads();
     }
     double *Li = Liall + (size_t)K * SB2, *Ui = Uiall + (size_t)K * SB2;
-    for (int e = t; e < SB2; e += kThreads) {
-        const int r = e / SB, c = e % SB;
-        double d = a[c * LS + c];
-        if (d == 0.0) d = 1.0;
-        D[e] = a[r * LS + c];
-        Li[e] = x[r * LS + c];
-        Ui[e] = z[r * LS + c] / d;             // U^-1 = U'^-1 diag(U)^-1
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const size_t o = (size_t)(r0 + i) * SB + c0;
+        *reinterpret_cast<double2 *>(D + o) = make_double2(a[i][0], a[i][1]);
+        *reinterpret_cast<double2 *>(D + o + 2) = make_double2(a[i][2], a[i][3]);
+        *reinterpret_cast<double2 *>(Li + o) = make_double2(x[i][0], x[i][1]);
+        *reinterpret_cast<double2 *>(Li + o + 2) = make_double2(x[i][2], x[i][3]);
+        *reinterpret_cast<double2 *>(Ui + o) = make_double2(y[i][0], y[i][1]);
+        *reinterpret_cast<double2 *>(Ui + o + 2) = make_double2(y[i][2], y[i][3]);
     }
 }
 
-// C = A B (SUB false) or C -= A B (SUB true) for SB x SB row-major blocks;
-// C may alias A or B (both are read into LDS before the barrier, C written
-// after it).  256 threads, each a 4 x 4 tile of C.
+// Strip products of SB x SB row-major blocks, one workgroup per 16-wide
+// strip of the output (1024 entries, 4 per thread), every global load issued
+// before the first use.  C may alias the operand whose strip is read: the
+// workgroup reads only the strip it writes from that operand.
+constexpr int STRIP = 16;
+constexpr int NSTRIP = SB / STRIP;
+
+// rows [s0, s0 + 16) of C (op)= A B: thread t owns row s0 + t / 16, columns
+// 4 (t % 16)..; A's strip and all of B go to LDS
 template <bool SUB>
-__device__ __forceinline__ void block_gemm(const double *A, const double *B, double *C, double *At, double *Bs)
+__device__ __forceinline__ void row_strip_gemm(const double *A, const double *B, double *C, int s0, double *As,
+                                               double *Bs)
 {
-    const int t = (int)threadIdx.x;
-    for (int e = t; e < SB2 / 2; e += kThreads) {
-        const int r = (2 * e) / SB, c = (2 * e) % SB;
-        const double2 va = *reinterpret_cast<const double2 *>(A + 2 * e);
-        const double2 vb = *reinterpret_cast<const double2 *>(B + 2 * e);
-        At[c * TS + r] = va.x;
-        At[(c + 1) * TS + r] = va.y;
-        *reinterpret_cast<double2 *>(Bs + r * TS + c) = vb;
+    const int t = (int)threadIdx.x, r = t >> 4, c0 = 4 * (t & 15);
+    double2 va[2], vb[8], vc[2];
+#pragma unroll
+    for (int m = 0; m < 2; m++) va[m] = *reinterpret_cast<const double2 *>(A + (size_t)s0 * SB + 2 * (m * kThreads + t));
+#pragma unroll
+    for (int m = 0; m < 8; m++) vb[m] = *reinterpret_cast<const double2 *>(B + 2 * (m * kThreads + t));
+    if (SUB) {
+        vc[0] = *reinterpret_cast<const double2 *>(C + (size_t)(s0 + r) * SB + c0);
+        vc[1] = *reinterpret_cast<const double2 *>(C + (size_t)(s0 + r) * SB + c0 + 2);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const int e = 2 * (m * kThreads + t);
+        *reinterpret_cast<double2 *>(As + (e / SB) * TS + e % SB) = va[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int e = 2 * (m * kThreads + t);
+        *reinterpret_cast<double2 *>(Bs + (e / SB) * TS + e % SB) = vb[m];
     }
     __syncthreads();
-    const int r0 = (t / 16) * 4, c0 = (t % 16) * 4;
-    double acc[4][4];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int k = 0; k < SB; k++) {
+        const double av = As[r * TS + k];
+        const double2 b01 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0);
+        const double2 b23 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0 + 2);
+        acc[0] = fma(av, b01.x, acc[0]);
+        acc[1] = fma(av, b01.y, acc[1]);
+        acc[2] = fma(av, b23.x, acc[2]);
+        acc[3] = fma(av, b23.y, acc[3]);
+    }
+    double *row = C + (size_t)(s0 + r) * SB + c0;
+    if (SUB) {
+        *reinterpret_cast<double2 *>(row) = make_double2(vc[0].x - acc[0], vc[0].y - acc[1]);
+        *reinterpret_cast<double2 *>(row + 2) = make_double2(vc[1].x - acc[2], vc[1].y - acc[3]);
+    } else {
+        *reinterpret_cast<double2 *>(row) = make_double2(acc[0], acc[1]);
+        *reinterpret_cast<double2 *>(row + 2) = make_double2(acc[2], acc[3]);
+    }
+}
+
+// columns [s0, s0 + 16) of C = A B: thread t owns column s0 + t % 16, rows
+// 4 (t / 16)..; all of A (transposed) and B's strip go to LDS
+__device__ __forceinline__ void col_strip_gemm(const double *A, const double *B, double *C, int s0, double *At,
+                                               double *Bs)
+{
+    const int t = (int)threadIdx.x, c = t & 15, r0 = 4 * (t >> 4);
+    double2 va[8], vb[2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int m = 0; m < 8; m++) va[m] = *reinterpret_cast<const double2 *>(A + 2 * (m * kThreads + t));
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int m = 0; m < 2; m++) {
+        const int e = m * kThreads + t, k = e / (STRIP / 2), cp = e % (STRIP / 2);
+        vb[m] = *reinterpret_cast<const double2 *>(B + (size_t)k * SB + s0 + 2 * cp);
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int e = 2 * (m * kThreads + t), rr = e / SB, kk = e % SB;
+        At[kk * TS + rr] = va[m].x;
+        At[(kk + 1) * TS + rr] = va[m].y;
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const int e = m * kThreads + t, k = e / (STRIP / 2), cp = e % (STRIP / 2);
+        *reinterpret_cast<double2 *>(Bs + k * (STRIP + 2) + 2 * cp) = vb[m];
+    }
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
     for (int k = 0; k < SB; k++) {
         const double2 a01 = *reinterpret_cast<const double2 *>(At + k * TS + r0);
         const double2 a23 = *reinterpret_cast<const double2 *>(At + k * TS + r0 + 2);
-        const double2 b01 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0);
-        const double2 b23 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0 + 2);
-        const double av[4] = {a01.x, a01.y, a23.x, a23.y};
-        const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        const double bv = Bs[k * (STRIP + 2) + c];
+        acc[0] = fma(a01.x, bv, acc[0]);
+        acc[1] = fma(a01.y, bv, acc[1]);
+        acc[2] = fma(a23.x, bv, acc[2]);
+        acc[3] = fma(a23.y, bv, acc[3]);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        double *row = C + (r0 + i) * SB + c0;
-        if (SUB) {
-            const double2 o01 = *reinterpret_cast<const double2 *>(row);
-            const double2 o23 = *reinterpret_cast<const double2 *>(row + 2);
-            *reinterpret_cast<double2 *>(row) = make_double2(o01.x - acc[i][0], o01.y - acc[i][1]);
-            *reinterpret_cast<double2 *>(row + 2) = make_double2(o23.x - acc[i][2], o23.y - acc[i][3]);
-        } else {
-            *reinterpret_cast<double2 *>(row) = make_double2(acc[i][0], acc[i][1]);
-            *reinterpret_cast<double2 *>(row + 2) = make_double2(acc[i][2], acc[i][3]);
-        }
-    }
+    for (int i = 0; i < 4; i++) C[(size_t)(r0 + i) * SB + s0 + c] = acc[i];
 }
 
 struct Layout {
@@ -192,89 +284,115 @@ struct Layout {
     __device__ double *D(int I) const { return F + (size_t)(2 * ns + I) * SB2; }
 };
 
-// (2) workgroup w < na: L_IK = A_IK U_KK^-1; else U_KJ = L_KK^-1 A_KJ
+// (2) workgroup w: block w / NSTRIP of the panel list, strip w % NSTRIP.
+// First na blocks: L_IK = A_IK U_KK^-1 by row strips; next na: U_KJ =
+// L_KK^-1 A_KJ by column strips (each in place, see the strip products)
 __global__ __launch_bounds__(256) void k_panel(int K, const int *__restrict__ act, int na, Layout Y,
                                                const double *__restrict__ Li, const double *__restrict__ Ui)
 {
-    __shared__ __align__(16) double At[SB * TS];
+    __shared__ __align__(16) double As[SB * TS];
     __shared__ __align__(16) double Bs[SB * TS];
-    const int w = (int)blockIdx.x;
-    if (w < na) {
-        double *c = Y.L(act[w], K);
-        block_gemm<false>(c, Ui + (size_t)K * SB2, c, At, Bs);
+    const int blk = (int)blockIdx.x / NSTRIP, s0 = STRIP * ((int)blockIdx.x % NSTRIP);
+    if (blk < na) {
+        double *c = Y.L(act[blk], K);
+        row_strip_gemm<false>(c, Ui + (size_t)K * SB2, c, s0, As, Bs);
     } else {
-        double *c = Y.U(K, act[w - na]);
-        block_gemm<false>(Li + (size_t)K * SB2, c, c, At, Bs);
+        double *c = Y.U(K, act[blk - na]);
+        col_strip_gemm(Li + (size_t)K * SB2, c, c, s0, As, Bs);
     }
 }
 
-// (3) A_IJ -= L_IK U_KJ for (I, J) in active x active
+// (3) A_IJ -= L_IK U_KJ for (I, J) in active x active, one row strip per workgroup
 __global__ __launch_bounds__(256) void k_update(int K, const int *__restrict__ act, int na, Layout Y)
 {
-    __shared__ __align__(16) double At[SB * TS];
+    __shared__ __align__(16) double As[STRIP * TS];
     __shared__ __align__(16) double Bs[SB * TS];
-    const int w = (int)blockIdx.x;
-    const int I = act[w / na], J = act[w % na];
+    const int blk = (int)blockIdx.x / NSTRIP, s0 = STRIP * ((int)blockIdx.x % NSTRIP);
+    const int I = act[blk / na], J = act[blk % na];
     double *c = (I > J) ? Y.L(I, J) : (I < J) ? Y.U(I, J) : Y.D(I);
-    block_gemm<true>(Y.L(I, K), Y.U(K, J), c, At, Bs);
+    row_strip_gemm<true>(Y.L(I, K), Y.U(K, J), c, s0, As, Bs);
 }
 
-// v = M y_K for one SB x SB block M (row-major) into LDS v[SB]; 4 threads per row
-__device__ __forceinline__ void block_gemv(const double *__restrict__ M, const double *__restrict__ yk, double *ys,
-                                           double *v)
+// Two SB x SB blocks into LDS (row stride TS) with every load issued up
+// front (a caller with one block passes it twice).  256 threads, 8 double2
+// each per block.
+__device__ __forceinline__ void stage_two(const double *M, const double *N, double *Ms, double *Ns)
 {
     const int t = (int)threadIdx.x;
-    if (t < SB) ys[t] = yk[t];
-    __syncthreads();
-    const int r = t / 4, q = t % 4;
+    double2 vm[8], vn[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        vm[m] = *reinterpret_cast<const double2 *>(M + 2 * (m * kThreads + t));
+        vn[m] = *reinterpret_cast<const double2 *>(N + 2 * (m * kThreads + t));
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int e = 2 * (m * kThreads + t), r = e / SB, c = e % SB;
+        *reinterpret_cast<double2 *>(Ms + r * TS + c) = vm[m];
+        *reinterpret_cast<double2 *>(Ns + r * TS + c) = vn[m];
+    }
+}
+
+// row r = t / 4 of M times v (LDS, SB values): each of the row's 4 threads
+// sums 16 columns, the quad reduces with lane exchanges
+__device__ __forceinline__ double row_dot(const double *Ms, const double *v)
+{
+    const int t = (int)threadIdx.x, r = t >> 2, q = t & 3;
     double s = 0.0;
-    for (int c = q; c < SB; c += 4) s = fma(M[r * SB + c], ys[c], s);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int c = 2 * q + 8 * m;
+        const double2 mv = *reinterpret_cast<const double2 *>(Ms + r * TS + c);
+        s = fma(mv.x, v[c], s);
+        s = fma(mv.y, v[c + 1], s);
+    }
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    if (q == 0) v[r] = s;
-    __syncthreads();
+    return s;
 }
 
 // forward step K: z_K = L_KK^-1 y_K; y_I -= L_IK z_K for the active I
 __global__ __launch_bounds__(256) void k_fwd(int K, const int *__restrict__ act, int na, Layout Y,
                                              const double *__restrict__ Li, double *y, double *zout)
 {
-    __shared__ double ys[SB], v[SB], u[SB];
-    block_gemv(Li + (size_t)K * SB2, y + (size_t)K * SB, ys, v);
+    __shared__ __align__(16) double Ms[SB * TS], Ns[SB * TS];
+    __shared__ double ys[SB], v[SB];
     const int w = (int)blockIdx.x, t = (int)threadIdx.x;
-    if (w == 0 && t < SB) zout[(size_t)K * SB + t] = v[t];
-    if (w >= na) return;
-    const int I = act[w];
-    const double *L = Y.L(I, K);
-    const int r = t / 4, q = t % 4;
-    double s = 0.0;
-    for (int c = q; c < SB; c += 4) s = fma(L[r * SB + c], v[c], s);
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (q == 0) u[r] = s;
+    const bool two = w < na;
+    const int I = two ? act[w] : 0;
+    const double *Lk = Li + (size_t)K * SB2;
+    stage_two(Lk, two ? Y.L(I, K) : Lk, Ms, Ns);
+    if (t < SB) ys[t] = y[(size_t)K * SB + t];
     __syncthreads();
-    if (t < SB) y[(size_t)I * SB + t] -= u[t];
+    const double s = row_dot(Ms, ys);
+    if ((t & 3) == 0) v[t >> 2] = s;
+    __syncthreads();
+    if (w == 0 && t < SB) zout[(size_t)K * SB + t] = v[t];
+    if (!two) return;
+    const double u = row_dot(Ns, v);
+    if ((t & 3) == 0) y[(size_t)I * SB + (t >> 2)] -= u;
 }
 
 // backward step K: x_K = U_KK^-1 z_K; z_I -= U_IK x_K for I in [FB[K], K)
 __global__ __launch_bounds__(256) void k_bwd(int K, Layout Y, const double *__restrict__ Ui, double *z,
                                              double *xout)
 {
-    __shared__ double ys[SB], v[SB], u[SB];
-    block_gemv(Ui + (size_t)K * SB2, z + (size_t)K * SB, ys, v);
+    __shared__ __align__(16) double Ms[SB * TS], Ns[SB * TS];
+    __shared__ double zs[SB], v[SB];
     const int w = (int)blockIdx.x, t = (int)threadIdx.x;
-    if (w == 0 && t < SB) xout[(size_t)K * SB + t] = v[t];
     const int I = Y.fb[K] + w;
-    if (I >= K) return;
-    const double *U = Y.U(I, K);
-    const int r = t / 4, q = t % 4;
-    double s = 0.0;
-    for (int c = q; c < SB; c += 4) s = fma(U[r * SB + c], v[c], s);
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (q == 0) u[r] = s;
+    const bool two = I < K;
+    const double *Uk = Ui + (size_t)K * SB2;
+    stage_two(Uk, two ? Y.U(I, K) : Uk, Ms, Ns);
+    if (t < SB) zs[t] = z[(size_t)K * SB + t];
     __syncthreads();
-    if (t < SB) z[(size_t)I * SB + t] -= u[t];
+    const double s = row_dot(Ms, zs);
+    if ((t & 3) == 0) v[t >> 2] = s;
+    __syncthreads();
+    if (w == 0 && t < SB) xout[(size_t)K * SB + t] = v[t];
+    if (!two) return;
+    const double u = row_dot(Ns, v);
+    if ((t & 3) == 0) z[(size_t)I * SB + (t >> 2)] -= u;
 }
 
 __global__ __launch_bounds__(256) void k_perm_in(int n, int npad, const int *__restrict__ inv,
@@ -419,6 +537,7 @@ int symbolic(pxg_ctx *ctx)
 {
     SolveState &S = ctx->solve;
     S.ready = 0;
+    S.drop_graphs();
     hipStream_t st = ctx->stream;
     const int n = ctx->n * ctx->csr_ndof;
     const long long nnz = ctx->nnz;
@@ -533,6 +652,63 @@ Layout layout(SolveState &S)
     return Layout{S.F.p, S.ns, S.fb.p, S.lp.p};
 }
 
+// the per-step launch chains depend only on the pattern and the solver's own
+// buffers: enqueued directly, or captured once into a HIP graph and replayed
+int enqueue_factor_steps(SolveState &S, hipStream_t st)
+{
+    const Layout Y = layout(S);
+    double *D = S.F.p + (size_t)(2 * S.ns) * SB2;
+    for (int K = 0; K < S.nb; K++) {
+        hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, K, D, S.Li.p, S.Ui.p, S.zpiv.p);
+        const int na = S.hap[K + 1] - S.hap[K];
+        if (na > 0) {
+            const int *act = S.al.p + S.hap[K];
+            hipLaunchKernelGGL(k_panel, dim3(2 * na * NSTRIP), dim3(256), 0, st, K, act, na, Y, S.Li.p, S.Ui.p);
+            hipLaunchKernelGGL(k_update, dim3(na * na * NSTRIP), dim3(256), 0, st, K, act, na, Y);
+        }
+    }
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int enqueue_sweeps(SolveState &S, hipStream_t st)
+{
+    const Layout Y = layout(S);
+    for (int K = 0; K < S.nb; K++) {
+        const int na = S.hap[K + 1] - S.hap[K];
+        hipLaunchKernelGGL(k_fwd, dim3(std::max(1, na)), dim3(256), 0, st, K, S.al.p + S.hap[K], na, Y, S.Li.p,
+                           S.y.p, S.z.p);
+    }
+    for (int K = S.nb - 1; K >= 0; K--) {
+        const int nu = K - S.hfb[K];
+        hipLaunchKernelGGL(k_bwd, dim3(std::max(1, nu)), dim3(256), 0, st, K, Y, S.Ui.p, S.z.p, S.w.p);
+    }
+    PXG_HIP(hipGetLastError());
+    return PXG_OK;
+}
+
+int run_chain(SolveState &S, hipStream_t st, hipGraphExec_t &exec, int (*enqueue)(SolveState &, hipStream_t))
+{
+    if (!S.use_graphs) return enqueue(S, st);
+    if (!exec) {
+        hipGraph_t g = nullptr;
+        PXG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(S, st);
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PXG_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) exec = nullptr;
+        PXG_HIP(ei);
+    }
+    PXG_HIP(hipGraphLaunch(exec, st));
+    return PXG_OK;
+}
+
 }  // namespace
 
 // numeric factorisation of the device CSR values (DefaultSolver.cpp:248-370)
@@ -551,21 +727,11 @@ extern "C" int pxg_solve_factor_dev(pxg_ctx *ctx, const double *kvals_dev)
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, S.zpiv.p, 1, INT_MAX);
     const long long npad = (long long)S.nb * SB;
     const long long work = std::max(S.nnz, npad - S.n);
-    Layout Y = layout(S);
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(work, 256)), dim3(256), 0, st, S.nnz, kvals_dev, S.loc.p, S.F.p,
                        S.n, (int)npad, S.F.p + (size_t)(2 * S.ns) * SB2);
     PXG_HIP(hipGetLastError());
-    double *D = S.F.p + (size_t)(2 * S.ns) * SB2;
-    for (int K = 0; K < S.nb; K++) {
-        hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, K, D, S.Li.p, S.Ui.p, S.zpiv.p);
-        const int na = S.hap[K + 1] - S.hap[K];
-        if (na > 0) {
-            const int *act = S.al.p + S.hap[K];
-            hipLaunchKernelGGL(k_panel, dim3(2 * na), dim3(256), 0, st, K, act, na, Y, S.Li.p, S.Ui.p);
-            hipLaunchKernelGGL(k_update, dim3(na * na), dim3(256), 0, st, K, act, na, Y);
-        }
-    }
-    PXG_HIP(hipGetLastError());
+    rc = run_chain(S, st, S.gfac, enqueue_factor_steps);
+    if (rc) return rc;
     int zp = INT_MAX;
     PXG_HIP(hipMemcpyAsync(&zp, S.zpiv.p, sizeof(int), hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
@@ -587,17 +753,9 @@ extern "C" int pxg_solve_apply_dev(pxg_ctx *ctx, const double *rhs_dev, double *
     SolveState &S = ctx->solve;
     hipStream_t st = ctx->stream;
     const int npad = S.nb * SB;
-    Layout Y = layout(S);
     hipLaunchKernelGGL(k_perm_in, dim3(grid_for(npad, 256)), dim3(256), 0, st, S.n, npad, S.inv.p, rhs_dev, S.y.p);
-    for (int K = 0; K < S.nb; K++) {
-        const int na = S.hap[K + 1] - S.hap[K];
-        hipLaunchKernelGGL(k_fwd, dim3(std::max(1, na)), dim3(256), 0, st, K, S.al.p + S.hap[K], na, Y, S.Li.p,
-                           S.y.p, S.z.p);
-    }
-    for (int K = S.nb - 1; K >= 0; K--) {
-        const int nu = K - S.hfb[K];
-        hipLaunchKernelGGL(k_bwd, dim3(std::max(1, nu)), dim3(256), 0, st, K, Y, S.Ui.p, S.z.p, S.w.p);
-    }
+    int rc = run_chain(S, st, S.gsub, enqueue_sweeps);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_perm_out, dim3(grid_for(S.n, 256)), dim3(256), 0, st, S.n, S.inv.p, S.w.p, x_dev);
     PXG_HIP(hipGetLastError());
     return PXG_OK;
@@ -629,12 +787,13 @@ extern "C" int pxg_solve(pxg_ctx *ctx, const double *kvals, const double *rhs, d
     return PXG_OK;
 }
 
-extern "C" int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib)
+extern "C" int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib, int flags)
 {
-    PXG_CHECK(ctx, PXG_ERR_ARG, "pxg_solve_options: bad arguments");
+    PXG_CHECK(ctx && (flags & ~PXG_SOLVE_NO_GRAPHS) == 0, PXG_ERR_ARG, "pxg_solve_options: bad arguments");
     SolveState &S = ctx->solve;
     S.use_rcm = use_rcm != 0;
     S.max_gib = max_profile_gib;
+    S.use_graphs = !(flags & PXG_SOLVE_NO_GRAPHS);
     S.ready = 0;                      // the next solve re-runs the symbolic phase
     S.factored = 0;
     return PXG_OK;

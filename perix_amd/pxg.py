@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 FP64 = 0
 FP32C = 1
 ERR_SINGULAR = -7      # zero pivot in the direct solve
+SOLVE_NO_GRAPHS = 1
 _LIB = None
 
 D = C.POINTER(C.c_double)
@@ -224,7 +225,7 @@ def load():
         L.pxg_frac_project_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.pxg_frac_state_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]
-        L.pxg_solve_options.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.pxg_solve_options.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         L.pxg_solve_factor_dev.argtypes = [C.c_void_p, C.c_void_p]
         L.pxg_solve_apply_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.pxg_solve_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -525,8 +526,8 @@ class Context:
         _check(load().pxg_csr_apply_bcs_dev(self.h, t, u_ptr, k_ptr, r_ptr))
 
     # -- direct solve of the implicit system (DefaultSolver on the device) -------
-    def solve_options(self, use_rcm=True, max_profile_gib=16.0):
-        _check(load().pxg_solve_options(self.h, int(use_rcm), max_profile_gib))
+    def solve_options(self, use_rcm=True, max_profile_gib=16.0, graphs=True):
+        _check(load().pxg_solve_options(self.h, int(use_rcm), max_profile_gib, 0 if graphs else SOLVE_NO_GRAPHS))
 
     def solve(self, k, r):
         """K x = r for host values k (nnz, BC rows applied) and r (rows)."""

@@ -79,6 +79,20 @@ def test_solve_device_pointers_and_refactor():
     ctx.close()
 
 
+def test_solve_graph_replays_equal_eager():
+    """The factor and sweep chains replay as HIP graphs captured once per
+    pattern: three solves on changing values (copies and a pivot readback
+    between the replays) equal the same solves launched kernel by kernel."""
+    ctx, k, r = _ch(32)
+    scales = (1.0, 1.25, 0.8)
+    xg = [ctx.solve(k * f, r * (2.0 - f)) for f in scales]
+    ctx.solve_options(graphs=False)
+    xe = [ctx.solve(k * f, r * (2.0 - f)) for f in scales]
+    for a, b in zip(xg, xe):
+        assert np.array_equal(a, b)
+    ctx.close()
+
+
 def test_solve_options_and_errors(oracle):
     ctx, k, r = _poisson(24)
     rp, cols = ctx.csr()
