@@ -31,6 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 METRIC = "explicit PDDO node-steps/s FP32-comp vs FP64 at stated err; adjoint/fwd cost"
@@ -179,7 +180,8 @@ def cpu_baselines():
             "one_thread_node_steps_per_s": one["node_steps_per_s"], "one_thread_steps": one["steps"],
             "block3d_node_steps_per_s": block["node_steps_per_s"], "block3d_steps": block["steps"],
             "block3d_setup_s": block["setup_s"], "block3d_kind": block["kind"],
-            "implicit_poisson_100x100": implicit_solve_split()}
+            "implicit_poisson_100x100": implicit_solve_split(),
+            "implicit_ch_128x128_ldu": reference_ldu_ch(128)}
 
 
 def implicit_solve_split(nx=100):
@@ -196,6 +198,118 @@ def implicit_solve_split(nx=100):
         _, rec = refpy.run_job(refpy.poisson_deck(nx, assemble), timeout=300)
         out[assemble] = {k: rec[k] for k in ("seconds", "assemble_s", "solve_s", "newton_iterations", "seam_ok")}
     out["deck"] = "Poisson %dx%d, static Newton, default LDU (reference driver)" % (nx, nx)
+    return out
+
+
+def reference_ldu_ch(nx=128):
+    """The reference's DefaultSolver (oracle/_ref, RCM + skyline LDU, one
+    thread) on config 3's first Newton system at nx x nx: seconds of one
+    factorisation and solve.  None when the reference build is absent."""
+    from oracle import refpy
+    if not refpy.available():
+        return None
+    c = refpy.Case(refpy.ch_deck(nx, 1, "openmp"))
+    u, _ = c.initial_state()
+    u = c.preset(u, 0.0)
+    k, r = c.assemble(u, u, 1e-3)
+    k, r = c.apply_bcs(u, 1e-3, k, r)
+    t0 = time.perf_counter()
+    rc, x, perm, jp = c.default_solve(k, r)
+    sec = time.perf_counter() - t0
+    c.close()
+    return {"seconds": sec, "equations": len(r), "profile": int(jp[-1]), "rc": rc,
+            "gflop": 2.0 * float(np.sum(np.diff(jp).astype(float) ** 2)) / 1e9,
+            "system": "Cahn-Hilliard %dx%d, first Newton system of the config-3 loop" % (nx, nx)}
+
+
+def implicit_case(problems, pxg, ch_nx=256, ch_steps=20):
+    """The implicit jobs with the linear solve on the device (the reference's
+    default LDU as a block-skyline LU): config 1 (Poisson 100 x 100, static
+    Newton) and config 3 (Cahn-Hilliard ch_nx^2, ch_steps backward-Euler
+    steps, Newton max 20 / abs 1e-9 / rel 1e-8), wall time of the whole job
+    after a warm-up solve that builds the ordering and captures the launch
+    graphs, and the factorisation / substitution split of one Newton system."""
+    import math
+    import statistics
+    import torch
+    from perix_amd import implicit
+    dev = torch.device("cuda", 0)
+    out = {}
+
+    def split(ctx, asm_sys, reps=5):
+        ws = implicit.Workspace(ctx)
+        asm_sys(ws)
+        ctx.sync()
+        tf, ta = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.solve_factor_dev(ws.K.data_ptr())
+            ctx.sync()
+            t1 = time.perf_counter()
+            ctx.solve_apply_dev(ws.R.data_ptr(), ws.dU.data_ptr())
+            ctx.sync()
+            t2 = time.perf_counter()
+            tf.append(t1 - t0)
+            ta.append(t2 - t1)
+        info = ctx.solve_info()
+        fms, ams = statistics.median(tf) * 1e3, statistics.median(ta) * 1e3
+        flop = info["block_products"] * 2.0 * info["block"] ** 3
+        return {"factor_ms": fms, "substitution_ms": ams, "equations": info["equations"],
+                "blocks": info["blocks"], "profile": info["profile"], "block_products": info["block_products"],
+                "factor_tflops": flop / (fms * 1e-3) / 1e12, "block_storage_gb": info["bytes"] / 1e9}
+
+    ctx, cloud, delta, f = problems.poisson_square(nx=100, ny=100)
+    f0 = 2.0 * math.pi ** 2
+
+    def asm_p(up, uop, kp, rp):
+        ctx.assemble_poisson_dev(1.0, up, kp, rp, f_const=f0)
+
+    u = torch.zeros(cloud.n, dtype=torch.float64, device=dev)
+    implicit.static_solve(ctx, asm_p, u)
+    u.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = implicit.static_solve(ctx, asm_p, u)
+    job = time.perf_counter() - t0
+
+    def sys_p(ws):
+        asm_p(u.data_ptr(), u.data_ptr(), ws.K.data_ptr(), ws.R.data_ptr())
+        ctx.csr_apply_bcs_dev(0.0, u.data_ptr(), ws.K.data_ptr(), ws.R.data_ptr())
+
+    out["poisson_100x100"] = dict(job_s=job, newton_iterations=res.iters, converged=res.converged,
+                                  **split(ctx, sys_p))
+    ctx.close()
+
+    ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=ch_nx, ny=ch_nx)
+
+    def asm_c(dt, up, uop, kp, rp):
+        ctx.assemble_ch_dev(1e-2, 3.0, 1.0, dt, up, uop, kp, rp)
+
+    u = torch.tensor(u0, device=dev)
+    torch.cuda.synchronize()
+    # warm-up: ordering, block storage and graph capture on the first system
+    wsw = implicit.Workspace(ctx)
+    asm_c(1e-3, u.data_ptr(), u.data_ptr(), wsw.K.data_ptr(), wsw.R.data_ptr())
+    ctx.csr_apply_bcs_dev(1e-3, u.data_ptr(), wsw.K.data_ptr(), wsw.R.data_ptr())
+    ctx.solve_dev(wsw.K.data_ptr(), wsw.R.data_ptr(), wsw.dU.data_ptr())
+    ctx.sync()
+    del wsw
+    t0 = time.perf_counter()
+    iters = implicit.transient(ctx, asm_c, u, 1e-3, ch_steps * 1e-3, max_iters=20, abs_tol=1e-9, rel_tol=1e-8)
+    job = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    uo = u.clone()
+    torch.cuda.synchronize()
+
+    def sys_c(ws):
+        asm_c(1e-3, u.data_ptr(), uo.data_ptr(), ws.K.data_ptr(), ws.R.data_ptr())
+        ctx.csr_apply_bcs_dev(1e-3, u.data_ptr(), ws.K.data_ptr(), ws.R.data_ptr())
+
+    sp = split(ctx, sys_c)
+    out["ch_%dx%d_%d_steps" % (ch_nx, ch_nx, ch_steps)] = dict(
+        job_s=job, newton_iterations=sum(iters), per_step_iterations=iters,
+        ms_per_newton_iteration=job * 1e3 / max(sum(iters), 1), **sp)
+    ctx.close()
     return out
 
 
@@ -913,6 +1027,9 @@ def main():
     ap.add_argument("--only-adjoint", action="store_true", help="adjoint case only (profiling)")
     ap.add_argument("--only-assembly", action="store_true", help="CSR assembly only (profiling)")
     ap.add_argument("--only-gweight", action="store_true", help="family + g-weight build only (profiling)")
+    ap.add_argument("--only-implicit", action="store_true", help="implicit jobs with the device solve only")
+    ap.add_argument("--ch-nx", type=int, default=256, help="with --only-implicit: Cahn-Hilliard grid")
+    ap.add_argument("--ch-steps", type=int, default=20, help="with --only-implicit: Cahn-Hilliard steps")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--stream-ceiling", action="store_true")
     args = ap.parse_args()
@@ -951,6 +1068,9 @@ def main():
     if args.only_gweight:
         print(json.dumps({"gweight_build": gweight_case(problems, pxg)}))
         return 0
+    if args.only_implicit:
+        print(json.dumps({"implicit": implicit_case(problems, pxg, args.ch_nx, args.ch_steps)}))
+        return 0
     pt = plate_throughput(problems, pxg, K, W)
     f32, f64 = pt["fp32c"], pt["fp64"]
     n, nb = f32["n"], f32["nb"]
@@ -986,6 +1106,8 @@ def main():
         asm = assembly_case(problems, pxg)
         log("family search and g-weight build")
         gw = gweight_case(problems, pxg)
+        log("implicit jobs with the device solve")
+        imp = implicit_case(problems, pxg)
     out = {
         "metric": METRIC,
         "value": value,
@@ -1043,6 +1165,20 @@ def main():
                                      if isinstance(v, dict) and "nnz_per_s" in v}
         out["assembly_nnz_per_s"]["unit"] = "1e9 nnz/s"
         out["gweight_build_ms"] = {k: round(v["gweight_ms"], 3) for k, v in gw.items()}
+        ip, ic = imp["poisson_100x100"], imp["ch_256x256_20_steps"]
+        ref_ch = (extra.get("cpu_baseline") or {}).get("implicit_ch_128x128_ldu")
+        out["implicit_device_solve"] = {
+            "poisson_100x100_job_s": ip["job_s"], "poisson_newton_iterations": ip["newton_iterations"],
+            "poisson_factor_ms": ip["factor_ms"], "poisson_substitution_ms": ip["substitution_ms"],
+            "ch256_20_steps_job_s": ic["job_s"], "ch256_newton_iterations": ic["newton_iterations"],
+            "ch256_factor_ms": ic["factor_ms"], "ch256_substitution_ms": ic["substitution_ms"],
+            "ch256_factor_tflops": ic["factor_tflops"],
+            "roofline": {"bound": "fp64", "achieved": ic["factor_tflops"], "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ic["factor_tflops"] / FP64_PEAK_TFLOPS,
+                         "kernel": "pxg_solve_factor_dev (k_diag, k_panel, k_update) on the CH 256x256 system",
+                         "note": "block products over the whole factorisation; the chain of n pivots through "
+                                 "the diagonal blocks bounds it, not the FP64 rate"},
+            "reference_ldu_ch128_s": ref_ch["seconds"] if ref_ch else None}
     sys.stderr.flush()
     print(json.dumps(out))
     sys.stdout.flush()

@@ -173,6 +173,26 @@ def poisson_deck(nx, assemble):
             "JobSystem": {"type": "static", "assemble": assemble}}
 
 
+def ch_deck(nx, steps, assemble):
+    """Config 3's loop on an nx x nx square: c0 = 0.5 + U(-0.01, 0.01)
+    (RandomIC, seed 12345), chi = 3, kappa = 1e-2, M = 1, backward Euler
+    dt = 1e-3 (fixed), Neumann c and mu on all ghost walls, Newton (the
+    mobility Picard-frozen per iteration, max 20, abs 1e-9, rel 1e-8) with the
+    default LDU solver."""
+    walls = ["left", "right", "bottom", "top"]
+    return {"Mesh": mesh_block(2, nx, nx), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
+            "DOFs": ["c", "mu"],
+            "ElmtSystem": {"ch": {"type": "cahnhilliard", "params": {"chi": 3.0, "kappa": 1e-2, "M": 1.0}}},
+            "BCSystem": {w: {"type": "neumann", "phygroup": w + "nodes_ghost", "dofs": ["c", "mu"],
+                             "value": [0.0, 0.0]} for w in walls},
+            "ICSystem": {"noisy": {"type": "random", "dof": "c", "min": 0.49, "max": 0.51, "seed": 12345},
+                         "mu0": {"type": "constant", "dof": "mu", "value": 0.0}},
+            "JobSystem": {"type": "transient", "assemble": assemble},
+            "TimeStepping": {"dt": 1e-3, "total_time": steps * 1e-3},
+            "NonlinearSolver": {"max_iters": 20, "abs_tol": 1e-9, "rel_tol": 1e-8},
+            "OutputSystem": {"interval": 100000}}
+
+
 def mesh_block(dim, nx, ny, nz=1, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
     m = {"type": "perix", "dim": dim, "nx": nx, "ny": ny, "xmin": lo[0], "xmax": hi[0],
          "ymin": lo[1], "ymax": hi[1], "meshtype": "quad4" if dim == 2 else "hex8",

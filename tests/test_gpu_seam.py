@@ -46,21 +46,10 @@ def poisson_deck(refpy, nx, assemble):
 
 
 def ch_deck(refpy, nx, steps, assemble):
-    """Config 3 loop: c0 = 0.5 + U(-0.01, 0.01) (RandomIC, seed 12345),
-    chi = 3, kappa = 1e-2, M = 1, backward Euler dt = 1e-3 (fixed), Neumann c
-    and mu on all ghost walls, Newton (the mobility Picard-frozen per
-    iteration) with the default LDU solver."""
-    return {"Mesh": refpy.mesh_block(2, nx, nx), "PDMesh": {"HorizonRadiusFactor": 3.015, "Order": 2},
-            "DOFs": ["c", "mu"],
-            "ElmtSystem": {"ch": {"type": "cahnhilliard", "params": {"chi": 3.0, "kappa": 1e-2, "M": 1.0}}},
-            "BCSystem": {w: {"type": "neumann", "phygroup": w + "nodes_ghost", "dofs": ["c", "mu"],
-                             "value": [0.0, 0.0]} for w in _walls(2)},
-            "ICSystem": {"noisy": {"type": "random", "dof": "c", "min": 0.49, "max": 0.51, "seed": 12345},
-                         "mu0": {"type": "constant", "dof": "mu", "value": 0.0}},
-            "JobSystem": {"type": "transient", "assemble": assemble},
-            "TimeStepping": {"dt": 1e-3, "total_time": steps * 1e-3},
-            "NonlinearSolver": {"max_iters": 20, "abs_tol": 1e-9, "rel_tol": 1e-8},
-            "OutputSystem": {"interval": 100000}}
+    """Config 3 loop (oracle/refpy.ch_deck): RandomIC seed 12345, chi = 3,
+    kappa = 1e-2, M = 1, backward Euler dt = 1e-3, Neumann c and mu ghosts,
+    Newton with the default LDU solver."""
+    return refpy.ch_deck(nx, steps, assemble)
 
 
 def plate_deck(refpy, nx, ny, steps, assemble):

@@ -26,8 +26,7 @@ def _ref():
 
 
 def _system(refpy, kind, nx):
-    import test_gpu_seam as seam
-    deck = refpy.poisson_deck(nx, "openmp") if kind == "poisson" else seam.ch_deck(refpy, nx, 1, "openmp")
+    deck = refpy.poisson_deck(nx, "openmp") if kind == "poisson" else refpy.ch_deck(nx, 1, "openmp")
     c = refpy.Case(deck)
     u, _ = c.initial_state()
     u = c.preset(u, 0.0)
