@@ -304,6 +304,13 @@ int pxg_csr_apply_bcs_dev(pxg_ctx *ctx, double t, const double *u_dev, double *k
  * storage, block products per factorisation}.  pxg_solve_ordering copies the
  * ordering (perm[n]: new -> old) and the reference skyline pointer
  * (jp[n+1]); either may be NULL. */
+/* pxg_solve_set_pattern gives the solver a square CSR pattern of its own
+ * (n rows, sorted distinct columns, every diagonal stored): the reference's
+ * SparseMatrix CSR as DefaultSolver::initSolver takes it (:428-462), for
+ * systems assembled elsewhere.  Every later pxg_solve* call factors that
+ * pattern (kvals index it) until pxg_build_csr or pxg_set_csr gives the
+ * context a new implicit CSR. */
+int pxg_solve_set_pattern(pxg_ctx *ctx, int n, const int *rowptr, const int *cols);
 #define PXG_SOLVE_NO_GRAPHS 1
 int pxg_solve_options(pxg_ctx *ctx, int use_rcm, double max_profile_gib, int flags);
 int pxg_solve_factor_dev(pxg_ctx *ctx, const double *kvals_dev);

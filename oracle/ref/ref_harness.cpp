@@ -506,3 +506,29 @@ int ref_default_solve(void *h, const double *kvals, const double *rhs, int use_r
     return ok ? 0 : 1;
 }
 } // extern "C"
+
+#ifdef PERIX_CUDA_ASSEMBLE
+#include "pxg_linear_solver.h"
+extern "C" {
+// The libpxg back-end of the reference's LinearSolver dispatcher
+// (integration/pxg_linear_solver.cpp) driven exactly as LinearSolver::init
+// and ::solve drive a back-end: initSolver on the case's SparseMatrix with the
+// JSON parameters `params`, then solveLinearSystem with caller values kvals
+// and right-hand side rhs.  Returns 1 when the solve succeeded.
+int ref_pxg_linear_solve(void *h, const double *kvals, const double *rhs, const char *params, double *x)
+{
+    auto &p = static_cast<RefCase *>(h)->prob;
+    const int n = p.m_U.getSize();
+    const int nnz = p.m_Matrix.getNNZNum();
+    std::copy(kvals, kvals + nnz, p.m_Matrix.getCSRValuesPtr());
+    VectorXd b(n), xs(n);
+    std::copy(rhs, rhs + n, b.getDataPtr());
+    nlohmann::ordered_json prm = nlohmann::ordered_json::parse(params);
+    PxgLinearSolver solver;
+    solver.initSolver(p.m_Matrix, prm);
+    const bool ok = solver.solveLinearSystem(p.m_Matrix, b, xs);
+    std::copy(xs.getDataPtr(), xs.getDataPtr() + n, x);
+    return ok ? 1 : 0;
+}
+} // extern "C"
+#endif

@@ -67,6 +67,14 @@ def pxg_lib():
                                      C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         _PXG_LIB.perix_cuda_pxg_stats.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         _PXG_LIB.perix_cuda_pxg_reset_stats.argtypes = [C.c_longlong]
+        _PXG_LIB.ref_open.restype = C.c_void_p
+        _PXG_LIB.ref_open.argtypes = [C.c_char_p]
+        _PXG_LIB.ref_horizon.restype = C.c_double
+        _PXG_LIB.ref_group.restype = C.c_int
+        _PXG_LIB.ref_default_solve.restype = C.c_int
+        _PXG_LIB.ref_default_solve.argtypes = [C.c_void_p, D, D, C.c_int, D, I, C.POINTER(C.c_longlong)]
+        _PXG_LIB.ref_pxg_linear_solve.restype = C.c_int
+        _PXG_LIB.ref_pxg_linear_solve.argtypes = [C.c_void_p, D, D, C.c_char_p, D]
     return _PXG_LIB
 
 
@@ -205,12 +213,15 @@ def mesh_block(dim, nx, ny, nz=1, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
 class Case:
     """One reference problem built from a JSON deck."""
 
-    def __init__(self, deck):
+    def __init__(self, deck, pxg=False):
+        """pxg: open the case in the PERIX_CUDA_ASSEMBLE build (libperix_ref_pxg.so),
+        which also carries the libpxg linear-solver back-end."""
+        self._L = pxg_lib() if pxg else lib()
         self._dir = tempfile.mkdtemp(prefix="pxref_")
         self.path = os.path.join(self._dir, "deck.json")
         with open(self.path, "w") as f:
             json.dump(deck, f, indent=1)
-        L = lib()
+        L = self._L
         self.h = C.c_void_p(L.ref_open(self.path.encode()))
         info = np.zeros(6, np.int64)
         L.ref_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong)))
@@ -219,26 +230,26 @@ class Case:
 
     def close(self):
         if self.h:
-            lib().ref_close(self.h)
+            self._L.ref_close(self.h)
             self.h = None
 
     def mesh(self):
         c = np.zeros(3 * self.n); v = np.zeros(self.n); m = np.zeros(self.n, np.int32)
-        lib().ref_mesh(self.h, _dp(c), _dp(v), _ip(m))
+        self._L.ref_mesh(self.h, _dp(c), _dp(v), _ip(m))
         return c, v, m
 
     def dxyz(self):
         d = np.zeros(3)
-        lib().ref_dxyz(self.h, _dp(d))
+        self._L.ref_dxyz(self.h, _dp(d))
         return d
 
     def families(self):
         off = np.zeros(self.n + 1, np.int32); nbr = np.zeros(self.nb, np.int32)
-        lib().ref_families(self.h, _ip(off), _ip(nbr))
+        self._L.ref_families(self.h, _ip(off), _ip(nbr))
         return off, nbr
 
     def group(self, name):
-        L = lib()
+        L = self._L
         k = L.ref_group(self.h, name.encode(), None)
         ids = np.zeros(k, np.int32)
         if k:
@@ -247,28 +258,28 @@ class Case:
 
     def ops(self, nop):
         o = np.zeros(self.nb * nop); fl = np.zeros(self.n, np.uint8)
-        lib().ref_ops(self.h, _dp(o), fl.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        self._L.ref_ops(self.h, _dp(o), fl.ctypes.data_as(C.POINTER(C.c_ubyte)))
         return o.reshape(self.nb, nop), fl
 
     def csr(self):
         rp = np.zeros(self.n * self.ndof + 1, np.int32); cols = np.zeros(self.nnz, np.int32)
-        lib().ref_csr(self.h, _ip(rp), _ip(cols))
+        self._L.ref_csr(self.h, _ip(rp), _ip(cols))
         return rp, cols
 
     def initial_state(self):
         u = np.zeros(self.n * self.ndof); v = np.zeros(self.n * self.ndof)
-        lib().ref_initial_state(self.h, _dp(u), _dp(v))
+        self._L.ref_initial_state(self.h, _dp(u), _dp(v))
         return u, v
 
     def preset(self, u, t):
         u = np.ascontiguousarray(u, float).copy()
-        lib().ref_preset(self.h, _dp(u), C.c_double(t))
+        self._L.ref_preset(self.h, _dp(u), C.c_double(t))
         return u
 
     def assemble(self, u, uold, dt, parallel=False):
         """formResidualAndJacobian (serial) or its OpenMP twin with `parallel`."""
         k = np.zeros(self.nnz); r = np.zeros(self.n * self.ndof)
-        f = lib().ref_assemble_parallel if parallel else lib().ref_assemble
+        f = self._L.ref_assemble_parallel if parallel else self._L.ref_assemble
         f(self.h, _dp(np.ascontiguousarray(u, float)), _dp(np.ascontiguousarray(uold, float)),
           C.c_double(dt), _dp(k), _dp(r))
         return k, r
@@ -276,7 +287,7 @@ class Case:
     def apply_bcs(self, u, t, k, r):
         """BCSystem::applyBCs (row elimination) on copies of K values and RHS."""
         k = np.ascontiguousarray(k, float).copy(); r = np.ascontiguousarray(r, float).copy()
-        lib().ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
+        self._L.ref_apply_bcs(self.h, _dp(np.ascontiguousarray(u, float)), C.c_double(t), _dp(k), _dp(r))
         return k, r
 
     def default_solve(self, k, r, use_rcm=True):
@@ -285,76 +296,86 @@ class Case:
         pivot, perm new -> old, jp the skyline pointer."""
         n = self.n * self.ndof
         x = np.zeros(n); perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
-        rc = lib().ref_default_solve(self.h, _dp(np.ascontiguousarray(k, float)),
+        rc = self._L.ref_default_solve(self.h, _dp(np.ascontiguousarray(k, float)),
                                      _dp(np.ascontiguousarray(r, float)), int(use_rcm), _dp(x), _ip(perm),
                                      jp.ctypes.data_as(C.POINTER(C.c_longlong)))
         return rc, x, perm, jp
 
+    def pxg_linear_solve(self, k, r, params="{}"):
+        """PxgLinearSolver (integration/pxg_linear_solver.cpp), the reference's
+        LinearSolverBase over libpxg, on the case's SparseMatrix with values k:
+        returns (ok, x).  Needs a case opened with pxg=True."""
+        n = self.n * self.ndof
+        x = np.zeros(n)
+        ok = self._L.ref_pxg_linear_solve(self.h, _dp(np.ascontiguousarray(k, float)),
+                                          _dp(np.ascontiguousarray(r, float)), params.encode(), _dp(x))
+        return bool(ok), x
+
     def last_bc_seconds(self):
         """Wall time of the applyBCs call alone in the last apply_bcs."""
-        return lib().ref_last_bc_seconds()
+        return self._L.ref_last_bc_seconds()
 
     def projection(self, name):
         """computeNodalProjection of every node (n x 1 or n x 9)."""
         nc = 9 if name in ("stress", "strain") else 1
         out = np.zeros(self.n * nc)
-        lib().ref_projection(self.h, name.encode(), _dp(out))
+        self._L.ref_projection(self.h, name.encode(), _dp(out))
         return out.reshape(self.n, nc)
 
     def projection_u(self, name, u, nc):
         """computeNodalProjection of every node on the state u (n x nc)."""
         out = np.zeros(self.n * nc)
-        lib().ref_projection_u(self.h, name.encode(), _dp(np.ascontiguousarray(u, float)), nc, _dp(out))
+        self._L.ref_projection_u(self.h, name.encode(), _dp(np.ascontiguousarray(u, float)), nc, _dp(out))
         return out.reshape(self.n, nc)
 
     def save_vtu(self, u, stem, step):
         """PDProblem::savePDResults of the state u; returns the written path."""
-        lib().ref_save_vtu(self.h, _dp(np.ascontiguousarray(u, float)), stem.encode(), step)
+        self._L.ref_save_vtu(self.h, _dp(np.ascontiguousarray(u, float)), stem.encode(), step)
         return "%s-PD-results-%05d.vtu" % (stem, step) if step > 0 else "%s-PD-results.vtu" % stem
 
     def assemble_step(self, u, uold, dt, timestep):
         """formResidualAndJacobian as time step `timestep` (per-step commits)."""
         k = np.zeros(self.nnz); r = np.zeros(self.n * self.ndof)
-        lib().ref_assemble_step(self.h, _dp(np.ascontiguousarray(u, float)),
+        self._L.ref_assemble_step(self.h, _dp(np.ascontiguousarray(u, float)),
                                 _dp(np.ascontiguousarray(uold, float)), C.c_double(dt), C.c_int(timestep),
                                 _dp(k), _dp(r))
         return k, r
 
     def dynfrac_state(self):
         h = np.zeros(self.nb); v = np.zeros(self.n * self.ndof)
-        lib().ref_dynfrac_state(self.h, _dp(h), _dp(v))
+        self._L.ref_dynfrac_state(self.h, _dp(h), _dp(v))
         return h, v
 
     def dynfrac_set(self, b, v0):
-        lib().ref_dynfrac_set(self.h, _dp(np.ascontiguousarray(b, float)), _dp(np.ascontiguousarray(v0, float)))
+        self._L.ref_dynfrac_set(self.h, _dp(np.ascontiguousarray(b, float)), _dp(np.ascontiguousarray(v0, float)))
 
     def stressch_consts(self):
         """C11, C12, C66, A, Kh, Ch, D, kappa, chi, Omega, cref of the element."""
         out = np.zeros(11)
-        lib().ref_stressch_consts(self.h, _dp(out))
+        self._L.ref_stressch_consts(self.h, _dp(out))
         return out
 
     def fsch_state(self, dim):
         """FracStressCahnHilliardElement: (health, vel, lapc, constants)."""
         h = np.zeros(self.nb); v = np.zeros(self.n * dim); lc = np.zeros(self.nb); k = np.zeros(14)
-        lib().ref_fsch_state(self.h, _dp(h), _dp(v), _dp(lc), _dp(k))
+        self._L.ref_fsch_state(self.h, _dp(h), _dp(v), _dp(lc), _dp(k))
         return h, v, lc, k
 
     def ch_geometry(self):
         lap = np.zeros(self.nb); rev = np.zeros(self.nb, np.int32); mn = np.zeros(self.n)
-        lib().ref_ch_geometry(self.h, _dp(lap), _ip(rev), _dp(mn))
+        self._L.ref_ch_geometry(self.h, _dp(lap), _ip(rev), _dp(mn))
         return lap, rev, mn
 
     def frac_consts(self):
         o = np.zeros(5)
-        lib().ref_frac_consts(self.h, _dp(o))
+        self._L.ref_frac_consts(self.h, _dp(o))
         return o
 
     def explicit_timed(self):
         """(steps, one-off setup seconds, loop seconds, final u)."""
         u = np.zeros(self.n * self.ndof)
         a = C.c_double(); b = C.c_double()
-        steps = lib().ref_explicit_timed(self.h, C.byref(a), C.byref(b), _dp(u))
+        steps = self._L.ref_explicit_timed(self.h, C.byref(a), C.byref(b), _dp(u))
         return steps, a.value, b.value, u
 
     def explicit(self):
@@ -362,7 +383,7 @@ class Case:
         nops = 9 if self.dim == 3 else 5
         u = np.zeros(nd); uo = np.zeros(nd); h = np.zeros(self.nb); opc = np.zeros(self.nb * nops)
         sc = C.c_double()
-        steps = lib().ref_explicit(self.h, _dp(u), _dp(uo), _dp(h), _dp(opc), C.byref(sc))
+        steps = self._L.ref_explicit(self.h, _dp(u), _dp(uo), _dp(h), _dp(opc), C.byref(sc))
         return steps, u, uo, h, opc.reshape(self.nb, nops), sc.value
 
 

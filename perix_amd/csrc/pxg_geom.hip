@@ -638,6 +638,7 @@ extern "C" int pxg_set_csr(pxg_ctx *ctx, int ndof, const int *rowptr, const int 
     PXG_FAULT(ctx, "pxg_set_csr");
     ctx->csrbc.set = 0;
     ctx->csr_gen++;
+    ctx->solve.own = 0;                 // the solver follows the new implicit CSR
     PXG_HIP(ctx->rowptr.alloc((size_t)rows + 1));
     PXG_HIP(ctx->cols.alloc((size_t)nnz));
     PXG_HIP(ctx->slot.alloc((size_t)ctx->nb));
@@ -707,6 +708,7 @@ extern "C" int pxg_build_csr(pxg_ctx *ctx, int ndof, long long *nnz_out)
     PXG_HIP(hipSetDevice(ctx->device));
     ctx->csrbc.set = 0;                 // a BC row program refers to the old pattern
     ctx->csr_gen++;
+    ctx->solve.own = 0;                 // the solver follows the new implicit CSR
     hipStream_t st = ctx->stream;
     const int n = ctx->n;
     const long long rows = (long long)n * ndof;

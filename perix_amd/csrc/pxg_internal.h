@@ -320,6 +320,9 @@ struct SolveState {
     dbuf<int> inv, fb, al, zpiv;
     dbuf<long long> loc, lp;
     dbuf<double> part, norms;       // Newton norms
+    int own = 0;                    // factor own_rp/own_cols instead of the context's CSR
+    long long own_gen = 0;
+    std::vector<int> own_rp, own_cols;
     int use_graphs = 1;             // replay the per-step launch chains as HIP graphs
     hipGraphExec_t gfac = nullptr;  // factor steps of the current pattern
     hipGraphExec_t gsub = nullptr;  // forward and backward sweeps

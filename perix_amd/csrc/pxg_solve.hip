@@ -533,18 +533,32 @@ void rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &cols, 
         for (int k = 0; k < n; k++) perm[k] = order[n - 1 - k];
 }
 
+// which pattern the solver factors: its own (pxg_solve_set_pattern) or the
+// context's implicit CSR, each with its own generation
+long long pattern_key(const pxg_ctx *ctx)
+{
+    return ctx->solve.own ? -1 - ctx->solve.own_gen : ctx->csr_gen;
+}
+
 int symbolic(pxg_ctx *ctx)
 {
     SolveState &S = ctx->solve;
     S.ready = 0;
     S.drop_graphs();
     hipStream_t st = ctx->stream;
-    const int n = ctx->n * ctx->csr_ndof;
-    const long long nnz = ctx->nnz;
-    std::vector<int> rp((size_t)n + 1), cols((size_t)nnz);
-    PXG_HIP(hipMemcpyAsync(rp.data(), ctx->rowptr.p, sizeof(int) * rp.size(), hipMemcpyDeviceToHost, st));
-    PXG_HIP(hipMemcpyAsync(cols.data(), ctx->cols.p, sizeof(int) * cols.size(), hipMemcpyDeviceToHost, st));
-    PXG_HIP(hipStreamSynchronize(st));
+    std::vector<int> rp, cols;
+    if (S.own) {
+        rp = S.own_rp;
+        cols = S.own_cols;
+    } else {
+        rp.resize((size_t)ctx->n * ctx->csr_ndof + 1);
+        cols.resize((size_t)ctx->nnz);
+        PXG_HIP(hipMemcpyAsync(rp.data(), ctx->rowptr.p, sizeof(int) * rp.size(), hipMemcpyDeviceToHost, st));
+        PXG_HIP(hipMemcpyAsync(cols.data(), ctx->cols.p, sizeof(int) * cols.size(), hipMemcpyDeviceToHost, st));
+        PXG_HIP(hipStreamSynchronize(st));
+    }
+    const int n = (int)rp.size() - 1;
+    const long long nnz = (long long)cols.size();
     std::vector<int> perm, inv(n);
     rcm_order(n, rp, cols, S.use_rcm != 0, perm);
     for (int k = 0; k < n; k++) inv[perm[k]] = k;
@@ -633,7 +647,7 @@ int symbolic(pxg_ctx *ctx)
     S.ns = ns;
     S.amax = amax;
     S.nnz = nnz;
-    S.gen = ctx->csr_gen;
+    S.gen = pattern_key(ctx);
     S.factored = 0;
     S.ready = 1;
     return PXG_OK;
@@ -641,9 +655,10 @@ int symbolic(pxg_ctx *ctx)
 
 int ensure_symbolic(pxg_ctx *ctx)
 {
-    PXG_CHECK(ctx && ctx->nnz > 0 && ctx->csr_ndof > 0, PXG_ERR_STATE, "pxg_solve: call pxg_build_csr first");
+    PXG_CHECK(ctx && (ctx->solve.own || (ctx->nnz > 0 && ctx->csr_ndof > 0)), PXG_ERR_STATE,
+              "pxg_solve: call pxg_build_csr, pxg_set_csr or pxg_solve_set_pattern first");
     const SolveState &S = ctx->solve;
-    if (S.ready && S.gen == ctx->csr_gen && S.nnz == ctx->nnz && S.n == ctx->n * ctx->csr_ndof) return PXG_OK;
+    if (S.ready && S.gen == pattern_key(ctx)) return PXG_OK;
     return symbolic(ctx);
 }
 
@@ -746,7 +761,7 @@ extern "C" int pxg_solve_factor_dev(pxg_ctx *ctx, const double *kvals_dev)
 extern "C" int pxg_solve_apply_dev(pxg_ctx *ctx, const double *rhs_dev, double *x_dev)
 {
     PXG_CHECK(ctx && rhs_dev && x_dev, PXG_ERR_ARG, "pxg_solve_apply_dev: bad arguments");
-    PXG_CHECK(ctx->solve.factored && ctx->solve.gen == ctx->csr_gen, PXG_ERR_STATE,
+    PXG_CHECK(ctx->solve.factored && ctx->solve.gen == pattern_key(ctx), PXG_ERR_STATE,
               "pxg_solve_apply_dev: call pxg_solve_factor_dev first");
     PXG_FAULT(ctx, "pxg_solve_apply_dev");
     PXG_HIP(hipSetDevice(ctx->device));
@@ -771,19 +786,45 @@ extern "C" int pxg_solve_dev(pxg_ctx *ctx, const double *kvals_dev, const double
 extern "C" int pxg_solve(pxg_ctx *ctx, const double *kvals, const double *rhs, double *x)
 {
     PXG_CHECK(ctx && kvals && rhs && x, PXG_ERR_ARG, "pxg_solve: bad arguments");
-    PXG_CHECK(ctx->nnz > 0 && ctx->csr_ndof > 0, PXG_ERR_STATE, "pxg_solve: call pxg_build_csr first");
     PXG_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_symbolic(ctx);
+    if (rc) return rc;
     hipStream_t st = ctx->stream;
-    const size_t rows = (size_t)ctx->n * ctx->csr_ndof, nnz = (size_t)ctx->nnz;
+    const size_t rows = (size_t)ctx->solve.n, nnz = (size_t)ctx->solve.nnz;
     PXG_HIP(ctx->sk.alloc(nnz));
     PXG_HIP(ctx->sr.alloc(rows));
     PXG_HIP(ctx->su.alloc(rows));
     PXG_HIP(hipMemcpyAsync(ctx->sk.p, kvals, sizeof(double) * nnz, hipMemcpyHostToDevice, st));
     PXG_HIP(hipMemcpyAsync(ctx->sr.p, rhs, sizeof(double) * rows, hipMemcpyHostToDevice, st));
-    int rc = pxg_solve_dev(ctx, ctx->sk.p, ctx->sr.p, ctx->su.p);
+    rc = pxg_solve_dev(ctx, ctx->sk.p, ctx->sr.p, ctx->su.p);
     if (rc) return rc;
     PXG_HIP(hipMemcpyAsync(x, ctx->su.p, sizeof(double) * rows, hipMemcpyDeviceToHost, st));
     PXG_HIP(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+extern "C" int pxg_solve_set_pattern(pxg_ctx *ctx, int n, const int *rowptr, const int *cols)
+{
+    PXG_CHECK(ctx && n > 0 && rowptr && cols, PXG_ERR_ARG, "pxg_solve_set_pattern: bad arguments");
+    PXG_CHECK(rowptr[0] == 0 && rowptr[n] > 0, PXG_ERR_ARG, "pxg_solve_set_pattern: bad row offsets");
+    for (int i = 0; i < n; i++) {
+        PXG_CHECK(rowptr[i + 1] >= rowptr[i], PXG_ERR_ARG, "pxg_solve_set_pattern: row offsets must not decrease");
+        bool diag = false;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; p++) {
+            PXG_CHECK(cols[p] >= 0 && cols[p] < n, PXG_ERR_ARG, "pxg_solve_set_pattern: column out of range");
+            PXG_CHECK(p == rowptr[i] || cols[p] > cols[p - 1], PXG_ERR_ARG,
+                      "pxg_solve_set_pattern: columns must be sorted and distinct within a row");
+            diag = diag || cols[p] == i;
+        }
+        PXG_CHECK(diag, PXG_ERR_ARG, "pxg_solve_set_pattern: a row without its diagonal");
+    }
+    SolveState &S = ctx->solve;
+    S.own_rp.assign(rowptr, rowptr + n + 1);
+    S.own_cols.assign(cols, cols + rowptr[n]);
+    S.own = 1;
+    S.own_gen++;
+    S.ready = 0;
+    S.factored = 0;
     return PXG_OK;
 }
 

@@ -92,7 +92,7 @@ SYMBOLS = [
     "pxg_assemble_diffusion", "pxg_assemble_diffusion_dev", "pxg_ch_setup",
     "pxg_ch_get_geometry", "pxg_assemble_ch", "pxg_assemble_ch_dev", "pxg_frac_setup",
     "pxg_frac_set_bcs", "pxg_csr_set_bcs", "pxg_csr_bc_rows", "pxg_csr_apply_bcs",
-    "pxg_csr_apply_bcs_dev", "pxg_solve_options", "pxg_solve_factor_dev", "pxg_solve_apply_dev",
+    "pxg_csr_apply_bcs_dev", "pxg_solve_set_pattern", "pxg_solve_options", "pxg_solve_factor_dev", "pxg_solve_apply_dev",
     "pxg_solve_dev", "pxg_solve", "pxg_solve_info", "pxg_solve_ordering", "pxg_csr_preset_dev",
     "pxg_newton_update_dev", "pxg_dynfrac_setup", "pxg_dynfrac_commit", "pxg_dynfrac_commit_dev",
     "pxg_assemble_dynfrac", "pxg_assemble_dynfrac_dev", "pxg_dynfrac_get_state",
@@ -226,6 +226,7 @@ def load():
         L.pxg_frac_state_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p),
                                           C.POINTER(C.c_void_p)]
         L.pxg_solve_options.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.pxg_solve_set_pattern.argtypes = [C.c_void_p, C.c_int, I, I]
         L.pxg_solve_factor_dev.argtypes = [C.c_void_p, C.c_void_p]
         L.pxg_solve_apply_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.pxg_solve_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -529,6 +530,11 @@ class Context:
     def solve_options(self, use_rcm=True, max_profile_gib=16.0, graphs=True):
         _check(load().pxg_solve_options(self.h, int(use_rcm), max_profile_gib, 0 if graphs else SOLVE_NO_GRAPHS))
 
+    def solve_set_pattern(self, rowptr, cols):
+        """A square CSR pattern of the solver's own (any system assembled elsewhere)."""
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        _check(load().pxg_solve_set_pattern(self.h, len(rp) - 1, _ip(rp), _ip(np.ascontiguousarray(cols, np.int32))))
+
     def solve(self, k, r):
         """K x = r for host values k (nnz, BC rows applied) and r (rows)."""
         x = np.zeros(len(r))
@@ -552,7 +558,7 @@ class Context:
         return dict(zip(keys, (int(v) for v in out)))
 
     def solve_ordering(self):
-        n = self.n * self.ndof
+        n = self.solve_info()["equations"]
         perm = np.zeros(n, np.int32); jp = np.zeros(n + 1, np.int64)
         _check(load().pxg_solve_ordering(self.h, _ip(perm), jp.ctypes.data_as(C.POINTER(C.c_longlong))))
         return perm, jp

@@ -204,3 +204,58 @@ def test_transient_ch_matches_reference_driver(refpy):
     assert sum(iters) == rec["newton_iterations"]
     assert err <= 1e-9
     ctx.close()
+
+
+def test_solve_own_pattern(oracle):
+    """pxg_solve_set_pattern on a context without a PD mesh: a system
+    assembled elsewhere (here the oracle's Cahn-Hilliard 24 x 24 system)."""
+    m = oracle.Mesh(oracle.make_grid(2, 24, 24))
+    ops, _ = oracle.operators(m)
+    rp, cols = oracle.csr_pattern(m, 2)
+    lap, rev, isg = oracle.ch_geometry(m, ops)
+    u = np.zeros(2 * m.n)
+    oracle.random_ic(12345, 0.49, 0.51, np.arange(1, m.n + 1), 2, [1], u)
+    mn = oracle.ch_mobility(m, u, 1.0)
+    k, r = oracle.assemble_ch(m, lap, rev, isg, mn, 1e-2, 3.0, 1e-3, rp, cols, u, u)
+    bcs = oracle.make_bcs([{"type": "neumann", "nodes": m.group(oracle.WALLS[w]), "comps": [0, 1]}
+                           for w in ("left", "right", "bottom", "top")])
+    _, k, r = oracle.bc_rows(m, 2, bcs, 0.0, u, rp, cols, k, r)
+    ctx = pxg.Context(0)
+    ctx.solve_set_pattern(rp, cols)
+    perm, jp = ctx.solve_ordering()
+    orc, ox, operm, ojp = oracle.ldu_solve(rp, cols, k, r)
+    assert np.array_equal(perm, operm) and np.array_equal(jp, ojp)
+    x = ctx.solve(k, r)
+    assert np.abs(x - ox).max() <= 1e-9 * np.abs(ox).max()
+    with pytest.raises(pxg.PxgError):
+        ctx.solve_set_pattern(rp, cols[::-1].copy())      # unsorted rows are refused
+    ctx.close()
+
+
+def test_reference_linear_solver_backend(refpy):
+    """integration/pxg_linear_solver.cpp, the reference's LinearSolverBase
+    over libpxg, driven as LinearSolver::init/solve drive a back-end on the
+    reference's own SparseMatrix: the reference DefaultSolver's solution to
+    1e-9, its parameters, and `false` on a zero pivot."""
+    if not refpy.pxg_available():
+        pytest.fail("oracle/_ref/libperix_ref_pxg.so is not built (make -C oracle/ref)")
+    c = refpy.Case(refpy.ch_deck(32, 1, "openmp"), pxg=True)
+    u, _ = c.initial_state()
+    u = c.preset(u, 0.0)
+    k, r = c.assemble(u, u, 1e-3)
+    k, r = c.apply_bcs(u, 1e-3, k, r)
+    rc, xref, perm, jp = c.default_solve(k, r)
+    assert rc == 0
+    ok, x = c.pxg_linear_solve(k, r)
+    assert ok and np.abs(x - xref).max() <= 1e-9 * np.abs(xref).max()
+    ok, x2 = c.pxg_linear_solve(k, r, '{"use_rcm": false}')
+    assert ok and np.abs(x2 - xref).max() <= 1e-9 * np.abs(xref).max()
+    ok, _ = c.pxg_linear_solve(k, r, '{"max_profile_gb": 1e-9}')
+    assert not ok
+    rp, cols = c.csr()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    kz = k.copy()
+    kz[(rows == 100) | (cols == 100)] = 0.0
+    ok, _ = c.pxg_linear_solve(kz, r)
+    assert not ok
+    c.close()
