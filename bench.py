@@ -204,19 +204,29 @@ def implicit_solve_split(nx=100):
 def reference_ldu_ch(nx=128):
     """The reference's DefaultSolver (oracle/_ref, RCM + skyline LDU, one
     thread) on config 3's first Newton system at nx x nx: seconds of one
-    factorisation and solve.  None when the reference build is absent."""
+    factorisation and solve.  None when the reference build is absent.  The
+    reference prints its progress to stdout; it goes to stderr here, so that
+    the bench's stdout stays one JSON line."""
     from oracle import refpy
     if not refpy.available():
         return None
-    c = refpy.Case(refpy.ch_deck(nx, 1, "openmp"))
-    u, _ = c.initial_state()
-    u = c.preset(u, 0.0)
-    k, r = c.assemble(u, u, 1e-3)
-    k, r = c.apply_bcs(u, 1e-3, k, r)
-    t0 = time.perf_counter()
-    rc, x, perm, jp = c.default_solve(k, r)
-    sec = time.perf_counter() - t0
-    c.close()
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        c = refpy.Case(refpy.ch_deck(nx, 1, "openmp"))
+        u, _ = c.initial_state()
+        u = c.preset(u, 0.0)
+        k, r = c.assemble(u, u, 1e-3)
+        k, r = c.apply_bcs(u, 1e-3, k, r)
+        t0 = time.perf_counter()
+        rc, x, perm, jp = c.default_solve(k, r)
+        sec = time.perf_counter() - t0
+        c.close()
+    finally:
+        ctypes.CDLL(None).fflush(None)      # the reference's buffered C stdio output
+        os.dup2(saved, 1)
+        os.close(saved)
     return {"seconds": sec, "equations": len(r), "profile": int(jp[-1]), "rc": rc,
             "gflop": 2.0 * float(np.sum(np.diff(jp).astype(float) ** 2)) / 1e9,
             "system": "Cahn-Hilliard %dx%d, first Newton system of the config-3 loop" % (nx, nx)}
