@@ -315,6 +315,7 @@ struct SolveState {
     std::vector<int> hperm;         // new -> old (reference RCM)
     std::vector<long long> hjp;     // reference skyline pointer (n + 1)
     std::vector<int> hap, hfb;      // active-set offsets (nb + 1), first envelope block
+    std::vector<int> hal;           // active sets (host copy)
     dbuf<double> F, Li, Ui;         // block storage, inverted diagonal factors
     dbuf<double> y, z, w;           // substitution vectors (nb * kSolveBlock)
     dbuf<int> inv, fb, al, zpiv;

@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_scatter(long long nnz, const double *__
 //   U^-1 by column operations: Y[:][k] /= a_kk, Y[:][j] -= Y[:][k] a_kj (j > k).
 // A zero pivot records the least failing equation (new order) in *zpiv and
 // continues with 1, so the arithmetic stays finite; the host reports it.
-__global__ __launch_bounds__(256) void k_diag(int K, double *__restrict__ Dall, double *__restrict__ Liall,
-                                              double *__restrict__ Uiall, int *zpiv)
+__device__ __forceinline__ void diag_factor(int K, double *__restrict__ Dall, double *__restrict__ Liall,
+                                            double *__restrict__ Uiall, int *zpiv)
 {
     __shared__ __align__(16) double rowA[2][SB], colA[2][SB], rowX[2][SB], colY[2][SB];
     const int t = (int)threadIdx.x, tr = t >> 4, tc = t & 15, r0 = 4 * tr, c0 = 4 * tc;
@@ -176,6 +176,12 @@ __global__ __launch_bounds__(256) void k_diag(int K, double *__restrict__ Dall, 
         *reinterpret_cast<double2 *>(Ui + o) = make_double2(y[i][0], y[i][1]);
         *reinterpret_cast<double2 *>(Ui + o + 2) = make_double2(y[i][2], y[i][3]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_diag(int K, double *__restrict__ Dall, double *__restrict__ Liall,
+                                              double *__restrict__ Uiall, int *zpiv)
+{
+    diag_factor(K, Dall, Liall, Uiall, zpiv);
 }
 
 // Strip products of SB x SB row-major blocks, one workgroup per 16-wide
@@ -308,6 +314,73 @@ __global__ __launch_bounds__(256) void k_update(int K, const int *__restrict__ a
     __shared__ __align__(16) double As[STRIP * TS];
     __shared__ __align__(16) double Bs[SB * TS];
     const int blk = (int)blockIdx.x / NSTRIP, s0 = STRIP * ((int)blockIdx.x % NSTRIP);
+    const int I = act[blk / na], J = act[blk % na];
+    double *c = (I > J) ? Y.L(I, J) : (I < J) ? Y.U(I, J) : Y.D(I);
+    row_strip_gemm<true>(Y.L(I, K), Y.U(K, J), c, s0, As, Bs);
+}
+
+// (3') the trailing update of step K when block K+1 is active (act[0] == K+1),
+// fused with the next step's diagonal factor: workgroup 0 applies all four
+// strips of A_{K+1,K+1} -= L_{K+1,K} U_{K,K+1} itself and then factors
+// D_{K+1} (as k_diag), while the other workgroups update the remaining tiles;
+// the next diagonal factor so runs beside this step's update instead of after it.
+__global__ __launch_bounds__(256) void k_update_diag(int K, const int *__restrict__ act, int na, Layout Y,
+                                                     double *__restrict__ Dall, double *__restrict__ Liall,
+                                                     double *__restrict__ Uiall, int *zpiv)
+{
+    __shared__ __align__(16) double As[STRIP * TS];
+    __shared__ __align__(16) double Bs[SB * TS];
+    if (blockIdx.x == 0) {
+        // D_{K+1} -= L_{K+1,K} U_{K,K+1}: U_{K,K+1} staged once, the four
+        // 16-row strips of L and of D loaded up front, then strip by strip
+        const int I = act[0], t = (int)threadIdx.x, r = t >> 4, c0 = 4 * (t & 15);
+        const double *Lb = Y.L(I, K), *Ub = Y.U(K, I);
+        double *Db = Y.D(I);
+        double2 vb[8], va[8], vc[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            vb[m] = *reinterpret_cast<const double2 *>(Ub + 2 * (m * kThreads + t));
+            va[m] = *reinterpret_cast<const double2 *>(Lb + 2 * (m * kThreads + t));
+        }
+#pragma unroll
+        for (int q = 0; q < NSTRIP; q++) {
+            vc[2 * q] = *reinterpret_cast<const double2 *>(Db + (size_t)(STRIP * q + r) * SB + c0);
+            vc[2 * q + 1] = *reinterpret_cast<const double2 *>(Db + (size_t)(STRIP * q + r) * SB + c0 + 2);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int e = 2 * (m * kThreads + t);
+            *reinterpret_cast<double2 *>(Bs + (e / SB) * TS + e % SB) = vb[m];
+        }
+#pragma unroll
+        for (int q = 0; q < NSTRIP; q++) {
+            // va[2q], va[2q+1] hold strip q of L (rows 16q..16q+15)
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const int e = 2 * (m * kThreads + t);
+                *reinterpret_cast<double2 *>(As + (e / SB) * TS + e % SB) = va[2 * q + m];
+            }
+            __syncthreads();
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+            for (int k = 0; k < SB; k++) {
+                const double av = As[r * TS + k];
+                const double2 b01 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0);
+                const double2 b23 = *reinterpret_cast<const double2 *>(Bs + k * TS + c0 + 2);
+                acc[0] = fma(av, b01.x, acc[0]);
+                acc[1] = fma(av, b01.y, acc[1]);
+                acc[2] = fma(av, b23.x, acc[2]);
+                acc[3] = fma(av, b23.y, acc[3]);
+            }
+            double *row = Db + (size_t)(STRIP * q + r) * SB + c0;
+            *reinterpret_cast<double2 *>(row) = make_double2(vc[2 * q].x - acc[0], vc[2 * q].y - acc[1]);
+            *reinterpret_cast<double2 *>(row + 2) = make_double2(vc[2 * q + 1].x - acc[2], vc[2 * q + 1].y - acc[3]);
+            __syncthreads();          // LDS reuse; the strips' stores precede the factor's loads
+        }
+        diag_factor(I, Dall, Liall, Uiall, zpiv);
+        return;
+    }
+    const int w = (int)blockIdx.x - 1, blk = 1 + w / NSTRIP, s0 = STRIP * (w % NSTRIP);
     const int I = act[blk / na], J = act[blk % na];
     double *c = (I > J) ? Y.L(I, J) : (I < J) ? Y.U(I, J) : Y.D(I);
     row_strip_gemm<true>(Y.L(I, K), Y.U(K, J), c, s0, As, Bs);
@@ -641,6 +714,7 @@ int symbolic(pxg_ctx *ctx)
     S.hperm = perm;
     S.hjp = jp;
     S.hap = ap;
+    S.hal = al;
     S.hfb = fb;
     S.n = n;
     S.nb = nb;
@@ -669,17 +743,32 @@ Layout layout(SolveState &S)
 
 // the per-step launch chains depend only on the pattern and the solver's own
 // buffers: enqueued directly, or captured once into a HIP graph and replayed
+// the next diagonal factor joins the trailing update when the update has at
+// least this many active blocks (below it the update is shorter than the
+// fused kernel's extra block product)
+constexpr int kFuseMinActive = 16;
+
 int enqueue_factor_steps(SolveState &S, hipStream_t st)
 {
     const Layout Y = layout(S);
     double *D = S.F.p + (size_t)(2 * S.ns) * SB2;
+    if (S.nb > 0) hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, 0, D, S.Li.p, S.Ui.p, S.zpiv.p);
     for (int K = 0; K < S.nb; K++) {
-        hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, K, D, S.Li.p, S.Ui.p, S.zpiv.p);
+        // D_K is factored (by k_diag or by the previous step's fused update)
         const int na = S.hap[K + 1] - S.hap[K];
-        if (na > 0) {
-            const int *act = S.al.p + S.hap[K];
-            hipLaunchKernelGGL(k_panel, dim3(2 * na * NSTRIP), dim3(256), 0, st, K, act, na, Y, S.Li.p, S.Ui.p);
+        const bool next = K + 1 < S.nb;
+        if (na == 0) {
+            if (next) hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, K + 1, D, S.Li.p, S.Ui.p, S.zpiv.p);
+            continue;
+        }
+        const int *act = S.al.p + S.hap[K];
+        hipLaunchKernelGGL(k_panel, dim3(2 * na * NSTRIP), dim3(256), 0, st, K, act, na, Y, S.Li.p, S.Ui.p);
+        if (S.hal[(size_t)S.hap[K]] == K + 1 && na >= kFuseMinActive) {
+            hipLaunchKernelGGL(k_update_diag, dim3(1 + (na * na - 1) * NSTRIP), dim3(256), 0, st, K, act, na, Y, D,
+                               S.Li.p, S.Ui.p, S.zpiv.p);
+        } else {
             hipLaunchKernelGGL(k_update, dim3(na * na * NSTRIP), dim3(256), 0, st, K, act, na, Y);
+            if (next) hipLaunchKernelGGL(k_diag, dim3(1), dim3(256), 0, st, K + 1, D, S.Li.p, S.Ui.p, S.zpiv.p);
         }
     }
     PXG_HIP(hipGetLastError());
