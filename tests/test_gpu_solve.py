@@ -291,3 +291,44 @@ def test_ch_256_config3_full_size():
     assert len(iters) == 20 and np.isfinite(U).all()
     assert abs(m1 / m0 - 1.0) <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("dim,nx,ny,nz", [(2, 20, 16, 1), (3, 10, 8, 6)])
+def test_solve_dynamic_fracture_systems(oracle, dim, nx, ny, nz):
+    """The direct solve on another element's system: the implicit dynamic
+    fracture element (dim dofs per node, a pre-cut crack, bonds broken by the
+    committed step) in 2D and 3D, against the oracle's LDU."""
+    import perix_amd
+    print("dynamic fracture system dim %d %dx%dx%d" % (dim, nx, ny, nz), flush=True)
+    h = 0.05 if dim == 2 else 0.1
+    g = perix_amd.StructuredGrid(dim, nx, ny, nz, hi=(nx * h, ny * h, nz * h if dim == 3 else 1.0))
+    cloud = perix_amd.build_structured_cloud(g)
+    delta = perix_amd.horizon_radius(g, 3.015)
+    ctx = pxg.Context(0)
+    ctx.set_points(cloud.coords, cloud.vols, cloud.mirror, cloud.dxyz, dim)
+    ctx.set_cracks(segs=[(0.5, 0.0, 0.5, 0.3)])
+    ctx.build_families(delta)
+    ctx.build_operators(delta, float(max(cloud.dxyz)))
+    ctx.build_csr(dim)
+    c11, c12, c66 = problems.elastic_constants(dim, False, problems.E_STEEL, problems.NU)
+    rs = np.random.RandomState(3)
+    nd = cloud.n * dim
+    uold = 1e-6 * rs.standard_normal(nd)
+    ctx.dynfrac_setup(pxg.FracParams(dim, c11, c12, c66, problems.RHO, 1e5, -2e5, 3e4, 2e-3, 1, 1), uold,
+                      v0=[0.3, -0.1, 0.2][:dim])
+    uold2 = uold + 2e-4 * rs.standard_normal(nd)
+    ctx.dynfrac_commit(uold2, 1e-7)
+    u = uold2 + 1e-7 * rs.standard_normal(nd)
+    k, r = ctx.assemble_dynfrac(1e-7, u, uold2)
+    rp, cols = ctx.csr()
+    perm, jp = ctx.solve_ordering()
+    orc, ox, operm, ojp = oracle.ldu_solve(rp, cols, k, r)
+    assert orc == 0
+    assert np.array_equal(perm, operm) and np.array_equal(jp, ojp)
+    x = ctx.solve(k, r)
+    A = sp.csr_matrix((k, cols, rp), shape=(len(r), len(r)))
+    err = np.abs(x - ox).max() / np.abs(ox).max()
+    res = np.linalg.norm(A @ x - r) / np.linalg.norm(r)
+    print("  %s: vs oracle %.2e, residual %.2e" % (ctx.solve_info(), err, res))
+    assert err <= 1e-9 and res <= 1e-12
+    ctx.close()
