@@ -69,112 +69,183 @@ __global__ __launch_bounds__(256) void k_scatter(long long nnz, const double *__
 }
 
 // (1) D_K = L U in place (unit L below the diagonal, U on and above), with
-// L^-1 (unit lower) and U^-1 (upper) formed in the same pivot walk.  Thread t
-// keeps a 4 x 4 tile (rows 4 (t >> 4).., columns 4 (t & 15)..) of the block
-// and of both inverses in registers.  Pivot k: the owners of row k and of
-// column k publish them to LDS (double-buffered, so one barrier per pivot),
-// then every thread applies
-//   A: l_i = a_ik / a_kk (stored as L), a_ij -= l_i a_kj  (i, j > k);
-//   L^-1 by row operations: X[i][:] -= l_i X[k][:]  (i > k; row k is final);
-//   U^-1 by column operations: Y[:][k] /= a_kk, Y[:][j] -= Y[:][k] a_kj (j > k).
+// L^-1 = X (unit lower) and U^-1 = Y (upper), blocked in 8 x 8 sub-blocks
+// (rows and columns b0..b0+7 of round kb), all three matrices in LDS:
+//   (a) wave 0 factors the diagonal sub-block, lane r holding its row r in
+//       registers, the pivot row read across lanes (readlane); its unit-L
+//       and U inverses (Lx, Uy) come out of the same walk;
+//   (b) the panels A_i,kb = A_i,kb Uy and A_kb,j = Lx A_kb,j, and the
+//       inverses' own block row / column X_kb,: = Lx X_kb,: and
+//       Y_:,kb = Y_:,kb Uy, one thread per row or column;
+//   (c) the trailing updates A_ij -= A_i,kb A_kb,j, X_i,: -= A_i,kb X_kb,:
+//       and Y_:,j -= Y_:,kb A_kb,j, four consecutive entries per item.
 // A zero pivot records the least failing equation (new order) in *zpiv and
 // continues with 1, so the arithmetic stays finite; the host reports it.
-__device__ __forceinline__ void diag_factor(int K, double *__restrict__ Dall, double *__restrict__ Liall,
-                                            double *__restrict__ Uiall, int *zpiv)
+// The pivots stay a chain of 64 per block: the walks take about 13 us of a
+// block's 29 us, the panels and updates the rest.
+constexpr int BD = 8;                 // sub-block edge of the blocked diagonal factor
+
+__device__ __forceinline__ double read_lane(double v, int lane)
 {
-    __shared__ __align__(16) double rowA[2][SB], colA[2][SB], rowX[2][SB], colY[2][SB];
-    const int t = (int)threadIdx.x, tr = t >> 4, tc = t & 15, r0 = 4 * tr, c0 = 4 * tc;
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(hi, lane), __builtin_amdgcn_readlane(lo, lane));
+}
+
+__device__ __forceinline__ void diag_factor(int K, double *__restrict__ Dall, double *__restrict__ Liall,
+                                                    double *__restrict__ Uiall, int *zpiv)
+{
+    __shared__ __align__(16) double A[SB * TS], X[SB * TS], Y[SB * TS];
+    __shared__ double Lx[BD * BD], Uy[BD * BD];
+    const int t = (int)threadIdx.x;
     double *D = Dall + (size_t)K * SB2;
-    double a[4][4], x[4][4], y[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const double2 v01 = *reinterpret_cast<const double2 *>(D + (r0 + i) * SB + c0);
-        const double2 v23 = *reinterpret_cast<const double2 *>(D + (r0 + i) * SB + c0 + 2);
-        a[i][0] = v01.x; a[i][1] = v01.y; a[i][2] = v23.x; a[i][3] = v23.y;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const double e = (r0 + i == c0 + j) ? 1.0 : 0.0;
-            x[i][j] = e;
-            y[i][j] = e;
-        }
+    for (int m = 0; m < 8; m++) {
+        const int e = 2 * (m * kThreads + t), r = e / SB, c = e % SB;
+        *reinterpret_cast<double2 *>(A + r * TS + c) = *reinterpret_cast<const double2 *>(D + e);
+        *reinterpret_cast<double2 *>(X + r * TS + c) = make_double2(r == c ? 1.0 : 0.0, r == c + 1 ? 1.0 : 0.0);
+        *reinterpret_cast<double2 *>(Y + r * TS + c) = make_double2(r == c ? 1.0 : 0.0, r == c + 1 ? 1.0 : 0.0);
     }
-    for (int kk = 0; kk < SB; kk += 4) {
-        const bool own_row = tr == (kk >> 2), own_col = tc == (kk >> 2);
+    __syncthreads();
+    int zfirst = INT_MAX;
+    for (int kb = 0; kb < SB / BD; kb++) {
+        const int b0 = BD * kb, nr = SB - b0 - BD;          // rows (columns) after the sub-block
+        // (a) the diagonal sub-block, wave 0 (lanes 0..7 carry its rows)
+        if (t < 64) {
+            const int r = t & (BD - 1);
+            double a[BD], x[BD], y[BD];
 #pragma unroll
-        for (int kr = 0; kr < 4; kr++) {
-            const int k = kk + kr, b = k & 1;
-            if (own_row) {
-                *reinterpret_cast<double2 *>(&rowA[b][c0]) = make_double2(a[kr][0], a[kr][1]);
-                *reinterpret_cast<double2 *>(&rowA[b][c0 + 2]) = make_double2(a[kr][2], a[kr][3]);
-                *reinterpret_cast<double2 *>(&rowX[b][c0]) = make_double2(x[kr][0], x[kr][1]);
-                *reinterpret_cast<double2 *>(&rowX[b][c0 + 2]) = make_double2(x[kr][2], x[kr][3]);
+            for (int j = 0; j < BD; j++) {
+                a[j] = A[(b0 + r) * TS + b0 + j];
+                x[j] = (r == j) ? 1.0 : 0.0;
+                y[j] = x[j];
             }
-            if (own_col) {
-                *reinterpret_cast<double2 *>(&colA[b][r0]) = make_double2(a[0][kr], a[1][kr]);
-                *reinterpret_cast<double2 *>(&colA[b][r0 + 2]) = make_double2(a[2][kr], a[3][kr]);
-                *reinterpret_cast<double2 *>(&colY[b][r0]) = make_double2(y[0][kr], y[1][kr]);
-                *reinterpret_cast<double2 *>(&colY[b][r0 + 2]) = make_double2(y[2][kr], y[3][kr]);
+#pragma unroll
+            for (int k = 0; k < BD; k++) {
+                double p = read_lane(a[k], k);
+                if (p == 0.0) {
+                    zfirst = min(zfirst, K * SB + b0 + k);
+                    p = 1.0;
+                }
+                double rinv = __builtin_amdgcn_rcp(p);
+                rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
+                rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
+                double uk[BD], xk[BD];
+#pragma unroll
+                for (int j = 0; j < BD; j++) {
+                    uk[j] = j > k ? read_lane(a[j], k) : 0.0;
+                    xk[j] = j <= k ? read_lane(x[j], k) : 0.0;
+                }
+                const double l = (r > k) ? a[k] * rinv : 0.0;
+                const double yk = y[k] * rinv;
+#pragma unroll
+                for (int j = 0; j < BD; j++) {
+                    if (j > k) {
+                        a[j] = fma(-l, uk[j], a[j]);
+                        y[j] = fma(-yk, uk[j], y[j]);
+                    } else {
+                        x[j] = fma(-l, xk[j], x[j]);
+                    }
+                }
+                if (r > k) a[k] = l;
+                y[k] = yk;
+            }
+            if (t < BD) {
+#pragma unroll
+                for (int j = 0; j < BD; j++) {
+                    A[(b0 + r) * TS + b0 + j] = a[j];
+                    Lx[r * BD + j] = x[j];
+                    Uy[r * BD + j] = y[j];
+                }
+            }
+        }
+        __syncthreads();
+        // (b) panels and the inverses' block row / column: 2 nr + 2 (b0 + 8) <= 128 items
+        {
+            double v[BD], o[BD];
+            int kind = -1, idx = 0;
+            if (t < nr) { kind = 0; idx = b0 + BD + t; }                               // row of A below
+            else if (t < 2 * nr) { kind = 1; idx = b0 + BD + t - nr; }                 // column of A to the right
+            else if (t < 2 * nr + b0 + BD) { kind = 2; idx = t - 2 * nr; }             // column of X (< b0+8)
+            else if (t < 2 * nr + 2 * (b0 + BD)) { kind = 3; idx = t - 2 * nr - b0 - BD; }  // row of Y (< b0+8)
+            if (kind == 0 || kind == 3) {
+                double *M = (kind == 0) ? A : Y;
+#pragma unroll
+                for (int m = 0; m < BD; m++) v[m] = M[idx * TS + b0 + m];
+#pragma unroll
+                for (int c = 0; c < BD; c++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < BD; m++) acc = fma(v[m], Uy[m * BD + c], acc);
+                    o[c] = acc;
+                }
+            } else if (kind == 1 || kind == 2) {
+                double *M = (kind == 1) ? A : X;
+#pragma unroll
+                for (int m = 0; m < BD; m++) v[m] = M[(b0 + m) * TS + idx];
+#pragma unroll
+                for (int c = 0; c < BD; c++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < BD; m++) acc = fma(Lx[c * BD + m], v[m], acc);
+                    o[c] = acc;
+                }
             }
             __syncthreads();
-            double p = rowA[b][k];
-            if (p == 0.0) {
-                if (t == 0) atomicMin(zpiv, K * SB + k);
-                p = 1.0;
-            }
-            // 1/p: hardware estimate and two Newton steps (full double accuracy)
-            double rinv = __builtin_amdgcn_rcp(p);
-            rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
-            rinv = fma(fma(-p, rinv, 1.0), rinv, rinv);
-            const double2 ca01 = *reinterpret_cast<const double2 *>(&colA[b][r0]);
-            const double2 ca23 = *reinterpret_cast<const double2 *>(&colA[b][r0 + 2]);
-            const double2 cy01 = *reinterpret_cast<const double2 *>(&colY[b][r0]);
-            const double2 cy23 = *reinterpret_cast<const double2 *>(&colY[b][r0 + 2]);
-            const double2 ra01 = *reinterpret_cast<const double2 *>(&rowA[b][c0]);
-            const double2 ra23 = *reinterpret_cast<const double2 *>(&rowA[b][c0 + 2]);
-            const double2 rx01 = *reinterpret_cast<const double2 *>(&rowX[b][c0]);
-            const double2 rx23 = *reinterpret_cast<const double2 *>(&rowX[b][c0 + 2]);
-            // multipliers masked to zero outside the update ranges, so every
-            // entry takes the same (exact no-op where masked) fused update
-            const double cl[4] = {ca01.x, ca01.y, ca23.x, ca23.y};
-            const double cyv[4] = {cy01.x, cy01.y, cy23.x, cy23.y};
-            const double rav[4] = {ra01.x, ra01.y, ra23.x, ra23.y};
-            const double rx[4] = {rx01.x, rx01.y, rx23.x, rx23.y};
-            double l[4], lm[4], yk[4], ra[4];
+            if (kind == 0 || kind == 3) {
+                double *M = (kind == 0) ? A : Y;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                l[i] = cl[i] * rinv;
-                lm[i] = (r0 + i > k) ? l[i] : 0.0;
-                yk[i] = cyv[i] * rinv;
-                ra[i] = (c0 + i > k) ? rav[i] : 0.0;
-            }
+                for (int c = 0; c < BD; c++) M[idx * TS + b0 + c] = o[c];
+            } else if (kind == 1 || kind == 2) {
+                double *M = (kind == 1) ? A : X;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    a[i][j] = fma(-lm[i], ra[j], a[i][j]);
-                    x[i][j] = fma(-lm[i], rx[j], x[i][j]);
-                    y[i][j] = fma(-yk[i], ra[j], y[i][j]);
-                }
-            }
-            if (own_col) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    if (r0 + i > k) a[i][kr] = l[i];
-                    y[i][kr] = yk[i];
-                }
+                for (int c = 0; c < BD; c++) M[(b0 + c) * TS + idx] = o[c];
             }
         }
+        __syncthreads();
+        // (c) trailing updates, four consecutive entries per item:
+        //   A rows/cols >= b0+8; X rows >= b0+8, cols < b0+8; Y rows < b0+8, cols >= b0+8
+        if (nr > 0) {
+            const int q4 = nr / 4, qx = (b0 + BD) / 4;
+            const int nA = nr * q4, nX = nr * qx, nY = (b0 + BD) * q4;
+            for (int w = t; w < nA + nX + nY; w += kThreads) {
+                const double *Lf;   // left factor row (8 values in the kb column block)
+                const double *Rf;   // right factor rows b0..b0+7, four columns
+                double *Cp;
+                if (w < nA) {
+                    const int i = b0 + BD + w / q4, j = b0 + BD + 4 * (w % q4);
+                    Lf = A + i * TS + b0; Rf = A + b0 * TS + j; Cp = A + i * TS + j;
+                } else if (w < nA + nX) {
+                    const int w2 = w - nA, i = b0 + BD + w2 / qx, j = 4 * (w2 % qx);
+                    Lf = A + i * TS + b0; Rf = X + b0 * TS + j; Cp = X + i * TS + j;
+                } else {
+                    const int w3 = w - nA - nX, i = w3 / q4, j = b0 + BD + 4 * (w3 % q4);
+                    Lf = Y + i * TS + b0; Rf = A + b0 * TS + j; Cp = Y + i * TS + j;
+                }
+                double2 c01 = *reinterpret_cast<const double2 *>(Cp), c23 = *reinterpret_cast<const double2 *>(Cp + 2);
+#pragma unroll
+                for (int m = 0; m < BD; m++) {
+                    const double lv = Lf[m];
+                    const double2 r01 = *reinterpret_cast<const double2 *>(Rf + m * TS);
+                    const double2 r23 = *reinterpret_cast<const double2 *>(Rf + m * TS + 2);
+                    c01.x = fma(-lv, r01.x, c01.x);
+                    c01.y = fma(-lv, r01.y, c01.y);
+                    c23.x = fma(-lv, r23.x, c23.x);
+                    c23.y = fma(-lv, r23.y, c23.y);
+                }
+                *reinterpret_cast<double2 *>(Cp) = c01;
+                *reinterpret_cast<double2 *>(Cp + 2) = c23;
+            }
+        }
+        __syncthreads();
     }
+    if (t == 0 && zfirst != INT_MAX) atomicMin(zpiv, zfirst);
     double *Li = Liall + (size_t)K * SB2, *Ui = Uiall + (size_t)K * SB2;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const size_t o = (size_t)(r0 + i) * SB + c0;
-        *reinterpret_cast<double2 *>(D + o) = make_double2(a[i][0], a[i][1]);
-        *reinterpret_cast<double2 *>(D + o + 2) = make_double2(a[i][2], a[i][3]);
-        *reinterpret_cast<double2 *>(Li + o) = make_double2(x[i][0], x[i][1]);
-        *reinterpret_cast<double2 *>(Li + o + 2) = make_double2(x[i][2], x[i][3]);
-        *reinterpret_cast<double2 *>(Ui + o) = make_double2(y[i][0], y[i][1]);
-        *reinterpret_cast<double2 *>(Ui + o + 2) = make_double2(y[i][2], y[i][3]);
+    for (int m = 0; m < 8; m++) {
+        const int e = 2 * (m * kThreads + t), r = e / SB, c = e % SB;
+        *reinterpret_cast<double2 *>(D + e) = *reinterpret_cast<const double2 *>(A + r * TS + c);
+        *reinterpret_cast<double2 *>(Li + e) = *reinterpret_cast<const double2 *>(X + r * TS + c);
+        *reinterpret_cast<double2 *>(Ui + e) = *reinterpret_cast<const double2 *>(Y + r * TS + c);
     }
 }
 
