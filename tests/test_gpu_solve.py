@@ -37,7 +37,7 @@ def _ch(nx):
     return ctx, k, r
 
 
-@pytest.mark.parametrize("kind,nx", [("poisson", 30), ("poisson", 100), ("ch", 24), ("ch", 64)])
+@pytest.mark.parametrize("kind,nx", [("poisson", 30), ("poisson", 100), ("ch", 24), ("ch", 64), ("ch", 128)])
 def test_solve_matches_oracle(oracle, kind, nx):
     print("solve parity %s %d" % (kind, nx), flush=True)
     ctx, k, r = _poisson(nx) if kind == "poisson" else _ch(nx)
