@@ -42,7 +42,8 @@ def log(*a):
 
 
 class quiet_stdout:
-    """Redirects fd 1 to fd 2 (the reference prints progress on stdout)."""
+    """Redirects fd 1 to fd 2 (the reference prints progress on stdout),
+    flushing the C library's buffered output before fd 1 is restored."""
 
     def __enter__(self):
         sys.stdout.flush()
@@ -51,6 +52,7 @@ class quiet_stdout:
 
     def __exit__(self, *exc):
         sys.stdout.flush()
+        ctypes.CDLL(None).fflush(None)
         os.dup2(self.saved, 1)
         os.close(self.saved)
 
@@ -210,10 +212,7 @@ def reference_ldu_ch(nx=128):
     from oracle import refpy
     if not refpy.available():
         return None
-    sys.stdout.flush()
-    saved = os.dup(1)
-    os.dup2(2, 1)
-    try:
+    with quiet_stdout():
         c = refpy.Case(refpy.ch_deck(nx, 1, "openmp"))
         u, _ = c.initial_state()
         u = c.preset(u, 0.0)
@@ -223,10 +222,6 @@ def reference_ldu_ch(nx=128):
         rc, x, perm, jp = c.default_solve(k, r)
         sec = time.perf_counter() - t0
         c.close()
-    finally:
-        ctypes.CDLL(None).fflush(None)      # the reference's buffered C stdio output
-        os.dup2(saved, 1)
-        os.close(saved)
     return {"seconds": sec, "equations": len(r), "profile": int(jp[-1]), "rc": rc,
             "gflop": 2.0 * float(np.sum(np.diff(jp).astype(float) ** 2)) / 1e9,
             "system": "Cahn-Hilliard %dx%d, first Newton system of the config-3 loop" % (nx, nx)}
