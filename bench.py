@@ -187,7 +187,7 @@ def cpu_baselines():
 
 
 def implicit_solve_split(nx=100):
-    """Config 1 run whole by the reference's own driver (oracle/_ref, Newton
+    """The Poisson 100 x 100 job run whole by the reference's own driver (oracle/_ref, Newton
     with its default LDU solver), once on its OpenMP assembler and once with
     "assemble": "cuda" through the seam over libpxg: the reference's own
     timers split each run into assembly and linear solve, the part that
@@ -205,7 +205,7 @@ def implicit_solve_split(nx=100):
 
 def reference_ldu_ch(nx=128):
     """The reference's DefaultSolver (oracle/_ref, RCM + skyline LDU, one
-    thread) on config 3's first Newton system at nx x nx: seconds of one
+    thread) on the Cahn-Hilliard spinodal loop's first Newton system at nx x nx: seconds of one
     factorisation and solve.  None when the reference build is absent.  The
     reference prints its progress to stdout; it goes to stderr here, so that
     the bench's stdout stays one JSON line."""
@@ -224,13 +224,13 @@ def reference_ldu_ch(nx=128):
         c.close()
     return {"seconds": sec, "equations": len(r), "profile": int(jp[-1]), "rc": rc,
             "gflop": 2.0 * float(np.sum(np.diff(jp).astype(float) ** 2)) / 1e9,
-            "system": "Cahn-Hilliard %dx%d, first Newton system of the config-3 loop" % (nx, nx)}
+            "system": "Cahn-Hilliard %dx%d, first Newton system of the spinodal loop (chi 3, kappa 1e-2, dt 1e-3)" % (nx, nx)}
 
 
 def implicit_case(problems, pxg, ch_nx=256, ch_steps=20):
     """The implicit jobs with the linear solve on the device (the reference's
-    default LDU as a block-skyline LU): config 1 (Poisson 100 x 100, static
-    Newton) and config 3 (Cahn-Hilliard ch_nx^2, ch_steps backward-Euler
+    default LDU as a block-skyline LU): Poisson 100 x 100 (static Newton)
+    and the Cahn-Hilliard spinodal job (ch_nx^2, ch_steps backward-Euler
     steps, Newton max 20 / abs 1e-9 / rel 1e-8), wall time of the whole job
     after a warm-up solve that builds the ordering and captures the launch
     graphs, and the factorisation / substitution split of one Newton system."""

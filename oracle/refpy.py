@@ -168,7 +168,7 @@ def _ip(a):
 
 
 def poisson_deck(nx, assemble):
-    """Config 1: Poisson on the nx x nx unit square, sigma = 1, scalar source
+    """Poisson on the nx x nx unit square, sigma = 1, scalar source
     2 pi^2 (PoissonElement.h:74), Dirichlet u = 0 on the four ghost walls,
     static Newton with the reference's default LDU solver; "assemble" is
     "openmp" (the reference's CPU assembler) or "cuda" (the seam)."""
@@ -182,7 +182,7 @@ def poisson_deck(nx, assemble):
 
 
 def ch_deck(nx, steps, assemble):
-    """Config 3's loop on an nx x nx square: c0 = 0.5 + U(-0.01, 0.01)
+    """The Cahn-Hilliard spinodal loop on an nx x nx square: c0 = 0.5 + U(-0.01, 0.01)
     (RandomIC, seed 12345), chi = 3, kappa = 1e-2, M = 1, backward Euler
     dt = 1e-3 (fixed), Neumann c and mu on all ghost walls, Newton (the
     mobility Picard-frozen per iteration, max 20, abs 1e-9, rel 1e-8) with the

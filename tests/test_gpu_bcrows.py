@@ -1,7 +1,7 @@
 """BC row elimination of the implicit CSR system on the device
 (pxg_csr_set_bcs / pxg_csr_apply_bcs[_dev]) against the reference's own
 output (golden fixtures) and the oracle's sequential restatement: bit for
-bit.  At full size (configs 1 and 3) the size-independent properties are
+bit.  At full size (Poisson 100 x 100, Cahn-Hilliard 256 x 256) the size-independent properties are
 checked: unconstrained rows untouched, constrained rows hold exactly their
 diagonal and mirror entry, and a second application changes nothing."""
 import os

@@ -40,20 +40,20 @@ def _walls(dim):
 
 
 def poisson_deck(refpy, nx, assemble):
-    """Config 1 (oracle/refpy.poisson_deck): Poisson on the 100 x 100 unit
+    """oracle/refpy.poisson_deck: Poisson on the 100 x 100 unit
     square, Dirichlet ghost walls, static Newton with the default LDU."""
     return refpy.poisson_deck(nx, assemble)
 
 
 def ch_deck(refpy, nx, steps, assemble):
-    """Config 3 loop (oracle/refpy.ch_deck): RandomIC seed 12345, chi = 3,
+    """The Cahn-Hilliard spinodal loop (oracle/refpy.ch_deck): RandomIC seed 12345, chi = 3,
     kappa = 1e-2, M = 1, backward Euler dt = 1e-3, Neumann c and mu ghosts,
     Newton with the default LDU solver."""
     return refpy.ch_deck(nx, steps, assemble)
 
 
 def plate_deck(refpy, nx, ny, steps, assemble):
-    """Config 2 geometry: the 0.2 x 0.1 m plate, two force-only edge notches,
+    """The 0.2 x 0.1 m plate geometry: two force-only edge notches,
     the 16.5 m/s impactor on the bottom ghosts with 75 <= x <= 125 mm,
     Neumann ghosts elsewhere, critical stretch 0.01 (G0 from
     FractureCriterion.h:30-43), 0.9 of the CFL step."""

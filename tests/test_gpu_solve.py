@@ -166,7 +166,7 @@ def refpy():
 
 
 def test_static_poisson_matches_reference_driver(refpy):
-    """Config 1 (Poisson 100 x 100, Dirichlet ghost walls, f = 2 pi^2) as the
+    """The Poisson 100 x 100 job (Dirichlet ghost walls, f = 2 pi^2) as the
     reference's static job with its default LDU solver, and on the device."""
     print("static poisson 100", flush=True)
     nx = 100
@@ -185,7 +185,7 @@ def test_static_poisson_matches_reference_driver(refpy):
 
 
 def test_transient_ch_matches_reference_driver(refpy):
-    """Config 3's loop on a 48 x 48 square for 5 backward-Euler steps (Newton
+    """The Cahn-Hilliard spinodal loop on a 48 x 48 square for 5 backward-Euler steps (Newton
     max 20, abs 1e-9, rel 1e-8, Neumann ghosts, RandomIC seed 12345)."""
     import test_gpu_seam as seam
     print("transient ch 48", flush=True)
@@ -261,13 +261,13 @@ def test_reference_linear_solver_backend(refpy):
     c.close()
 
 
-def test_ch_256_config3_full_size():
-    """Config 3 at its own size (Cahn-Hilliard 256 x 256, 133,120 equations):
+def test_ch_256_spinodal_full_size():
+    """The Cahn-Hilliard spinodal job at 256 x 256 (133,120 equations):
     the first Newton system's device solve leaves a residual below 1e-11 of
     |rhs|, and the whole 20-step job converges every step and conserves the
     bulk concentration (Neumann ghosts: the reference conserves it to the last
     bit; here within 1e-12)."""
-    print("config 3 at 256 x 256", flush=True)
+    print("Cahn-Hilliard spinodal 256 x 256", flush=True)
     ctx, cloud, delta, u0 = problems.cahnhilliard_square(nx=256, ny=256)
     k, r = ctx.assemble_ch(1e-2, 3.0, 1.0, 1e-3, u0, u0)
     k, r = ctx.csr_apply_bcs(1e-3, u0, k, r)
