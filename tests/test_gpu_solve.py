@@ -332,3 +332,28 @@ def test_solve_dynamic_fracture_systems(oracle, dim, nx, ny, nz):
     print("  %s: vs oracle %.2e, residual %.2e" % (ctx.solve_info(), err, res))
     assert err <= 1e-9 and res <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("n", [1, 50, 64, 130])
+def test_solve_closed_form_laplacian(n):
+    """-u'' = 1 on n interior points with u = 0 at both ends: the difference
+    equations' exact solution x_i = i (n + 1 - i) h^2 / 2, through the
+    solver's own pattern -- one partial block, exactly one block, and three
+    blocks with padded equations."""
+    h = 1.0 / (n + 1)
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        for j in (i - 1, i, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(2.0 / h ** 2 if i == j else -1.0 / h ** 2)
+        rp.append(len(cols))
+    ctx = pxg.Context(0)
+    ctx.solve_set_pattern(np.array(rp, np.int32), np.array(cols, np.int32))
+    x = ctx.solve(np.array(vals), np.ones(n))
+    i = np.arange(1, n + 1)
+    exact = i * (n + 1 - i) * h * h / 2.0
+    assert np.abs(x - exact).max() <= 1e-12 * exact.max()
+    info = ctx.solve_info()
+    assert info["equations"] == n and info["profile"] == n - 1 and info["blocks"] == (n + 63) // 64
+    ctx.close()
