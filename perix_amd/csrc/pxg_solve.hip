@@ -21,8 +21,11 @@
 //           U_KI = L_KK^-1 A_KI for the active set {I > K : FB[I] <= K};
 //           (3) trailing update A_IJ -= L_IK U_KJ over active x active.
 // Each step's work is SB x SB x SB products (the panel and update kernels are
-// plain FP64 GEMM tiles, one workgroup per block); the chain of n pivots is the
-// serial part, as it is in the reference.  Substitution walks the blocks with
+// plain FP64 GEMM tiles, one workgroup per 16-row strip); the chain of n
+// pivots is the serial part, as it is in the reference.  When the update has
+// many blocks, the next step's diagonal factor runs inside it (k_update_diag).
+// The per-step launch chains depend only on the pattern and are replayed as
+// HIP graphs.  Substitution walks the blocks with
 // the inverted diagonal factors: forward y_K = L_KK^-1 y_K, then y_I -= L_IK
 // y_K for the active I; backward x_K = U_KK^-1 z_K, then z_I -= U_IK x_K for
 // I in [FB[K], K).  The orderings (RCM, envelope) are computed on the host
