@@ -617,6 +617,8 @@ extern "C" int pxg_frac_setup(pxg_ctx *ctx, const pxg_frac_params *p, int precis
     PXG_HIP(F.bc.inv_off.alloc((size_t)ctx->n + 1));
     PXG_HIP(hipMemsetAsync(F.bc.inv_off.p, 0, sizeof(int) * ((size_t)ctx->n + 1), st));
     PXG_HIP(F.bc.inv_q.alloc(1));
+    PXG_HIP(F.bc.node.alloc(1));               // the step reads entry 0 whatever the program
+    PXG_HIP(hipMemsetAsync(F.bc.node.p, 0, sizeof(int), st));
     PXG_HIP(F.gstate.alloc(4));
     if (!F.gstate_host) PXG_HIP(hipHostMalloc((void **)&F.gstate_host, 4 * sizeof(int), hipHostMallocDefault));
     PXG_HIP(F.anybroke.alloc(1));

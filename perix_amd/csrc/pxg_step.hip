@@ -301,7 +301,7 @@ struct GroupData {
 // reads its own operators once per step, so keeping them in the caches only
 // evicts the staged rows and the other streams)
 template <int DIM, bool F32, int MODE, bool FUSED, bool NTO = false>
-__global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_frac_step(StepArgs a)
 {
     constexpr int S = StepCfg<DIM, F32>::S;
     constexpr int G = StepCfg<DIM, F32>::G;
@@ -317,6 +317,7 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     __shared__ unsigned long long s_gmax;     // guard maximum of the block
     double *s_f = s_dyn;
     double *s_rec = s_dyn + FROWS;
+    double *s_park = s_rec + (size_t)a.gcap * RS;   // 2D FP32C: 4 x 256 parked update inputs
 
     if constexpr (!FUSED) {
         if (a.gstate && a.gstate[0]) return;  // blow-up guard tripped earlier
@@ -343,19 +344,28 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
             if (a.step >= 2) { if (wv == 0) g_max = guard_slot_load(a.step - 1, a.gmax); g_past = guard_past(a.step - 1, a.ghist64); }
         }
     }
-    int4 nr = make_int4(0, 0, 0, 0);
+    // The loads of both rounds take clamped indices rather than branches: a
+    // load under a branch makes the compiler consume its value inside the
+    // branch, which waits for it before any later load can issue and splits
+    // the round in two.  A lane past the last node reads the last slot's
+    // records and is given an empty bond range.
+    const int gq = nvalid ? gi : a.n - 1;
+    const int4 nr = a.nrec[gq];
     int4 br = make_int4(-1, 0, 0, -1);
-    if (nvalid) {
-        nr = a.nrec[gi];
-        if constexpr (FUSED) br = a.brec[gi];
-    }
+    if constexpr (FUSED) br = a.brec[gq];
     const int *rows = a.ulist + (size_t)blk * a.gcap;
-    const int ida0 = tid < a.gcap ? rows[tid] : -1, idb0 = tid + 256 < a.gcap ? rows[tid + 256] : -1;
+    // the second 256 row slots exist only where some block has more rows
+    // (3D): the 2D plate's blocks stage under 256 rows in one load per thread
+    const bool two = a.gcap > 256;
+    const int ra0 = rows[min(tid, a.gcap - 1)];
+    int rb0 = -1;
+    if (two) rb0 = rows[min(tid + 256, a.gcap - 1)];
     const int nrows = a.rcnt[blk];
     const bool govf = a.bovf[blk];
+    const int ida0 = tid < a.gcap ? ra0 : -1, idb0 = tid + 256 < a.gcap ? rb0 : -1;
 
     // [b0, b1): the node's bonds in the stream layout; po: FP32C shared run
-    const int i = nr.x, b0 = nr.y, b1 = nr.z;
+    const int i = nr.x, b0 = nr.y, b1 = nvalid ? nr.z : nr.y;
     const int po = F32 ? nr.w : 0;
     // BC program of the node (fused path): own entry bq and its mode m_own;
     // the bcnt entries bp0.. whose mirror source it is, the first of them
@@ -369,32 +379,40 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
     double ui[DIM], xi[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-        ui[c] = nvalid ? a.u[(size_t)i * DIM + c] : 0.0;
-        xi[c] = (nvalid && live_dmg) ? a.xp[(size_t)i * DIM + c] : 0.0;
+        ui[c] = a.u[(size_t)i * DIM + c];
+        xi[c] = a.xp[(size_t)i * DIM + c];
     }
     // The update's own inputs: u^{n-1} of the lane's component (when out
     // aliases uold the lane reads its own entry before writing it; preset
     // targets written by other lanes do not use theirs), and g(0), dg/dt of
     // the one program entry whose value the lane needs first (its own direct
-    // value, else the first reflection it writes).  2D loads them here, so
-    // that their latency hides behind the staging: its grids are one round
-    // of resident waves, bound by the latency of each wave.  3D loads them
+    // value, else the first reflection it writes).  2D grids are one round of
+    // resident waves, bound by the latency of each wave, so 2D issues these
+    // loads with the staging loads (after them, so that the BC loads, which
+    // stay under branches, cannot hold the staging back), and FP32C, whose
+    // staging phase also holds its first bond group (below), parks them in
+    // LDS once the staging writes have waited for them: held through the
+    // family loop they would cost a resident wave per SIMD.  3D loads them
     // at the update: its grids run many rounds, and holding them would cost
-    // a resident wave per SIMD (100 VGPRs against 92).
+    // a resident wave per SIMD (100 VGPRs against 92).  The early form reads
+    // u^{n-1} on every lane (lanes sl >= DIM read the node's last component
+    // and drop it).
+    constexpr bool PRE = F32 && DIM == 2;     // first bond group before the barrier
     constexpr bool EARLY = DIM == 2;
     double uo_pre = 0.0, pv0 = 0.0, pvel = 0.0;
     int tn0 = -1;
     auto load_update = [&]() {
         if constexpr (MODE != MODE_ACCEL) {
-            if (upd) uo_pre = a.uold[(size_t)i * DIM + sl];
+            if (EARLY || upd) uo_pre = a.uold[(size_t)i * DIM + min(sl, DIM - 1)];
         }
         if constexpr (FUSED) {
-            if (m_inv0 == 1 || m_inv0 == 2) tn0 = a.bnode[qq0];
             const int qv = (m_own == 3) ? bq : (m_inv0 == 2 ? qq0 : -1);
             if (qv >= 0) { pv0 = a.bval0[qv * 3 + sl]; pvel = a.bvel[qv * 3 + sl]; }
+            // read on every lane (entry 0 stands in for none): under a branch
+            // the compiler consumes it there and waits for it
+            tn0 = a.bnode[max(qq0, 0)];
         }
     };
-    if constexpr (EARLY) load_update();
     const int A = b0 & ~(G - 1);
     int broke = 0;
 
@@ -406,56 +424,93 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         if constexpr (F32) P32c[c] = a.P32 + c * a.pst;
         else P64c[c] = a.P64 + c * a.pst;
     }
+    // The compiler tests the bond states inside the frozen/live branch,
+    // which waits for every load issued so far.  FP32C reads them after its
+    // operators (its first group goes out with the staging loads, and the
+    // wait then covers them all); FP64 reads them first (measured on the
+    // 400x200 plate: 10.75 us against 11.10 us for FP32C, 17.27 us against
+    // 18.17 us for FP64).
     auto load_group = [&](unsigned t0, GD &d) {
+        auto load_state = [&]() {
+            if (a.frozen) {
+                int bs[G];
+                vload(a.break_step + t0, bs);
+#pragma unroll
+                for (int k = 0; k < G; k++) d.ok[k] = bs[k] > a.frozen_k;
+            } else {
+                unsigned char hv[G];
+                vload(a.health + t0, hv);
+#pragma unroll
+                for (int k = 0; k < G; k++) d.ok[k] = hv[k] != 0;
+            }
+        };
         vload(a.gslot + t0, d.sv);
-        if (a.frozen) {
-            int bs[G];
-            vload(a.break_step + t0, bs);
-#pragma unroll
-            for (int k = 0; k < G; k++) d.ok[k] = bs[k] > a.frozen_k;
-        } else {
-            unsigned char hv[G];
-            vload(a.health + t0, hv);
-#pragma unroll
-            for (int k = 0; k < G; k++) d.ok[k] = hv[k] != 0;
-        }
+        if constexpr (!F32) load_state();
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             if constexpr (F32) vload<NTO>(P32c[c] + (unsigned)((int)t0 + po), d.p32[c]);
             else vload<NTO>(P64c[c] + t0, d.p64[c]);
         }
+        if constexpr (F32) load_state();
     };
     GD d;
     const int tfirst = A + G * sl;
+    // FP32C reads its first bond group on every lane, outside the family
+    // loop: it lies inside the node's 32-aligned run (or, for the last run,
+    // the zeroed slack after it), and out of the loop its loads go out
+    // together, where the loop's first load would wait for register reuse
+    // of its later rounds.  2D issues them with the staging, before the
+    // barrier (one memory latency fewer in its latency-bound step).
     // nbr keeps CSR order: a block without LDS rows reads bond t at t - sh
-    const int sh = (govf && nvalid) ? a.bsh[i] : 0;
+    const int sh = govf ? a.bsh[i] : 0;
 
     // stage the block's neighbour rows as LDS records {x_j, u_j}: two rows
     // per thread and round, every load of a round issued before its LDS
     // writes (the row ids are sorted node ids, so neighbouring lanes read
-    // neighbouring nodes); rounds beyond the first read their ids first
-    for (int q0 = 0; q0 == 0 || q0 < nrows; q0 += 512) {
-        const int qa = q0 + tid, qb = qa + 256;
-        const int ida = q0 == 0 ? ida0 : (qa < a.gcap ? rows[qa] : -1);
-        const int idb = q0 == 0 ? idb0 : (qb < a.gcap ? rows[qb] : -1);
-        const bool ha = ida >= 0, hb = idb >= 0;
-        const size_t ja = ha ? (size_t)ida * DIM : 0, jb = hb ? (size_t)idb * DIM : 0;
-        double ra[2 * DIM], rb[2 * DIM];
+    // neighbouring nodes); a slot past the block's rows (id -1) reads the
+    // lane's own node (already on its way to the same cache) and writes
+    // nothing.  Rounds beyond the first read their ids first.
+    double ra[2 * DIM], rb[2 * DIM];
+    auto stage_load = [&](int ida, int idb) {
+        const size_t ja = (size_t)(ida >= 0 ? ida : i) * DIM, jb = (size_t)(idb >= 0 ? idb : i) * DIM;
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
-            ra[c] = (ha && live_dmg) ? a.xp[ja + c] : 0.0;
-            ra[DIM + c] = ha ? a.u[ja + c] : 0.0;
-            rb[c] = (hb && live_dmg) ? a.xp[jb + c] : 0.0;
-            rb[DIM + c] = hb ? a.u[jb + c] : 0.0;
+            ra[c] = a.xp[ja + c];
+            ra[DIM + c] = a.u[ja + c];
         }
-        if (ha) {
+        if (two) {
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                rb[c] = a.xp[jb + c];
+                rb[DIM + c] = a.u[jb + c];
+            }
+        }
+    };
+    auto stage_store = [&](int qa, int ida, int idb) {
+        if (ida >= 0) {
 #pragma unroll
             for (int c = 0; c < 2 * DIM; c++) s_rec[qa * RS + c] = ra[c];
         }
-        if (hb) {
+        if (two && idb >= 0) {
 #pragma unroll
-            for (int c = 0; c < 2 * DIM; c++) s_rec[qb * RS + c] = rb[c];
+            for (int c = 0; c < 2 * DIM; c++) s_rec[(qa + 256) * RS + c] = rb[c];
         }
+    };
+    stage_load(ida0, idb0);
+    if constexpr (EARLY) load_update();
+    if constexpr (PRE) load_group((unsigned)tfirst, d);
+    stage_store(tid, ida0, idb0);
+    if constexpr (PRE) {
+        s_park[tid] = uo_pre;
+        s_park[256 + tid] = pv0;
+        s_park[512 + tid] = pvel;
+        reinterpret_cast<int *>(s_park + 768)[tid] = tn0;
+    }
+    for (int q0 = 512; q0 < nrows; q0 += 512) {
+        const int qa = q0 + tid, qb = qa + 256;
+        const int ida = qa < a.gcap ? rows[qa] : -1, idb = qb < a.gcap ? rows[qb] : -1;
+        stage_load(ida, idb);
+        stage_store(qa, ida, idb);
     }
     if constexpr (FUSED) {
         // wave 0 reduces the guard slots for the block, overlapped with the
@@ -487,7 +542,9 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
 #pragma unroll
         for (int k = 0; k < G; k++) {
             const int t = (int)t0 + k;
-            bool live = dd.ok[k] && t >= b0 && t < b1;
+            // a group lies inside the node's run and the dead padding after it
+            // (runs start on 32-bond lines), so its state bytes alone decide
+            bool live = dd.ok[k];
             double du[DIM];
 #pragma unroll
             for (int c = 0; c < DIM; c++) du[c] = 0.0;
@@ -588,9 +645,11 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         F32G acc;              // the lane's FP64 family sum per component
 #pragma unroll
         for (int c = 0; c < DIM; c++) acc[c] = 0.0;
-        for (int t0 = tfirst; t0 < b1; t0 += W) {
+        F64G f64;
+        if constexpr (!PRE) load_group((unsigned)tfirst, d);
+        if (tfirst < b1) compute_group((unsigned)tfirst, d, f64, acc);
+        for (int t0 = tfirst + W; t0 < b1; t0 += W) {
             load_group((unsigned)t0, d);
-            F64G f64;
             compute_group((unsigned)t0, d, f64, acc);
         }
         butterfly64<1, S, DIM>(acc);
@@ -636,6 +695,12 @@ __global__ void __launch_bounds__(256) k_frac_step(StepArgs a)
         mymax = bits > mymax ? bits : mymax;
     };
     if constexpr (!EARLY) load_update();
+    if constexpr (PRE) {
+        uo_pre = s_park[tid];
+        pv0 = s_park[256 + tid];
+        pvel = s_park[512 + tid];
+        tn0 = reinterpret_cast<const int *>(s_park + 768)[tid];
+    }
     if (upd) {
         const int c = sl;
         const size_t idx = (size_t)i * DIM + c;
@@ -724,6 +789,13 @@ constexpr size_t step_force_rows()
     return FF ? 0 : (size_t)4 * (64 / StepCfg<D, FF>::S) * (StepCfg<D, FF>::G * StepCfg<D, FF>::S * D + 1);
 }
 
+// LDS doubles where the 2D FP32C step parks its update inputs (4 per thread)
+template <int D, bool FF>
+constexpr size_t step_park()
+{
+    return (D == 2 && FF) ? 4 * 256 : 0;
+}
+
 // nodes per step block; both precisions tile nodes alike, so one LDS
 // neighbour table serves either
 template <int D>
@@ -780,7 +852,7 @@ static int launch_any(pxg_ctx *ctx, int mode, const double *u, const double *uol
     const bool nto = fused && !(f32 && F.pofs.p) && (f32 || opbytes >= ((size_t)128 << 20));
 #define PXG_LAUNCH(D, FF, M, FU, NT)                                                                \
     hipLaunchKernelGGL((k_frac_step<D, FF, M, FU, NT>), grid, dim3(256),                              \
-                       sizeof(double) * (step_force_rows<D, FF>() + (size_t)F.gcap * rec_stride(D)), st, a)
+                       sizeof(double) * (step_force_rows<D, FF>() + (size_t)F.gcap * rec_stride(D) + step_park<D, FF>()), st, a)
 #define PXG_FUSED(D, FF, M)                                                                         \
     if (!fused) PXG_LAUNCH(D, FF, M, false, false);                                                 \
     else if (nto) PXG_LAUNCH(D, FF, M, true, true);                                                 \
