@@ -427,9 +427,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     // The compiler tests the bond states inside the frozen/live branch,
     // which waits for every load issued so far.  FP32C reads them after its
     // operators (its first group goes out with the staging loads, and the
-    // wait then covers them all); FP64 reads them first (measured on the
+    // wait then covers them all); 2D FP64 reads them first (measured on the
     // 400x200 plate: 10.75 us against 11.10 us for FP32C, 17.27 us against
-    // 18.17 us for FP64).
+    // 18.17 us for FP64), 3D FP64 after (160x80x40 block: 592 us against
+    // 611 us).
     auto load_group = [&](unsigned t0, GD &d) {
         auto load_state = [&]() {
             if (a.frozen) {
@@ -445,13 +446,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
             }
         };
         vload(a.gslot + t0, d.sv);
-        if constexpr (!F32) load_state();
+        if constexpr (!F32 && DIM == 2) load_state();
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             if constexpr (F32) vload<NTO>(P32c[c] + (unsigned)((int)t0 + po), d.p32[c]);
             else vload<NTO>(P64c[c] + t0, d.p64[c]);
         }
-        if constexpr (F32) load_state();
+        if constexpr (F32 || DIM == 3) load_state();
     };
     GD d;
     const int tfirst = A + G * sl;
