@@ -575,25 +575,48 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                     double xs[DIM];
 #pragma unroll
                     for (int c = 0; c < DIM; c++) xs[c] = xj[c] - xi[c];
-                    double rr = xs[0] * xs[0] + xs[1] * xs[1];
-                    if (DIM == 3) rr = rr + xs[DIM - 1] * xs[DIM - 1];
+                    // 2D FP32C forms q and the band with fused multiply-adds
+                    // (plate 400x200: 10.36 us against 10.55 us; the other
+                    // forms measured slower with them, the 3D ones by up to
+                    // 20 %): they move q by rounding far inside the band, and
+                    // inside it the reference's expression, rounded as the
+                    // reference rounds it, decides
+                    constexpr bool FMAQ = DIM == 2 && F32;
+                    double yv[DIM];
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) yv[c] = xs[c] + du[c];
+                    double rr, yy2;
+                    if constexpr (FMAQ) {
+                        rr = fma(xs[0], xs[0], xs[1] * xs[1]);
+                        yy2 = fma(yv[0], yv[0], yv[1] * yv[1]);
+                    } else {
+                        rr = xs[0] * xs[0] + xs[1] * xs[1];
+                        yy2 = yv[0] * yv[0] + yv[1] * yv[1];
+                        if (DIM == 3) {
+                            rr = rr + xs[DIM - 1] * xs[DIM - 1];
+                            yy2 = yy2 + yv[DIM - 1] * yv[DIM - 1];
+                        }
+                    }
                     if (rr > 0.0) {
-                        const double yx = xs[0] + du[0], yy = xs[1] + du[1];
-                        double yy2 = yx * yx + yy * yy;
-                        if (DIM == 3) { const double yz = xs[DIM - 1] + du[DIM - 1]; yy2 = yy2 + yz * yz; }
                         const double band = 1.0e-9 * rr;
-                        const double qh = yy2 - rr * a.chi;
+                        const double qh = FMAQ ? fma(-rr, a.chi, yy2) : yy2 - rr * a.chi;
                         bool brk = qh > 0.0;
                         bool unsure = fabs(qh) <= band;
                         if (!a.tension_only) {
-                            const double ql = yy2 - rr * a.clo;
+                            const double ql = FMAQ ? fma(-rr, a.clo, yy2) : yy2 - rr * a.clo;
                             brk = brk || ql < 0.0;
                             unsure = unsure || fabs(ql) <= band;
                         }
                         if (unsure) {
                             // near the threshold: the reference's own expression
-                            const double rl = sqrt(rr);
-                            const double s = (sqrt(yy2) - rl) / rl;
+                            double re = xs[0] * xs[0] + xs[1] * xs[1];
+                            double ye = yv[0] * yv[0] + yv[1] * yv[1];
+                            if (DIM == 3) {
+                                re = re + xs[DIM - 1] * xs[DIM - 1];
+                                ye = ye + yv[DIM - 1] * yv[DIM - 1];
+                            }
+                            const double rl = sqrt(re);
+                            const double s = (sqrt(ye) - rl) / rl;
                             brk = a.tension_only ? (s > a.sc) : (fabs(s) > a.sc);
                         }
                         if (brk) {
